@@ -1,0 +1,456 @@
+/*
+ * gq_cost_model.h -- the Postgres-12 join cost model of the reference, in
+ * float, written once for the GPU kernels and the host drivers.
+ *
+ * Follows src/backend/optimizer/gpuqo/gpuqo_cost.cuh:43-158,
+ * gpuqo_cost_postgres.cuh:89-863 and gpuqo_row_estimation.cuh:26-158:
+ * every float operation happens in the reference's order, so results are
+ * bit-identical when compiled with -ffp-contract=off and IEEE division.
+ *
+ * Layout difference (not an arithmetic one): instead of testing every
+ * equivalence class (EC) per join, the caller passes the bitmask of ECs that
+ * have members on both sides, built from per-relation incidence masks, and
+ * only those are visited, in ascending EC index.  Classes with no member on
+ * one side contribute selectivity 1.0f (an exact multiply), no qual and no
+ * bucket-size or index candidate in the reference, so skipping them changes
+ * no bit.
+ */
+#ifndef GQ_COST_MODEL_H
+#define GQ_COST_MODEL_H
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define GQ_HD __host__ __device__ __forceinline__
+#else
+#define GQ_HD inline
+#endif
+
+#define GQ_MAX_ECW 8            /* EC masks: up to 512 equivalence classes */
+
+namespace gq {
+
+/* ---- relation-set operations for the set types used (u32 / u64 / SetN) -- */
+GQ_HD int popc(uint32_t x) { return __builtin_popcount(x); }
+GQ_HD int popc(uint64_t x) { return __builtin_popcountll(x); }
+GQ_HD int ctz(uint32_t x) { return __builtin_ctz(x); }
+GQ_HD int ctz(uint64_t x) { return __builtin_ctzll(x); }
+GQ_HD bool empty(uint32_t x) { return x == 0; }
+GQ_HD bool empty(uint64_t x) { return x == 0; }
+GQ_HD bool bit(uint32_t x, int i) { return (x >> i) & 1u; }
+GQ_HD bool bit(uint64_t x, int i) { return (x >> i) & 1ull; }
+/* number of members of `y` below the lowest member of `x` */
+GQ_HD int rank_below(uint32_t y, uint32_t x) { return popc(y & ((x & (0u - x)) - 1u)); }
+GQ_HD int rank_below(uint64_t y, uint64_t x) {
+    uint64_t low = x & ((uint64_t)0 - x);
+    return __builtin_popcountll(y & (low - (uint64_t)1));
+}
+GQ_HD uint32_t lowbit(uint32_t x) { return x & (0u - x); }
+GQ_HD uint64_t lowbit(uint64_t x) { return x & ((uint64_t)0 - x); }
+
+/* Multi-word set for host drivers above 64 relations (bit i = relation i). */
+template <int W>
+struct SetN {
+    uint64_t w[W];
+    GQ_HD SetN() { for (int i = 0; i < W; i++) w[i] = 0; }
+    GQ_HD static SetN nth(int n) { SetN s; s.w[n >> 6] = 1ull << (n & 63); return s; }
+    GQ_HD SetN operator&(const SetN &o) const { SetN r; for (int i = 0; i < W; i++) r.w[i] = w[i] & o.w[i]; return r; }
+    GQ_HD SetN operator|(const SetN &o) const { SetN r; for (int i = 0; i < W; i++) r.w[i] = w[i] | o.w[i]; return r; }
+    GQ_HD SetN operator-(const SetN &o) const { SetN r; for (int i = 0; i < W; i++) r.w[i] = w[i] & ~o.w[i]; return r; }
+    GQ_HD SetN &operator|=(const SetN &o) { for (int i = 0; i < W; i++) w[i] |= o.w[i]; return *this; }
+    GQ_HD SetN &operator&=(const SetN &o) { for (int i = 0; i < W; i++) w[i] &= o.w[i]; return *this; }
+    GQ_HD SetN &operator-=(const SetN &o) { for (int i = 0; i < W; i++) w[i] &= ~o.w[i]; return *this; }
+    GQ_HD bool operator==(const SetN &o) const { for (int i = 0; i < W; i++) if (w[i] != o.w[i]) return false; return true; }
+    GQ_HD bool operator!=(const SetN &o) const { return !(*this == o); }
+    GQ_HD bool operator<(const SetN &o) const {
+        for (int i = W - 1; i >= 0; i--) if (w[i] != o.w[i]) return w[i] < o.w[i];
+        return false;
+    }
+    GQ_HD void set(int n) { w[n >> 6] |= 1ull << (n & 63); }
+    GQ_HD void unset(int n) { w[n >> 6] &= ~(1ull << (n & 63)); }
+};
+template <int W> GQ_HD int popc(const SetN<W> &x) { int s = 0; for (int i = 0; i < W; i++) s += popc(x.w[i]); return s; }
+template <int W> GQ_HD bool empty(const SetN<W> &x) { for (int i = 0; i < W; i++) if (x.w[i]) return false; return true; }
+template <int W> GQ_HD int ctz(const SetN<W> &x) { for (int i = 0; i < W; i++) if (x.w[i]) return i * 64 + ctz(x.w[i]); return -1; }
+template <int W> GQ_HD bool bit(const SetN<W> &x, int i) { return (x.w[i >> 6] >> (i & 63)) & 1ull; }
+template <int W> GQ_HD SetN<W> lowbit(const SetN<W> &x) { SetN<W> r; int p = ctz(x); if (p >= 0) r.set(p); return r; }
+template <int W> GQ_HD int rank_below(const SetN<W> &y, const SetN<W> &x) {
+    int p = ctz(x), s = 0;
+    for (int i = 0; i < W; i++) {
+        uint64_t m;
+        if (p >= (i + 1) * 64) m = ~0ull;
+        else if (p > i * 64) m = (1ull << (p - i * 64)) - 1;
+        else m = 0;
+        s += popc(y.w[i] & m);
+    }
+    return s;
+}
+
+/* ---- query description seen by the cost model ------------------------- */
+struct Params {             /* GpuqoPlannerInfoParams subset (float), planner_info.cuh:79-96 */
+    float seq_page_cost;
+    float cpu_tuple_cost;
+    float cpu_operator_cost;
+    float disable_cost;
+    int work_mem;
+    int enable_nestloop;
+    int enable_hashjoin;
+};
+
+/* Member flags */
+#define GQ_MF_BUCKET_DEFAULT 1u   /* stadistinct resolves to <= 0: bucket size 0.1 */
+#define GQ_MF_INDEX 2u            /* index available                               */
+#define GQ_MF_UNIQUE 4u           /* index unique                                  */
+
+template <class S>
+struct View {
+    int n_rels, n_ec, ecw;
+    Params p;
+    const float *rel_rows;        /* [n_rels] base rows (bucket-size path)         */
+    const float *rel_fksel;       /* [n_rels] float(1.0 / max(1.0f, tuples))      */
+    S composite;                  /* relations that are composites (UnionDP)       */
+    const uint64_t *inc;          /* [n_rels * ecw]: ECs containing the relation   */
+    const S *ec_relids;           /* [n_ec]                                        */
+    const int *ec_moff;           /* [n_ec] first member index                     */
+    const int *ec_soff;           /* [n_ec] first selectivity index                */
+    const uint8_t *ec_hasfk;      /* [n_ec] some member holds an FK                */
+    const S *mem_fks;             /* [n_members]                                   */
+    const float *mem_nd;          /* ndistinct after the rows/tuples adjustment    */
+    const float *mem_avgfreq;
+    const float *mem_mcv;
+    const uint8_t *mem_flags;
+    const float *mem_idx_rows, *mem_idx_startup, *mem_idx_total;
+    const float *sels;
+    const float *qualtab;         /* qualtab[k] = cpu_operator_cost added k times  */
+};
+
+struct Rel {                /* value part of JoinRelation (planner_info.cuh:22-28) */
+    float rows, startup, total;
+    int width;
+};
+
+GQ_HD int eq_class_index(int l, int r, int size) {    /* gpuqo_common.h:71-79 */
+    if (l > r) { int t = l; l = r; r = t; }
+    return l * size - l * (l + 1) / 2 + (r - l - 1);
+}
+
+/* float -> int as the reference's x86 host build converts it (cvttss2si):
+ * values outside int32 range and NaN become INT_MIN.  Used where the
+ * reference passes a float to an int (cost_postgres.cuh:370, :573, :615). */
+GQ_HD int f2i_x86(float f) {
+    if (f >= -2147483648.0f && f < 2147483648.0f) return (int)f;
+    return (int)0x80000000u;
+}
+
+GQ_HD int floor_log2_l(long num) { return 63 - __builtin_clzll((unsigned long long)num); }
+GQ_HD int ceil_log2_l(long num) {
+    int r = floor_log2_l(num);
+    return (1L << r) == num ? r : r + 1;
+}
+
+GQ_HD float clamp_row_est(float nrows) {               /* cost_postgres.cuh:850-863 */
+    return nrows <= 1.0f ? 1.0f : rintf(nrows);
+}
+GQ_HD unsigned long maxalign8(long len) { return ((unsigned long)len + 7ul) & ~7ul; }
+GQ_HD float relation_byte_size(float tuples, int width) {   /* :822-826 */
+    return tuples * (float)(maxalign8(width) + 24ul);
+}
+GQ_HD float page_size(float tuples, int width) {            /* :832-840 */
+    return ceilf(relation_byte_size(tuples, width) / 8192.0f);
+}
+
+struct HashSize { int nbuckets, nbatches; };
+
+/* ExecChooseHashTableSize with NTUP_PER_BUCKET 1, SKEW_WORK_MEM_PERCENT 2,
+ * SKEW_OVERHEAD 84, TUPLE_OVERHEAD 32, MaxAllocSize 0x3fffffff and 8-byte
+ * pointers (cost_postgres.cuh:472-618).  min/max on float are fminf/fmaxf. */
+GQ_HD HashSize hash_table_size(float ntuples, int tupwidth, int work_mem) {
+    HashSize res;
+    res.nbatches = 1;
+    if (ntuples <= 0.0f) ntuples = 1000.0f;
+    int tupsize = (int)(32ul + maxalign8(tupwidth));
+    float inner_rel_bytes = ntuples * (float)tupsize;
+    long hash_table_bytes = (long)work_mem * 1024L;
+    float space_allowed = (float)hash_table_bytes;
+    long skew_table_bytes = hash_table_bytes * 2 / 100;
+    long num_skew_mcvs = skew_table_bytes / (long)(tupsize + 84);
+    if ((int)num_skew_mcvs > 0) hash_table_bytes -= skew_table_bytes;
+    long max_pointers = (long)(space_allowed / 8.0f);
+    const long alloc_ptrs = 0x3fffffffL / 8;
+    if (alloc_ptrs < max_pointers) max_pointers = alloc_ptrs;
+    long mppow2 = 1L << ceil_log2_l(max_pointers);
+    if (max_pointers != mppow2) max_pointers = mppow2 / 2;
+    if (max_pointers > 0x7fffffffL / 2) max_pointers = 0x7fffffffL / 2;
+    float dbuckets = ceilf(ntuples);
+    dbuckets = fminf(dbuckets, (float)max_pointers);
+    int nb = f2i_x86(dbuckets);
+    if (nb < 1024) nb = 1024;
+    nb = 1 << ceil_log2_l(nb);
+    long bucket_bytes = 8L * (long)nb;
+    if (inner_rel_bytes + (float)bucket_bytes > (float)hash_table_bytes) {
+        long bucket_size = (long)tupsize + 8L;
+        long lbuckets = 1L << ceil_log2_l(hash_table_bytes / bucket_size);
+        if (max_pointers < lbuckets) lbuckets = max_pointers;
+        nb = (int)lbuckets;
+        nb = 1 << ceil_log2_l(nb);
+        bucket_bytes = (long)nb * 8L;
+        float dbatch = ceilf(inner_rel_bytes / (float)(hash_table_bytes - bucket_bytes));
+        dbatch = fminf(dbatch, (float)max_pointers);
+        int minbatch = f2i_x86(dbatch);
+        int p2 = 1 << ceil_log2_l(minbatch);
+        res.nbatches = p2 > 2 ? p2 : 2;
+    }
+    res.nbuckets = nb;
+    return res;
+}
+
+/* Match mask of the ECs with members on both sides. */
+template <class S>
+GQ_HD int match_mask(const View<S> &q, const S &l, const S &r, uint64_t *m) {
+    uint64_t ml[GQ_MAX_ECW], mr[GQ_MAX_ECW];
+    for (int w = 0; w < q.ecw; w++) { ml[w] = 0; mr[w] = 0; }
+    S t = l;
+    while (!empty(t)) {
+        int i = ctz(t);
+        t = t - lowbit(t);
+        for (int w = 0; w < q.ecw; w++) ml[w] |= q.inc[i * q.ecw + w];
+    }
+    t = r;
+    while (!empty(t)) {
+        int i = ctz(t);
+        t = t - lowbit(t);
+        for (int w = 0; w < q.ecw; w++) mr[w] |= q.inc[i * q.ecw + w];
+    }
+    int n = 0;
+    for (int w = 0; w < q.ecw; w++) { m[w] = ml[w] & mr[w]; n += popc(m[w]); }
+    return n;
+}
+
+/* fkselc_of_ec (row_estimation.cuh:26-45): returns false when no FK applies. */
+template <class S>
+GQ_HD bool fk_sel(const View<S> &q, int e, const S &outer_m, const S &inner_m, float *out) {
+    S ec = q.ec_relids[e];
+    S t = outer_m;
+    while (!empty(t)) {
+        S o = lowbit(t);
+        int idx = rank_below(ec, o);
+        S match = inner_m & q.mem_fks[q.ec_moff[e] + idx];
+        if (!empty(match)) { *out = q.rel_fksel[ctz(match)]; return true; }
+        t = t - o;
+    }
+    return false;
+}
+
+/* estimate_ec_selectivity for a class with members on both sides
+ * (row_estimation.cuh:48-86). */
+template <class S>
+GQ_HD float ec_selectivity(const View<S> &q, int e, const S &l, const S &r) {
+    S ec = q.ec_relids[e];
+    S ml = ec & l, mr = ec & r;
+    float s;
+    if (q.ec_hasfk[e]) {
+        if (fk_sel(q, e, ml, mr, &s)) return s;
+        if (fk_sel(q, e, mr, ml, &s)) return s;
+    }
+    int il = rank_below(ec, ml), ir = rank_below(ec, mr);
+    return q.sels[q.ec_soff[e] + eq_class_index(il, ir, popc(ec))];
+}
+
+/* Visit the classes of mask `m` in ascending index. */
+template <class S, class F>
+GQ_HD void for_each_ec(const View<S> &q, const uint64_t *m, F f) {
+    for (int w = 0; w < q.ecw; w++) {
+        uint64_t b = m[w];
+        while (b) {
+            int e = w * 64 + __builtin_ctzll(b);
+            b &= b - 1;
+            f(e);
+        }
+    }
+}
+
+/* estimate_join_selectivity (row_estimation.cuh:90-117) */
+template <class S>
+GQ_HD float join_selectivity(const View<S> &q, const S &l, const S &r, const uint64_t *m) {
+    float sel = 1.0f;
+    for_each_ec(q, m, [&](int e) { sel *= ec_selectivity(q, e, l, r); });
+    return sel;
+}
+
+/* estimate_join_rows (row_estimation.cuh:120-133) */
+GQ_HD float join_rows(float sel, float lrows, float rrows) {
+    float rows = sel * lrows * rrows;
+    return rows > 1.0f ? roundf(rows) : 1.0f;
+}
+
+struct Extra { bool inner_unique, indexed; float joinrows, outer_match_frac, match_count; };
+
+/* cost_nestloop (cost_postgres.cuh:110-275); qual_per_tuple is
+ * cost_qual_eval's per-tuple cost (its startup is 0). */
+template <class S>
+GQ_HD void nestloop(const View<S> &q, const Rel &outer, float in_rows, float in_startup, float in_total,
+                    const Extra &ex, float qual_per_tuple, float *o_startup, float *o_total) {
+    float startup_cost = 0.0f, run_cost = 0.0f, ntuples;
+    float outer_path_rows = outer.rows, inner_path_rows = in_rows;
+    if (!q.p.enable_nestloop) startup_cost += q.p.disable_cost;
+    startup_cost += in_startup + outer.startup;
+    run_cost += outer.total - outer.startup;
+    if (outer_path_rows > 1.0f) run_cost += (outer_path_rows - 1.0f) * in_startup;
+    float inner_run_cost = in_total - in_startup;
+    float inner_rescan_run_cost = inner_run_cost;
+    if (ex.inner_unique) {
+        float outer_matched_rows = rintf(outer_path_rows * ex.outer_match_frac);
+        float outer_unmatched_rows = outer_path_rows - outer_matched_rows;
+        float inner_scan_frac = 2.0f / (ex.match_count + 1.0f);
+        ntuples = outer_matched_rows * inner_path_rows * inner_scan_frac;
+        if (ex.indexed) {
+            run_cost += inner_run_cost * inner_scan_frac;
+            if (outer_matched_rows > 1.0f)
+                run_cost += (outer_matched_rows - 1.0f) * inner_rescan_run_cost * inner_scan_frac;
+            run_cost += outer_unmatched_rows * inner_rescan_run_cost / inner_path_rows;
+        } else {
+            ntuples += outer_unmatched_rows * inner_path_rows;
+            run_cost += inner_run_cost;
+            if (outer_unmatched_rows >= 1.0f) outer_unmatched_rows -= 1.0f;
+            else outer_matched_rows -= 1.0f;
+            if (outer_matched_rows > 0.0f)
+                run_cost += outer_matched_rows * inner_rescan_run_cost * inner_scan_frac;
+            if (outer_unmatched_rows > 0.0f)
+                run_cost += outer_unmatched_rows * inner_rescan_run_cost;
+        }
+    } else {
+        run_cost += inner_run_cost;
+        if (outer_path_rows > 1.0f) run_cost += (outer_path_rows - 1.0f) * inner_rescan_run_cost;
+        ntuples = outer_path_rows * inner_path_rows;
+    }
+    startup_cost += 0.0f;   /* cost_qual_eval startup (always 0); keeps -0.0 -> +0.0 */
+    float cpu_per_tuple = q.p.cpu_tuple_cost + qual_per_tuple;
+    run_cost += cpu_per_tuple * ntuples;
+    *o_startup = startup_cost;
+    *o_total = startup_cost + run_cost;
+}
+
+/* __estimate_hash_bucketsize with the nbuckets-independent part precomputed
+ * per member (cost_postgres.cuh:701-775). */
+template <class S>
+GQ_HD float bucketsize(const View<S> &q, int mi, int nbuckets) {
+    if (q.mem_flags[mi] & GQ_MF_BUCKET_DEFAULT) return 0.1f;
+    float nd = q.mem_nd[mi], avgfreq = q.mem_avgfreq[mi], mcv = q.mem_mcv[mi];
+    float fb = (float)nbuckets;
+    float est = nd > fb ? 1.0f / fb : 1.0f / nd;
+    if (avgfreq > 0.0f && mcv > avgfreq) est *= mcv / avgfreq;
+    if (est < 1.0e-6f) est = 1.0e-6f;
+    else if (est > 1.0f) est = 1.0f;
+    return est;
+}
+
+/* cost_hashjoin (cost_postgres.cuh:287-470) */
+template <class S>
+GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const Rel &inner, const Extra &ex,
+                    int n_quals, float qual_per_tuple, const uint64_t *m, float *o_startup, float *o_total) {
+    float startup_cost = 0.0f, run_cost = 0.0f;
+    float outer_path_rows = outer.rows, inner_path_rows = inner.rows;
+    if (!q.p.enable_hashjoin) startup_cost += q.p.disable_cost;
+    startup_cost += outer.startup;
+    run_cost += outer.total - outer.startup;
+    startup_cost += inner.total;
+    startup_cost += (q.p.cpu_operator_cost * (float)n_quals + q.p.cpu_tuple_cost) * inner_path_rows;
+    run_cost += q.p.cpu_operator_cost * (float)n_quals * outer_path_rows;
+    HashSize hs = hash_table_size(inner_path_rows, inner.width, q.p.work_mem);
+    if (hs.nbatches > 1) {
+        float outerpages = page_size(outer_path_rows, outer.width);
+        float innerpages = page_size(inner_path_rows, inner.width);
+        startup_cost += q.p.seq_page_cost * innerpages;
+        run_cost += q.p.seq_page_cost * (innerpages + 2.0f * outerpages);
+    }
+    float virtualbuckets = (float)hs.nbuckets * (float)hs.nbatches;
+    int nbk = f2i_x86(virtualbuckets);
+    /* estimate_hash_bucket_stats (:620-668) */
+    float bsize = 1.0f, mcvfreq = 1.0f;
+    for_each_ec(q, m, [&](int e) {
+        S ec = q.ec_relids[e];
+        S mr = ec & inner_id;
+        while (!empty(mr)) {
+            S in_id = lowbit(mr);
+            int mi = q.ec_moff[e] + rank_below(ec, in_id);
+            float t = bucketsize(q, mi, nbk);
+            if (bsize > t) bsize = t;
+            if (mcvfreq > q.mem_mcv[mi]) mcvfreq = q.mem_mcv[mi];
+            mr = mr - in_id;
+        }
+    });
+    if (relation_byte_size(clamp_row_est(inner_path_rows * mcvfreq), inner.width) >
+        (float)((long)q.p.work_mem * 1024L))
+        startup_cost += q.p.disable_cost;
+    float hashjointuples;
+    startup_cost += 0.0f;   /* hash_qual_cost.startup (always 0), both branches */
+    if (ex.inner_unique) {
+        float outer_matched_rows = rintf(outer_path_rows * ex.outer_match_frac);
+        float inner_scan_frac = 2.0f / (ex.match_count + 1.0f);
+        run_cost += qual_per_tuple * outer_matched_rows *
+                    clamp_row_est(inner_path_rows * bsize * inner_scan_frac) * 0.5f;
+        run_cost += qual_per_tuple * (outer_path_rows - outer_matched_rows) *
+                    clamp_row_est(inner_path_rows / virtualbuckets) * 0.05f;
+        hashjointuples = outer_matched_rows;
+    } else {
+        run_cost += qual_per_tuple * outer_path_rows * clamp_row_est(inner_path_rows * bsize) * 0.5f;
+        hashjointuples = ex.joinrows;
+    }
+    run_cost += q.p.cpu_tuple_cost * hashjointuples;
+    *o_startup = startup_cost;
+    *o_total = startup_cost + run_cost;
+}
+
+/* One join: rows (estimate_join_rows) and cost (calc_join_cost,
+ * gpuqo_cost.cuh:76-158) of outer `l` joined with inner `r`. */
+template <class S>
+GQ_HD Rel join(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R) {
+    uint64_t m[GQ_MAX_ECW];
+    int n_quals = match_mask(q, l, r, m);
+    Rel out;
+    out.rows = join_rows(join_selectivity(q, l, r, m), L.rows, R.rows);
+    out.width = L.width + R.width;
+    float qual_pt = q.qualtab[n_quals];
+    bool single = popc(r) == 1;
+    Extra ex;
+    ex.inner_unique = false;
+    if (single) {                                   /* is_inner_unique :43-74 */
+        for_each_ec(q, m, [&](int e) {
+            int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
+            if ((q.mem_flags[mi] & (GQ_MF_INDEX | GQ_MF_UNIQUE)) == (GQ_MF_INDEX | GQ_MF_UNIQUE))
+                ex.inner_unique = true;
+        });
+    }
+    ex.indexed = false;
+    ex.joinrows = out.rows;
+    ex.outer_match_frac = out.rows / (L.rows * R.rows);
+    ex.match_count = R.rows;
+    float bs, bt;
+    nestloop(q, L, R.rows, R.startup, R.total, ex, qual_pt, &bs, &bt);
+    if (q.p.enable_hashjoin) {
+        float hs, ht;
+        hashjoin(q, r, L, R, ex, n_quals, qual_pt, m, &hs, &ht);
+        if (ht < bt) { bs = hs; bt = ht; }
+    }
+    if (single && !bit(q.composite, ctz(r))) {       /* index nested loop :116-147 */
+        ex.indexed = true;
+        for_each_ec(q, m, [&](int e) {
+            int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
+            if (q.mem_flags[mi] & GQ_MF_INDEX) {
+                float is, it;
+                nestloop(q, L, q.mem_idx_rows[mi], q.mem_idx_startup[mi], q.mem_idx_total[mi], ex, qual_pt,
+                         &is, &it);
+                if (it < bt) { bs = is; bt = it; }
+            }
+        });
+    }
+    out.startup = bs;
+    out.total = bt;
+    return out;
+}
+
+}  // namespace gq
+
+#endif /* GQ_COST_MODEL_H */

@@ -1,0 +1,58 @@
+/*
+ * gq_dp.h -- host-side interface of the GPU DP (DPsub / MPDP / DPsize) over
+ * one query of at most 32 relations, already relabelled.  Internal to the
+ * library; the public C ABI is include/gpuqo_amd.h.
+ */
+#ifndef GQ_DP_H
+#define GQ_DP_H
+
+#include <stdint.h>
+#include <vector>
+
+#include "gq_cost_model.h"
+
+namespace gq {
+
+/* Flattened query for the device (32-bit relation sets, relation i = bit i).
+ * Layout of the blob (32-bit words) described by BlobHdr; staged into LDS by
+ * every kernel. */
+struct BlobHdr {
+    int n_rels, n_ec, n_mem, n_sels, ecw;
+    Params p;
+    uint32_t composite;
+    int words;
+    int o_rel_rows, o_rel_fksel, o_adj, o_inc, o_ec_relids, o_ec_moff, o_ec_soff, o_ec_hasfk;
+    int o_mem_fks, o_mem_nd, o_mem_avgfreq, o_mem_mcv, o_mem_flags, o_mem_idx_rows, o_mem_idx_startup,
+        o_mem_idx_total, o_sels, o_qualtab, o_base;
+};
+
+struct DPInput {
+    BlobHdr hdr;
+    std::vector<uint32_t> blob;
+    /* base relation values (memo level 1) */
+    std::vector<Rel> base;
+};
+
+struct DPNode {                 /* plan node in DP numbering, pre-order */
+    uint32_t set;
+    float rows, startup, total;
+    int32_t width, left, right, pad;
+};
+
+struct DPStats {
+    uint64_t candidates = 0, sets = 0, pairs = 0, memo_bytes = 0;
+    double gpu_ms = 0.0;
+    int memo_kind = 1;
+};
+
+enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
+
+/* Run one DP on the current device.  `n_iters` < n_rels selects the IDP1
+ * level cap (the tree returned is the cheapest set of that size).  Returns a
+ * gq_status. */
+int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vector<DPNode> &tree,
+           DPStats &stats);
+
+}  // namespace gq
+
+#endif

@@ -1,0 +1,653 @@
+/*
+ * gq_dp.hip -- DPsub / MPDP level loop on CDNA4 (gfx950, wave64).
+ *
+ * One DP over n <= 32 relations (relation i = bit i of a uint32_t set).
+ * Per level k (k = 2..n) three kernels are enqueued back to back on one
+ * stream, with no host synchronisation between levels:
+ *
+ *   k_filter      unrank the C(n,k) candidate sets (combinadic rank ->
+ *                 k-subset in increasing numeric order, then Gosper's next
+ *                 permutation), keep the connected ones (closure over the
+ *                 LDS adjacency table), compact them with one atomic per wave
+ *                 and mark them in the connectivity bitmap.
+ *   k_eval_*      evaluate the splits of every kept set with the cost model
+ *                 staged in LDS, reading the memo of the smaller levels.
+ *   k_finalize    (split-parallel mode only) rebuild the winning split.
+ *
+ * The memo is dense: entry S lives at index S (a perfect hash; 2^n entries
+ * of 16 B plus a 4 B left child), which fits HBM up to n = 32.  Ties and NaN
+ * costs follow the reference CPU oracles: per set the first valid split in
+ * their enumeration order is kept unless a later one is strictly cheaper
+ * (gpuqo_cpu_common.cuh:189-211), which the kernels reproduce either with a
+ * sequential scan inside one lane or with an associative (first, firstmin)
+ * reduction over packed (cost, rank) keys.
+ *
+ * Reference counterparts: gpuqo_dpsub_common.cu:177-392 (level loop),
+ * gpuqo_dpsub_filtered_kernels.cuh:30-240 (unrank/filter/evaluate),
+ * gpuqo_dpsub_enum_all_subs.cuh:21-72 (all-subsets splits),
+ * gpuqo_dpsub_bicc.cuh:35-313 (MPDP), gpuqo_cpu_dpsub_bicc.cu:65-236 (the
+ * block order reproduced here).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "gq_dp.h"
+#include "gq_internal.h"
+
+namespace gq {
+
+#define WAVE 64
+#define BLOCK 256
+
+__constant__ uint32_t c_binom[33 * 33];   /* C(n,k) at [n*33+k], n,k <= 32 */
+
+struct DevBufs {
+    float4 *vals;          /* memo: rows, startup, total, width(bits) */
+    uint32_t *left;        /* memo: left child set                    */
+    uint32_t *conn;        /* connectivity bitmap, 2^n bits           */
+    uint32_t *lvl_sets;    /* sets of the current level               */
+    unsigned long long *lvl_best;
+    uint32_t *lvl_first;
+    uint32_t *counters;    /* [0..32] sets per level, [40] pairs lo, [41] pairs hi */
+    uint32_t *blob;
+    DPNode *nodes;
+};
+
+/* ---------------------------------------------------------------- helpers */
+
+__device__ __forceinline__ View<uint32_t> make_view(const BlobHdr &h, const uint32_t *s) {
+    View<uint32_t> v;
+    v.n_rels = h.n_rels;
+    v.n_ec = h.n_ec;
+    v.ecw = h.ecw;
+    v.p = h.p;
+    v.rel_rows = (const float *)(s + h.o_rel_rows);
+    v.rel_fksel = (const float *)(s + h.o_rel_fksel);
+    v.composite = h.composite;
+    v.inc = (const uint64_t *)(s + h.o_inc);
+    v.ec_relids = s + h.o_ec_relids;
+    v.ec_moff = (const int *)(s + h.o_ec_moff);
+    v.ec_soff = (const int *)(s + h.o_ec_soff);
+    v.ec_hasfk = (const uint8_t *)(s + h.o_ec_hasfk);
+    v.mem_fks = s + h.o_mem_fks;
+    v.mem_nd = (const float *)(s + h.o_mem_nd);
+    v.mem_avgfreq = (const float *)(s + h.o_mem_avgfreq);
+    v.mem_mcv = (const float *)(s + h.o_mem_mcv);
+    v.mem_flags = (const uint8_t *)(s + h.o_mem_flags);
+    v.mem_idx_rows = (const float *)(s + h.o_mem_idx_rows);
+    v.mem_idx_startup = (const float *)(s + h.o_mem_idx_startup);
+    v.mem_idx_total = (const float *)(s + h.o_mem_idx_total);
+    v.sels = (const float *)(s + h.o_sels);
+    v.qualtab = (const float *)(s + h.o_qualtab);
+    return v;
+}
+
+__device__ __forceinline__ void stage(const uint32_t *g, uint32_t *s, int words) {
+    for (int i = threadIdx.x; i < words; i += blockDim.x) s[i] = g[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ bool is_conn(const uint32_t *conn, uint32_t x) {
+    return (conn[x >> 5] >> (x & 31)) & 1u;
+}
+
+__device__ __forceinline__ Rel load_rel(const float4 *vals, uint32_t x) {
+    float4 v = vals[x];
+    Rel r;
+    r.rows = v.x;
+    r.startup = v.y;
+    r.total = v.z;
+    r.width = __float_as_int(v.w);
+    return r;
+}
+
+__device__ __forceinline__ void store_rel(float4 *vals, uint32_t x, const Rel &r) {
+    vals[x] = make_float4(r.rows, r.startup, r.total, __int_as_float(r.width));
+}
+
+/* closure of `from` inside `subset` over the adjacency table (gpuqo_filter.cuh:113-146) */
+__device__ __forceinline__ uint32_t grow(uint32_t from, uint32_t subset, const uint32_t *adj) {
+    uint32_t V = 0, N = from;
+    while (N) {
+        int i = __builtin_ctz(N);
+        V |= 1u << i;
+        N |= adj[i];
+        N &= subset;
+        N &= ~V;
+    }
+    return V;
+}
+
+__device__ __forceinline__ uint32_t pdep32(uint32_t j, uint32_t mask) {
+    uint32_t res = 0;
+    while (mask) {
+        uint32_t b = mask & (0u - mask);
+        if (j & 1u) res |= b;
+        j >>= 1;
+        mask &= mask - 1;
+    }
+    return res;
+}
+
+/* unsigned order of non-NaN floats; -0 and +0 map to the same key */
+__device__ __forceinline__ uint32_t ord_key(float x) {
+    if (x == 0.0f) x = 0.0f;
+    uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+/* r-th k-subset of {0..n-1} in increasing numeric order (colex ranking) */
+__device__ __forceinline__ uint32_t unrank(uint32_t r, int k, int n) {
+    uint32_t s = 0;
+    int j = k;
+    for (int pos = n - 1; pos >= 0 && j > 0; pos--) {
+        uint32_t c = c_binom[pos * 33 + j];
+        if (c <= r) {
+            s |= 1u << pos;
+            r -= c;
+            j--;
+        }
+    }
+    return s;
+}
+
+__device__ __forceinline__ uint32_t gosper(uint32_t v) {
+    uint64_t x = v;
+    uint64_t t = x | (x - 1);
+    uint64_t nx = (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(x) + 1));
+    return (uint32_t)nx;
+}
+
+__device__ __forceinline__ void add_pairs(uint32_t *counters, uint32_t n) {
+    if (n) atomicAdd((unsigned long long *)(counters + 40), (unsigned long long)n);
+}
+
+/* ---------------------------------------------------------------- kernels */
+
+__global__ void k_clear(uint32_t *conn, uint32_t words, uint32_t *counters) {
+    uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t i = tid; i < words; i += gridDim.x * blockDim.x) conn[i] = 0;
+    if (tid < 64) counters[tid] = 0;
+}
+
+__global__ void k_base(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left, uint32_t *conn) {
+    int i = threadIdx.x;
+    if (i < h.n_rels) {
+        const float *b = (const float *)(blob + h.o_base) + 4 * i;
+        uint32_t s = 1u << i;
+        vals[s] = make_float4(b[0], b[1], b[2], b[3]);
+        left[s] = 0;
+        atomicOr(&conn[s >> 5], 1u << (s & 31));
+    }
+}
+
+/* Candidates of level k -> connected sets (lvl_sets), counters[k]. */
+__global__ void __launch_bounds__(BLOCK) k_filter(BlobHdr h, const uint32_t *blob, int k, uint32_t n_cand,
+                                                   uint32_t per, uint32_t *lvl_sets,
+                                                   unsigned long long *lvl_best, uint32_t *lvl_first,
+                                                   uint32_t *conn, uint32_t *counters) {
+    __shared__ uint32_t adj[32];
+    if (threadIdx.x < 32) adj[threadIdx.x] = threadIdx.x < (unsigned)h.n_rels ? blob[h.o_adj + threadIdx.x] : 0u;
+    __syncthreads();
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t r0 = (uint64_t)gtid * per;
+    uint32_t s = r0 < n_cand ? unrank((uint32_t)r0, k, h.n_rels) : 0u;
+    for (uint32_t i = 0; i < per; i++) {
+        bool valid = (r0 + i) < n_cand;
+        bool keep = false;
+        if (valid) keep = grow(s & (0u - s), s, adj) == s;
+        unsigned long long m = __ballot(keep);
+        if (m) {
+            int leader = __builtin_ctzll(m);
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (keep) {
+                uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                lvl_sets[pos] = s;
+                lvl_best[pos] = ~0ull;
+                lvl_first[pos] = ~0u;
+                atomicOr(&conn[s >> 5], 1u << (s & 31));
+            }
+        }
+        if (valid) s = gosper(s);
+    }
+}
+
+/* DPsub, one lane per set: the reference's sequential scan over l
+ * ascending (gpuqo_cpu_dpsub.cu:32-79), exact tie/NaN behaviour. */
+__global__ void __launch_bounds__(BLOCK) k_eval_dpsub_lane(BlobHdr h, const uint32_t *blob, int k,
+                                                            const uint32_t *lvl_sets, float4 *vals,
+                                                            uint32_t *left, const uint32_t *conn,
+                                                            uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    stage(blob, smem, h.words);
+    View<uint32_t> q = make_view(h, smem);
+    uint32_t count = counters[k];
+    uint32_t npairs = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+        uint32_t S = lvl_sets[idx];
+        bool have = false;
+        Rel best;
+        uint32_t best_l = 0;
+        uint32_t l = S & (0u - S);
+        while (l != S) {
+            uint32_t r = S ^ l;
+            if (is_conn(conn, l) && is_conn(conn, r)) {
+                Rel L = load_rel(vals, l), R = load_rel(vals, r);
+                Rel j = join(q, l, L, r, R);
+                npairs++;
+                if (!have || j.total < best.total) {
+                    best = j;
+                    best_l = l;
+                    have = true;
+                }
+            }
+            l = S & (l - S);
+        }
+        if (have) {
+            store_rel(vals, S, best);
+            left[S] = best_l;
+        }
+    }
+    add_pairs(counters, npairs);
+}
+
+/* DPsub, split-parallel: a wave owns `ipl * 64` consecutive split ranks of
+ * one set; lane t scans ranks [j0 + t*ipl, j0 + (t+1)*ipl) in order.  Each
+ * lane keeps (first valid rank, NaN flag) and the first minimum among
+ * non-NaN costs; both reduce associatively across lanes and waves. */
+__global__ void __launch_bounds__(BLOCK) k_eval_dpsub_chunk(BlobHdr h, const uint32_t *blob, int k, uint32_t cps,
+                                                             uint32_t ipl, const uint32_t *lvl_sets,
+                                                             unsigned long long *lvl_best, uint32_t *lvl_first,
+                                                             const float4 *vals, const uint32_t *conn,
+                                                             uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    stage(blob, smem, h.words);
+    View<uint32_t> q = make_view(h, smem);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t T = (1u << k) - 2u;
+    uint64_t nwaves = (uint64_t)counters[k] * cps;
+    uint64_t wstride = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    uint32_t npairs = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; w < nwaves; w += wstride) {
+        uint32_t idx = (uint32_t)(w / cps), part = (uint32_t)(w % cps);
+        uint32_t S = lvl_sets[idx];
+        uint64_t j0 = 1 + (uint64_t)part * ipl * WAVE + (uint64_t)lane * ipl;
+        uint64_t j1 = j0 + ipl;
+        if (j1 > (uint64_t)T + 1) j1 = (uint64_t)T + 1;
+        uint32_t first = ~0u;
+        unsigned long long best = ~0ull;
+        if (j0 <= T) {
+            uint32_t l = pdep32((uint32_t)j0, S);
+            for (uint64_t j = j0; j < j1; j++) {
+                uint32_t r = S ^ l;
+                if (is_conn(conn, l) && is_conn(conn, r)) {
+                    Rel L = load_rel(vals, l), R = load_rel(vals, r);
+                    Rel jr = join(q, l, L, r, R);
+                    npairs++;
+                    bool nan = jr.total != jr.total;
+                    if (first == ~0u) first = ((uint32_t)j << 1) | (nan ? 1u : 0u);
+                    if (!nan) {
+                        unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | (uint32_t)j;
+                        if (key < best) best = key;
+                    }
+                }
+                l = S & (l - S);
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            uint32_t f2 = __shfl_xor(first, off);
+            unsigned long long b2 = __shfl_xor(best, off);
+            first = f2 < first ? f2 : first;
+            best = b2 < best ? b2 : best;
+        }
+        if (lane == 0) {
+            if (first != ~0u) atomicMin(&lvl_first[idx], first);
+            if (best != ~0ull) atomicMin(&lvl_best[idx], best);
+        }
+    }
+    add_pairs(counters, npairs);
+}
+
+/* Winner of each set after k_eval_dpsub_chunk: the first valid split if its
+ * cost is NaN, else the first minimum.  The join is recomputed. */
+__global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint32_t *blob, int k,
+                                                           const uint32_t *lvl_sets,
+                                                           const unsigned long long *lvl_best,
+                                                           const uint32_t *lvl_first, float4 *vals,
+                                                           uint32_t *left, const uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    stage(blob, smem, h.words);
+    View<uint32_t> q = make_view(h, smem);
+    uint32_t count = counters[k];
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+        uint32_t S = lvl_sets[idx];
+        uint32_t f = lvl_first[idx];
+        if (f == ~0u) continue;
+        uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(lvl_best[idx] & 0xffffffffull);
+        uint32_t l = pdep32(rank, S), r = S ^ l;
+        Rel j = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
+        store_rel(vals, S, j);
+        left[S] = l;
+    }
+}
+
+/* MPDP, one lane per set.  Blocks come from a DFS from the lowest vertex
+ * with neighbours in ascending order, emitted when a tree edge (p,c)
+ * finishes with low[c] >= disc[p] -- the emission order of find_blocks
+ * (gpuqo_cpu_dpsub_bicc.cu:65-169).  The block's vertices are p plus the
+ * vertices discovered at or after c still on the vertex stack.  Each block
+ * is enumerated as soon as it is emitted: lb ascends over its non-empty
+ * proper subsets; L = closure of lb's lowest vertex in S - rb, R likewise;
+ * (L, R) is valid iff L | R == S (:172-236). */
+__global__ void __launch_bounds__(BLOCK) k_eval_bicc_lane(BlobHdr h, const uint32_t *blob, int k,
+                                                           const uint32_t *lvl_sets, float4 *vals,
+                                                           uint32_t *left, uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    uint32_t *dfs = smem + ((h.words + 3) & ~3);      /* [32][BLOCK]: disc | low<<8 | parent<<16 */
+    stage(blob, smem, h.words);
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t *adj = smem + h.o_adj;
+    const int tx = threadIdx.x;
+#define DFS(v) dfs[(v) * BLOCK + tx]
+    uint32_t count = counters[k];
+    uint32_t npairs = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+        uint32_t S = lvl_sets[idx];
+        bool have = false;
+        Rel best;
+        uint32_t best_l = 0;
+        int root = __builtin_ctz(S);
+        uint32_t visited = 1u << root, onstack = 1u << root;
+        uint32_t t = 1;
+        DFS(root) = 0u | (255u << 8) | (255u << 16);
+        int u = root, cursor = -1;
+        while (true) {
+            uint32_t gt = cursor < 0 ? 0xffffffffu : (uint32_t)(0xffffffffull << (cursor + 1));
+            uint32_t rem = adj[u] & S & gt;
+            uint32_t du = DFS(u);
+            if (rem) {
+                int v = __builtin_ctz(rem);
+                if (!((visited >> v) & 1u)) {
+                    DFS(v) = t | (255u << 8) | ((uint32_t)u << 16);
+                    t++;
+                    visited |= 1u << v;
+                    onstack |= 1u << v;
+                    u = v;
+                    cursor = -1;
+                } else {
+                    uint32_t dv = DFS(v);
+                    if ((int)(du >> 16) != v && (dv & 0xffu) < (du & 0xffu)) {
+                        uint32_t low = (du >> 8) & 0xffu;
+                        uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
+                        DFS(u) = (du & 0xffff00ffu) | (nl << 8);
+                    }
+                    cursor = v;
+                }
+                continue;
+            }
+            if (u == root) break;
+            int p = (int)(du >> 16);
+            uint32_t dp = DFS(p);
+            uint32_t lowu = (du >> 8) & 0xffu, discu = du & 0xffu;
+            if (lowu >= (dp & 0xffu)) {
+                uint32_t rest = 0, os = onstack;
+                while (os) {
+                    int w = __builtin_ctz(os);
+                    os &= os - 1;
+                    if ((DFS(w) & 0xffu) >= discu) rest |= 1u << w;
+                }
+                onstack &= ~rest;
+                uint32_t block = rest | (1u << p);
+                uint32_t lb = block & (0u - block);
+                while (lb != block) {
+                    uint32_t rb = block ^ lb;
+                    uint32_t L = grow(lb & (0u - lb), S & ~rb, adj);
+                    uint32_t R = grow(rb & (0u - rb), S & ~lb, adj);
+                    if ((L | R) == S) {
+                        Rel j = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+                        npairs++;
+                        if (!have || j.total < best.total) {
+                            best = j;
+                            best_l = L;
+                            have = true;
+                        }
+                    }
+                    lb = block & (lb - block);
+                }
+            }
+            uint32_t lowp = (dp >> 8) & 0xffu;
+            if (lowu < lowp) DFS(p) = (dp & 0xffff00ffu) | (lowu << 8);
+            cursor = u;
+            u = p;
+        }
+        if (have) {
+            store_rel(vals, S, best);
+            left[S] = best_l;
+        }
+    }
+#undef DFS
+    add_pairs(counters, npairs);
+}
+
+/* Pre-order walk of the plan from `root_set` (dpsub_buildQueryTree,
+ * gpuqo_query_tree.cuh:46-72, done on the device in one pass). */
+__global__ void k_tree(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes, int cap,
+                       uint32_t *status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t st_set[66];
+    int st_parent[66], st_side[66];
+    int sp = 0, n = 0;
+    uint32_t bad = 0;
+    st_set[sp] = root_set; st_parent[sp] = -1; st_side[sp] = 0; sp++;
+    while (sp > 0) {
+        sp--;
+        uint32_t S = st_set[sp];
+        int par = st_parent[sp], side = st_side[sp];
+        if (n >= cap) { bad = 1; break; }
+        int me = n++;
+        float4 v = vals[S];
+        DPNode nd;
+        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
+        nd.left = -1; nd.right = -1; nd.pad = 0;
+        nodes[me] = nd;
+        if (par >= 0) {
+            if (side == 0) nodes[par].left = me;
+            else nodes[par].right = me;
+        }
+        if (__builtin_popcount(S) > 1) {
+            uint32_t l = left[S], r = S ^ l;
+            /* the left child must be a proper non-empty subset */
+            if (l == 0 || (l & ~S) != 0 || l == S || sp + 2 > 64) { bad = 1; break; }
+            st_set[sp] = r; st_parent[sp] = me; st_side[sp] = 1; sp++;
+            st_set[sp] = l; st_parent[sp] = me; st_side[sp] = 0; sp++;
+        }
+    }
+    status[0] = bad ? 0xffffffffu : (uint32_t)n;
+}
+
+/* ------------------------------------------------------------ host side */
+
+static uint64_t binom64(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
+    return r;
+}
+
+struct DeviceCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBufs b{};
+    size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
+           cap_nodes = 0, cap_counters = 0;
+    bool binom_ready = false;
+    std::mutex mu;
+};
+
+static DeviceCtx g_ctx;
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #x); \
+            return GQ_ERR_DEVICE;                                                          \
+        }                                                                                  \
+    } while (0)
+
+template <class T>
+static int ensure(T **p, size_t &cap, size_t need) {
+    if (*p && cap >= need) return GQ_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+    if (hipMalloc((void **)p, bytes) != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc of %zu bytes failed", bytes);
+        return GQ_ERR_NOMEM;
+    }
+    cap = std::max<size_t>(need, 1);
+    return GQ_OK;
+}
+
+static int ctx_init(DeviceCtx &c, int device) {
+    if (c.device == device && c.stream) return GQ_OK;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c.ev0));
+    HIPCHK(hipEventCreate(&c.ev1));
+    uint32_t tab[33 * 33];
+    for (int n = 0; n <= 32; n++)
+        for (int k = 0; k <= 32; k++) tab[n * 33 + k] = (uint32_t)binom64(n, k);
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tab, sizeof(tab)));
+    c.device = device;
+    c.binom_ready = true;
+    int rc = ensure(&c.b.counters, c.cap_counters, 64);
+    if (rc) return rc;
+    return ensure(&c.b.nodes, c.cap_nodes, 2 * 32);
+}
+
+int device_init(int device) {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    return ctx_init(g_ctx, device);
+}
+
+int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vector<DPNode> &tree, DPStats &st) {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    DeviceCtx &c = g_ctx;
+    int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
+    if (rc) return rc;
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    if (n < 1 || n > 32) {
+        set_error("GPU DP supports 1..32 relations, got %d", n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (algo == DP_DPSIZE) {
+        set_error("DPsize GPU path not available");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (n_iters != n) {
+        set_error("IDP1 level cap not available on the GPU path");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    const size_t blob_bytes = (size_t)h.words * 4;
+    const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
+    const size_t lds_bicc = lds_eval + (size_t)32 * BLOCK * 4;
+    if (lds_bicc > 64 * 1024) {
+        set_error("query description too large for LDS (%zu bytes)", blob_bytes);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    (void)memo_kind;
+    const uint64_t nsets_all = 1ull << n;
+    uint64_t maxC = 1;
+    for (int k = 1; k <= n; k++) maxC = std::max(maxC, binom64(n, k));
+    if ((rc = ensure(&c.b.vals, c.cap_memo, nsets_all))) return rc;
+    if ((rc = ensure(&c.b.left, c.cap_left, nsets_all))) return rc;
+    if ((rc = ensure(&c.b.conn, c.cap_conn, std::max<uint64_t>(1, nsets_all / 32)))) return rc;
+    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, maxC))) return rc;
+    if ((rc = ensure(&c.b.lvl_best, c.cap_best, maxC))) return rc;
+    if ((rc = ensure(&c.b.lvl_first, c.cap_first, maxC))) return rc;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, (size_t)h.words))) return rc;
+    if ((rc = ensure(&c.b.nodes, c.cap_nodes, (size_t)2 * n))) return rc;
+
+    hipStream_t s = c.stream;
+    HIPCHK(hipMemcpyAsync(c.b.blob, in.blob.data(), blob_bytes, hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(c.ev0, s));
+    uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
+    {
+        uint32_t blocks = (uint32_t)std::min<uint64_t>((conn_words + BLOCK - 1) / BLOCK, 4096);
+        hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, c.b.conn, conn_words, c.b.counters);
+        hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, c.b.blob, c.b.vals, c.b.left, c.b.conn);
+    }
+    uint64_t cand_total = 0;
+    for (int k = 2; k <= n; k++) {
+        uint64_t Ck = binom64(n, k);
+        cand_total += Ck;
+        uint64_t threads = std::min<uint64_t>(Ck, 1ull << 19);
+        uint32_t per = (uint32_t)((Ck + threads - 1) / threads);
+        threads = (Ck + per - 1) / per;
+        uint32_t fblocks = (uint32_t)((threads + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL(k_filter, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, k, (uint32_t)Ck, per,
+                           c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.conn, c.b.counters);
+        uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
+        if (algo == DP_BICC) {
+            hipLaunchKernelGGL(k_eval_bicc_lane, dim3(lblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k,
+                               c.b.lvl_sets, c.b.vals, c.b.left, c.b.counters);
+        } else {
+            uint64_t T = (1ull << k) - 2;
+            bool lane_mode = T <= 256 || Ck * T <= (1ull << 16);
+            if (lane_mode) {
+                hipLaunchKernelGGL(k_eval_dpsub_lane, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                                   c.b.lvl_sets, c.b.vals, c.b.left, c.b.conn, c.b.counters);
+            } else {
+                uint32_t ipl = 16;
+                uint64_t per_wave = (uint64_t)ipl * WAVE;
+                uint32_t cps = (uint32_t)((T + per_wave - 1) / per_wave);
+                uint64_t waves = Ck * cps;
+                uint32_t wblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 16384);
+                hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                                   cps, ipl, c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.conn,
+                                   c.b.counters);
+                hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                                   c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.left, c.b.counters);
+            }
+        }
+    }
+    uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, c.b.vals, c.b.left, full, c.b.nodes, 2 * n - 1,
+                       c.b.counters + 48);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, s));
+    tree.resize(2 * n - 1);
+    uint32_t cnt[64];
+    HIPCHK(hipMemcpyAsync(tree.data(), c.b.nodes, sizeof(DPNode) * (2 * n - 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cnt, c.b.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (cnt[48] != (uint32_t)(2 * n - 1)) {
+        set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
+        return GQ_ERR_DEVICE;
+    }
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    st.gpu_ms = ms;
+    st.candidates = cand_total;
+    st.sets = 0;
+    for (int k = 2; k <= n; k++) st.sets += cnt[k];
+    st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+    st.memo_bytes = nsets_all * (sizeof(float4) + 4);
+    st.memo_kind = 1;
+    return GQ_OK;
+}
+
+}  // namespace gq

@@ -1,0 +1,51 @@
+/*
+ * gq_drivers.h -- BFS relabelling, the Remapper, and the DP / IDP1 / IDP2 /
+ * UnionDP drivers (host side).
+ */
+#ifndef GQ_DRIVERS_H
+#define GQ_DRIVERS_H
+
+#include <deque>
+#include <memory>
+#include <vector>
+
+#include "gq_host.h"
+
+namespace gq {
+
+/* Per-call state: configuration, counters, and an arena owning every plan
+ * node built during the call (trees share composite subtrees). */
+struct RunCtx {
+    int algorithm, bicc, idp_type, idp_n_iters, memo_kind;
+    DPStats stats;
+    int dp_calls = 0;
+    std::deque<QT<1>> arena1;
+    std::deque<QT<4>> arena4;
+    explicit RunCtx(const gq_config *cfg)
+        : algorithm(cfg->gpuqo_algorithm), bicc(cfg->gpuqo_dpsub_bicc), idp_type(cfg->gpuqo_idp_type),
+          idp_n_iters(cfg->gpuqo_idp_n_iters), memo_kind(cfg->memo_kind) {}
+    template <int W>
+    QT<W> *node();
+    void add(const DPStats &s) {
+        stats.candidates += s.candidates;
+        stats.sets += s.sets;
+        stats.pairs += s.pairs;
+        stats.gpu_ms += s.gpu_ms;
+        if (s.memo_bytes > stats.memo_bytes) stats.memo_bytes = s.memo_bytes;
+        stats.memo_kind = s.memo_kind;
+    }
+};
+template <>
+inline QT<1> *RunCtx::node<1>() { arena1.emplace_back(); return &arena1.back(); }
+template <>
+inline QT<4> *RunCtx::node<4>() { arena4.emplace_back(); return &arena4.back(); }
+
+/* __gpuqo_run (gpuqo_main_internal.cu:81-147). *out is arena-owned. */
+template <int W>
+int gpuqo_run_t(RunCtx &ctx, const HQuery<W> &h, QT<W> **out);
+
+int build_dp_input(const HQuery<1> &q, DPInput &in);
+
+}  // namespace gq
+
+#endif

@@ -1,0 +1,473 @@
+/*
+ * gq_host.cpp -- C ABI of the optimizer core and the host-side drivers.
+ *
+ * gq_plan follows __gpuqo_run (src/backend/optimizer/gpuqo/
+ * gpuqo_main_internal.cu:81-147): convert the input, relabel relations in
+ * BFS order from relation 0, run the DP (or IDP1 / IDP2 / UnionDP), map the
+ * tree back.  The DPs run on the GPU (gq_dp.hip); the drivers, the Remapper
+ * and GOO run here.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <queue>
+#include <unordered_map>
+
+#include "gq_drivers.h"
+#include "gq_host.h"
+
+namespace gq {
+
+/* ------------------------------------------------------------ errors */
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+static std::atomic<int> g_interrupt(0);
+bool interrupted() { return g_interrupt.load() != 0; }
+
+/* ------------------------------------------------------------ cost view */
+
+template <int W>
+void HostView<W>::build(const HQuery<W> &q) {
+    const int n = q.n, n_ec = (int)q.ec_relids.size();
+    const int ecw = std::max(1, (n_ec + 63) / 64);
+    rel_rows = q.rows;
+    rel_fksel.resize(n);
+    for (int i = 0; i < n; i++)   /* fkselc_of_ec: double division narrowed to float (row_estimation.cuh:39) */
+        rel_fksel[i] = (float)(1.0 / (double)fmaxf(1.0f, q.tuples[i]));
+    inc.assign((size_t)n * ecw, 0);
+    ec_moff.resize(n_ec);
+    ec_soff.resize(n_ec);
+    ec_hasfk.assign(n_ec, 0);
+    mem_rel.clear();
+    int moff = 0, soff = 0;
+    for (int e = 0; e < n_ec; e++) {
+        ec_moff[e] = moff;
+        ec_soff[e] = soff;
+        Set<W> t = q.ec_relids[e];
+        int s = popc(t);
+        while (!empty(t)) {
+            int r = ctz(t);
+            t.unset(r);
+            inc[(size_t)r * ecw + (e >> 6)] |= 1ull << (e & 63);
+            mem_rel.push_back(r);
+        }
+        for (int m = moff; m < moff + s; m++)
+            if (!empty(q.fks[m])) ec_hasfk[e] = 1;
+        moff += s;
+        soff += s * (s - 1) / 2;
+    }
+    const int nm = moff;
+    mem_nd.assign(nm, 0.0f);
+    mem_avgfreq.assign(nm, 0.0f);
+    mem_mcv.assign(nm, 0.0f);
+    mem_flags.assign(nm, 0);
+    mem_idx_rows.assign(nm, 0.0f);
+    mem_idx_startup.assign(nm, 0.0f);
+    mem_idx_total.assign(nm, 0.0f);
+    for (int m = 0; m < nm; m++) {
+        const gq_var_info &var = q.vars[m];
+        const int r = mem_rel[m];
+        /* __estimate_hash_bucketsize up to the nbuckets test (cost_postgres.cuh:711-742) */
+        float nd = var.stadistinct;
+        if (nd < 0.0f) nd = -nd * q.tuples[r];
+        if (nd <= 0.0f) {
+            mem_flags[m] |= GQ_MF_BUCKET_DEFAULT;
+        } else {
+            float avgfreq = (1.0f - var.stanullfrac) / nd;
+            if (q.tuples[r] > 0.0f) {
+                nd *= q.rows[r] / q.tuples[r];
+                nd = clamp_row_est(nd);
+            }
+            mem_nd[m] = nd;
+            mem_avgfreq[m] = avgfreq;
+        }
+        mem_mcv[m] = var.mcvfreq;
+        if (var.index_available) mem_flags[m] |= GQ_MF_INDEX;
+        if (var.index_unique) mem_flags[m] |= GQ_MF_UNIQUE;
+        mem_idx_rows[m] = var.index_rows;
+        mem_idx_startup[m] = var.index_cost.startup;
+        mem_idx_total[m] = var.index_cost.total;
+    }
+    qualtab.assign(n_ec + 1, 0.0f);
+    float acc = 0.0f;                /* cost_qual_eval: repeated addition (cost_postgres.cuh:801-812) */
+    for (int k = 1; k <= n_ec; k++) {
+        acc += q.p.cpu_operator_cost;
+        qualtab[k] = acc;
+    }
+    v.n_rels = n;
+    v.n_ec = n_ec;
+    v.ecw = ecw;
+    v.p = q.p;
+    v.rel_rows = rel_rows.data();
+    v.rel_fksel = rel_fksel.data();
+    Set<W> comp;
+    for (int i = 0; i < n; i++)
+        if (q.composite[i]) comp.set(i);
+    v.composite = comp;
+    v.inc = inc.data();
+    v.ec_relids = q.ec_relids.data();
+    v.ec_moff = ec_moff.data();
+    v.ec_soff = ec_soff.data();
+    v.ec_hasfk = ec_hasfk.data();
+    v.mem_fks = q.fks.data();
+    v.mem_nd = mem_nd.data();
+    v.mem_avgfreq = mem_avgfreq.data();
+    v.mem_mcv = mem_mcv.data();
+    v.mem_flags = mem_flags.data();
+    v.mem_idx_rows = mem_idx_rows.data();
+    v.mem_idx_startup = mem_idx_startup.data();
+    v.mem_idx_total = mem_idx_total.data();
+    v.sels = q.sels.data();
+    v.qualtab = qualtab.data();
+}
+
+template struct HostView<1>;
+template struct HostView<4>;
+
+/* ------------------------------------------------------------ device blob */
+
+int build_dp_input(const HQuery<1> &q, DPInput &in) {
+    const int n = q.n;
+    if (n > 32) {
+        set_error("GPU DP supports at most 32 relations (got %d)", n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    HostView<1> hv;
+    hv.build(q);
+    const int n_ec = hv.v.n_ec, ecw = hv.v.ecw, nm = (int)q.vars.size();
+    if (ecw > GQ_MAX_ECW) {
+        set_error("GPU DP supports at most %d equivalence classes (got %d)", 64 * GQ_MAX_ECW, n_ec);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    BlobHdr &h = in.hdr;
+    std::vector<uint32_t> &b = in.blob;
+    b.clear();
+    auto take = [&](int words) {
+        int o = (int)b.size();
+        b.resize(b.size() + words, 0u);
+        return o;
+    };
+    auto align2 = [&]() { if (b.size() & 1) b.push_back(0u); };
+    auto put_f = [&](int o, int i, float x) { memcpy(&b[o + i], &x, 4); };
+    h.n_rels = n;
+    h.n_ec = n_ec;
+    h.n_mem = nm;
+    h.n_sels = (int)q.sels.size();
+    h.ecw = ecw;
+    h.p = q.p;
+    h.composite = (uint32_t)hv.v.composite.w[0];
+    h.o_rel_rows = take(32);
+    h.o_rel_fksel = take(32);
+    h.o_adj = take(32);
+    for (int i = 0; i < n; i++) {
+        put_f(h.o_rel_rows, i, hv.rel_rows[i]);
+        put_f(h.o_rel_fksel, i, hv.rel_fksel[i]);
+        b[h.o_adj + i] = (uint32_t)q.adj[i].w[0];
+    }
+    align2();
+    h.o_inc = take(2 * 32 * ecw);
+    for (int i = 0; i < n; i++)
+        for (int w = 0; w < ecw; w++) {
+            uint64_t x = hv.inc[(size_t)i * ecw + w];
+            b[h.o_inc + 2 * (i * ecw + w)] = (uint32_t)x;
+            b[h.o_inc + 2 * (i * ecw + w) + 1] = (uint32_t)(x >> 32);
+        }
+    h.o_ec_relids = take(std::max(1, n_ec));
+    h.o_ec_moff = take(std::max(1, n_ec));
+    h.o_ec_soff = take(std::max(1, n_ec));
+    h.o_ec_hasfk = take(std::max(1, (n_ec + 3) / 4));
+    for (int e = 0; e < n_ec; e++) {
+        b[h.o_ec_relids + e] = (uint32_t)q.ec_relids[e].w[0];
+        b[h.o_ec_moff + e] = (uint32_t)hv.ec_moff[e];
+        b[h.o_ec_soff + e] = (uint32_t)hv.ec_soff[e];
+        ((uint8_t *)&b[h.o_ec_hasfk])[e] = hv.ec_hasfk[e];
+    }
+    h.o_mem_fks = take(std::max(1, nm));
+    h.o_mem_nd = take(std::max(1, nm));
+    h.o_mem_avgfreq = take(std::max(1, nm));
+    h.o_mem_mcv = take(std::max(1, nm));
+    h.o_mem_flags = take(std::max(1, (nm + 3) / 4));
+    h.o_mem_idx_rows = take(std::max(1, nm));
+    h.o_mem_idx_startup = take(std::max(1, nm));
+    h.o_mem_idx_total = take(std::max(1, nm));
+    for (int m = 0; m < nm; m++) {
+        b[h.o_mem_fks + m] = (uint32_t)q.fks[m].w[0];
+        put_f(h.o_mem_nd, m, hv.mem_nd[m]);
+        put_f(h.o_mem_avgfreq, m, hv.mem_avgfreq[m]);
+        put_f(h.o_mem_mcv, m, hv.mem_mcv[m]);
+        ((uint8_t *)&b[h.o_mem_flags])[m] = hv.mem_flags[m];
+        put_f(h.o_mem_idx_rows, m, hv.mem_idx_rows[m]);
+        put_f(h.o_mem_idx_startup, m, hv.mem_idx_startup[m]);
+        put_f(h.o_mem_idx_total, m, hv.mem_idx_total[m]);
+    }
+    h.o_sels = take(std::max<int>(1, (int)q.sels.size()));
+    for (size_t i = 0; i < q.sels.size(); i++) put_f(h.o_sels, (int)i, q.sels[i]);
+    h.o_qualtab = take(n_ec + 1);
+    for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
+    h.o_base = take(4 * n);
+    in.base.resize(n);
+    for (int i = 0; i < n; i++) {
+        put_f(h.o_base, 4 * i, q.rows[i]);
+        put_f(h.o_base, 4 * i + 1, q.startup[i]);
+        put_f(h.o_base, 4 * i + 2, q.total[i]);
+        b[h.o_base + 4 * i + 3] = (uint32_t)q.width[i];
+        in.base[i] = Rel{q.rows[i], q.startup[i], q.total[i], q.width[i]};
+    }
+    while (b.size() & 3) b.push_back(0u);
+    h.words = (int)b.size();
+    return GQ_OK;
+}
+
+/* ------------------------------------------------------------ conversion */
+
+static Params make_params(const gq_config *c) {     /* setParams, gpuqo_planner_info.cu:113-131 */
+    Params p;
+    p.seq_page_cost = (float)c->seq_page_cost;
+    p.cpu_tuple_cost = (float)c->cpu_tuple_cost;
+    p.cpu_operator_cost = (float)c->cpu_operator_cost;
+    p.disable_cost = (float)c->disable_cost;
+    p.work_mem = c->work_mem;
+    p.enable_nestloop = c->enable_nestloop;
+    p.enable_hashjoin = c->enable_hashjoin;
+    return p;
+}
+
+template <int W>
+static Set<W> set_from_words(const uint64_t *w, int nw) {
+    Set<W> s;
+    for (int i = 0; i < nw && i < W; i++) s.w[i] = w[i];
+    return s;
+}
+
+static int validate(const gq_query *q) {
+    if (!q || q->n_rels < 1 || q->n_rels > GQ_MAX_RELS) {
+        set_error("n_rels must be in 1..%d", GQ_MAX_RELS);
+        return GQ_ERR_INVALID;
+    }
+    if (q->n_words != (q->n_rels + 63) / 64) {
+        set_error("n_words must be ceil(n_rels/64)");
+        return GQ_ERR_INVALID;
+    }
+    if (!q->base_rels || !q->edge_table || (q->n_eq_classes > 0 && (!q->ec_member_offsets || !q->ec_members ||
+                                                                     !q->ec_sels || !q->ec_fks || !q->ec_vars))) {
+        set_error("null array in query");
+        return GQ_ERR_INVALID;
+    }
+    for (int e = 0; e < q->n_eq_classes; e++) {
+        int b0 = q->ec_member_offsets[e], b1 = q->ec_member_offsets[e + 1];
+        if (b1 - b0 < 1) {
+            set_error("equivalence class %d has no members", e);
+            return GQ_ERR_INVALID;
+        }
+        for (int m = b0; m < b1; m++) {
+            if (q->ec_members[m] < 0 || q->ec_members[m] >= q->n_rels || (m > b0 && q->ec_members[m] <= q->ec_members[m - 1])) {
+                set_error("equivalence class %d: members must be ascending relation indices", e);
+                return GQ_ERR_INVALID;
+            }
+        }
+    }
+    return GQ_OK;
+}
+
+template <int W>
+static void convert(const gq_query *q, const gq_config *cfg, HQuery<W> &h) {  /* convertGpuqoPlannerInfo :133-209 */
+    h.n = q->n_rels;
+    h.n_iters = q->n_rels;
+    h.p = make_params(cfg);
+    h.rows.resize(h.n);
+    h.tuples.resize(h.n);
+    h.startup.resize(h.n);
+    h.total.resize(h.n);
+    h.width.resize(h.n);
+    h.composite.assign(h.n, 0);
+    h.adj.resize(h.n);
+    for (int i = 0; i < h.n; i++) {
+        h.rows[i] = q->base_rels[i].rows;
+        h.tuples[i] = q->base_rels[i].tuples;
+        h.startup[i] = q->base_rels[i].cost.startup;
+        h.total[i] = q->base_rels[i].cost.total;
+        h.width[i] = q->base_rels[i].width;
+        h.adj[i] = set_from_words<W>(q->edge_table + (size_t)i * q->n_words, q->n_words);
+        h.adj[i].unset(i);
+    }
+    int soff = 0;
+    for (int e = 0; e < q->n_eq_classes; e++) {
+        int b0 = q->ec_member_offsets[e], b1 = q->ec_member_offsets[e + 1];
+        Set<W> ec;
+        for (int m = b0; m < b1; m++) ec.set(q->ec_members[m]);
+        h.ec_relids.push_back(ec);
+        int s = b1 - b0;
+        for (int j = 0; j < s * (s - 1) / 2; j++) h.sels.push_back(q->ec_sels[soff + j]);
+        soff += s * (s - 1) / 2;
+        for (int m = b0; m < b1; m++) {
+            h.fks.push_back(set_from_words<W>(q->ec_fks + (size_t)m * q->n_words, q->n_words));
+            h.vars.push_back(q->ec_vars[m]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------ output */
+
+template <int W>
+static int emit(const QT<W> *t, gq_plan_out *out) {
+    if (!t) return -1;
+    if (out->n_nodes >= out->capacity) return -2;
+    int me = out->n_nodes++;
+    gq_plan_node &nd = out->nodes[me];
+    memset(&nd, 0, sizeof(nd));
+    for (int i = 0; i < W && i < GQ_MAX_WORDS; i++) nd.relids[i] = t->id.w[i];
+    nd.rows = t->rows;
+    nd.cost.startup = t->startup;
+    nd.cost.total = t->total;
+    nd.width = t->width;
+    int l = emit(t->left, out);
+    if (l == -2) return -2;
+    int r = emit(t->right, out);
+    if (r == -2) return -2;
+    out->nodes[me].left = l;
+    out->nodes[me].right = r;
+    return me;
+}
+
+template <int W>
+static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    auto t0 = std::chrono::steady_clock::now();
+    HQuery<W> h;
+    convert(q, cfg, h);
+    RunCtx ctx(cfg);
+    QT<W> *t = nullptr;
+    int rc = gpuqo_run_t(ctx, h, &t);
+    auto t1 = std::chrono::steady_clock::now();
+    out->n_nodes = 0;
+    memset(&out->stats, 0, sizeof(out->stats));
+    out->stats.candidates = ctx.stats.candidates;
+    out->stats.sets = ctx.stats.sets;
+    out->stats.pairs = ctx.stats.pairs;
+    out->stats.memo_bytes = ctx.stats.memo_bytes;
+    out->stats.gpu_ms = ctx.stats.gpu_ms;
+    out->stats.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    out->stats.n_dp_calls = ctx.dp_calls;
+    out->stats.memo_kind = ctx.stats.memo_kind;
+    if (rc != GQ_OK) return rc;
+    if (!t) {
+        set_error("no join tree (is the join graph connected?)");
+        return GQ_ERR_NO_PLAN;
+    }
+    int r = emit(t, out);
+    if (r == -2) {
+        set_error("output capacity %d < %d nodes", out->capacity, 2 * q->n_rels - 1);
+        return GQ_ERR_CAPACITY;
+    }
+    return GQ_OK;
+}
+
+}  // namespace gq
+
+/* ================================================================ C ABI */
+
+extern "C" {
+
+gq_status gq_config_default(gq_config *c) {
+    if (!c) return GQ_ERR_INVALID;
+    memset(c, 0, sizeof(*c));
+    c->gpuqo_algorithm = GQ_ALGO_DPSIZE;                 /* guc.c:4810-4816 */
+    c->gpuqo_dpsub_bicc = 0;                             /* :1153-1161 */
+    c->gpuqo_dpsub_filter = 1;                           /* :1113-1121 */
+    c->gpuqo_dpsub_filter_threshold = 16;                /* :2190-2197 */
+    c->gpuqo_dpsub_filter_cpu_enum_threshold = 1024;     /* :2200-2207 */
+    c->gpuqo_dpsub_filter_keys_overprovisioning = 100;   /* :2210-2217 */
+    c->gpuqo_dpsub_ccc = 1;                              /* :1123-1131 */
+    c->gpuqo_dpsub_csg = 1;                              /* :1133-1141 */
+    c->gpuqo_dpsub_csg_threshold = 128;                  /* :2220-2227 */
+    c->gpuqo_n_parallel = 10240;                         /* :2180-2187 */
+    c->gpuqo_scratchpad_size_mb = 10;                    /* :2150-2157 */
+    c->gpuqo_min_memo_size_mb = 1;                       /* :2170-2177 */
+    c->gpuqo_max_memo_size_mb = 1024;                    /* :2160-2167 */
+    c->gpuqo_idp_type = GQ_IDP2;                         /* :4819-4825 */
+    c->gpuqo_idp_n_iters = 0;                            /* :2260-2267 */
+    c->gpuqo_dpe_n_threads = 4;                          /* :2230-2237 */
+    c->gpuqo_cpu_dpsub_parallel_chunk_size = 64;         /* :2240-2247 */
+    c->gpuqo_dpe_pairs_per_depbuf = 128;                 /* :2250-2257 */
+    c->seq_page_cost = 1.0;
+    c->random_page_cost = 4.0;
+    c->cpu_tuple_cost = 0.01;
+    c->cpu_index_tuple_cost = 0.005;
+    c->cpu_operator_cost = 0.0025;
+    c->disable_cost = 1.0e10;
+    c->effective_cache_size = 524288;
+    c->work_mem = 4096;
+    c->enable_seqscan = c->enable_indexscan = c->enable_tidscan = c->enable_sort = 1;
+    c->enable_hashagg = c->enable_nestloop = c->enable_hashjoin = 1;
+    c->enable_mergejoin = 0;
+    c->memo_kind = GQ_MEMO_AUTO;
+    c->device = 0;
+    return GQ_OK;
+}
+
+gq_status gq_init(int32_t device) { return (gq_status)gq::device_init(device); }
+
+gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    int rc = gq::validate(q);
+    if (rc) return (gq_status)rc;
+    if (!cfg || !out || !out->nodes) {
+        gq::set_error("null config or output");
+        return GQ_ERR_INVALID;
+    }
+    if (cfg->gpuqo_algorithm != GQ_ALGO_DPSUB && cfg->gpuqo_algorithm != GQ_ALGO_DPSIZE) {
+        gq::set_error("algorithm %d is not a GPU algorithm (use dpsub or dpsize)", cfg->gpuqo_algorithm);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if ((rc = gq::device_init(cfg->device))) return (gq_status)rc;
+    if (q->n_rels <= 64) return (gq_status)gq::plan_t<1>(q, cfg, out);
+    return (gq_status)gq::plan_t<4>(q, cfg, out);
+}
+
+gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, gq_plan_out *outs) {
+    if (n_queries < 0 || (n_queries > 0 && (!qs || !outs))) {
+        gq::set_error("bad batch arguments");
+        return GQ_ERR_INVALID;
+    }
+    for (int i = 0; i < n_queries; i++) {
+        gq_status st = gq_plan(qs[i], cfg, &outs[i]);
+        if (st != GQ_OK) return st;
+    }
+    return GQ_OK;
+}
+
+struct gq_server {
+    int dummy;
+};
+
+gq_status gq_server_create(const gq_config *, uint64_t, int32_t, gq_server **out) {
+    if (out) *out = nullptr;
+    gq::set_error("server not available");
+    return GQ_ERR_UNSUPPORTED;
+}
+gq_status gq_server_submit(gq_server *, const gq_query *, uint64_t) { return GQ_ERR_UNSUPPORTED; }
+gq_status gq_server_poll(gq_server *, int32_t, int32_t *done, uint64_t *, gq_status *, gq_plan_out *) {
+    if (done) *done = 0;
+    return GQ_ERR_UNSUPPORTED;
+}
+gq_status gq_server_destroy(gq_server *) { return GQ_OK; }
+
+void gq_interrupt(int32_t on) { gq::g_interrupt.store(on ? 1 : 0); }
+
+const char *gq_last_error(void) { return gq::g_err; }
+
+const char *gq_version(void) { return "gpuqo-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

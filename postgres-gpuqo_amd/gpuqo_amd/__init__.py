@@ -1,0 +1,97 @@
+"""Host-side Python binding of the MI355X join-order optimizer core.
+
+The compute path is the C ABI library lib/libgpuqo_amd.so (HIP kernels for
+gfx950 plus the C++ host drivers).  This module only marshals arguments; it
+has no fallback: if the library is missing, calls raise.
+"""
+import ctypes as C
+import os
+import struct
+
+from . import abi, qgen  # noqa: F401
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libgpuqo_amd.so")
+_lib = None
+
+
+class GpuqoError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"gpuqo status {status}: {msg}")
+        self.status = status
+
+
+def lib():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.gq_config_default.argtypes = [C.POINTER(abi.Config)]
+        L.gq_init.argtypes = [C.c_int32]
+        L.gq_plan.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.POINTER(abi.PlanOut)]
+        L.gq_plan_batch.argtypes = [C.c_int32, C.POINTER(C.POINTER(abi.Query)), C.POINTER(abi.Config),
+                                    C.POINTER(abi.PlanOut)]
+        L.gq_last_error.restype = C.c_char_p
+        L.gq_version.restype = C.c_char_p
+        L.gq_interrupt.argtypes = [C.c_int32]
+        L.gq_interrupt.restype = None
+        L.gq_server_create.argtypes = [C.POINTER(abi.Config), C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]
+        L.gq_server_submit.argtypes = [C.c_void_p, C.POINTER(abi.Query), C.c_uint64]
+        L.gq_server_poll.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_int32), C.POINTER(abi.PlanOut)]
+        L.gq_server_destroy.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        msg = lib().gq_last_error()
+        raise GpuqoError(st, msg.decode() if msg else "")
+
+
+def fbits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def plan_tuples(out):
+    """Plan nodes as tuples (relids tuple, rows, startup, total, width, left, right)."""
+    res = []
+    for i in range(out.n_nodes):
+        n = out.nodes[i]
+        res.append((tuple(n.relids), n.rows, n.cost.startup, n.cost.total, n.width, n.left, n.right))
+    return res
+
+
+def plan_bits(nodes):
+    """Same tuples with floats replaced by their IEEE-754 bit patterns."""
+    return [(r, fbits(a), fbits(b), fbits(c), w, l, rr) for (r, a, b, c, w, l, rr) in nodes]
+
+
+def default_config(**kw):
+    return abi.default_config(**kw)
+
+
+def plan(query, cfg):
+    """Plan one query on the GPU; returns (nodes, stats)."""
+    n = query.n
+    cap = 2 * n
+    nodes = (abi.PlanNode * cap)()
+    out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
+    _check(lib().gq_plan(C.byref(query.to_c()), C.byref(cfg), C.byref(out)))
+    return plan_tuples(out), out.stats
+
+
+def plan_batch(queries, cfg):
+    """Plan many queries in one batched device pass; returns [(nodes, stats)]."""
+    k = len(queries)
+    qptrs = (C.POINTER(abi.Query) * k)(*[C.pointer(q.to_c()) for q in queries])
+    bufs = [(abi.PlanNode * (2 * q.n))() for q in queries]
+    outs = (abi.PlanOut * k)()
+    for i, q in enumerate(queries):
+        outs[i].nodes = C.cast(bufs[i], C.POINTER(abi.PlanNode))
+        outs[i].capacity = 2 * q.n
+    _check(lib().gq_plan_batch(k, qptrs, C.byref(cfg), outs))
+    return [(plan_tuples(outs[i]), outs[i].stats) for i in range(k)]
