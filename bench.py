@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU join-order search (one JSON line on stdout).
+
+Default workload (N=1): MPDP (DPsub with gpuqo_dpsub_bicc on) on the seeded
+20-relation snowflake -- the headline configuration.  A step is one complete
+plan of that query on the GPU.  `value` is csg-cmp pairs per second of
+device time (inputs resident in HBM: the query blob is uploaded before the
+first timed kernel; the plan tree is read back after the last); the
+PCIe-inclusive wall time per call is reported beside it.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Other workloads (for measurement, not the headline line): star12-dpsub,
+clique18-dpsub, clique18-dpsize.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "postgres-gpuqo_amd"))
+
+METRIC = "MPDP plan ms + csg-cmp pairs/s (1 GPU); batched queries/s and p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "mpdp-snowflake20": ("config2", "dpsub", 1),
+    "star12-dpsub": ("config1", "dpsub", 0),
+    "clique18-dpsub": ("config3", "dpsub", 0),
+    "clique18-dpsize": ("config3", "dpsize", 0),
+}
+
+
+def gather_roofline():
+    """Measured random-access rate (GB/s of record bytes) per regime."""
+    path = os.path.join(ROOT, "postgres-gpuqo_amd", "lib", "libgq_bench.so")
+    L = C.CDLL(path)
+    L.gq_gather_bench.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    out = {}
+    for name, fp in (("l2_2MB", 2 << 20), ("mall_64MB", 64 << 20), ("hbm_4GB", 4 << 30)):
+        for rec in (16, 40):
+            g, ms = C.c_double(), C.c_double()
+            rc = L.gq_gather_bench(fp, rec, 1 << 26, C.byref(g), C.byref(ms))
+            if rc == 0:
+                out[f"{rec}B@{name}"] = round(g.value, 1)
+    return out
+
+
+def cpu_baseline(g, algo, bicc, budget_s=12.0):
+    """The oracle's CPU restatement timed on this host (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # test infrastructure: CPU baseline leg only
+    from gpuqo_amd import abi
+    res = {}
+    seq_algo = {("dpsub", 1): "cpu_dpsub_bicc", ("dpsub", 0): "cpu_dpsub", ("dpsize", 0): "cpu_dpsize"}[(algo, bicc)]
+    cfg = abi.default_config(algorithm=seq_algo)
+    times, pairs = [], 0
+    t_start = time.time()
+    while True:
+        t0 = time.perf_counter()
+        st, nodes, stats = oracle.plan(g, cfg)
+        times.append(time.perf_counter() - t0)
+        pairs = stats.pairs
+        if time.time() - t_start > budget_s or len(times) >= 20:
+            break
+    med = statistics.median(times)
+    res["seq"] = {"algorithm": seq_algo, "runs": len(times), "median_ms": med * 1e3, "pairs": pairs,
+                  "pairs_per_s": pairs / med}
+    if algo == "dpsub":
+        ncpu = os.cpu_count() or 1
+        par_algo = "parallel_cpu_dpsub_bicc" if bicc else "parallel_cpu_dpsub"
+        pcfg = abi.default_config(algorithm=par_algo, dpe_n_threads=ncpu, cpu_dpsub_parallel_chunk_size=256)
+        ptimes = []
+        t_start = time.time()
+        while True:
+            t0 = time.perf_counter()
+            st, nodes, stats = oracle.plan(g, pcfg)
+            ptimes.append(time.perf_counter() - t0)
+            if time.time() - t_start > budget_s / 2 or len(ptimes) >= 20:
+                break
+        pm = statistics.median(ptimes)
+        res["parallel"] = {"algorithm": par_algo, "threads": ncpu, "chunk": 256, "runs": len(ptimes),
+                           "median_ms": pm * 1e3, "pairs_per_s": pairs / pm}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mpdp-snowflake20", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+
+    import gpuqo_amd
+    from gpuqo_amd import abi, qgen
+    cfg_name, algo, bicc = WORKLOADS[args.workload]
+    g = getattr(qgen, cfg_name)()
+    cfg = abi.default_config(algorithm=algo, dpsub_bicc=bicc, profile_kernels=1)
+    print(f"# workload {args.workload}: {g.shape}-{g.n}, algorithm {algo} bicc={bicc}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        gpuqo_amd.plan(g, cfg)
+    dev_ms, wall_ms, f_ms, e_ms, f_n, e_n = [], [], 0.0, 0.0, 0, 0
+    nodes = stats = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        nodes, stats = gpuqo_amd.plan(g, cfg)
+        wall_ms.append((time.perf_counter() - t1) * 1e3)
+        dev_ms.append(stats.gpu_ms)
+        f_ms += stats.filter_ms
+        e_ms += stats.eval_ms
+        f_n += stats.filter_launches
+        e_n += stats.eval_launches
+    total_wall = time.perf_counter() - t0
+    print(f"# timed {args.steps} steps in {total_wall:.3f} s", file=sys.stderr, flush=True)
+
+    P, S, Cn = stats.pairs, stats.sets, stats.candidates
+    dev_total_s = sum(dev_ms) / 1e3
+    value = P * args.steps / dev_total_s
+    # algorithmic bytes (80 P + 40 S + 16 C): two memo probes of an 8-byte key
+    # and a 32-byte JoinRelation per pair, one memo write per set, one 8-byte
+    # candidate write + read per unranked set.
+    eval_bytes = 80.0 * P + 40.0 * S
+    filt_bytes = 16.0 * Cn
+    if e_ms >= f_ms:
+        kern, kbytes, kms, kn = ("k_eval_bicc_lane" if bicc else "k_eval_dpsub_*"), eval_bytes, e_ms, e_n
+    else:
+        kern, kbytes, kms, kn = "k_filter", filt_bytes, f_ms, f_n
+    per_launch_bytes = kbytes * args.steps / kn
+    avg_launch_ms = kms / kn
+    achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
+            "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_ms,
+            "launches_per_step": kn // args.steps,
+            "phase_ms_per_step": {"filter": f_ms / args.steps, "eval": e_ms / args.steps}}
+    if not args.no_roofline:
+        ra = gather_roofline()
+        roof["measured_random_access_gbps"] = ra
+        whole = (eval_bytes + filt_bytes) / (dev_total_s / args.steps) / 1e9
+        roof["whole_dp_algorithmic_gbps"] = round(whole, 2)
+        if "40B@mall_64MB" in ra:
+            roof["frac_of_measured_mall_40B"] = round(whole / ra["40B@mall_64MB"], 4)
+        if "40B@hbm_4GB" in ra:
+            roof["frac_of_measured_hbm_40B"] = round(whole / ra["40B@hbm_4GB"], 4)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "csg-cmp pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": statistics.mean(dev_ms),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "shape": g.shape, "relations": g.n, "algorithm": algo,
+                   "gpuqo_dpsub_bicc": bicc, "pairs": P, "sets": S, "candidates": Cn,
+                   "parallelism": "single-gpu"},
+        "plan_ms_device_median": statistics.median(dev_ms),
+        "plan_ms_wall_median": statistics.median(wall_ms),
+        "plan_ms_wall_p99": sorted(wall_ms)[min(len(wall_ms) - 1, int(0.99 * len(wall_ms)))],
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(g, algo, bicc)
+        seq = cb["seq"]
+        out["cpu_baseline"] = {"value": seq["pairs_per_s"], "unit": "csg-cmp pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"{seq['runs']} full plans of the same query with {seq['algorithm']} "
+                                         f"(median {seq['median_ms']:.1f} ms)",
+                               "plan_ms": seq["median_ms"], "host": platform.processor() or platform.machine(),
+                               "nproc": os.cpu_count()}
+        if "parallel" in cb:
+            out["cpu_baseline"]["parallel"] = cb["parallel"]
+        # parity of the last GPU plan against the oracle (outside the timed region)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        st, want, _ = oracle.plan(g, abi.default_config(algorithm=algo, dpsub_bicc=bicc))
+        out["parity_vs_oracle"] = "bit-exact" if gpuqo_amd.plan_bits(nodes) == gpuqo_amd.plan_bits(want) else "MISMATCH"
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

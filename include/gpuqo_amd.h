@@ -163,6 +163,8 @@ typedef struct gq_config {
     /* Build-specific knobs (no reference counterpart). */
     int32_t memo_kind;                            /* GQ_MEMO_AUTO              */
     int32_t device;                               /* HIP device ordinal, 0     */
+    int32_t profile_kernels;                      /* time each phase with HIP
+                                                     events (stats.*_ms)       */
 } gq_config;
 
 /* One node of the returned join tree (QueryTreeC,
@@ -188,6 +190,11 @@ typedef struct gq_plan_stats {
     double wall_ms;           /* host wall time of the whole call          */
     int32_t n_dp_calls;       /* DPs run (1 for a plain query)             */
     int32_t memo_kind;        /* GQ_MEMO_DENSE or GQ_MEMO_HASH used        */
+    /* with cfg->profile_kernels: summed device time per phase */
+    double filter_ms;         /* unrank + connectivity filter kernels      */
+    double eval_ms;           /* split evaluation (+ finalize) kernels     */
+    int32_t filter_launches;
+    int32_t eval_launches;
 } gq_plan_stats;
 
 typedef struct gq_plan_out {

@@ -21,17 +21,27 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp
          "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
 
-def build(verbose=False):
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+BENCH_LIB = os.path.join(HERE, "lib", "libgq_bench.so")
+BENCH_SOURCES = ["gq_gather_bench.hip"]
+
+
+def _compile(out, sources, verbose):
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in sources]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose=False):
+    """Compile the planner library and the roofline microbenchmark."""
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    _compile(LIB, SOURCES, verbose)
+    _compile(BENCH_LIB, BENCH_SOURCES, verbose)
     return LIB
 
 

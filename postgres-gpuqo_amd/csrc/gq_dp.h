@@ -41,7 +41,8 @@ struct DPNode {                 /* plan node in DP numbering, pre-order */
 
 struct DPStats {
     uint64_t candidates = 0, sets = 0, pairs = 0, memo_bytes = 0;
-    double gpu_ms = 0.0;
+    double gpu_ms = 0.0, filter_ms = 0.0, eval_ms = 0.0;
+    int filter_launches = 0, eval_launches = 0;
     int memo_kind = 1;
 };
 
@@ -50,7 +51,7 @@ enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
 /* Run one DP on the current device.  `n_iters` < n_rels selects the IDP1
  * level cap (the tree returned is the cheapest set of that size).  Returns a
  * gq_status. */
-int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vector<DPNode> &tree,
+int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
            DPStats &stats);
 
 }  // namespace gq

@@ -490,6 +490,7 @@ struct DeviceCtx {
     size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
            cap_nodes = 0, cap_counters = 0;
     bool binom_ready = false;
+    std::vector<hipEvent_t> prof_ev;
     std::mutex mu;
 };
 
@@ -542,7 +543,8 @@ int device_init(int device) {
     return ctx_init(g_ctx, device);
 }
 
-int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vector<DPNode> &tree, DPStats &st) {
+int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
+           DPStats &st) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
     DeviceCtx &c = g_ctx;
     int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
@@ -591,7 +593,16 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vect
         hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, c.b.blob, c.b.vals, c.b.left, c.b.conn);
     }
     uint64_t cand_total = 0;
+    /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
+    if (profile && (int)c.prof_ev.size() < 3 * 33) {
+        while ((int)c.prof_ev.size() < 3 * 33) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.prof_ev.push_back(e);
+        }
+    }
     for (int k = 2; k <= n; k++) {
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         uint64_t Ck = binom64(n, k);
         cand_total += Ck;
         uint64_t threads = std::min<uint64_t>(Ck, 1ull << 19);
@@ -601,6 +612,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vect
         hipLaunchKernelGGL(k_filter, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, k, (uint32_t)Ck, per,
                            c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.conn, c.b.counters);
         uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         if (algo == DP_BICC) {
             hipLaunchKernelGGL(k_eval_bicc_lane, dim3(lblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k,
                                c.b.lvl_sets, c.b.vals, c.b.left, c.b.counters);
@@ -623,6 +635,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vect
                                    c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.left, c.b.counters);
             }
         }
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
     uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, c.b.vals, c.b.left, full, c.b.nodes, 2 * n - 1,
@@ -641,6 +654,17 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, std::vect
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     st.gpu_ms = ms;
+    if (profile) {
+        for (int k = 2; k <= n; k++) {
+            float a = 0.0f, b = 0.0f;
+            HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
+            HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
+            st.filter_ms += a;
+            st.eval_ms += b;
+            st.filter_launches++;
+            st.eval_launches++;
+        }
+    }
     st.candidates = cand_total;
     st.sets = 0;
     for (int k = 2; k <= n; k++) st.sets += cnt[k];

@@ -260,7 +260,7 @@ int run_switch<1>(RunCtx &ctx, const HQuery<1> &h, QT<1> **out) {
     DPAlgo algo = ctx.algorithm == GQ_ALGO_DPSIZE ? DP_DPSIZE : (ctx.bicc ? DP_BICC : DP_DPSUB);
     std::vector<DPNode> nodes;
     DPStats st;
-    rc = run_dp(in, algo, h.n_iters, ctx.memo_kind, nodes, st);
+    rc = run_dp(in, algo, h.n_iters, ctx.memo_kind, ctx.profile != 0, nodes, st);
     ctx.dp_calls++;
     if (rc) return rc;
     ctx.add(st);

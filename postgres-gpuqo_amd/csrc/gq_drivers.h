@@ -16,14 +16,14 @@ namespace gq {
 /* Per-call state: configuration, counters, and an arena owning every plan
  * node built during the call (trees share composite subtrees). */
 struct RunCtx {
-    int algorithm, bicc, idp_type, idp_n_iters, memo_kind;
+    int algorithm, bicc, idp_type, idp_n_iters, memo_kind, profile;
     DPStats stats;
     int dp_calls = 0;
     std::deque<QT<1>> arena1;
     std::deque<QT<4>> arena4;
     explicit RunCtx(const gq_config *cfg)
         : algorithm(cfg->gpuqo_algorithm), bicc(cfg->gpuqo_dpsub_bicc), idp_type(cfg->gpuqo_idp_type),
-          idp_n_iters(cfg->gpuqo_idp_n_iters), memo_kind(cfg->memo_kind) {}
+          idp_n_iters(cfg->gpuqo_idp_n_iters), memo_kind(cfg->memo_kind), profile(cfg->profile_kernels) {}
     template <int W>
     QT<W> *node();
     void add(const DPStats &s) {
@@ -31,6 +31,10 @@ struct RunCtx {
         stats.sets += s.sets;
         stats.pairs += s.pairs;
         stats.gpu_ms += s.gpu_ms;
+        stats.filter_ms += s.filter_ms;
+        stats.eval_ms += s.eval_ms;
+        stats.filter_launches += s.filter_launches;
+        stats.eval_launches += s.eval_launches;
         if (s.memo_bytes > stats.memo_bytes) stats.memo_bytes = s.memo_bytes;
         stats.memo_kind = s.memo_kind;
     }

@@ -362,6 +362,10 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     out->stats.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     out->stats.n_dp_calls = ctx.dp_calls;
     out->stats.memo_kind = ctx.stats.memo_kind;
+    out->stats.filter_ms = ctx.stats.filter_ms;
+    out->stats.eval_ms = ctx.stats.eval_ms;
+    out->stats.filter_launches = ctx.stats.filter_launches;
+    out->stats.eval_launches = ctx.stats.eval_launches;
     if (rc != GQ_OK) return rc;
     if (!t) {
         set_error("no join tree (is the join graph connected?)");

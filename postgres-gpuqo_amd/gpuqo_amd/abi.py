@@ -90,7 +90,8 @@ class Config(C.Structure):
                 ("enable_hashagg", C.c_uint8), ("enable_nestloop", C.c_uint8),
                 ("enable_mergejoin", C.c_uint8), ("enable_hashjoin", C.c_uint8),
                 ("memo_kind", C.c_int32),
-                ("device", C.c_int32)]
+                ("device", C.c_int32),
+                ("profile_kernels", C.c_int32)]
 
 
 class PlanNode(C.Structure):
@@ -103,7 +104,9 @@ class PlanStats(C.Structure):
     _fields_ = [("candidates", C.c_uint64), ("sets", C.c_uint64),
                 ("pairs", C.c_uint64), ("memo_bytes", C.c_uint64),
                 ("gpu_ms", C.c_double), ("wall_ms", C.c_double),
-                ("n_dp_calls", C.c_int32), ("memo_kind", C.c_int32)]
+                ("n_dp_calls", C.c_int32), ("memo_kind", C.c_int32),
+                ("filter_ms", C.c_double), ("eval_ms", C.c_double),
+                ("filter_launches", C.c_int32), ("eval_launches", C.c_int32)]
 
 
 class PlanOut(C.Structure):
@@ -154,6 +157,7 @@ def default_config(**overrides):
     c.enable_hashjoin = 1
     c.memo_kind = MEMO["auto"]
     c.device = 0
+    c.profile_kernels = 0
     for k, v in overrides.items():
         if k == "algorithm":
             v = ALGO[v] if isinstance(v, str) else v
