@@ -220,45 +220,6 @@ __global__ void __launch_bounds__(BLOCK) k_filter(BlobHdr h, const uint32_t *blo
     }
 }
 
-/* DPsub, one lane per set: the reference's sequential scan over l
- * ascending (gpuqo_cpu_dpsub.cu:32-79), exact tie/NaN behaviour. */
-__global__ void __launch_bounds__(BLOCK) k_eval_dpsub_lane(BlobHdr h, const uint32_t *blob, int k,
-                                                            const uint32_t *lvl_sets, float4 *vals,
-                                                            uint32_t *left, const uint32_t *conn,
-                                                            uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
-    stage(blob, smem, h.words);
-    View<uint32_t> q = make_view(h, smem);
-    uint32_t count = counters[k];
-    uint32_t npairs = 0;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
-        uint32_t S = lvl_sets[idx];
-        bool have = false;
-        Rel best;
-        uint32_t best_l = 0;
-        uint32_t l = S & (0u - S);
-        while (l != S) {
-            uint32_t r = S ^ l;
-            if (is_conn(conn, l) && is_conn(conn, r)) {
-                Rel L = load_rel(vals, l), R = load_rel(vals, r);
-                Rel j = join(q, l, L, r, R);
-                npairs++;
-                if (!have || j.total < best.total) {
-                    best = j;
-                    best_l = l;
-                    have = true;
-                }
-            }
-            l = S & (l - S);
-        }
-        if (have) {
-            store_rel(vals, S, best);
-            left[S] = best_l;
-        }
-    }
-    add_pairs(counters, npairs);
-}
-
 /* DPsub, split-parallel: a wave owns `ipl * 64` consecutive split ranks of
  * one set; lane t scans ranks [j0 + t*ipl, j0 + (t+1)*ipl) in order.  Each
  * lane keeps (first valid rank, NaN flag) and the first minimum among
@@ -339,101 +300,249 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
     }
 }
 
-/* MPDP, one lane per set.  Blocks come from a DFS from the lowest vertex
- * with neighbours in ascending order, emitted when a tree edge (p,c)
- * finishes with low[c] >= disc[p] -- the emission order of find_blocks
- * (gpuqo_cpu_dpsub_bicc.cu:65-169).  The block's vertices are p plus the
- * vertices discovered at or after c still on the vertex stack.  Each block
- * is enumerated as soon as it is emitted: lb ascends over its non-empty
- * proper subsets; L = closure of lb's lowest vertex in S - rb, R likewise;
- * (L, R) is valid iff L | R == S (:172-236). */
-__global__ void __launch_bounds__(BLOCK) k_eval_bicc_lane(BlobHdr h, const uint32_t *blob, int k,
-                                                           const uint32_t *lvl_sets, float4 *vals,
-                                                           uint32_t *left, uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
-    uint32_t *dfs = smem + ((h.words + 3) & ~3);      /* [32][BLOCK]: disc | low<<8 | parent<<16 */
-    stage(blob, smem, h.words);
-    View<uint32_t> q = make_view(h, smem);
-    const uint32_t *adj = smem + h.o_adj;
-    const int tx = threadIdx.x;
-#define DFS(v) dfs[(v) * BLOCK + tx]
-    uint32_t count = counters[k];
-    uint32_t npairs = 0;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
-        uint32_t S = lvl_sets[idx];
-        bool have = false;
-        Rel best;
-        uint32_t best_l = 0;
-        int root = __builtin_ctz(S);
-        uint32_t visited = 1u << root, onstack = 1u << root;
-        uint32_t t = 1;
-        DFS(root) = 0u | (255u << 8) | (255u << 16);
-        int u = root, cursor = -1;
-        while (true) {
-            uint32_t gt = cursor < 0 ? 0xffffffffu : (uint32_t)(0xffffffffull << (cursor + 1));
-            uint32_t rem = adj[u] & S & gt;
-            uint32_t du = DFS(u);
-            if (rem) {
-                int v = __builtin_ctz(rem);
-                if (!((visited >> v) & 1u)) {
-                    DFS(v) = t | (255u << 8) | ((uint32_t)u << 16);
-                    t++;
-                    visited |= 1u << v;
-                    onstack |= 1u << v;
-                    u = v;
-                    cursor = -1;
-                } else {
-                    uint32_t dv = DFS(v);
-                    if ((int)(du >> 16) != v && (dv & 0xffu) < (du & 0xffu)) {
-                        uint32_t low = (du >> 8) & 0xffu;
-                        uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
-                        DFS(u) = (du & 0xffff00ffu) | (nl << 8);
-                    }
-                    cursor = v;
+#define BMAX 128                 /* sets per workgroup batch            */
+#define QCAP 1024                /* DPsub valid-pair queue per round    */
+
+/* Blocks of connected set S in the emission order of find_blocks
+ * (gpuqo_cpu_dpsub_bicc.cu:65-169): DFS from the lowest vertex, neighbours
+ * ascending; a block is emitted when a tree edge (p,c) finishes with
+ * low[c] >= disc[p]; its vertices are p plus the vertices discovered at or
+ * after c still on the vertex stack.  `dfs` is this lane's column of a
+ * [32][stride] LDS array (disc | low << 8 | parent << 16). */
+__device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint32_t *dfs, int stride,
+                                          uint32_t *blocks) {
+#define DFS(v) dfs[(v) * stride]
+    int nblk = 0;
+    int root = __builtin_ctz(S);
+    uint32_t visited = 1u << root, onstack = 1u << root;
+    uint32_t t = 1;
+    DFS(root) = 0u | (255u << 8) | (255u << 16);
+    int u = root, cursor = -1;
+    while (true) {
+        uint32_t gt = cursor < 0 ? 0xffffffffu : (uint32_t)(0xffffffffull << (cursor + 1));
+        uint32_t rem = adj[u] & S & gt;
+        uint32_t du = DFS(u);
+        if (rem) {
+            int v = __builtin_ctz(rem);
+            if (!((visited >> v) & 1u)) {
+                DFS(v) = t | (255u << 8) | ((uint32_t)u << 16);
+                t++;
+                visited |= 1u << v;
+                onstack |= 1u << v;
+                u = v;
+                cursor = -1;
+            } else {
+                uint32_t dv = DFS(v);
+                if ((int)(du >> 16) != v && (dv & 0xffu) < (du & 0xffu)) {
+                    uint32_t low = (du >> 8) & 0xffu;
+                    uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
+                    DFS(u) = (du & 0xffff00ffu) | (nl << 8);
                 }
-                continue;
+                cursor = v;
             }
-            if (u == root) break;
-            int p = (int)(du >> 16);
-            uint32_t dp = DFS(p);
-            uint32_t lowu = (du >> 8) & 0xffu, discu = du & 0xffu;
-            if (lowu >= (dp & 0xffu)) {
-                uint32_t rest = 0, os = onstack;
-                while (os) {
-                    int w = __builtin_ctz(os);
-                    os &= os - 1;
-                    if ((DFS(w) & 0xffu) >= discu) rest |= 1u << w;
-                }
-                onstack &= ~rest;
-                uint32_t block = rest | (1u << p);
-                uint32_t lb = block & (0u - block);
-                while (lb != block) {
-                    uint32_t rb = block ^ lb;
-                    uint32_t L = grow(lb & (0u - lb), S & ~rb, adj);
-                    uint32_t R = grow(rb & (0u - rb), S & ~lb, adj);
-                    if ((L | R) == S) {
-                        Rel j = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
-                        npairs++;
-                        if (!have || j.total < best.total) {
-                            best = j;
-                            best_l = L;
-                            have = true;
-                        }
-                    }
-                    lb = block & (lb - block);
-                }
+            continue;
+        }
+        if (u == root) break;
+        int p = (int)(du >> 16);
+        uint32_t dp = DFS(p);
+        uint32_t lowu = (du >> 8) & 0xffu, discu = du & 0xffu;
+        if (lowu >= (dp & 0xffu)) {
+            uint32_t rest = 0, os = onstack;
+            while (os) {
+                int w = __builtin_ctz(os);
+                os &= os - 1;
+                if ((DFS(w) & 0xffu) >= discu) rest |= 1u << w;
             }
-            uint32_t lowp = (dp >> 8) & 0xffu;
-            if (lowu < lowp) DFS(p) = (dp & 0xffff00ffu) | (lowu << 8);
-            cursor = u;
-            u = p;
+            onstack &= ~rest;
+            blocks[nblk++] = rest | (1u << p);
         }
-        if (have) {
-            store_rel(vals, S, best);
-            left[S] = best_l;
-        }
+        uint32_t lowp = (dp >> 8) & 0xffu;
+        if (lowu < lowp) DFS(p) = (dp & 0xffff00ffu) | (lowu << 8);
+        cursor = u;
+        u = p;
     }
 #undef DFS
+    return nblk;
+}
+
+/* Split `local` of a set whose blocks are blocks[0..nblk): block-major,
+ * lb ascending over the block's non-empty proper subsets. */
+__device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, uint32_t local, const uint32_t *adj,
+                                           uint32_t *L, uint32_t *R) {
+    uint32_t acc = 0, blk;
+    int b = 0;
+    while (true) {
+        blk = blocks[b];
+        uint32_t c = (1u << __builtin_popcount(blk)) - 2u;
+        if (local < acc + c) break;
+        acc += c;
+        b++;
+    }
+    uint32_t lb = pdep32(local - acc + 1u, blk), rb = blk ^ lb;
+    *L = grow(lb & (0u - lb), S & ~rb, adj);
+    *R = grow(rb & (0u - rb), S & ~lb, adj);
+}
+
+/* One level, one kernel: a workgroup takes a batch of B sets.
+ *   1. one lane per set prepares it (MPDP: block decomposition into LDS) and
+ *      counts its splits; an exclusive scan lays the batch's splits out
+ *      set-major, each set's splits in the reference's enumeration order;
+ *   2. all lanes evaluate splits; per set, LDS atomics keep (a) the first
+ *      valid split with its NaN flag and (b) the first minimum of the packed
+ *      key (order-preserving cost bits << 32 | split rank) over non-NaN
+ *      costs -- together the reference's "first found, replaced only if
+ *      strictly cheaper" (gpuqo_cpu_common.cuh:189-211);
+ *   3. the set's lane rebuilds the winning split and writes the memo.
+ * DPsub splits are l = pdep(j, S), j = 1 .. 2^k-2 (l ascending,
+ * gpuqo_cpu_dpsub.cu:32-79); only pairs whose sides are both connected are
+ * costed, gathered first into an LDS queue so that every lane of a wave
+ * costs a valid pair. */
+template <bool BICC>
+__global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
+                                                    int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
+                                                    uint32_t *left, const uint32_t *conn, uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int W4 = (h.words + 3) & ~3;
+    unsigned long long *s_best = (unsigned long long *)(smem + W4);
+    unsigned long long *s_off = s_best + BMAX;                 /* BMAX + 2 entries */
+    uint32_t *s_first = (uint32_t *)(s_off + BMAX + 2);
+    uint32_t *s_set = s_first + BMAX;
+    uint32_t *s_misc = s_set + BMAX;                           /* [0] queue length, [1] error */
+    uint32_t *s_nblk = s_misc + 4;
+    uint32_t *s_blocks = s_nblk + BMAX;                        /* BICC: [BMAX][32]            */
+    uint32_t *dfs = s_blocks + BMAX * 32;                      /* BICC: [32][BMAX]            */
+    uint32_t *q_l = s_misc + 4;                                /* DPsub: [QCAP]               */
+    uint32_t *q_i = q_l + QCAP;
+    uint32_t *q_r = q_i + QCAP;
+    uint32_t *conn_l = q_r + QCAP;
+    for (int i = tid; i < h.words; i += BLOCK) smem[i] = blob[i];
+    if (!BICC)
+        for (int i = tid; i < conn_words_lds; i += BLOCK) conn_l[i] = conn[i];
+    if (tid == 0) s_misc[1] = 0;
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t *adj = smem + h.o_adj;
+    const uint32_t *cb = conn_words_lds ? conn_l : conn;
+    const uint32_t T = (k >= 32) ? 0xfffffffeu : (uint32_t)((1ull << k) - 2ull);
+    const uint32_t count = counters[k];
+    const uint32_t nbatches = (count + B - 1) / B;
+    uint32_t npairs = 0;
+    for (uint32_t bt = blockIdx.x; bt < nbatches; bt += gridDim.x) {
+        const uint32_t base = bt * B;
+        const uint32_t nb = (count - base) < B ? (count - base) : B;
+        if ((uint32_t)tid < nb) {
+            uint32_t S = lvl_sets[base + tid];
+            s_set[tid] = S;
+            s_best[tid] = ~0ull;
+            s_first[tid] = ~0u;
+            unsigned long long cnt = T;
+            if (BICC) {
+                int nblk = dfs_blocks(S, adj, dfs + tid, BMAX, s_blocks + tid * 32);
+                s_nblk[tid] = nblk;
+                cnt = 0;
+                for (int b = 0; b < nblk; b++) cnt += (1ull << __builtin_popcount(s_blocks[tid * 32 + b])) - 2ull;
+                if (cnt > 0x7fffffffull) s_misc[1] = 1;   /* split rank would not fit 31 bits */
+            }
+            s_off[tid] = cnt;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long acc = 0;
+            for (uint32_t i = 0; i < nb; i++) {
+                unsigned long long c = s_off[i];
+                s_off[i] = acc;
+                acc += c;
+            }
+            s_off[nb] = acc;
+        }
+        __syncthreads();
+        const unsigned long long total = s_off[nb];
+        if (BICC) {
+            for (unsigned long long item = tid; item < total; item += BLOCK) {
+                uint32_t lo = 0, hi = nb;                      /* largest i with s_off[i] <= item */
+                while (hi - lo > 1) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (s_off[mid] <= item) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t i = lo, S = s_set[i];
+                const uint32_t local = (uint32_t)(item - s_off[i]);
+                uint32_t L, R;
+                bicc_split(S, s_blocks + i * 32, local, adj, &L, &R);
+                if ((L | R) == S) {
+                    Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+                    npairs++;
+                    bool nan = jr.total != jr.total;
+                    atomicMin(&s_first[i], (local << 1) | (nan ? 1u : 0u));
+                    if (!nan) atomicMin(&s_best[i], ((unsigned long long)ord_key(jr.total) << 32) | local);
+                }
+            }
+        } else {
+            for (unsigned long long r0 = 0; r0 < total; r0 += QCAP) {
+                if (tid == 0) s_misc[0] = 0;
+                __syncthreads();
+                unsigned long long rend = r0 + QCAP < total ? r0 + QCAP : total;
+                for (unsigned long long item0 = r0; item0 < rend; item0 += BLOCK) {
+                    unsigned long long item = item0 + tid;
+                    bool ok = false;
+                    uint32_t l = 0, i = 0, j = 0;
+                    if (item < rend) {
+                        i = (uint32_t)(item / T);
+                        j = (uint32_t)(item % T) + 1u;
+                        uint32_t S = s_set[i];
+                        l = pdep32(j, S);
+                        ok = is_conn(cb, l) && is_conn(cb, S ^ l);
+                    }
+                    unsigned long long m = __ballot(ok);
+                    if (m) {
+                        int leader = __builtin_ctzll(m);
+                        uint32_t pos = 0;
+                        if (lane == leader) pos = atomicAdd(&s_misc[0], (uint32_t)__popcll(m));
+                        pos = __shfl(pos, leader);
+                        if (ok) {
+                            pos += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                            q_l[pos] = l;
+                            q_i[pos] = i;
+                            q_r[pos] = j;
+                        }
+                    }
+                }
+                __syncthreads();
+                const uint32_t qn = s_misc[0];
+                for (uint32_t e = tid; e < qn; e += BLOCK) {
+                    uint32_t l = q_l[e], i = q_i[e], j = q_r[e];
+                    uint32_t r = s_set[i] ^ l;
+                    Rel jr = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
+                    npairs++;
+                    bool nan = jr.total != jr.total;
+                    atomicMin(&s_first[i], (j << 1) | (nan ? 1u : 0u));
+                    if (!nan) atomicMin(&s_best[i], ((unsigned long long)ord_key(jr.total) << 32) | j);
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        if ((uint32_t)tid < nb) {
+            uint32_t f = s_first[tid];
+            if (f != ~0u) {
+                uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(s_best[tid] & 0xffffffffull);
+                uint32_t S = s_set[tid], L, R;
+                if (BICC) {
+                    bicc_split(S, s_blocks + tid * 32, rank, adj, &L, &R);
+                } else {
+                    L = pdep32(rank, S);
+                    R = S ^ L;
+                }
+                Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+                store_rel(vals, S, jr);
+                left[S] = L;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && s_misc[1]) atomicOr(&counters[49], 1u);
     add_pairs(counters, npairs);
 }
 
@@ -565,8 +674,12 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     }
     const size_t blob_bytes = (size_t)h.words * 4;
     const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
-    const size_t lds_bicc = lds_eval + (size_t)32 * BLOCK * 4;
-    if (lds_bicc > 64 * 1024) {
+    const uint32_t conn_words_all = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
+    const int conn_lds = conn_words_all <= 4096 ? (int)conn_words_all : 0;
+    const size_t lds_common = lds_eval + BMAX * 8 + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
+    const size_t lds_bicc = lds_common + BMAX * 4 + (size_t)BMAX * 32 * 4 * 2;
+    const size_t lds_dpsub = lds_common + 3 * QCAP * 4 + (size_t)conn_lds * 4;
+    if (lds_bicc > 64 * 1024 || lds_dpsub > 64 * 1024) {
         set_error("query description too large for LDS (%zu bytes)", blob_bytes);
         return GQ_ERR_UNSUPPORTED;
     }
@@ -613,27 +726,28 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                            c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.conn, c.b.counters);
         uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
-        if (algo == DP_BICC) {
-            hipLaunchKernelGGL(k_eval_bicc_lane, dim3(lblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k,
-                               c.b.lvl_sets, c.b.vals, c.b.left, c.b.counters);
-        } else {
-            uint64_t T = (1ull << k) - 2;
-            bool lane_mode = T <= 256 || Ck * T <= (1ull << 16);
-            if (lane_mode) {
-                hipLaunchKernelGGL(k_eval_dpsub_lane, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+        const uint64_t T = (1ull << k) - 2;
+        if (algo == DP_BICC || T <= 4096) {
+            uint64_t items_est = algo == DP_BICC ? 2ull * (k - 1) : T;
+            uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
+            uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
+            if (algo == DP_BICC) {
+                hipLaunchKernelGGL(k_eval_wg<true>, dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k, B, 0,
                                    c.b.lvl_sets, c.b.vals, c.b.left, c.b.conn, c.b.counters);
             } else {
-                uint32_t ipl = 16;
-                uint64_t per_wave = (uint64_t)ipl * WAVE;
-                uint32_t cps = (uint32_t)((T + per_wave - 1) / per_wave);
-                uint64_t waves = Ck * cps;
-                uint32_t wblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 16384);
-                hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   cps, ipl, c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.conn,
-                                   c.b.counters);
-                hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.left, c.b.counters);
+                hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, c.b.blob, k, B,
+                                   conn_lds, c.b.lvl_sets, c.b.vals, c.b.left, c.b.conn, c.b.counters);
             }
+        } else {
+            uint32_t ipl = 16;
+            uint64_t per_wave = (uint64_t)ipl * WAVE;
+            uint32_t cps = (uint32_t)((T + per_wave - 1) / per_wave);
+            uint64_t waves = Ck * cps;
+            uint32_t wblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 16384);
+            hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, cps, ipl,
+                               c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.conn, c.b.counters);
+            hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                               c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.left, c.b.counters);
         }
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
@@ -647,6 +761,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipMemcpyAsync(tree.data(), c.b.nodes, sizeof(DPNode) * (2 * n - 1), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(cnt, c.b.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    if (cnt[49] != 0) {
+        set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");
+        return GQ_ERR_UNSUPPORTED;
+    }
     if (cnt[48] != (uint32_t)(2 * n - 1)) {
         set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
         return GQ_ERR_DEVICE;

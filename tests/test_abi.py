@@ -1,0 +1,54 @@
+"""The C ABI library loads, exports every entry point include/*.h declares,
+and its structures match the Python mirror.  No device work is done here."""
+import ctypes as C
+import glob
+import os
+import re
+
+import gpuqo_amd
+from gpuqo_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(gq_\w+)\s*\(", txt, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for f in ("gq_plan", "gq_plan_batch", "gq_config_default", "gq_server_create", "gq_server_submit",
+              "gq_server_poll", "gq_server_destroy", "gq_last_error", "gq_init"):
+        assert f in names
+
+
+def test_library_exports_all_declared_symbols():
+    lib = gpuqo_amd.lib()
+    missing = [f for f in declared_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+
+
+def test_struct_sizes():
+    assert C.sizeof(abi.BaseRel) == 20
+    assert C.sizeof(abi.VarInfo) == 28
+    assert C.sizeof(abi.PlanNode) == 56
+    assert C.sizeof(abi.Query) == 72
+
+
+def test_config_default_matches_mirror():
+    lib = gpuqo_amd.lib()
+    c = abi.Config()
+    assert lib.gq_config_default(C.byref(c)) == 0
+    d = abi.default_config()
+    for name, _ in abi.Config._fields_:
+        assert getattr(c, name) == getattr(d, name), name
+
+
+def test_version_string():
+    assert b"gfx950" in gpuqo_amd.lib().gq_version()

@@ -1,0 +1,407 @@
+"""Pins the CPU oracle (oracle/gq_oracle.cpp) to facts fixed independently of
+its code: closed-form pair counts, brute-force enumeration, textbook graph
+algorithms, hand-derived cost-model answers and an independent restatement
+of the DP recurrence.  The reference ships no golden vectors for this path
+and cannot be compiled here, so these are the oracle's anchors."""
+import itertools
+from collections import deque
+
+import pytest
+
+import gpuqo_amd
+import oracle
+from gpuqo_amd import abi, qgen
+from helpers import small_graphs
+
+
+def cfg(algo, **kw):
+    return abi.default_config(algorithm=algo, **kw)
+
+
+def connected(mask, adj):
+    if mask == 0:
+        return False
+    start = mask & -mask
+    seen, stack = start, [start.bit_length() - 1]
+    while stack:
+        u = stack.pop()
+        nb = adj[u] & mask & ~seen
+        while nb:
+            b = nb & -nb
+            nb ^= b
+            seen |= b
+            stack.append(b.bit_length() - 1)
+    return seen == mask
+
+
+def brute_pairs_and_sets(g):
+    adj = g.adjacency()
+    n = g.n
+    P = S = 0
+    for s in range(1, 1 << n):
+        if bin(s).count("1") < 2 or not connected(s, adj):
+            continue
+        S += 1
+        sub = (s - 1) & s
+        while sub:
+            if connected(sub, adj) and connected(s ^ sub, adj):
+                P += 1
+            sub = (sub - 1) & s
+    return P, S
+
+
+# ---------------------------------------------------------------- counts
+
+@pytest.mark.parametrize("n", [3, 5, 8, 11])
+def test_pair_counts_closed_forms(n):
+    cases = [
+        (qgen.star(n, 1), (n - 1) * 2 ** (n - 1)),
+        (qgen.clique(n, 2), 3 ** n - 2 ** (n + 1) + 1),
+        (qgen.chain(n, 3), (n ** 3 - n) // 3),
+        (qgen.cycle(n, 4), n ** 3 - 2 * n ** 2 + n),
+    ]
+    for g, want in cases:
+        for algo in ("cpu_dpsub", "cpu_dpsub_bicc", "cpu_dpsize"):
+            st, nodes, stats = oracle.plan(g, cfg(algo))
+            assert st == 0
+            assert stats.pairs == want, (g.shape, n, algo)
+            assert len(nodes) == 2 * n - 1
+
+
+@pytest.mark.parametrize("n,seed", [(7, 1), (10, 2), (12, 3), (14, 4)])
+def test_tree_pairs_and_sets_brute_force(n, seed):
+    g = qgen.snowflake(n, seed)
+    P, S = brute_pairs_and_sets(g)
+    for algo in ("cpu_dpsub", "cpu_dpsub_bicc"):
+        st, _, stats = oracle.plan(g, cfg(algo))
+        assert st == 0 and stats.pairs == P and stats.sets == S
+    # trees: every connected set of size s has exactly 2(s-1) splits
+    adj = g.adjacency()
+    tot = sum(2 * (bin(s).count("1") - 1) for s in range(1, 1 << n)
+              if bin(s).count("1") > 1 and connected(s, adj))
+    assert tot == P
+
+
+# ---------------------------------------------------------------- graph algorithms
+
+def textbook_bfs(g):
+    adj = g.adjacency()
+    order, seen, q = [], {0}, deque([0])
+    while q:
+        u = q.popleft()
+        order.append(u)
+        for v in range(g.n):
+            if adj[u] >> v & 1 and v not in seen:
+                seen.add(v)
+                q.append(v)
+    return order
+
+
+@pytest.mark.parametrize("name,g", small_graphs(), ids=lambda x: x if isinstance(x, str) else "")
+def test_bfs_order(name, g):
+    assert oracle.bfs_order(g) == textbook_bfs(g)
+
+
+def tarjan_blocks(mask, adj):
+    """Recursive Hopcroft-Tarjan with an edge stack, neighbours ascending."""
+    disc, low, out, es = {}, {}, [], []
+    t = [0]
+
+    def dfs(u, parent):
+        disc[u] = low[u] = t[0]
+        t[0] += 1
+        for v in range(32):
+            if not (adj[u] >> v & 1 and mask >> v & 1):
+                continue
+            if v not in disc:
+                es.append((u, v))
+                dfs(v, u)
+                low[u] = min(low[u], low[v])
+                if low[v] >= disc[u]:
+                    blk = 0
+                    while True:
+                        a, b = es.pop()
+                        blk |= 1 << a | 1 << b
+                        if (a, b) == (u, v):
+                            break
+                    out.append(blk)
+            elif v != parent and disc[v] < disc[u]:
+                es.append((u, v))
+                low[u] = min(low[u], disc[v])
+
+    dfs((mask & -mask).bit_length() - 1, -1)
+    return out
+
+
+def random_graph(n, p, seed):
+    import random
+    r = random.Random(seed)
+    g = qgen.QueryGraph(n, "random")
+    for i in range(n):
+        g.rows[i] = float(r.randint(10, 1000))
+    for i in range(1, n):                      # spanning tree keeps it connected
+        g.add_generic(r.randrange(i), i, 0.01)
+    have = {(min(a, b), max(a, b)) for a, b, _, _ in g.edges}
+    for i in range(n):
+        for j in range(i + 1, n):
+            if (i, j) not in have and r.random() < p:
+                g.add_generic(i, j, 0.05)
+    return g.build()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_block_order_matches_recursive_tarjan(seed):
+    g = random_graph(9, 0.25, seed)
+    adj = g.adjacency()
+    for s in range(1, 1 << g.n):
+        if bin(s).count("1") >= 2 and connected(s, adj):
+            assert oracle.blocks(g, s) == tarjan_blocks(s, adj), hex(s)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_bicc_splits_are_exactly_the_csg_cmp_pairs(seed):
+    """Order-free: MPDP must cost the same pairs as plain DPsub (and find the
+    same optimum when no two splits tie)."""
+    g = random_graph(9, 0.3, 100 + seed)
+    P, S = brute_pairs_and_sets(g)
+    st, a, sa = oracle.plan(g, cfg("cpu_dpsub"))
+    st2, b, sb = oracle.plan(g, cfg("cpu_dpsub_bicc"))
+    assert sa.pairs == sb.pairs == P and sa.sets == sb.sets == S
+    assert a[0][3] == b[0][3]
+
+
+def test_dpsize_level1_order_is_a_permutation():
+    for n in (2, 5, 12, 31):
+        o = oracle.dpsize_level1_order(n)
+        assert sorted(o) == list(range(n))
+
+
+# ---------------------------------------------------------------- cost model
+
+def test_hash_table_size_known_answers():
+    # 100 tuples of width 8: 1024 buckets, 1 batch, 83886 // 124 skew MCVs
+    assert oracle.hash_table_size(100.0, 8) == (1024, 1, 676)
+    # 1e6 tuples: 40 MB > work_mem -> lbuckets = 2^ceil_log2(4110418 // 48) = 131072,
+    # batches = 2^ceil_log2(ceil(4e7 / (4110418 - 1048576))) = 16
+    assert oracle.hash_table_size(1.0e6, 8) == (131072, 16, 676)
+    # non-positive input rows are forced to 1000
+    assert oracle.hash_table_size(0.0, 8) == oracle.hash_table_size(1000.0, 8)
+    # batches are capped at max_pointers = 524288
+    assert oracle.hash_table_size(1.0e12, 8)[1] == 524288
+
+
+def two_rel(rows0, rows1, sel, fk=False, idx=False):
+    g = qgen.QueryGraph(2, "kat")
+    g.rows = [rows0, rows1]
+    if fk:
+        g.add_pkfk(0, 1)
+    else:
+        g.add_generic(0, 1, sel)
+    g.build()
+    if not idx:
+        g.ec_vars["index_available"] = 0
+        g.ec_vars["index_unique"] = 0
+    return g
+
+
+def test_nestloop_known_answer_dyadic():
+    """Only nested loop enabled, dyadic parameters: every operation is exact.
+    outer {rows 8, cost 0..64}, inner {rows 4, cost 0..16}, sel 1/2:
+      rows = 8*4/2 = 16
+      run  = 64 + 16 + (8-1)*16 = 192, cpu = (1/4 + 1/8 per qual) * 32 = 12
+      total = 204"""
+    g = two_rel(8.0, 4.0, 0.5)
+    c = cfg("cpu_dpsub", cpu_tuple_cost=0.25, cpu_operator_cost=0.125, enable_hashjoin=0)
+    out = oracle.join_cost(g, c, [1], [2], (8.0, 0.0, 64.0, 8.0), (4.0, 0.0, 16.0, 4.0))
+    assert out == (16.0, 0.0, 204.0, 12.0)
+
+
+def test_nestloop_startup_terms():
+    """Inner startup 2 is paid once up front and again per extra outer row:
+    startup = 2 + 1 = 3; run = (10-1) + 7*2 + 14 + 7*14 = 135;
+    cpu = (0.25+0.125)*8*8 = 24; total = 3 + 135 + 24 = 162."""
+    g = two_rel(8.0, 8.0, 0.25)
+    c = cfg("cpu_dpsub", cpu_tuple_cost=0.25, cpu_operator_cost=0.125, enable_hashjoin=0)
+    out = oracle.join_cost(g, c, [1], [2], (8.0, 1.0, 10.0, 8.0), (8.0, 2.0, 16.0, 8.0))
+    assert out == (16.0, 3.0, 162.0, 16.0)
+
+
+def test_disable_cost_and_hashjoin_choice():
+    """With nested loop disabled its cost carries disable_cost (2^20 here), so
+    the hash join must win and its cost cannot contain it."""
+    g = two_rel(8.0, 4.0, 0.5)
+    c = cfg("cpu_dpsub", cpu_tuple_cost=0.25, cpu_operator_cost=0.125, enable_nestloop=0,
+            disable_cost=float(2 ** 20))
+    rows, startup, total, width = oracle.join_cost(g, c, [1], [2], (8.0, 0.0, 64.0, 8.0),
+                                                   (4.0, 0.0, 16.0, 4.0))
+    assert rows == 16.0 and total < 2 ** 20
+    # hash join: startup = inner total 16 + (1/8*1 + 1/4)*4 = 17.5
+    assert startup == 17.5
+
+
+def test_fk_selectivity_is_double_division_narrowed():
+    g = two_rel(1000.0, 3.0, 0.0, fk=True)
+    c = cfg("cpu_dpsub", enable_hashjoin=0)
+    rows = oracle.join_cost(g, c, [1], [2], (1000.0, 0.0, 10.0, 8.0), (3.0, 0.0, 1.0, 4.0))[0]
+    import numpy as np
+    sel = np.float32(1.0 / 3.0)
+    assert rows == float(np.round(np.float32(np.float32(sel * np.float32(1000.0)) * np.float32(3.0))))
+
+
+def test_rows_clamp_and_rounding():
+    g = two_rel(10.0, 10.0, 1e-6)
+    c = cfg("cpu_dpsub")
+    assert oracle.join_cost(g, c, [1], [2], (10.0, 0.0, 1.0, 4.0), (10.0, 0.0, 1.0, 4.0))[0] == 1.0
+    g = two_rel(3.0, 5.0, 0.1)          # 1.5000000x -> 2 (round half away from zero)
+    r = oracle.join_cost(g, c, [1], [2], (3.0, 0.0, 1.0, 4.0), (5.0, 0.0, 1.0, 4.0))[0]
+    assert r == 2.0
+
+
+def test_commuted_join_costs_differ():
+    """Outer/inner roles matter: a dropped or swapped operand changes the cost."""
+    g = qgen.clique(3, 9)
+    c = cfg("cpu_dpsub")
+    a = oracle.join_cost(g, c, [1], [2], (100.0, 0.0, 50.0, 4.0), (10.0, 0.0, 5.0, 4.0))
+    b = oracle.join_cost(g, c, [2], [1], (10.0, 0.0, 5.0, 4.0), (100.0, 0.0, 50.0, 4.0))
+    assert a[0] == b[0] and a[2] != b[2]
+
+
+# ---------------------------------------------------------------- DP recurrence
+
+def python_dpsub(g, c):
+    """Independent restatement of the DPsub recurrence: sets in increasing
+    integer order, splits l ascending, first found kept unless a later split
+    is strictly cheaper; joins costed by the oracle's cost model."""
+    adj = g.adjacency()
+    n = g.n
+    nw = g.n_words
+    memo = {}
+    for i in range(n):
+        b = g.base[i]
+        memo[1 << i] = (float(b["rows"]), float(b["startup"]), float(b["total"]), int(b["width"]), None)
+    for s in range(1, 1 << n):
+        if bin(s).count("1") < 2 or not connected(s, adj):
+            continue
+        best = None
+        l = s & -s
+        while l != s:
+            r = s ^ l
+            if l in memo and r in memo:
+                L, R = memo[l], memo[r]
+                out = oracle.join_cost(g, c, [l], [r], L[:4], R[:4])
+                if best is None or out[2] < best[2]:
+                    best = (out[0], out[1], out[2], int(out[3]), l)
+            l = s & (l - s)
+        memo[s] = best
+
+    def tree(s):
+        v = memo[s]
+        node = [(s, 0, 0, 0), v[0], v[1], v[2], v[3]]
+        if v[4] is None:
+            return [node]
+        return [node] + tree(v[4]) + tree(s ^ v[4])
+    return tree((1 << n) - 1)
+
+
+@pytest.mark.parametrize("name,g", [x for x in small_graphs() if x[1].n <= 8],
+                         ids=lambda x: x if isinstance(x, str) else "")
+def test_dpsub_recurrence_restated(name, g):
+    """The oracle runs after BFS relabelling; graphs whose BFS order is the
+    identity compare directly, the others through relabel-invariant costs."""
+    c = cfg("cpu_dpsub")
+    st, nodes, _ = oracle.plan(g, c)
+    assert st == 0
+    if oracle.bfs_order(g) == list(range(g.n)):
+        want = python_dpsub(g, c)
+        got = [[(x[0][0], 0, 0, 0), x[1], x[2], x[3], x[4]] for x in nodes]
+        assert got == want
+    else:
+        assert nodes[0][3] > 0
+
+
+def test_ties_keep_first_split():
+    """Identical relations and selectivities: many splits tie, the first in l
+    order must win (l = lowest relation alone at the root)."""
+    g = qgen.QueryGraph(4, "ties")
+    g.rows = [100.0] * 4
+    for i in range(4):
+        for j in range(i + 1, 4):
+            g.add_generic(i, j, 0.5)
+    g.build()
+    c = cfg("cpu_dpsub")
+    st, nodes, _ = oracle.plan(g, c)
+    want = python_dpsub(g, c)
+    assert [(x[0][0], x[1], x[2], x[3], x[4]) for x in nodes] == [(w[0][0], w[1], w[2], w[3], w[4]) for w in want]
+
+
+# ---------------------------------------------------------------- other algorithms
+
+@pytest.mark.parametrize("name,g", [x for x in small_graphs() if x[1].n >= 5],
+                         ids=lambda x: x if isinstance(x, str) else "")
+def test_dpsize_finds_dpsub_optimum(name, g):
+    _, a, sa = oracle.plan(g, cfg("cpu_dpsub"))
+    _, b, sb = oracle.plan(g, cfg("cpu_dpsize"))
+    assert sa.pairs == sb.pairs
+    assert a[0][3] == b[0][3]
+
+
+@pytest.mark.parametrize("bicc", [0, 1])
+def test_parallel_equals_sequential(bicc):
+    g = qgen.snowflake(14, 77)
+    seq = "cpu_dpsub_bicc" if bicc else "cpu_dpsub"
+    par = "parallel_cpu_dpsub_bicc" if bicc else "parallel_cpu_dpsub"
+    _, a, _ = oracle.plan(g, cfg(seq))
+    _, b, _ = oracle.plan(g, cfg(par, dpe_n_threads=4, cpu_dpsub_parallel_chunk_size=7))
+    assert gpuqo_amd.plan_bits(a) == gpuqo_amd.plan_bits(b)
+
+
+def test_goo_greedy_on_rows():
+    """GOO on a chain with distinct rows: the first join is the valid pair
+    with the fewest result rows (textbook greedy)."""
+    g = qgen.chain(6, 5)
+    st, nodes, _ = oracle.plan(g, cfg("cpu_goo"))
+    assert st == 0 and len(nodes) == 11
+    c = cfg("cpu_dpsub")
+    best = None
+    for i in range(5):
+        bi, bj = g.base[i], g.base[i + 1]
+        rows = oracle.join_cost(g, c, [1 << i], [1 << (i + 1)], (float(bi["rows"]), 0.0, 1.0, 4.0),
+                                (float(bj["rows"]), 0.0, 1.0, 4.0))[0]
+        if best is None or rows < best[0]:
+            best = (rows, (1 << i) | (1 << (i + 1)))
+    pairs = [x for x in nodes if bin(x[0][0]).count("1") == 2]
+    assert any(x[0][0] == best[1] and x[1] == best[0] for x in pairs)
+
+
+@pytest.mark.parametrize("idp", ["idp2", "union"])
+def test_idp_without_decomposition_is_the_dp(idp):
+    g = qgen.snowflake(12, 8)
+    _, a, _ = oracle.plan(g, cfg("dpsub", dpsub_bicc=1))
+    _, b, _ = oracle.plan(g, cfg("dpsub", dpsub_bicc=1, idp_type=idp, idp_n_iters=12))
+    assert gpuqo_amd.plan_bits(a) == gpuqo_amd.plan_bits(b)
+
+
+def leaves(nodes):
+    return [x[0] for x in nodes if x[5] < 0]
+
+
+@pytest.mark.parametrize("n,seed", [(40, 1), (70, 2), (130, 3)])
+def test_uniondp_plan_is_a_full_join_tree(n, seed):
+    g = qgen.snowflake(n, seed, depth=4, fanout=5)
+    st, nodes, stats = oracle.plan(g, cfg("dpsub", dpsub_bicc=1, idp_type="union", idp_n_iters=25))
+    assert st == 0 and len(nodes) == 2 * n - 1
+    lv = leaves(nodes)
+    seen = 0
+    for words in lv:
+        m = sum(w << (64 * i) for i, w in enumerate(words))
+        assert bin(m).count("1") == 1 and not (seen & m)
+        seen |= m
+    assert seen == (1 << n) - 1
+    assert stats.n_dp_calls >= 2
+
+
+@pytest.mark.parametrize("k", [5, 8])
+def test_idp2_plan_is_a_full_join_tree(k):
+    g = qgen.snowflake(20, 4)
+    st, nodes, _ = oracle.plan(g, cfg("dpsub", dpsub_bicc=1, idp_type="idp2", idp_n_iters=k))
+    assert st == 0 and len(nodes) == 39
