@@ -31,6 +31,8 @@ struct DPInput {
     std::vector<uint32_t> blob;
     /* base relation values (memo level 1) */
     std::vector<Rel> base;
+    /* DPsize: order of the level-1 list (relation indices) */
+    std::vector<int> level1;
 };
 
 struct DPNode {                 /* plan node in DP numbering, pre-order */

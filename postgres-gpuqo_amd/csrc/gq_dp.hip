@@ -546,6 +546,267 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
     add_pairs(counters, npairs);
 }
 
+/* ================================================================ DPsize
+ *
+ * gpuqo_algorithm = dpsize against the oracle cpu_dpsize
+ * (gpuqo_cpu_dpsize.cu:27-93).  Level lists keep the CPU's creation order:
+ * level 1 in the order the host supplies (libstdc++ unordered_map iteration
+ * over the base relations, :43-47), level k in the order the sets are first
+ * created by the pair scan
+ *     for t = 0..k-2 (left size k-1-t):  for a in list[k-1-t]:  for b in list[t+1]
+ * (:50-73).  The scan runs tile-parallel; each valid pair carries its global
+ * rank (t, a, b).  Per set, atomics keep the first valid rank (with a NaN
+ * flag) and the first cheapest pair; the first valid ranks then order the
+ * new list: sets are bucketed by the row (t, a) of their first pair, rows
+ * are laid out by an exclusive scan, and each row is sorted by b. */
+
+#define DS_TA 64
+#define DS_TB 64
+#define DS_RANK_BITS 27
+
+struct DsBufs {
+    unsigned long long *best, *first;   /* [2^n]                                   */
+    uint32_t *lset, *lnbr;              /* level lists, all levels concatenated    */
+    uint32_t *raw, *tmp;                /* [maxC] new sets of the level            */
+    uint32_t *rowcnt, *rowfill, *rowoff;/* [2^n (+1)]                              */
+    uint32_t *sz;                       /* [0..32] size, [40..72] offset, [80..112] rowbase,
+                                           [120..152] tile prefix, [160] rows, [161] tiles */
+};
+
+__global__ void k_ds_init(BlobHdr h, const uint32_t *blob, const int *level1, DsBufs d, uint64_t nsets,
+                          float4 *vals, uint32_t *left, uint32_t *counters) {
+    uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = tid; i < nsets; i += (uint64_t)gridDim.x * blockDim.x) {
+        d.best[i] = ~0ull;
+        d.first[i] = ~0ull;
+    }
+    if (tid < 64) counters[tid] = 0;
+    if (tid < 256) d.sz[tid] = 0;
+    if (tid < (uint64_t)h.n_rels) {
+        const float *b = (const float *)(blob + h.o_base) + 4 * tid;
+        uint32_t s1 = 1u << tid;
+        vals[s1] = make_float4(b[0], b[1], b[2], b[3]);
+        left[s1] = 0;
+        int r = level1[tid];
+        d.lset[tid] = 1u << r;
+        d.lnbr[tid] = blob[h.o_adj + r];
+    }
+}
+
+/* Sizes, offsets, row bases and tile counts of level k (one thread). */
+__global__ void k_ds_prep(int k, int n, DsBufs d) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t *sz = d.sz;
+    if (k == 2) { sz[1] = (uint32_t)n; sz[40 + 1] = 0; }
+    sz[40 + k] = sz[40 + k - 1] + sz[k - 1];
+    uint32_t rows = 0, tiles = 0;
+    for (int t = 0; t <= k - 2; t++) {
+        int ls = k - 1 - t, rs = t + 1;
+        sz[80 + t] = rows;
+        rows += sz[ls];
+        sz[120 + t] = tiles;
+        tiles += ((sz[ls] + DS_TA - 1) / DS_TA) * ((sz[rs] + DS_TB - 1) / DS_TB);
+    }
+    sz[120 + k - 1] = tiles;
+    sz[160] = rows;
+    sz[161] = tiles;
+}
+
+__global__ void k_ds_zero_rows(DsBufs d) {
+    uint32_t rows = d.sz[160];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= rows; i += gridDim.x * blockDim.x) {
+        d.rowcnt[i] = 0;
+        d.rowfill[i] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_ds_pairs(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
+                                                     const float4 *vals, uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int W4 = (h.words + 3) & ~3;
+    uint32_t *tl = smem + W4;                 /* left sets  [DS_TA] */
+    uint32_t *tn = tl + DS_TA;                /* left nbrs  [DS_TA] */
+    uint32_t *tr = tn + DS_TA;                /* right sets [DS_TB] */
+    uint32_t *qq = tr + DS_TB;                /* queue [DS_TA*DS_TB]: a | b << 16 */
+    uint32_t *qn = qq + DS_TA * DS_TB;
+    for (int i = tid; i < h.words; i += BLOCK) smem[i] = blob[i];
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t *sz = d.sz;
+    const uint32_t tiles = sz[161];
+    uint32_t checked = 0, npairs = 0;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        int t = 0;
+        while ((uint32_t)sz[120 + t + 1] <= tile) t++;
+        const int ls = k - 1 - t, rs = t + 1;
+        const uint32_t nl = sz[ls], nr = sz[rs], ol = sz[40 + ls], orr = sz[40 + rs];
+        const uint32_t ntb = (nr + DS_TB - 1) / DS_TB;
+        const uint32_t local = tile - sz[120 + t];
+        const uint32_t a0 = (local / ntb) * DS_TA, b0 = (local % ntb) * DS_TB;
+        const uint32_t na = nl - a0 < DS_TA ? nl - a0 : DS_TA, nb = nr - b0 < DS_TB ? nr - b0 : DS_TB;
+        if (tid < DS_TA) {
+            tl[tid] = tid < (int)na ? d.lset[ol + a0 + tid] : 0u;
+            tn[tid] = tid < (int)na ? d.lnbr[ol + a0 + tid] : 0u;
+        } else if (tid < DS_TA + DS_TB) {
+            int j = tid - DS_TA;
+            tr[j] = j < (int)nb ? d.lset[orr + b0 + j] : 0u;
+        }
+        if (tid == 0) *qn = 0;
+        __syncthreads();
+        for (int p0 = 0; p0 < DS_TA * DS_TB; p0 += BLOCK) {
+            int p = p0 + tid, a = p / DS_TB, b = p % DS_TB;
+            bool ok = false;
+            if (a < (int)na && b < (int)nb) {
+                checked++;
+                uint32_t l = tl[a], r = tr[b];
+                ok = (l & r) == 0u && (tn[a] & r) != 0u;
+            }
+            unsigned long long m = __ballot(ok);
+            if (m) {
+                int leader = __builtin_ctzll(m);
+                uint32_t pos = 0;
+                if (lane == leader) pos = atomicAdd(qn, (uint32_t)__popcll(m));
+                pos = __shfl(pos, leader);
+                if (ok) qq[pos + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)a | ((uint32_t)b << 16);
+            }
+        }
+        __syncthreads();
+        const uint32_t nq = *qn;
+        for (uint32_t e = tid; e < nq; e += BLOCK) {
+            uint32_t a = qq[e] & 0xffffu, b = qq[e] >> 16;
+            uint32_t l = tl[a], r = tr[b], S = l | r;
+            uint32_t ag = a0 + a, bg = b0 + b;
+            Rel jr = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
+            npairs++;
+            bool nan = jr.total != jr.total;
+            unsigned long long g = ((unsigned long long)t << (2 * DS_RANK_BITS)) |
+                                   ((unsigned long long)ag << DS_RANK_BITS) | bg;
+            unsigned long long f = (g << 1) | (nan ? 1ull : 0ull);
+            if (f < d.first[S]) {
+                unsigned long long old = atomicMin(&d.first[S], f);
+                if (old == ~0ull) {
+                    uint32_t pos = atomicAdd(&counters[k], 1u);
+                    d.raw[pos] = S;
+                }
+            }
+            if (!nan) {
+                unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) |
+                                         ((uint32_t)t << DS_RANK_BITS) | ag;
+                if (key < d.best[S]) atomicMin(&d.best[S], key);
+            }
+        }
+        __syncthreads();
+    }
+    add_pairs(counters, npairs);
+    if (checked) atomicAdd((unsigned long long *)(counters + 44), (unsigned long long)checked);
+}
+
+__device__ __forceinline__ uint32_t ds_row(const DsBufs &d, unsigned long long f) {
+    unsigned long long g = f >> 1;
+    uint32_t t = (uint32_t)(g >> (2 * DS_RANK_BITS));
+    uint32_t a = (uint32_t)(g >> DS_RANK_BITS) & ((1u << DS_RANK_BITS) - 1u);
+    return d.sz[80 + t] + a;
+}
+
+__global__ void k_ds_rowcount(int k, DsBufs d, const uint32_t *counters) {
+    uint32_t cnt = counters[k];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x)
+        atomicAdd(&d.rowcnt[ds_row(d, d.first[d.raw[i]])], 1u);
+}
+
+/* exclusive scan of rowcnt[0..rows) into rowoff, one workgroup of 1024 */
+__global__ void __launch_bounds__(1024) k_ds_scan(DsBufs d) {
+    __shared__ uint32_t part[1024];
+    const uint32_t rows = d.sz[160];
+    const uint32_t per = (rows + 1023) / 1024;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b0 + per && i < rows; i++) sum += d.rowcnt[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t acc = part[threadIdx.x] - sum;
+    for (uint32_t i = b0; i < b0 + per && i < rows; i++) {
+        d.rowoff[i] = acc;
+        acc += d.rowcnt[i];
+    }
+}
+
+__global__ void k_ds_scatter(int k, DsBufs d, const uint32_t *counters) {
+    uint32_t cnt = counters[k];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        uint32_t S = d.raw[i];
+        uint32_t row = ds_row(d, d.first[S]);
+        d.tmp[d.rowoff[row] + atomicAdd(&d.rowfill[row], 1u)] = S;
+    }
+}
+
+/* rows with several new sets: order them by the b of their first pair */
+__global__ void k_ds_rowsort(DsBufs d) {
+    const uint32_t rows = d.sz[160];
+    const unsigned long long bm = (1ull << DS_RANK_BITS) - 1ull;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+        uint32_t c = d.rowcnt[r];
+        if (c < 2) continue;
+        uint32_t *seg = d.tmp + d.rowoff[r];
+        for (uint32_t i = 1; i < c; i++) {
+            uint32_t x = seg[i];
+            unsigned long long bx = (d.first[x] >> 1) & bm;
+            uint32_t j = i;
+            while (j > 0 && ((d.first[seg[j - 1]] >> 1) & bm) > bx) {
+                seg[j] = seg[j - 1];
+                j--;
+            }
+            seg[j] = x;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_ds_finalize(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
+                                                        float4 *vals, uint32_t *left, uint32_t *counters) {
+    extern __shared__ uint32_t smem[];
+    for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t *adj = smem + h.o_adj;
+    const uint32_t cnt = counters[k];
+    const uint32_t base = d.sz[40 + k];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < cnt; p += gridDim.x * blockDim.x) {
+        uint32_t S = d.tmp[p];
+        uint32_t nbr = 0, t = S;
+        while (t) {
+            int i = __builtin_ctz(t);
+            t &= t - 1;
+            nbr |= adj[i];
+        }
+        d.lset[base + p] = S;
+        d.lnbr[base + p] = nbr;
+        unsigned long long f = d.first[S];
+        uint32_t tt, a;
+        if (f & 1ull) {
+            unsigned long long g = f >> 1;
+            tt = (uint32_t)(g >> (2 * DS_RANK_BITS));
+            a = (uint32_t)(g >> DS_RANK_BITS) & ((1u << DS_RANK_BITS) - 1u);
+        } else {
+            uint32_t r32 = (uint32_t)(d.best[S] & 0xffffffffull);
+            tt = r32 >> DS_RANK_BITS;
+            a = r32 & ((1u << DS_RANK_BITS) - 1u);
+        }
+        int ls = k - 1 - (int)tt;
+        uint32_t l = d.lset[d.sz[40 + ls] + a], r = S ^ l;
+        Rel jr = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
+        store_rel(vals, S, jr);
+        left[S] = l;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.sz[k] = cnt;
+}
+
 /* Pre-order walk of the plan from `root_set` (dpsub_buildQueryTree,
  * gpuqo_query_tree.cuh:46-72, done on the device in one pass). */
 __global__ void k_tree(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes, int cap,
@@ -600,6 +861,10 @@ struct DeviceCtx {
            cap_nodes = 0, cap_counters = 0;
     bool binom_ready = false;
     std::vector<hipEvent_t> prof_ev;
+    DsBufs ds{};
+    int *l1 = nullptr;
+    size_t cap_ds_best = 0, cap_ds_first = 0, cap_ds_lset = 0, cap_ds_lnbr = 0, cap_ds_raw = 0, cap_ds_tmp = 0,
+           cap_ds_rc = 0, cap_ds_rf = 0, cap_ds_ro = 0, cap_ds_sz = 0, cap_l1 = 0;
     std::mutex mu;
 };
 
@@ -664,10 +929,6 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         set_error("GPU DP supports 1..32 relations, got %d", n);
         return GQ_ERR_UNSUPPORTED;
     }
-    if (algo == DP_DPSIZE) {
-        set_error("DPsize GPU path not available");
-        return GQ_ERR_UNSUPPORTED;
-    }
     if (n_iters != n) {
         set_error("IDP1 level cap not available on the GPU path");
         return GQ_ERR_UNSUPPORTED;
@@ -696,6 +957,14 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     if ((rc = ensure(&c.b.blob, c.cap_blob, (size_t)h.words))) return rc;
     if ((rc = ensure(&c.b.nodes, c.cap_nodes, (size_t)2 * n))) return rc;
 
+    /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
+    if (profile && (int)c.prof_ev.size() < 3 * 33) {
+        while ((int)c.prof_ev.size() < 3 * 33) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.prof_ev.push_back(e);
+        }
+    }
     hipStream_t s = c.stream;
     HIPCHK(hipMemcpyAsync(c.b.blob, in.blob.data(), blob_bytes, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(c.ev0, s));
@@ -705,15 +974,99 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, c.b.conn, conn_words, c.b.counters);
         hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, c.b.blob, c.b.vals, c.b.left, c.b.conn);
     }
-    uint64_t cand_total = 0;
-    /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
-    if (profile && (int)c.prof_ev.size() < 3 * 33) {
-        while ((int)c.prof_ev.size() < 3 * 33) {
-            hipEvent_t e;
-            HIPCHK(hipEventCreate(&e));
-            c.prof_ev.push_back(e);
+    if (algo == DP_DPSIZE) {
+        if (n > 28) {
+            set_error("GPU DPsize supports at most 28 relations (got %d)", n);
+            return GQ_ERR_UNSUPPORTED;
         }
+        if ((int)in.level1.size() != n) {
+            set_error("DPsize level-1 order missing");
+            return GQ_ERR_INVALID;
+        }
+        DsBufs &d = c.ds;
+        if ((rc = ensure(&d.best, c.cap_ds_best, nsets_all))) return rc;
+        if ((rc = ensure(&d.first, c.cap_ds_first, nsets_all))) return rc;
+        if ((rc = ensure(&d.lset, c.cap_ds_lset, nsets_all))) return rc;
+        if ((rc = ensure(&d.lnbr, c.cap_ds_lnbr, nsets_all))) return rc;
+        if ((rc = ensure(&d.raw, c.cap_ds_raw, maxC))) return rc;
+        if ((rc = ensure(&d.tmp, c.cap_ds_tmp, maxC))) return rc;
+        if ((rc = ensure(&d.rowcnt, c.cap_ds_rc, nsets_all + 1))) return rc;
+        if ((rc = ensure(&d.rowfill, c.cap_ds_rf, nsets_all + 1))) return rc;
+        if ((rc = ensure(&d.rowoff, c.cap_ds_ro, nsets_all + 1))) return rc;
+        if ((rc = ensure(&d.sz, c.cap_ds_sz, 256))) return rc;
+        if ((rc = ensure(&c.l1, c.cap_l1, 32))) return rc;
+        HIPCHK(hipMemcpyAsync(c.l1, in.level1.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        const size_t lds_pairs = lds_eval + (DS_TA * 2 + DS_TB + DS_TA * DS_TB + 4) * 4;
+        {
+            uint32_t blocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK - 1) / BLOCK, 4096);
+            hipLaunchKernelGGL(k_ds_init, dim3(blocks), dim3(BLOCK), 0, s, h, c.b.blob, c.l1, d, nsets_all, c.b.vals,
+                               c.b.left, c.b.counters);
+        }
+        uint64_t cand_ub = 0;
+        for (int k = 2; k <= n; k++) {
+            if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
+            hipLaunchKernelGGL(k_ds_prep, dim3(1), dim3(64), 0, s, k, n, d);
+            uint32_t rblocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK) / BLOCK, 4096);
+            hipLaunchKernelGGL(k_ds_zero_rows, dim3(rblocks), dim3(BLOCK), 0, s, d);
+            /* grid: upper bound on the tiles of this level from binomials */
+            uint64_t tiles_ub = 0;
+            for (int t = 0; t <= k - 2; t++) {
+                uint64_t nl = binom64(n, k - 1 - t), nr = binom64(n, t + 1);
+                tiles_ub += ((nl + DS_TA - 1) / DS_TA) * ((nr + DS_TB - 1) / DS_TB);
+                cand_ub += nl * nr;
+            }
+            uint32_t pblocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tiles_ub, 1), 8192);
+            if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
+            hipLaunchKernelGGL(k_ds_pairs, dim3(pblocks), dim3(BLOCK), lds_pairs, s, h, c.b.blob, k, d, c.b.vals,
+                               c.b.counters);
+            uint64_t Ck = binom64(n, k);
+            uint32_t sblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 4096);
+            hipLaunchKernelGGL(k_ds_rowcount, dim3(sblocks), dim3(BLOCK), 0, s, k, d, c.b.counters);
+            hipLaunchKernelGGL(k_ds_scan, dim3(1), dim3(1024), 0, s, d);
+            hipLaunchKernelGGL(k_ds_scatter, dim3(sblocks), dim3(BLOCK), 0, s, k, d, c.b.counters);
+            hipLaunchKernelGGL(k_ds_rowsort, dim3(rblocks), dim3(BLOCK), 0, s, d);
+            hipLaunchKernelGGL(k_ds_finalize, dim3(sblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, d, c.b.vals,
+                               c.b.left, c.b.counters);
+            if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+        }
+        (void)cand_ub;
+        uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, c.b.vals, c.b.left, full, c.b.nodes, 2 * n - 1,
+                           c.b.counters + 48);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c.ev1, s));
+        tree.resize(2 * n - 1);
+        uint32_t cnt[64];
+        HIPCHK(hipMemcpyAsync(tree.data(), c.b.nodes, sizeof(DPNode) * (2 * n - 1), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(cnt, c.b.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (cnt[48] != (uint32_t)(2 * n - 1)) {
+            set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
+            return GQ_ERR_DEVICE;
+        }
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+        st.gpu_ms = ms;
+        if (profile) {
+            for (int k = 2; k <= n; k++) {
+                float a = 0.0f, b = 0.0f;
+                HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
+                HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
+                st.filter_ms += a;
+                st.eval_ms += b;
+                st.filter_launches++;
+                st.eval_launches++;
+            }
+        }
+        st.candidates = (uint64_t)cnt[44] | ((uint64_t)cnt[45] << 32);
+        st.sets = 0;
+        for (int k = 2; k <= n; k++) st.sets += cnt[k];
+        st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+        st.memo_bytes = nsets_all * (sizeof(float4) + 4 + 16 + 8 + 12);
+        st.memo_kind = 1;
+        return GQ_OK;
     }
+    uint64_t cand_total = 0;
     for (int k = 2; k <= n; k++) {
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         uint64_t Ck = binom64(n, k);

@@ -229,6 +229,14 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     }
     while (b.size() & 3) b.push_back(0u);
     h.words = (int)b.size();
+    /* Level-1 list of cpu_dpsize: iteration order of a libstdc++
+     * unordered_map holding the base relations, inserted in index order, keyed
+     * by the reference's relation ids (relation i -> bit i+1), whose hash is
+     * the identity (gpuqo_cpu_dpsize.cu:43-47, gpuqo_cpu_sequential.cu:73-85). */
+    std::unordered_map<uint64_t, int> m;
+    for (int i = 0; i < n; i++) m.insert(std::make_pair(1ull << (i + 1), i));
+    in.level1.clear();
+    for (auto &kv : m) in.level1.push_back(kv.second);
     return GQ_OK;
 }
 

@@ -51,3 +51,64 @@ def test_mixed_batch_members(seed):
         want, _ = oracle_bits(g, c)
         got, _ = gpu_bits(g, c)
         assert_same_plan(got, want, f"{g.shape}{g.n}")
+
+
+@pytest.mark.parametrize("name,g", small_graphs(), ids=lambda x: x if isinstance(x, str) else "")
+def test_dpsize_small_graphs(name, g):
+    c = cfg("dpsize")
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"{name} dpsize")
+    assert gst.pairs == ost.pairs
+
+
+def test_dpsize_star12():
+    g = qgen.config1()
+    c = cfg("dpsize")
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, "star-12 dpsize")
+
+
+def test_dpsize_clique12():
+    g = qgen.clique(12, 0x55)
+    c = cfg("dpsize")
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, "clique-12 dpsize")
+    assert gst.pairs == ost.pairs == 3 ** 12 - 2 ** 13 + 1
+
+
+def test_dpsize_ties_follow_creation_order():
+    """All relations and selectivities identical: ties everywhere, so the
+    plan is fixed by cpu_dpsize's list order and pair order alone."""
+    g = qgen.QueryGraph(7, "ties")
+    g.rows = [1000.0] * 7
+    for i in range(7):
+        for j in range(i + 1, 7):
+            g.add_generic(i, j, 0.5)
+    g.build()
+    for algo in ("dpsize", "dpsub"):
+        c = cfg(algo)
+        want, _ = oracle_bits(g, c)
+        got, _ = gpu_bits(g, c)
+        assert_same_plan(got, want, f"ties {algo}")
+
+
+@pytest.mark.parametrize("n,seed", [(40, 1), (70, 2), (130, 3)])
+def test_uniondp_mpdp_large_snowflakes(n, seed):
+    g = qgen.snowflake(n, seed, depth=4, fanout=5)
+    c = cfg("dpsub", 1, idp_type="union", idp_n_iters=25)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"uniondp snowflake-{n}")
+    assert gst.n_dp_calls == ost.n_dp_calls
+
+
+@pytest.mark.parametrize("k", [5, 8, 12])
+def test_idp2_snowflake20(k):
+    g = qgen.snowflake(20, 4)
+    c = cfg("dpsub", 1, idp_type="idp2", idp_n_iters=k)
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, f"idp2 k={k}")
