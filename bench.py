@@ -51,6 +51,44 @@ def gather_roofline():
     return out
 
 
+def batched_section(gpuqo_amd, abi, qgen, reps=3):
+    """Config 4: 256 seeded mixed queries (n in 8..22; 40% snowflake, 30%
+    star, 15% chain, 15% cycle), MPDP, planned as one batch (one launch per
+    DP level bucket) vs one query at a time on the same GPU."""
+    qs = qgen.config4()
+    cfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+    gpuqo_amd.plan_batch(qs, cfg)                       # warm-up
+    bt_wall, bt_dev = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = gpuqo_amd.plan_batch(qs, cfg)
+        bt_wall.append(time.perf_counter() - t0)
+        bt_dev.append(res[0][1].gpu_ms / 1e3)
+    for g in qs[:8]:
+        gpuqo_amd.plan(g, cfg)                          # warm-up
+    lat, dev = [], []
+    for g in qs:
+        t0 = time.perf_counter()
+        _, st = gpuqo_amd.plan(g, cfg)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        dev.append(st.gpu_ms)
+    same = all(gpuqo_amd.plan_bits(a[0]) == gpuqo_amd.plan_bits(gpuqo_amd.plan(g, cfg)[0])
+               for a, g in zip(res[:32], qs[:32]))
+    lat_sorted = sorted(lat)
+    bq = len(qs) / statistics.median(bt_wall)
+    sq = len(qs) / (sum(lat) / 1e3)
+    return {"workload": "batch256-mpdp", "queries": len(qs),
+            "batched_queries_per_s": bq,
+            "batched_queries_per_s_device": len(qs) / statistics.median(bt_dev),
+            "batch_ms_wall": statistics.median(bt_wall) * 1e3,
+            "single_queries_per_s": sq,
+            "single_queries_per_s_device": len(qs) / (sum(dev) / 1e3),
+            "speedup_batched_vs_single": bq / sq,
+            "single_p50_ms": lat_sorted[len(lat) // 2],
+            "single_p99_ms": lat_sorted[min(len(lat) - 1, int(0.99 * len(lat)))],
+            "batch_equals_single_first32": same}
+
+
 def cpu_baseline(g, algo, bicc, budget_s=12.0):
     """The oracle's CPU restatement timed on this host (bounded sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -97,6 +135,7 @@ def main():
     ap.add_argument("--workload", default="mpdp-snowflake20", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
@@ -135,10 +174,13 @@ def main():
     # candidate write + read per unranked set.
     eval_bytes = 80.0 * P + 40.0 * S
     filt_bytes = 16.0 * Cn
+    eval_kernel = {"mpdp-snowflake20": "k_eval_tree", "star12-dpsub": "k_eval_tree",
+                   "clique18-dpsub": "k_eval_wg<false> + k_eval_dpsub_chunk",
+                   "clique18-dpsize": "k_ds_pairs"}[args.workload]
     if e_ms >= f_ms:
-        kern, kbytes, kms, kn = ("k_eval_bicc_lane" if bicc else "k_eval_dpsub_*"), eval_bytes, e_ms, e_n
+        kern, kbytes, kms, kn = eval_kernel, eval_bytes, e_ms, e_n
     else:
-        kern, kbytes, kms, kn = "k_filter", filt_bytes, f_ms, f_n
+        kern, kbytes, kms, kn = "k_filter_all", filt_bytes, f_ms, f_n
     per_launch_bytes = kbytes * args.steps / kn
     avg_launch_ms = kms / kn
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
@@ -178,6 +220,8 @@ def main():
         "plan_ms_wall_p99": sorted(wall_ms)[min(len(wall_ms) - 1, int(0.99 * len(wall_ms)))],
         "roofline": roof,
     }
+    if not args.no_batch and args.workload == "mpdp-snowflake20":
+        out["batched"] = batched_section(gpuqo_amd, abi, qgen)
     if not args.no_cpu:
         cb = cpu_baseline(g, algo, bicc)
         seq = cb["seq"]

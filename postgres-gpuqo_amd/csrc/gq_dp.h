@@ -24,6 +24,11 @@ struct BlobHdr {
     int o_rel_rows, o_rel_fksel, o_adj, o_inc, o_ec_relids, o_ec_moff, o_ec_soff, o_ec_hasfk;
     int o_mem_fks, o_mem_nd, o_mem_avgfreq, o_mem_mcv, o_mem_flags, o_mem_idx_rows, o_mem_idx_startup,
         o_mem_idx_total, o_sels, o_qualtab, o_base;
+    /* Tree graphs whose every vertex v > 0 has exactly one lower-numbered
+     * neighbour (its parent; true after BFS relabelling of a tree): o_desc
+     * holds each vertex's descendants (itself included), o_lglob the vertices
+     * of the subtrees of smaller siblings of v and of its ancestors. */
+    int is_tree, o_desc, o_lglob;
 };
 
 struct DPInput {
@@ -55,6 +60,12 @@ enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
  * gq_status. */
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
            DPStats &stats);
+
+/* Run several DPs in one fused launch (one workgroup per DP).  Every input
+ * must satisfy fused_eligible(). */
+bool fused_eligible(const DPInput &in, DPAlgo algo);
+int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAlgo> &algos,
+                 std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &stats, double *gpu_ms);
 
 }  // namespace gq
 

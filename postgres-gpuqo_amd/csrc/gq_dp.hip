@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -43,6 +44,12 @@ namespace gq {
 
 #define WAVE 64
 #define BLOCK 256
+#define BMAX 128                 /* sets per workgroup batch            */
+#define FILT_CHUNK 4096          /* candidates per filter block pass    */
+#define QCAP 1024                /* DPsub valid-pair queue per round    */
+/* Dynamic LDS holds 64-bit slots updated with LDS atomics, which need
+ * natural alignment; kernels using it declare no static __shared__. */
+#define GQ_LDS_ALIGN __attribute__((aligned(16)))
 
 __constant__ uint32_t c_binom[33 * 33];   /* C(n,k) at [n*33+k], n,k <= 32 */
 
@@ -186,38 +193,109 @@ __global__ void k_base(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *
     }
 }
 
-/* Candidates of level k -> connected sets (lvl_sets), counters[k]. */
-__global__ void __launch_bounds__(BLOCK) k_filter(BlobHdr h, const uint32_t *blob, int k, uint32_t n_cand,
-                                                   uint32_t per, uint32_t *lvl_sets,
-                                                   unsigned long long *lvl_best, uint32_t *lvl_first,
-                                                   uint32_t *conn, uint32_t *counters) {
-    __shared__ uint32_t adj[32];
-    if (threadIdx.x < 32) adj[threadIdx.x] = threadIdx.x < (unsigned)h.n_rels ? blob[h.o_adj + threadIdx.x] : 0u;
+/* All levels at once: every integer s in [0, 2^n) is a candidate set; the
+ * connected ones (singletons included) are marked in the bitmap -- each wave
+ * covers 64 consecutive integers, i.e. two whole bitmap words, so the words
+ * are written without atomics -- and sets of two or more relations are
+ * appended to the list of their level (offset lvl_off[k]) with one atomic
+ * per (wave, level). */
+/* One chunk of FILT_CHUNK candidate sets (integers ch*FILT_CHUNK + i):
+ * connected ones are marked in the bitmap -- a wave owns 64 consecutive
+ * candidates, i.e. two whole bitmap words, written without atomics -- and
+ * those of two or more members are compacted in LDS, then appended to their
+ * level list with one global atomic per (chunk, level).  Trees test
+ * connectivity in O(1): S is connected iff |S & children(S)| = |S| - 1,
+ * children(S) coming from a per-byte table `lut`; other graphs grow the
+ * closure of the lowest member. */
+struct FilterSmem {
+    uint32_t adj[32];
+    uint32_t lut[4 * 256];
+    uint32_t kept[FILT_CHUNK];
+    uint32_t lcnt[33], lbase[33], nkept;
+};
+
+__device__ void filter_setup(FilterSmem &F, const BlobHdr &h, const uint32_t *blob) {
+    const int tid = threadIdx.x, n = h.n_rels;
+    if (tid < 32) F.adj[tid] = tid < n ? blob[h.o_adj + tid] : 0u;
     __syncthreads();
-    const int lane = threadIdx.x & (WAVE - 1);
-    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t r0 = (uint64_t)gtid * per;
-    uint32_t s = r0 < n_cand ? unrank((uint32_t)r0, k, h.n_rels) : 0u;
-    for (uint32_t i = 0; i < per; i++) {
-        bool valid = (r0 + i) < n_cand;
+    if (h.is_tree)
+        for (int e = tid; e < 4 * 256; e += BLOCK) {
+            int byte = e >> 8, x = e & 255;
+            uint32_t m = 0;
+            for (int i = 0; i < 8; i++) {
+                int v = byte * 8 + i;
+                if (((x >> i) & 1) && v < n) m |= F.adj[v] & ~((2u << v) - 1u);
+            }
+            F.lut[e] = m;
+        }
+    __syncthreads();
+}
+
+__device__ void filter_chunk(FilterSmem &F, const BlobHdr &h, uint64_t ch, const uint32_t *lvl_off,
+                             uint32_t *lvl_sets, unsigned long long *lvl_best, uint32_t *lvl_first, uint32_t *conn,
+                             uint32_t conn_words, uint32_t *counters) {
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const bool tree = h.is_tree != 0;
+    const uint64_t total = 1ull << h.n_rels;
+    if (tid < 33) F.lcnt[tid] = 0;
+    if (tid == 0) F.nkept = 0;
+    __syncthreads();
+    for (uint32_t o = (uint32_t)(tid / WAVE) * WAVE; o < FILT_CHUNK; o += BLOCK) {
+        const uint64_t s64 = ch * FILT_CHUNK + o + lane;
+        const uint32_t s = (uint32_t)s64;
         bool keep = false;
-        if (valid) keep = grow(s & (0u - s), s, adj) == s;
-        unsigned long long m = __ballot(keep);
-        if (m) {
-            int leader = __builtin_ctzll(m);
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&counters[k], (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (keep) {
-                uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                lvl_sets[pos] = s;
-                lvl_best[pos] = ~0ull;
-                lvl_first[pos] = ~0u;
-                atomicOr(&conn[s >> 5], 1u << (s & 31));
+        if (s64 < total && s != 0u) {
+            if (tree) {
+                uint32_t chs = F.lut[s & 255u] | F.lut[256 + ((s >> 8) & 255u)] | F.lut[512 + ((s >> 16) & 255u)] |
+                               F.lut[768 + (s >> 24)];
+                keep = __builtin_popcount(s & chs) == __builtin_popcount(s) - 1;
+            } else {
+                keep = grow(s & (0u - s), s, F.adj) == s;
             }
         }
-        if (valid) s = gosper(s);
+        unsigned long long m = __ballot(keep);
+        const uint64_t wd = (ch * FILT_CHUNK + o) / 32;
+        if (lane == 0 && wd < conn_words) conn[wd] = (uint32_t)m;
+        if (lane == 0 && wd + 1 < conn_words) conn[wd + 1] = (uint32_t)(m >> 32);
+        const bool pend = keep && __builtin_popcount(s) >= 2;
+        unsigned long long pm = __ballot(pend);
+        if (pm) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&F.nkept, (uint32_t)__popcll(pm));
+            base = __shfl(base, 0);
+            if (pend) {
+                F.kept[base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = s;
+                atomicAdd(&F.lcnt[__builtin_popcount(s)], 1u);
+            }
+        }
     }
+    __syncthreads();
+    if (tid < 33) {
+        F.lbase[tid] = F.lcnt[tid] ? atomicAdd(&counters[tid], F.lcnt[tid]) : 0u;
+        F.lcnt[tid] = 0;
+    }
+    __syncthreads();
+    const uint32_t nk = F.nkept;
+    for (uint32_t i = tid; i < nk; i += BLOCK) {
+        const uint32_t s = F.kept[i];
+        const int k = __builtin_popcount(s);
+        const uint32_t pos = lvl_off[k] + F.lbase[k] + atomicAdd(&F.lcnt[k], 1u);
+        lvl_sets[pos] = s;
+        lvl_best[pos] = ~0ull;
+        lvl_first[pos] = ~0u;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(BLOCK) k_filter_all(BlobHdr h, const uint32_t *blob, const uint32_t *lvl_off,
+                                                       uint32_t *lvl_sets, unsigned long long *lvl_best,
+                                                       uint32_t *lvl_first, uint32_t *conn, uint32_t conn_words,
+                                                       uint32_t *counters) {
+    __shared__ FilterSmem F;
+    filter_setup(F, h, blob);
+    const uint64_t nchunks = ((1ull << h.n_rels) + FILT_CHUNK - 1) / FILT_CHUNK;
+    for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x)
+        filter_chunk(F, h, ch, lvl_off, lvl_sets, lvl_best, lvl_first, conn, conn_words, counters);
 }
 
 /* DPsub, split-parallel: a wave owns `ipl * 64` consecutive split ranks of
@@ -229,7 +307,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_dpsub_chunk(BlobHdr h, const uin
                                                              unsigned long long *lvl_best, uint32_t *lvl_first,
                                                              const float4 *vals, const uint32_t *conn,
                                                              uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     stage(blob, smem, h.words);
     View<uint32_t> q = make_view(h, smem);
     const int lane = threadIdx.x & (WAVE - 1);
@@ -284,7 +362,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
                                                            const unsigned long long *lvl_best,
                                                            const uint32_t *lvl_first, float4 *vals,
                                                            uint32_t *left, const uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     stage(blob, smem, h.words);
     View<uint32_t> q = make_view(h, smem);
     uint32_t count = counters[k];
@@ -300,8 +378,6 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
     }
 }
 
-#define BMAX 128                 /* sets per workgroup batch            */
-#define QCAP 1024                /* DPsub valid-pair queue per round    */
 
 /* Blocks of connected set S in the emission order of find_blocks
  * (gpuqo_cpu_dpsub_bicc.cu:65-169): DFS from the lowest vertex, neighbours
@@ -365,6 +441,27 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
     return nblk;
 }
 
+__device__ __forceinline__ uint32_t pext32(uint32_t x, uint32_t mask) {
+    uint32_t res = 0, bitpos = 1;
+    while (mask) {
+        uint32_t b = mask & (0u - mask);
+        if (x & b) res |= bitpos;
+        bitpos <<= 1;
+        mask &= mask - 1;
+    }
+    return res;
+}
+
+/* Tree graphs (BlobHdr::is_tree): the split of connected S at the edge
+ * (parent(c), c).  `sub` = the subtree of c inside S; o = 0 gives
+ * (S - sub, sub), o = 1 gives (sub, S - sub).  In MPDP order that split has
+ * rank 2 * post(c) + o, post(c) being c's position in the post-order of the
+ * DFS from the lowest member of S with ascending neighbours (the block
+ * emission order of find_blocks); in DPsub order its rank is pext(L, S). */
+__device__ __forceinline__ uint32_t tree_post(uint32_t S, int c, const uint32_t *desc, const uint32_t *lglob) {
+    return (uint32_t)(__builtin_popcount(S & desc[c]) - 1 + __builtin_popcount(S & lglob[c]));
+}
+
 /* Split `local` of a set whose blocks are blocks[0..nblk): block-major,
  * lb ascending over the block's non-empty proper subsets. */
 __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, uint32_t local, const uint32_t *adj,
@@ -397,11 +494,101 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
  * gpuqo_cpu_dpsub.cu:32-79); only pairs whose sides are both connected are
  * costed, gathered first into an LDS queue so that every lane of a wave
  * costs a valid pair. */
+/* Tree graphs, wave per set(s): a set of level k has per = 2(k-1) splits;
+ * a segment of P = nextpow2(per) lanes owns one set and lane t evaluates
+ * split t.  The segment reduces the first valid split (rank and NaN flag)
+ * and the first minimum of the packed (cost, rank) key with shuffles; the
+ * winning lane writes its own result to the memo.  Waves w0, w0 + wstride,
+ * ... cover the level. */
+__device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
+                                    bool bicc, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
+                                    uint32_t *left, uint64_t w0, uint64_t wstride) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t per = 2u * (k - 1);
+    uint32_t P = 1;
+    while (P < per) P <<= 1;
+    const uint32_t spw = WAVE / P;
+    const uint32_t seg = lane / P, t = lane % P;
+    const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
+    const uint64_t nwaves = (count + spw - 1) / spw;
+    uint32_t npairs = 0;
+    for (uint64_t w = w0; w < nwaves; w += wstride) {
+        const uint64_t idx = w * spw + seg;
+        const bool active = idx < count && t < per;
+        uint32_t first = ~0u;
+        unsigned long long key = ~0ull;
+        Rel jr;
+        uint32_t S = 0, L = 0;
+        if (active) {
+            S = lvl_sets[idx];
+            const uint32_t cb = pdep32(2u << (t >> 1), S);   /* skip the root (lowest member) */
+            const int c = __builtin_ctz(cb);
+            const uint32_t sub = S & desc[c];
+            const uint32_t o = t & 1u;
+            L = o ? sub : (S ^ sub);
+            const uint32_t R = S ^ L;
+            const uint32_t rank = bicc ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
+            jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+            npairs++;
+            const bool nan = jr.total != jr.total;
+            first = (rank << 1) | (nan ? 1u : 0u);
+            if (!nan) key = ((unsigned long long)ord_key(jr.total) << 32) | rank;
+        }
+        uint32_t sf = first;
+        unsigned long long sk = key;
+        for (uint32_t off = P >> 1; off > 0; off >>= 1) {
+            uint32_t f2 = __shfl_xor(sf, (int)off);
+            unsigned long long k2 = __shfl_xor(sk, (int)off);
+            sf = f2 < sf ? f2 : sf;
+            sk = k2 < sk ? k2 : sk;
+        }
+        if (active) {
+            const bool win = (sf & 1u) ? (first == sf) : (key == sk);
+            if (win) {
+                store_rel(vals, S, jr);
+                left[S] = L;
+            }
+        }
+    }
+    return npairs;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
+                                                      const uint32_t *lvl_sets, float4 *vals, uint32_t *left,
+                                                      uint32_t *counters) {
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+    uint32_t np = eval_level_tree(h, smem, q, k, bicc != 0, lvl_sets, counters[k], vals, left, w0,
+                                  (uint64_t)gridDim.x * (BLOCK / WAVE));
+    add_pairs(counters, np);
+}
+
+/* LDS layout of the level evaluation, after the staged blob (W4 words):
+ *   s_best[BMAX] u64, s_off[BMAX+2] u64, s_first[BMAX], s_set[BMAX], s_misc[4]
+ *   MPDP (general graphs): s_nblk[BMAX], s_blocks[BMAX][32], dfs[32][BMAX]
+ *   DPsub: q_l, q_i, q_r [QCAP] each, then the staged connectivity bitmap. */
+__host__ __device__ constexpr size_t eval_lds_common(int words) {
+    return (size_t)(((words + 3) & ~3) * 4) + BMAX * 8 + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
+}
+__host__ __device__ constexpr size_t eval_lds_bicc(int words) {
+    return eval_lds_common(words) + BMAX * 4 + (size_t)BMAX * 32 * 4 * 2;
+}
+__host__ __device__ constexpr size_t eval_lds_dpsub(int words, int conn_words_lds) {
+    return eval_lds_common(words) + 3 * QCAP * 4 + (size_t)conn_words_lds * 4;
+}
+
+/* Evaluate level k: batches bt0, bt0 + bstride, ... of B sets each.  The
+ * blob is staged at smem[0..words) (and, for DPsub, `cb` points to the
+ * connectivity bitmap, staged or global).  Returns the pairs costed;
+ * raises *err when a set has more than 2^31 splits. */
 template <bool BICC>
-__global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
-                                                    int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
-                                                    uint32_t *left, const uint32_t *conn, uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
+__device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k, uint32_t B,
+                               const uint32_t *cb, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
+                               uint32_t *left, uint32_t bt0, uint32_t bstride) {
+    const bool tree = h.is_tree != 0;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int W4 = (h.words + 3) & ~3;
@@ -416,20 +603,12 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
     uint32_t *q_l = s_misc + 4;                                /* DPsub: [QCAP]               */
     uint32_t *q_i = q_l + QCAP;
     uint32_t *q_r = q_i + QCAP;
-    uint32_t *conn_l = q_r + QCAP;
-    for (int i = tid; i < h.words; i += BLOCK) smem[i] = blob[i];
-    if (!BICC)
-        for (int i = tid; i < conn_words_lds; i += BLOCK) conn_l[i] = conn[i];
-    if (tid == 0) s_misc[1] = 0;
-    __syncthreads();
-    View<uint32_t> q = make_view(h, smem);
     const uint32_t *adj = smem + h.o_adj;
-    const uint32_t *cb = conn_words_lds ? conn_l : conn;
     const uint32_t T = (k >= 32) ? 0xfffffffeu : (uint32_t)((1ull << k) - 2ull);
-    const uint32_t count = counters[k];
     const uint32_t nbatches = (count + B - 1) / B;
+    (void)lane; (void)s_nblk; (void)q_r;
     uint32_t npairs = 0;
-    for (uint32_t bt = blockIdx.x; bt < nbatches; bt += gridDim.x) {
+    for (uint32_t bt = bt0; bt < nbatches; bt += bstride) {
         const uint32_t base = bt * B;
         const uint32_t nb = (count - base) < B ? (count - base) : B;
         if ((uint32_t)tid < nb) {
@@ -437,8 +616,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
             s_set[tid] = S;
             s_best[tid] = ~0ull;
             s_first[tid] = ~0u;
-            unsigned long long cnt = T;
-            if (BICC) {
+            unsigned long long cnt = tree ? 2ull * (k - 1) : T;
+            if (BICC && !tree) {
                 int nblk = dfs_blocks(S, adj, dfs + tid, BMAX, s_blocks + tid * 32);
                 s_nblk[tid] = nblk;
                 cnt = 0;
@@ -459,7 +638,25 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
         }
         __syncthreads();
         const unsigned long long total = s_off[nb];
-        if (BICC) {
+        if (tree) {
+            const uint32_t per = 2u * (k - 1);
+            const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
+            for (unsigned long long item = tid; item < total; item += BLOCK) {
+                const uint32_t i = (uint32_t)(item / per), local = (uint32_t)(item % per);
+                const uint32_t S = s_set[i];
+                const uint32_t cb = pdep32(2u << (local >> 1), S);     /* skips the root (lowest) */
+                const int c = __builtin_ctz(cb);
+                const uint32_t sub = S & desc[c];
+                const uint32_t o = local & 1u;
+                const uint32_t L = o ? sub : (S ^ sub), R = S ^ L;
+                const uint32_t rank = BICC ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
+                Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+                npairs++;
+                bool nan = jr.total != jr.total;
+                atomicMin(&s_first[i], (rank << 1) | (nan ? 1u : 0u));
+                if (!nan) atomicMin(&s_best[i], ((unsigned long long)ord_key(jr.total) << 32) | rank);
+            }
+        } else if (BICC) {
             for (unsigned long long item = tid; item < total; item += BLOCK) {
                 uint32_t lo = 0, hi = nb;                      /* largest i with s_off[i] <= item */
                 while (hi - lo > 1) {
@@ -529,7 +726,20 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
             if (f != ~0u) {
                 uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(s_best[tid] & 0xffffffffull);
                 uint32_t S = s_set[tid], L, R;
-                if (BICC) {
+                if (tree && BICC) {
+                    const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
+                    uint32_t rest = S & (S - 1u), sub = 0;
+                    while (rest) {
+                        int c = __builtin_ctz(rest);
+                        rest &= rest - 1u;
+                        if (tree_post(S, c, desc, lglob) == (rank >> 1)) {
+                            sub = S & desc[c];
+                            break;
+                        }
+                    }
+                    L = (rank & 1u) ? sub : (S ^ sub);
+                    R = S ^ L;
+                } else if (BICC) {
                     bicc_split(S, s_blocks + tid * 32, rank, adj, &L, &R);
                 } else {
                     L = pdep32(rank, S);
@@ -542,8 +752,218 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
         }
         __syncthreads();
     }
+    return npairs;
+}
+
+template <bool BICC>
+__global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
+                                                    int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
+                                                    uint32_t *left, const uint32_t *conn, uint32_t *counters) {
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    const int tid = threadIdx.x;
+    const int W4 = (h.words + 3) & ~3;
+    uint32_t *s_misc = smem + W4 + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
+    uint32_t *conn_l = s_misc + 4 + 3 * QCAP;
+    for (int i = tid; i < h.words; i += BLOCK) smem[i] = blob[i];
+    if (!BICC)
+        for (int i = tid; i < conn_words_lds; i += BLOCK) conn_l[i] = conn[i];
+    if (tid == 0) s_misc[1] = 0;
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t *cb = conn_words_lds ? conn_l : conn;
+    uint32_t npairs = eval_level<BICC>(h, smem, q, k, B, cb, lvl_sets, counters[k], vals, left, blockIdx.x,
+                                       gridDim.x);
     if (tid == 0 && s_misc[1]) atomicOr(&counters[49], 1u);
     add_pairs(counters, npairs);
+}
+/* ================================================================ batches
+ *
+ * Many DPs at once, level bucket by level bucket: one filter launch over the
+ * candidates of every query, one planning launch that turns the level sizes
+ * into per-level work tasks, then for each level k one launch evaluating the
+ * level-k sets of all queries that have that level, and one plan-walk
+ * launch.  A task is a slice of one query's level: its workgroup stages that
+ * query's cost-model description in LDS and runs the same evaluators as the
+ * single-query path. */
+
+struct BatchJob {
+    BlobHdr h;
+    const uint32_t *blob;
+    float4 *vals;
+    uint32_t *left, *conn, *lvl_sets, *counters;
+    unsigned long long *lvl_best;
+    uint32_t *lvl_first;
+    DPNode *nodes;
+    uint32_t lvl_off[33];
+    uint32_t conn_words;
+    int bicc;
+};
+
+#define TREE_ITERS 4
+
+/* sets per task of level k */
+__device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
+    if (J.h.is_tree) {
+        uint32_t per = 2u * (k - 1), P = 1;
+        while (P < per) P <<= 1;
+        return (BLOCK / WAVE) * (WAVE / P) * TREE_ITERS;
+    }
+    uint64_t items = J.bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
+    uint64_t B = (768 + items - 1) / items;
+    return (uint32_t)(B < 1 ? 1 : (B > BMAX ? BMAX : B));
+}
+
+__global__ void k_batch_init(const BatchJob *jobs, int nj) {
+    for (int j = blockIdx.x; j < nj; j += gridDim.x) {
+        const BatchJob &J = jobs[j];
+        const int tid = threadIdx.x;
+        if (tid < 64) J.counters[tid] = 0;
+        if (tid < J.h.n_rels) {
+            const float *b = (const float *)(J.blob + J.h.o_base) + 4 * tid;
+            uint32_t s1 = 1u << tid;
+            J.vals[s1] = make_float4(b[0], b[1], b[2], b[3]);
+            J.left[s1] = 0;
+        }
+    }
+}
+
+/* chunk_prefix[j] = first filter chunk of job j (host-computed) */
+__global__ void __launch_bounds__(BLOCK) k_batch_filter(const BatchJob *jobs, int nj, const uint32_t *chunk_prefix) {
+    __shared__ FilterSmem F;
+    __shared__ int cur;
+    const uint32_t total = chunk_prefix[nj];
+    int have = -1;
+    for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+        if (threadIdx.x == 0) {
+            int lo = 0, hi = nj;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (chunk_prefix[mid] <= c) lo = mid;
+                else hi = mid;
+            }
+            cur = lo;
+        }
+        __syncthreads();
+        const int j = cur;
+        const BatchJob &J = jobs[j];
+        if (j != have) {
+            filter_setup(F, J.h, J.blob);
+            have = j;
+        }
+        filter_chunk(F, J.h, c - chunk_prefix[j], J.lvl_off, J.lvl_sets, J.lvl_best, J.lvl_first, J.conn,
+                     J.conn_words, J.counters);
+    }
+}
+
+/* task_prefix[k * (nj + 1) + j] = first task of job j at level k */
+__global__ void k_batch_plan(const BatchJob *jobs, int nj, int kmax, uint32_t *task_prefix) {
+    for (int k = 2 + (int)threadIdx.x; k <= kmax; k += blockDim.x) {
+        uint32_t acc = 0;
+        for (int j = 0; j < nj; j++) {
+            task_prefix[k * (nj + 1) + j] = acc;
+            const BatchJob &J = jobs[j];
+            if (k <= J.h.n_rels) {
+                uint32_t cnt = J.counters[k], spt = task_sets(J, k);
+                acc += (cnt + spt - 1) / spt;
+            }
+        }
+        task_prefix[k * (nj + 1) + nj] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int nj, int k,
+                                                       const uint32_t *task_prefix) {
+    /* The dynamic LDS must be the kernel's only LDS: a static __shared__
+     * variable would move its base off the 8-byte alignment that the 64-bit
+     * LDS atomics of eval_level need.  Each thread therefore finds its job
+     * itself (the search is uniform across the workgroup). */
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    const uint32_t *tp = task_prefix + k * (nj + 1);
+    const uint32_t total = tp[nj];
+    int have = -1;
+    uint32_t np = 0;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = nj;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (tp[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int j = lo;
+        const BatchJob &J = jobs[j];
+        const BlobHdr &h = J.h;
+        if (j != have) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = J.blob[i];
+            if (threadIdx.x == 0) smem[((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2 + 1] = 0;
+            __syncthreads();
+            have = j;
+        }
+        View<uint32_t> q = make_view(h, smem);
+        const uint32_t spt = task_sets(J, k);
+        const uint32_t first = (t - tp[j]) * spt, cnt = J.counters[k];
+        const uint32_t m = cnt - first < spt ? cnt - first : spt;
+        const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
+        if (h.is_tree) {
+            np += eval_level_tree(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                                  BLOCK / WAVE);
+        } else if (J.bicc) {
+            np += eval_level<true>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1);
+        } else {
+            np += eval_level<false>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t *misc = smem + ((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
+            if (!h.is_tree && misc[1]) atomicOr(&J.counters[49], 1u);
+        }
+        if (np) {
+            atomicAdd((unsigned long long *)(J.counters + 40), (unsigned long long)np);
+            np = 0;
+        }
+    }
+}
+
+/* Pre-order plan walk from `root_set` (dpsub_buildQueryTree,
+ * gpuqo_query_tree.cuh:46-72); returns the node count or ~0 on an
+ * inconsistent memo. */
+__device__ uint32_t walk_tree(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes, int cap) {
+    uint32_t st_set[66];
+    int st_parent[66], st_side[66];
+    int sp = 0, n = 0;
+    st_set[sp] = root_set; st_parent[sp] = -1; st_side[sp] = 0; sp++;
+    while (sp > 0) {
+        sp--;
+        uint32_t S = st_set[sp];
+        int par = st_parent[sp], side = st_side[sp];
+        if (n >= cap) return ~0u;
+        int me = n++;
+        float4 v = vals[S];
+        DPNode nd;
+        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
+        nd.left = -1; nd.right = -1; nd.pad = 0;
+        nodes[me] = nd;
+        if (par >= 0) {
+            if (side == 0) nodes[par].left = me;
+            else nodes[par].right = me;
+        }
+        if (__builtin_popcount(S) > 1) {
+            uint32_t l = left[S], r = S ^ l;
+            if (l == 0 || (l & ~S) != 0 || l == S || sp + 2 > 64) return ~0u;
+            st_set[sp] = r; st_parent[sp] = me; st_side[sp] = 1; sp++;
+            st_set[sp] = l; st_parent[sp] = me; st_side[sp] = 0; sp++;
+        }
+    }
+    return (uint32_t)n;
+}
+
+__global__ void k_batch_tree(const BatchJob *jobs, int nj) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nj) return;
+    const BatchJob &J = jobs[j];
+    const int n = J.h.n_rels;
+    uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    J.counters[48] = walk_tree(J.vals, J.left, full, J.nodes, 2 * n - 1);
 }
 
 /* ================================================================ DPsize
@@ -622,7 +1042,7 @@ __global__ void k_ds_zero_rows(DsBufs d) {
 
 __global__ void __launch_bounds__(BLOCK) k_ds_pairs(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
                                                      const float4 *vals, uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
     const int W4 = (h.words + 3) & ~3;
     uint32_t *tl = smem + W4;                 /* left sets  [DS_TA] */
@@ -770,7 +1190,7 @@ __global__ void k_ds_rowsort(DsBufs d) {
 
 __global__ void __launch_bounds__(BLOCK) k_ds_finalize(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
                                                         float4 *vals, uint32_t *left, uint32_t *counters) {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
@@ -863,6 +1283,14 @@ struct DeviceCtx {
     std::vector<hipEvent_t> prof_ev;
     DsBufs ds{};
     int *l1 = nullptr;
+    uint32_t *lvl_off = nullptr;
+    size_t cap_lvl_off = 0;
+    /* batches */
+    char *ws = nullptr;
+    uint32_t *bblob = nullptr;
+    BatchJob *jobs = nullptr;
+    uint32_t *results = nullptr, *bprefix = nullptr;
+    size_t cap_ws = 0, cap_bblob = 0, cap_jobs = 0, cap_results = 0, cap_bprefix = 0;
     size_t cap_ds_best = 0, cap_ds_first = 0, cap_ds_lset = 0, cap_ds_lnbr = 0, cap_ds_raw = 0, cap_ds_tmp = 0,
            cap_ds_rc = 0, cap_ds_rf = 0, cap_ds_ro = 0, cap_ds_sz = 0, cap_l1 = 0;
     std::mutex mu;
@@ -917,6 +1345,146 @@ int device_init(int device) {
     return ctx_init(g_ctx, device);
 }
 
+
+/* ------------------------------------------------------------ batches */
+
+#define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 nodes per job */
+
+bool fused_eligible(const DPInput &in, DPAlgo algo) {
+    const BlobHdr &h = in.hdr;
+    if (algo == DP_DPSIZE || h.n_rels < 2 || h.n_rels > 30) return false;
+    if (algo == DP_DPSUB && !h.is_tree && h.n_rels > 12) return false;
+    return std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)) <= 64 * 1024;
+}
+
+static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std::vector<DPAlgo> &algos,
+                             std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &stats, double *gpu_ms) {
+    DeviceCtx &c = g_ctx;
+    int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
+    if (rc) return rc;
+    const int nj = (int)ins.size();
+    trees.assign(nj, std::vector<DPNode>());
+    stats.assign(nj, DPStats());
+    if (nj == 0) return GQ_OK;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<BatchJob> jobs(nj);
+    std::vector<uint32_t> blobs, chunk_prefix(nj + 1, 0);
+    std::vector<size_t> ws_off(nj), blob_off(nj);
+    size_t ws = 0, lds = 0;
+    int kmax = 2;
+    for (int j = 0; j < nj; j++) {
+        const BlobHdr &h = ins[j]->hdr;
+        if (!fused_eligible(*ins[j], algos[j])) {
+            set_error("DP %d cannot run in a batch", j);
+            return GQ_ERR_INVALID;
+        }
+        const uint64_t ns = 1ull << h.n_rels;
+        ws_off[j] = ws;
+        ws += al(ns * 16) + al(ns * 4) + al(std::max<uint64_t>(1, ns / 32) * 4) + al(ns * 4) + al(ns * 8) +
+              al(ns * 4);
+        blob_off[j] = blobs.size();
+        blobs.insert(blobs.end(), ins[j]->blob.begin(), ins[j]->blob.end());
+        while (blobs.size() & 63) blobs.push_back(0u);
+        lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
+        chunk_prefix[j + 1] = chunk_prefix[j] + (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK);
+        kmax = std::max(kmax, h.n_rels);
+    }
+    if ((rc = ensure(&c.ws, c.cap_ws, ws))) return rc;
+    if ((rc = ensure(&c.bblob, c.cap_bblob, blobs.size()))) return rc;
+    if ((rc = ensure(&c.jobs, c.cap_jobs, (size_t)nj))) return rc;
+    if ((rc = ensure(&c.results, c.cap_results, (size_t)nj * RES_WORDS))) return rc;
+    if ((rc = ensure(&c.bprefix, c.cap_bprefix, (size_t)(nj + 1) * 34))) return rc;
+    for (int j = 0; j < nj; j++) {
+        const BlobHdr &h = ins[j]->hdr;
+        const uint64_t ns = 1ull << h.n_rels;
+        BatchJob &J = jobs[j];
+        J.h = h;
+        J.blob = c.bblob + blob_off[j];
+        char *p = c.ws + ws_off[j];
+        J.vals = (float4 *)p;
+        p += al(ns * 16);
+        J.left = (uint32_t *)p;
+        p += al(ns * 4);
+        J.conn = (uint32_t *)p;
+        J.conn_words = (uint32_t)std::max<uint64_t>(1, ns / 32);
+        p += al(std::max<uint64_t>(1, ns / 32) * 4);
+        J.lvl_sets = (uint32_t *)p;
+        p += al(ns * 4);
+        J.lvl_best = (unsigned long long *)p;
+        p += al(ns * 8);
+        J.lvl_first = (uint32_t *)p;
+        J.counters = c.results + (size_t)j * RES_WORDS;
+        J.nodes = (DPNode *)(J.counters + 64);
+        uint64_t acc = 0;
+        for (int k = 0; k <= 32; k++) {
+            J.lvl_off[k] = (uint32_t)acc;
+            if (k >= 2 && k <= h.n_rels) acc += binom64(h.n_rels, k);
+        }
+        J.bicc = algos[j] == DP_BICC ? 1 : 0;
+    }
+    hipStream_t s = c.stream;
+    HIPCHK(hipMemcpyAsync(c.bblob, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c.jobs, jobs.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c.bprefix, chunk_prefix.data(), sizeof(uint32_t) * (nj + 1), hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(c.ev0, s));
+    hipLaunchKernelGGL(k_batch_init, dim3(std::min(nj, 1024)), dim3(64), 0, s, c.jobs, nj);
+    const uint32_t fgrid = std::min<uint32_t>(chunk_prefix[nj], 4096);
+    hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, c.jobs, nj,
+                       c.bprefix);
+    uint32_t *task_prefix = c.bprefix + (nj + 1);        /* [33][nj+1] after the chunk prefix */
+    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, s, c.jobs, nj, kmax, task_prefix);
+    for (int k = 2; k <= kmax; k++) {
+        /* grid: enough workgroups for the largest level, capped */
+        uint64_t ub = 0;
+        for (int j = 0; j < nj; j++) {
+            const BlobHdr &h = ins[j]->hdr;
+            if (k > h.n_rels) continue;
+            ub += binom64(h.n_rels, k);
+        }
+        uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
+        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds, s, c.jobs, nj, k, task_prefix);
+    }
+    hipLaunchKernelGGL(k_batch_tree, dim3((nj + 63) / 64), dim3(64), 0, s, c.jobs, nj);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, s));
+    std::vector<uint32_t> res((size_t)nj * RES_WORDS);
+    HIPCHK(hipMemcpyAsync(res.data(), c.results, res.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    if (gpu_ms) *gpu_ms = ms;
+    for (int j = 0; j < nj; j++) {
+        const int n = ins[j]->hdr.n_rels;
+        const uint32_t *cnt = res.data() + (size_t)j * RES_WORDS;
+        if (cnt[49]) {
+            set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");
+            return GQ_ERR_UNSUPPORTED;
+        }
+        if (cnt[48] != (uint32_t)(2 * n - 1)) {
+            set_error("plan reconstruction failed in batch job %d (status %u)", j, cnt[48]);
+            return GQ_ERR_DEVICE;
+        }
+        const DPNode *nodes = (const DPNode *)(cnt + 64);
+        trees[j].assign(nodes, nodes + 2 * n - 1);
+        DPStats &st = stats[j];
+        st.candidates = (1ull << n) - (uint64_t)n - 1ull;
+        for (int k = 2; k <= n; k++) st.sets += cnt[k];
+        st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+        st.memo_bytes = (1ull << n) * 40;
+        st.memo_kind = 1;
+        st.gpu_ms = ms / nj;
+        st.eval_ms = ms / nj;
+        st.eval_launches = 1;
+    }
+    return GQ_OK;
+}
+
+int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAlgo> &algos,
+                 std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &stats, double *gpu_ms) {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
+}
+
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
            DPStats &st) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
@@ -937,9 +1505,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
     const uint32_t conn_words_all = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
     const int conn_lds = conn_words_all <= 4096 ? (int)conn_words_all : 0;
-    const size_t lds_common = lds_eval + BMAX * 8 + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
-    const size_t lds_bicc = lds_common + BMAX * 4 + (size_t)BMAX * 32 * 4 * 2;
-    const size_t lds_dpsub = lds_common + 3 * QCAP * 4 + (size_t)conn_lds * 4;
+    const size_t lds_bicc = eval_lds_bicc(h.words);
+    const size_t lds_dpsub = eval_lds_dpsub(h.words, conn_lds);
     if (lds_bicc > 64 * 1024 || lds_dpsub > 64 * 1024) {
         set_error("query description too large for LDS (%zu bytes)", blob_bytes);
         return GQ_ERR_UNSUPPORTED;
@@ -951,9 +1518,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     if ((rc = ensure(&c.b.vals, c.cap_memo, nsets_all))) return rc;
     if ((rc = ensure(&c.b.left, c.cap_left, nsets_all))) return rc;
     if ((rc = ensure(&c.b.conn, c.cap_conn, std::max<uint64_t>(1, nsets_all / 32)))) return rc;
-    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, maxC))) return rc;
-    if ((rc = ensure(&c.b.lvl_best, c.cap_best, maxC))) return rc;
-    if ((rc = ensure(&c.b.lvl_first, c.cap_first, maxC))) return rc;
+    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, nsets_all))) return rc;
+    if ((rc = ensure(&c.b.lvl_best, c.cap_best, nsets_all))) return rc;
+    if ((rc = ensure(&c.b.lvl_first, c.cap_first, nsets_all))) return rc;
+    if ((rc = ensure(&c.lvl_off, c.cap_lvl_off, 33))) return rc;
     if ((rc = ensure(&c.b.blob, c.cap_blob, (size_t)h.words))) return rc;
     if ((rc = ensure(&c.b.nodes, c.cap_nodes, (size_t)2 * n))) return rc;
 
@@ -1066,30 +1634,51 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         st.memo_kind = 1;
         return GQ_OK;
     }
-    uint64_t cand_total = 0;
+    /* level lists: level k at offset sum_{2<=j<k} C(n,j) */
+    uint32_t off_h[33];
+    memset(off_h, 0, sizeof(off_h));
+    {
+        uint64_t acc = 0;
+        for (int k = 2; k <= n; k++) {
+            off_h[k] = (uint32_t)acc;
+            acc += binom64(n, k);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(c.lvl_off, off_h, sizeof(off_h), hipMemcpyHostToDevice, s));
+    const uint64_t cand_total = nsets_all - (uint64_t)n - 1ull;
+    if (profile) HIPCHK(hipEventRecord(c.prof_ev[0], s));
+    {
+        uint64_t waves = (nsets_all + WAVE - 1) / WAVE;
+        uint32_t fblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
+        hipLaunchKernelGGL(k_filter_all, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, c.lvl_off, c.b.lvl_sets,
+                           c.b.lvl_best, c.b.lvl_first, c.b.conn, conn_words, c.b.counters);
+    }
+    if (profile) HIPCHK(hipEventRecord(c.prof_ev[1], s));
     for (int k = 2; k <= n; k++) {
-        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         uint64_t Ck = binom64(n, k);
-        cand_total += Ck;
-        uint64_t threads = std::min<uint64_t>(Ck, 1ull << 19);
-        uint32_t per = (uint32_t)((Ck + threads - 1) / threads);
-        threads = (Ck + per - 1) / per;
-        uint32_t fblocks = (uint32_t)((threads + BLOCK - 1) / BLOCK);
-        hipLaunchKernelGGL(k_filter, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, k, (uint32_t)Ck, per,
-                           c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.conn, c.b.counters);
+        uint32_t *sets_k = c.b.lvl_sets + off_h[k];
+        unsigned long long *best_k = c.b.lvl_best + off_h[k];
+        uint32_t *first_k = c.b.lvl_first + off_h[k];
         uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         const uint64_t T = (1ull << k) - 2;
-        if (algo == DP_BICC || T <= 4096) {
-            uint64_t items_est = algo == DP_BICC ? 2ull * (k - 1) : T;
+        if (h.is_tree) {
+            uint64_t per = 2ull * (k - 1), P = 1;
+            while (P < per) P <<= 1;
+            uint64_t waves = (Ck + (WAVE / P) - 1) / (WAVE / P);
+            uint32_t tblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
+            hipLaunchKernelGGL(k_eval_tree, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                               algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
+        } else if (algo == DP_BICC || T <= 4096) {
+            uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
             if (algo == DP_BICC) {
                 hipLaunchKernelGGL(k_eval_wg<true>, dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k, B, 0,
-                                   c.b.lvl_sets, c.b.vals, c.b.left, c.b.conn, c.b.counters);
+                                   sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters);
             } else {
                 hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, c.b.blob, k, B,
-                                   conn_lds, c.b.lvl_sets, c.b.vals, c.b.left, c.b.conn, c.b.counters);
+                                   conn_lds, sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters);
             }
         } else {
             uint32_t ipl = 16;
@@ -1098,9 +1687,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             uint64_t waves = Ck * cps;
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 16384);
             hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, cps, ipl,
-                               c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.conn, c.b.counters);
-            hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                               c.b.lvl_sets, c.b.lvl_best, c.b.lvl_first, c.b.vals, c.b.left, c.b.counters);
+                               sets_k, best_k, first_k, c.b.vals, c.b.conn, c.b.counters);
+            hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sets_k,
+                               best_k, first_k, c.b.vals, c.b.left, c.b.counters);
         }
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
@@ -1126,13 +1715,14 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     st.gpu_ms = ms;
     if (profile) {
+        float a = 0.0f;
+        HIPCHK(hipEventElapsedTime(&a, c.prof_ev[0], c.prof_ev[1]));
+        st.filter_ms = a;
+        st.filter_launches = 1;
         for (int k = 2; k <= n; k++) {
-            float a = 0.0f, b = 0.0f;
-            HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
+            float b = 0.0f;
             HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
-            st.filter_ms += a;
             st.eval_ms += b;
-            st.filter_launches++;
             st.eval_launches++;
         }
     }

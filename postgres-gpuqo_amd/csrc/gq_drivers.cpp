@@ -237,33 +237,7 @@ int run_switch<4>(RunCtx &, const HQuery<4> &h, QT<4> **out) {
     return GQ_ERR_UNSUPPORTED;
 }
 
-template <>
-int run_switch<1>(RunCtx &ctx, const HQuery<1> &h, QT<1> **out) {
-    *out = nullptr;
-    if (interrupted()) {
-        set_error("interrupted");
-        return GQ_ERR_INTERRUPTED;
-    }
-    if (h.n == 1) {
-        QT<1> *t = ctx.node<1>();
-        t->id = Set<1>::nth(0);
-        t->rows = h.rows[0];
-        t->startup = h.startup[0];
-        t->total = h.total[0];
-        t->width = h.width[0];
-        *out = t;
-        return GQ_OK;
-    }
-    DPInput in;
-    int rc = build_dp_input(h, in);
-    if (rc) return rc;
-    DPAlgo algo = ctx.algorithm == GQ_ALGO_DPSIZE ? DP_DPSIZE : (ctx.bicc ? DP_BICC : DP_DPSUB);
-    std::vector<DPNode> nodes;
-    DPStats st;
-    rc = run_dp(in, algo, h.n_iters, ctx.memo_kind, ctx.profile != 0, nodes, st);
-    ctx.dp_calls++;
-    if (rc) return rc;
-    ctx.add(st);
+static QT<1> *nodes_to_tree(RunCtx &ctx, const std::vector<DPNode> &nodes) {
     std::vector<QT<1> *> made(nodes.size(), nullptr);
     for (size_t i = 0; i < nodes.size(); i++) {
         QT<1> *t = ctx.node<1>();
@@ -278,8 +252,78 @@ int run_switch<1>(RunCtx &ctx, const HQuery<1> &h, QT<1> **out) {
         made[i]->left = nodes[i].left >= 0 ? made[nodes[i].left] : nullptr;
         made[i]->right = nodes[i].right >= 0 ? made[nodes[i].right] : nullptr;
     }
-    *out = made.empty() ? nullptr : made[0];
+    return made.empty() ? nullptr : made[0];
+}
+
+static QT<1> *single_leaf(RunCtx &ctx, const HQuery<1> &h) {
+    QT<1> *t = ctx.node<1>();
+    t->id = Set<1>::nth(0);
+    t->rows = h.rows[0];
+    t->startup = h.startup[0];
+    t->total = h.total[0];
+    t->width = h.width[0];
+    return t;
+}
+
+static DPAlgo dp_algo(const RunCtx &ctx) {
+    return ctx.algorithm == GQ_ALGO_DPSIZE ? DP_DPSIZE : (ctx.bicc ? DP_BICC : DP_DPSUB);
+}
+
+/* Run the DPs of several relabelled queries: those the fused kernel takes
+ * go through one batched launch, the others one by one. */
+static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::vector<QT<1> *> &out) {
+    out.assign(hs.size(), nullptr);
+    if (interrupted()) {
+        set_error("interrupted");
+        return GQ_ERR_INTERRUPTED;
+    }
+    const DPAlgo algo = dp_algo(ctx);
+    std::vector<DPInput> ins(hs.size());
+    std::vector<const DPInput *> batch;
+    std::vector<DPAlgo> algos;
+    std::vector<size_t> where;
+    for (size_t i = 0; i < hs.size(); i++) {
+        if (hs[i]->n == 1) {
+            out[i] = single_leaf(ctx, *hs[i]);
+            continue;
+        }
+        int rc = build_dp_input(*hs[i], ins[i]);
+        if (rc) return rc;
+        if (hs.size() > 1 && hs[i]->n_iters == hs[i]->n && fused_eligible(ins[i], algo)) {
+            batch.push_back(&ins[i]);
+            algos.push_back(algo);
+            where.push_back(i);
+            continue;
+        }
+        std::vector<DPNode> nodes;
+        DPStats st;
+        rc = run_dp(ins[i], algo, hs[i]->n_iters, ctx.memo_kind, ctx.profile != 0, nodes, st);
+        ctx.dp_calls++;
+        if (rc) return rc;
+        ctx.add(st);
+        out[i] = nodes_to_tree(ctx, nodes);
+    }
+    if (!batch.empty()) {
+        std::vector<std::vector<DPNode>> trees;
+        std::vector<DPStats> sts;
+        double ms = 0.0;
+        int rc = run_dp_batch(batch, algos, trees, sts, &ms);
+        if (rc) return rc;
+        for (size_t b = 0; b < batch.size(); b++) {
+            ctx.dp_calls++;
+            ctx.add(sts[b]);
+            out[where[b]] = nodes_to_tree(ctx, trees[b]);
+        }
+    }
     return GQ_OK;
+}
+
+template <>
+int run_switch<1>(RunCtx &ctx, const HQuery<1> &h, QT<1> **out) {
+    std::vector<QT<1> *> o;
+    int rc = run_many(ctx, {&h}, o);
+    *out = rc ? nullptr : o[0];
+    return rc;
 }
 
 /* Relabel `lst` out of `info` and run the DP on the result
@@ -540,12 +584,24 @@ static int union_rec(RunCtx &ctx, const HQuery<WO> &info, const std::vector<Rema
         if (!subset_of(g, seen)) parts.push_back(g);
         seen |= g;
     }
+    /* the partitions' DPs are independent: relabel each (:386-419) and run
+     * them together */
     std::vector<RemapEl<WI>> next;
-    for (size_t i = 0; i < parts.size(); i++) {                      /* :386-419 */
-        QT<WI> *r = nullptr;
-        std::vector<RemapEl<WI>> rl = members_list(parts[i]);
-        int rc = rl.size() < 64 ? sub_dp<WI, 1>(ctx, ni, rl, &r) : sub_dp<WI, 4>(ctx, ni, rl, &r);
-        if (rc) return rc;
+    std::vector<Remapper<WI, 1>> rms;
+    std::vector<HQuery<1>> subs(parts.size());
+    rms.reserve(parts.size());
+    for (size_t i = 0; i < parts.size(); i++) {
+        rms.emplace_back(members_list(parts[i]));
+        rms.back().remap(ni, subs[i]);
+        subs[i].n_iters = subs[i].n;
+    }
+    std::vector<const HQuery<1> *> hs;
+    for (auto &x : subs) hs.push_back(&x);
+    std::vector<QT<1> *> res1;
+    int rc0 = run_many(ctx, hs, res1);
+    if (rc0) return rc0;
+    for (size_t i = 0; i < parts.size(); i++) {
+        QT<WI> *r = rms[i].back(ctx, res1[i]);
         next.push_back(RemapEl<WI>{r->id, (int)i, r});
     }
     QT<WI> *res = nullptr;
@@ -593,6 +649,22 @@ int gpuqo_run_t(RunCtx &ctx, const HQuery<W> &h, QT<W> **out) {
 }
 
 template int gpuqo_run_t<1>(RunCtx &, const HQuery<1> &, QT<1> **);
+
+int prepare_plain(const HQuery<1> &h, PlainPrepared &p) {
+    p.bfs = bfs_list(h);
+    if ((int)p.bfs.size() != h.n) {
+        set_error("join graph is disconnected (%zu of %d relations reachable)", p.bfs.size(), h.n);
+        return GQ_ERR_NO_PLAN;
+    }
+    Remapper<1, 1> rm(p.bfs);
+    rm.remap(h, p.ri);
+    return GQ_OK;
+}
+
+QT<1> *finish_plain(RunCtx &ctx, const PlainPrepared &p, const std::vector<DPNode> &nodes) {
+    Remapper<1, 1> rm(p.bfs);
+    return rm.back(ctx, nodes_to_tree(ctx, nodes));
+}
 template int gpuqo_run_t<4>(RunCtx &, const HQuery<4> &, QT<4> **);
 
 }  // namespace gq

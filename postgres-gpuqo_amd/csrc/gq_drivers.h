@@ -50,6 +50,15 @@ int gpuqo_run_t(RunCtx &ctx, const HQuery<W> &h, QT<W> **out);
 
 int build_dp_input(const HQuery<1> &q, DPInput &in);
 
+/* Plain-DP pieces for batching: relabel a query in BFS order, and map a DP
+ * tree (in the relabelled numbering) back to the caller's. */
+struct PlainPrepared {
+    HQuery<1> ri;
+    std::vector<RemapEl<1>> bfs;
+};
+int prepare_plain(const HQuery<1> &h, PlainPrepared &p);
+QT<1> *finish_plain(RunCtx &ctx, const PlainPrepared &p, const std::vector<DPNode> &nodes);
+
 }  // namespace gq
 
 #endif

@@ -218,6 +218,30 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     for (size_t i = 0; i < q.sels.size(); i++) put_f(h.o_sels, (int)i, q.sels[i]);
     h.o_qualtab = take(n_ec + 1);
     for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
+    h.o_desc = take(32);
+    h.o_lglob = take(32);
+    h.is_tree = n >= 2 ? 1 : 0;
+    for (int v = 1; v < n; v++)
+        if (popc((uint32_t)q.adj[v].w[0] & ((1u << v) - 1u)) != 1) h.is_tree = 0;
+    if (h.is_tree) {
+        std::vector<int> parent(n, -1);
+        std::vector<uint32_t> desc(n, 0), lg(n, 0);
+        for (int v = 1; v < n; v++) parent[v] = ctz((uint32_t)q.adj[v].w[0] & ((1u << v) - 1u));
+        for (int v = n - 1; v >= 0; v--) {
+            desc[v] |= 1u << v;
+            if (v > 0) desc[parent[v]] |= desc[v];
+        }
+        for (int v = 1; v < n; v++) {          /* parents precede children */
+            uint32_t l = lg[parent[v]];
+            for (int s2 = 1; s2 < v; s2++)
+                if (parent[s2] == parent[v]) l |= desc[s2];
+            lg[v] = l;
+        }
+        for (int v = 0; v < n; v++) {
+            b[h.o_desc + v] = desc[v];
+            b[h.o_lglob + v] = lg[v];
+        }
+    }
     h.o_base = take(4 * n);
     in.base.resize(n);
     for (int i = 0; i < n; i++) {
@@ -449,11 +473,73 @@ gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
 }
 
 gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, gq_plan_out *outs) {
-    if (n_queries < 0 || (n_queries > 0 && (!qs || !outs))) {
+    if (n_queries < 0 || (n_queries > 0 && (!qs || !outs)) || !cfg) {
         gq::set_error("bad batch arguments");
         return GQ_ERR_INVALID;
     }
+    int rc = gq::device_init(cfg->device);
+    if (rc) return (gq_status)rc;
+    /* Plain DPs (no IDP decomposition) that the fused kernel takes run in one
+     * launch; every other query goes through gq_plan. */
+    const bool plain_cfg = cfg->gpuqo_algorithm == GQ_ALGO_DPSUB &&
+                           (cfg->gpuqo_idp_n_iters <= 1);
+    std::vector<gq::PlainPrepared> preps(n_queries);
+    std::vector<gq::DPInput> ins(n_queries);
+    std::vector<const gq::DPInput *> batch;
+    std::vector<gq::DPAlgo> algos;
+    std::vector<int> where;
+    const gq::DPAlgo algo = cfg->gpuqo_dpsub_bicc ? gq::DP_BICC : gq::DP_DPSUB;
+    auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n_queries; i++) {
+        if ((rc = gq::validate(qs[i]))) return (gq_status)rc;
+        if (!outs[i].nodes) {
+            gq::set_error("null output nodes for query %d", i);
+            return GQ_ERR_INVALID;
+        }
+        if (!plain_cfg || qs[i]->n_rels > 30 || qs[i]->n_rels < 2) continue;
+        gq::HQuery<1> h;
+        gq::convert(qs[i], cfg, h);
+        if (gq::prepare_plain(h, preps[i])) continue;
+        if (gq::build_dp_input(preps[i].ri, ins[i])) continue;
+        if (!gq::fused_eligible(ins[i], algo)) continue;
+        batch.push_back(&ins[i]);
+        algos.push_back(algo);
+        where.push_back(i);
+    }
+    std::vector<char> done(n_queries, 0);
+    if (!batch.empty()) {
+        std::vector<std::vector<gq::DPNode>> trees;
+        std::vector<gq::DPStats> sts;
+        double ms = 0.0;
+        if ((rc = gq::run_dp_batch(batch, algos, trees, sts, &ms))) return (gq_status)rc;
+        auto t1 = std::chrono::steady_clock::now();
+        double wall = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        for (size_t b = 0; b < batch.size(); b++) {
+            int i = where[b];
+            gq::RunCtx ctx(cfg);
+            gq::QT<1> *t = gq::finish_plain(ctx, preps[i], trees[b]);
+            gq_plan_out *out = &outs[i];
+            out->n_nodes = 0;
+            memset(&out->stats, 0, sizeof(out->stats));
+            if (gq::emit(t, out) == -2) {
+                gq::set_error("output capacity %d < %d nodes", out->capacity, 2 * qs[i]->n_rels - 1);
+                return GQ_ERR_CAPACITY;
+            }
+            out->stats.candidates = sts[b].candidates;
+            out->stats.sets = sts[b].sets;
+            out->stats.pairs = sts[b].pairs;
+            out->stats.memo_bytes = sts[b].memo_bytes;
+            out->stats.gpu_ms = ms;            /* the whole batch launch */
+            out->stats.wall_ms = wall;
+            out->stats.n_dp_calls = 1;
+            out->stats.memo_kind = sts[b].memo_kind;
+            out->stats.eval_ms = ms;
+            out->stats.eval_launches = 1;
+            done[i] = 1;
+        }
+    }
+    for (int i = 0; i < n_queries; i++) {
+        if (done[i]) continue;
         gq_status st = gq_plan(qs[i], cfg, &outs[i]);
         if (st != GQ_OK) return st;
     }

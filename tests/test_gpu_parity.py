@@ -112,3 +112,30 @@ def test_idp2_snowflake20(k):
     want, _ = oracle_bits(g, c)
     got, _ = gpu_bits(g, c)
     assert_same_plan(got, want, f"idp2 k={k}")
+
+
+@pytest.mark.parametrize("bicc", [1, 0])
+def test_batch_matches_oracle_and_single(bicc):
+    """A mixed batch in one fused launch: every plan equals the oracle's and
+    the plan of the same query run alone."""
+    import gpuqo_amd
+    qs = qgen.mixed_batch(40, 4242 + bicc, nmin=4, nmax=15)
+    c = cfg("dpsub", bicc)
+    res = gpuqo_amd.plan_batch(qs, c)
+    for g, (nodes, st) in zip(qs, res):
+        got = gpuqo_amd.plan_bits(nodes)
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(got, want, f"batch {g.shape}{g.n}")
+        alone, _ = gpu_bits(g, c)
+        assert_same_plan(got, alone, f"alone {g.shape}{g.n}")
+
+
+def test_fused_and_per_level_paths_agree():
+    """Queries just above and below the fused-kernel size limit."""
+    import gpuqo_amd
+    for n in (13, 14, 15, 16):
+        for g in (qgen.snowflake(n, 900 + n), qgen.star(n, 800 + n), qgen.cycle(n, 700 + n)):
+            c = cfg("dpsub", 1)
+            want, _ = oracle_bits(g, c)
+            got, _ = gpu_bits(g, c)
+            assert_same_plan(got, want, f"{g.shape}{n}")
