@@ -136,11 +136,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
+
+    if args.copy_calibration:
+        L = C.CDLL(os.path.join(ROOT, "postgres-gpuqo_amd", "lib", "libgq_bench.so"))
+        L.gq_copy_bench.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        g, ms = C.c_double(), C.c_double()
+        nbytes = args.copy_calibration << 20
+        rc = L.gq_copy_bench(nbytes, 1, C.byref(g), C.byref(ms))
+        print(json.dumps({"copy_bytes_each_way": nbytes, "launches": 2, "ms": ms.value, "gbps": g.value, "rc": rc}))
+        return rc
 
     import gpuqo_amd
     from gpuqo_amd import abi, qgen

@@ -170,8 +170,11 @@ __device__ __forceinline__ uint32_t gosper(uint32_t v) {
     return (uint32_t)nx;
 }
 
+/* Adds the wave's pair count to counters[40..41]: one atomic per wave.  All
+ * lanes of the wave must call it. */
 __device__ __forceinline__ void add_pairs(uint32_t *counters, uint32_t n) {
-    if (n) atomicAdd((unsigned long long *)(counters + 40), (unsigned long long)n);
+    for (int off = WAVE / 2; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && n) atomicAdd((unsigned long long *)(counters + 40), (unsigned long long)n);
 }
 
 /* ---------------------------------------------------------------- kernels */
@@ -566,6 +569,41 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     add_pairs(counters, np);
 }
 
+/* One DP's device state, as the batch kernels and the big-set kernels see it
+ * (a single-query run is a batch of one). */
+struct BatchJob {
+    BlobHdr h;
+    const uint32_t *blob;
+    float4 *vals;
+    uint32_t *left, *conn, *lvl_sets, *counters;
+    unsigned long long *lvl_best;
+    uint32_t *lvl_first;
+    DPNode *nodes;
+    uint32_t lvl_off[33];
+    uint32_t conn_words;
+    int bicc;
+};
+
+/* MPDP sets whose block decomposition yields more than BIG_SPLITS splits
+ * (a biconnected block of 15 or more relations, e.g. a whole cycle) are not
+ * evaluated by the workgroup that owns their batch: they are appended to a
+ * list and their split range is spread over many workgroups, BIG_CHUNK
+ * splits each (k_big_scan, k_big_eval). */
+#define BIG_SPLITS 16384u
+#define BIG_CHUNK 16384u
+
+struct BigEnt {
+    uint32_t job, gidx, cnt, done;   /* gidx: index into the job's level arrays */
+};
+
+struct BigList {
+    BigEnt *ents;        /* nullptr: evaluate every set in place */
+    uint32_t *count;
+    uint32_t cap;
+    uint32_t job;
+    uint32_t gidx0;      /* level-array index of lvl_sets[0] of this call */
+};
+
 /* LDS layout of the level evaluation, after the staged blob (W4 words):
  *   s_best[BMAX] u64, s_off[BMAX+2] u64, s_first[BMAX], s_set[BMAX], s_misc[4]
  *   MPDP (general graphs): s_nblk[BMAX], s_blocks[BMAX][32], dfs[32][BMAX]
@@ -587,7 +625,7 @@ __host__ __device__ constexpr size_t eval_lds_dpsub(int words, int conn_words_ld
 template <bool BICC>
 __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k, uint32_t B,
                                const uint32_t *cb, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
-                               uint32_t *left, uint32_t bt0, uint32_t bstride) {
+                               uint32_t *left, uint32_t bt0, uint32_t bstride, const BigList &bl) {
     const bool tree = h.is_tree != 0;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -622,7 +660,21 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                 s_nblk[tid] = nblk;
                 cnt = 0;
                 for (int b = 0; b < nblk; b++) cnt += (1ull << __builtin_popcount(s_blocks[tid * 32 + b])) - 2ull;
-                if (cnt > 0x7fffffffull) s_misc[1] = 1;   /* split rank would not fit 31 bits */
+                if (cnt > 0x7fffffffull) {          /* split rank would not fit 31 bits */
+                    s_misc[1] = 1;
+                    cnt = 0;
+                } else if (bl.ents && cnt > BIG_SPLITS) {
+                    const uint32_t slot = atomicAdd(bl.count, 1u);
+                    if (slot < bl.cap) {
+                        BigEnt e;
+                        e.job = bl.job;
+                        e.gidx = bl.gidx0 + base + (uint32_t)tid;
+                        e.cnt = (uint32_t)cnt;
+                        e.done = 0;
+                        bl.ents[slot] = e;
+                        cnt = 0;                    /* evaluated by k_big_eval */
+                    }
+                }
             }
             s_off[tid] = cnt;
         }
@@ -758,7 +810,8 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
 template <bool BICC>
 __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
                                                     int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
-                                                    uint32_t *left, const uint32_t *conn, uint32_t *counters) {
+                                                    uint32_t *left, const uint32_t *conn, uint32_t *counters,
+                                                    BigList bl) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const int tid = threadIdx.x;
     const int W4 = (h.words + 3) & ~3;
@@ -772,9 +825,172 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
     View<uint32_t> q = make_view(h, smem);
     const uint32_t *cb = conn_words_lds ? conn_l : conn;
     uint32_t npairs = eval_level<BICC>(h, smem, q, k, B, cb, lvl_sets, counters[k], vals, left, blockIdx.x,
-                                       gridDim.x);
+                                       gridDim.x, bl);
     if (tid == 0 && s_misc[1]) atomicOr(&counters[49], 1u);
     add_pairs(counters, npairs);
+}
+
+/* Exclusive scan of one value per thread over a workgroup of up to 1024
+ * threads; `part` is 17 words of LDS.  Returns the thread's prefix and sets
+ * *total. */
+__device__ uint32_t block_scan_excl(uint32_t v, uint32_t *part, uint32_t *total) {
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+    uint32_t x = v;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == WAVE - 1) part[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < nw; i++) {
+            uint32_t t = part[i];
+            part[i] = acc;
+            acc += t;
+        }
+        part[16] = acc;
+    }
+    __syncthreads();
+    const uint32_t r = x - v + part[w];
+    *total = part[16];
+    __syncthreads();
+    return r;
+}
+
+/* Lays out the chunks of the deferred big sets: prefix[i] = first chunk of
+ * entry i, prefix[nb] = total.  `hdr[0]` is the append counter (reset here
+ * for the next level), `hdr[1]` receives the entry count. */
+__global__ void __launch_bounds__(1024) k_big_scan(const BigEnt *ents, uint32_t *hdr, uint32_t cap,
+                                                    uint32_t *prefix) {
+    __shared__ uint32_t part[17];
+    const uint32_t raw = hdr[0];
+    const uint32_t nb = raw < cap ? raw : cap;
+    const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = b0; i < b0 + per && i < nb; i++) sum += (ents[i].cnt + BIG_CHUNK - 1) / BIG_CHUNK;
+    uint32_t total;
+    uint32_t acc = block_scan_excl(sum, part, &total);
+    for (uint32_t i = b0; i < b0 + per && i < nb; i++) {
+        prefix[i] = acc;
+        acc += (ents[i].cnt + BIG_CHUNK - 1) / BIG_CHUNK;
+    }
+    if (threadIdx.x == 0) {
+        prefix[nb] = total;
+        hdr[1] = nb;
+        hdr[0] = 0;
+    }
+}
+
+/* Big MPDP sets, chunk-parallel: workgroup-iteration c takes BIG_CHUNK
+ * consecutive split ranks of one deferred set (its block decomposition is
+ * rebuilt in LDS when the set changes), keeps per lane the first valid
+ * split (rank, NaN flag) and the first minimum of (cost, rank), reduces them
+ * per wave and merges them into the set's lvl_first / lvl_best with global
+ * atomics.  The workgroup finishing a set's last chunk (counted in
+ * BigEnt::done) rebuilds the winning split and writes the memo. */
+__global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt *ents, const uint32_t *hdr,
+                                                     const uint32_t *prefix) {
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    const uint32_t nb = hdr[1];
+    if (nb == 0) return;
+    const uint32_t total = prefix[nb];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    int have_job = -1;
+    uint32_t have_b = ~0u;
+    for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+        uint32_t lo = 0, hi = nb;                      /* largest b with prefix[b] <= c */
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (prefix[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t b = lo;
+        const uint32_t job = ents[b].job, gidx = ents[b].gidx, cnt = ents[b].cnt;
+        const BatchJob &J = jobs[job];
+        const BlobHdr &h = J.h;
+        const int W4 = (h.words + 3) & ~3;
+        uint32_t *s_blocks = smem + W4;                /* [32] */
+        uint32_t *s_dfs = s_blocks + 32;               /* [32] */
+        uint32_t *s_misc = s_dfs + 32;                 /* [0] nblk, [1] S, [2] last */
+        if ((int)job != have_job) {
+            __syncthreads();
+            for (int i = tid; i < h.words; i += BLOCK) smem[i] = J.blob[i];
+            __syncthreads();
+            have_job = (int)job;
+            have_b = ~0u;
+        }
+        const uint32_t *adj = smem + h.o_adj;
+        if (b != have_b) {
+            if (tid == 0) {
+                const uint32_t S0 = J.lvl_sets[gidx];
+                s_misc[1] = S0;
+                s_misc[0] = (uint32_t)dfs_blocks(S0, adj, s_dfs, 1, s_blocks);
+            }
+            __syncthreads();
+            have_b = b;
+        }
+        const uint32_t S = s_misc[1];
+        View<uint32_t> q = make_view(h, smem);
+        const uint32_t c0 = (c - prefix[b]) * BIG_CHUNK;
+        const uint32_t c1 = (cnt - c0) < BIG_CHUNK ? cnt : c0 + BIG_CHUNK;
+        uint32_t first = ~0u, npairs = 0;
+        unsigned long long best = ~0ull;
+        for (uint32_t local = c0 + (uint32_t)tid; local < c1; local += BLOCK) {
+            uint32_t L, R;
+            bicc_split(S, s_blocks, local, adj, &L, &R);
+            if ((L | R) == S) {
+                Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
+                npairs++;
+                const bool nan = jr.total != jr.total;
+                if (first == ~0u) first = (local << 1) | (nan ? 1u : 0u);
+                if (!nan) {
+                    const unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | local;
+                    if (key < best) best = key;
+                }
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            const uint32_t f2 = __shfl_xor(first, off);
+            const unsigned long long b2 = __shfl_xor(best, off);
+            const uint32_t p2 = __shfl_xor(npairs, off);
+            first = f2 < first ? f2 : first;
+            best = b2 < best ? b2 : best;
+            npairs += p2;
+        }
+        if (lane == 0) {
+            if (first != ~0u) atomicMin(&J.lvl_first[gidx], first);
+            if (best != ~0ull) atomicMin(&J.lvl_best[gidx], best);
+            if (npairs) atomicAdd((unsigned long long *)(J.counters + 40), (unsigned long long)npairs);
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t nch = (cnt + BIG_CHUNK - 1) / BIG_CHUNK;
+            const uint32_t old = atomicAdd(&ents[b].done, 1u);
+            s_misc[2] = (old == nch - 1) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (tid == 0 && s_misc[2]) {
+            __threadfence();
+            const uint32_t f = atomicOr(&J.lvl_first[gidx], 0u);
+            if (f != ~0u) {
+                const unsigned long long bst = atomicOr(&J.lvl_best[gidx], 0ull);
+                const uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(bst & 0xffffffffull);
+                uint32_t L, R;
+                bicc_split(S, s_blocks, rank, adj, &L, &R);
+                Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
+                store_rel(J.vals, S, jr);
+                J.left[S] = L;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__host__ __device__ constexpr size_t big_lds(int words) {
+    return (size_t)(((words + 3) & ~3) * 4) + (32 + 32 + 4) * 4;
 }
 /* ================================================================ batches
  *
@@ -786,20 +1002,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
  * query's cost-model description in LDS and runs the same evaluators as the
  * single-query path. */
 
-struct BatchJob {
-    BlobHdr h;
-    const uint32_t *blob;
-    float4 *vals;
-    uint32_t *left, *conn, *lvl_sets, *counters;
-    unsigned long long *lvl_best;
-    uint32_t *lvl_first;
-    DPNode *nodes;
-    uint32_t lvl_off[33];
-    uint32_t conn_words;
-    int bicc;
-};
-
-#define TREE_ITERS 4
+#define TREE_ITERS 16
 
 /* sets per task of level k */
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
@@ -855,33 +1058,53 @@ __global__ void __launch_bounds__(BLOCK) k_batch_filter(const BatchJob *jobs, in
     }
 }
 
-/* task_prefix[k * (nj + 1) + j] = first task of job j at level k */
-__global__ void k_batch_plan(const BatchJob *jobs, int nj, int kmax, uint32_t *task_prefix) {
-    for (int k = 2 + (int)threadIdx.x; k <= kmax; k += blockDim.x) {
-        uint32_t acc = 0;
-        for (int j = 0; j < nj; j++) {
-            task_prefix[k * (nj + 1) + j] = acc;
-            const BatchJob &J = jobs[j];
-            if (k <= J.h.n_rels) {
-                uint32_t cnt = J.counters[k], spt = task_sets(J, k);
-                acc += (cnt + spt - 1) / spt;
-            }
-        }
-        task_prefix[k * (nj + 1) + nj] = acc;
-    }
+__device__ __forceinline__ uint32_t job_tasks(const BatchJob &J, int k) {
+    if (k > J.h.n_rels) return 0;
+    const uint32_t spt = task_sets(J, k);
+    return (J.counters[k] + spt - 1) / spt;
 }
 
+/* task_prefix[k * (nj + 1) + j] = first task of job j at level k; one
+ * workgroup per level k = 2 + blockIdx.x. */
+__global__ void __launch_bounds__(1024) k_batch_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix) {
+    __shared__ uint32_t part[17];
+    const int k = 2 + (int)blockIdx.x;
+    uint32_t *tp = task_prefix + (size_t)k * (nj + 1);
+    const uint32_t per = ((uint32_t)nj + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], k);
+    uint32_t total;
+    uint32_t acc = block_scan_excl(sum, part, &total);
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) {
+        tp[j] = acc;
+        acc += job_tasks(jobs[j], k);
+    }
+    if (threadIdx.x == 0) tp[nj] = total;
+}
+
+/* Level k of every job.  The task prefix of this level is staged in LDS at
+ * word `tp_lds` (0: read from global memory).  Non-tree MPDP sets with more
+ * than BIG_SPLITS splits go to the big-set list (`big`, `big_hdr`). */
 __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int nj, int k,
-                                                       const uint32_t *task_prefix) {
+                                                       const uint32_t *task_prefix, uint32_t tp_lds, BigEnt *big,
+                                                       uint32_t big_cap, uint32_t *big_hdr) {
     /* The dynamic LDS must be the kernel's only LDS: a static __shared__
      * variable would move its base off the 8-byte alignment that the 64-bit
      * LDS atomics of eval_level need.  Each thread therefore finds its job
      * itself (the search is uniform across the workgroup). */
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
-    const uint32_t *tp = task_prefix + k * (nj + 1);
-    const uint32_t total = tp[nj];
+    const uint32_t *tpg = task_prefix + (size_t)k * (nj + 1);
+    const uint32_t total = tpg[nj];
+    if (blockIdx.x >= total) return;
+    const uint32_t *tp = tpg;
+    if (tp_lds) {
+        uint32_t *tl = smem + tp_lds;
+        for (int i = threadIdx.x; i <= nj; i += BLOCK) tl[i] = tpg[i];
+        __syncthreads();
+        tp = tl;
+    }
     int have = -1;
-    uint32_t np = 0;
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nj;
         while (hi - lo > 1) {
@@ -904,23 +1127,33 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         const uint32_t first = (t - tp[j]) * spt, cnt = J.counters[k];
         const uint32_t m = cnt - first < spt ? cnt - first : spt;
         const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
+        uint32_t np;
         if (h.is_tree) {
-            np += eval_level_tree(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
-                                  BLOCK / WAVE);
+            np = eval_level_tree(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                                 BLOCK / WAVE);
         } else if (J.bicc) {
-            np += eval_level<true>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1);
+            BigList bl;
+            bl.ents = big;
+            bl.count = big_hdr;
+            bl.cap = big_cap;
+            bl.job = (uint32_t)j;
+            bl.gidx0 = J.lvl_off[k] + first;
+            np = eval_level<true>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1, bl);
         } else {
-            np += eval_level<false>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1);
+            BigList bl;
+            bl.ents = nullptr;
+            bl.count = nullptr;
+            bl.cap = 0;
+            bl.job = 0;
+            bl.gidx0 = 0;
+            np = eval_level<false>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1, bl);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t *misc = smem + ((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
             if (!h.is_tree && misc[1]) atomicOr(&J.counters[49], 1u);
         }
-        if (np) {
-            atomicAdd((unsigned long long *)(J.counters + 40), (unsigned long long)np);
-            np = 0;
-        }
+        add_pairs(J.counters, np);
     }
 }
 
@@ -1291,6 +1524,11 @@ struct DeviceCtx {
     BatchJob *jobs = nullptr;
     uint32_t *results = nullptr, *bprefix = nullptr;
     size_t cap_ws = 0, cap_bblob = 0, cap_jobs = 0, cap_results = 0, cap_bprefix = 0;
+    /* big MPDP sets (both paths) and the single-query job descriptor */
+    BigEnt *big_ents = nullptr;
+    uint32_t *big_prefix = nullptr;
+    BatchJob *sjob = nullptr;
+    size_t cap_big = 0, cap_big_prefix = 0, cap_sjob = 0;
     size_t cap_ds_best = 0, cap_ds_first = 0, cap_ds_lset = 0, cap_ds_lnbr = 0, cap_ds_raw = 0, cap_ds_tmp = 0,
            cap_ds_rc = 0, cap_ds_rf = 0, cap_ds_ro = 0, cap_ds_sz = 0, cap_l1 = 0;
     std::mutex mu;
@@ -1370,8 +1608,9 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     std::vector<BatchJob> jobs(nj);
     std::vector<uint32_t> blobs, chunk_prefix(nj + 1, 0);
     std::vector<size_t> ws_off(nj), blob_off(nj);
-    size_t ws = 0, lds = 0;
-    int kmax = 2;
+    size_t ws = 0, lds = 0, big_lds_max = 0;
+    uint64_t big_cap = 0;
+    int kmax = 2, big_kmax = 0;
     for (int j = 0; j < nj; j++) {
         const BlobHdr &h = ins[j]->hdr;
         if (!fused_eligible(*ins[j], algos[j])) {
@@ -1386,6 +1625,13 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         blobs.insert(blobs.end(), ins[j]->blob.begin(), ins[j]->blob.end());
         while (blobs.size() & 63) blobs.push_back(0u);
         lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
+        if (algos[j] == DP_BICC && !h.is_tree && h.n_rels >= 15) {
+            uint64_t mc = 0;
+            for (int k = 2; k <= h.n_rels; k++) mc = std::max(mc, binom64(h.n_rels, k));
+            big_cap += mc;
+            big_lds_max = std::max(big_lds_max, big_lds(h.words));
+            big_kmax = std::max(big_kmax, h.n_rels);
+        }
         chunk_prefix[j + 1] = chunk_prefix[j] + (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK);
         kmax = std::max(kmax, h.n_rels);
     }
@@ -1394,6 +1640,14 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     if ((rc = ensure(&c.jobs, c.cap_jobs, (size_t)nj))) return rc;
     if ((rc = ensure(&c.results, c.cap_results, (size_t)nj * RES_WORDS))) return rc;
     if ((rc = ensure(&c.bprefix, c.cap_bprefix, (size_t)(nj + 1) * 34))) return rc;
+    if (big_cap > 0xffffffffull) {
+        set_error("batch too large for the big-set list");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (big_cap) {
+        if ((rc = ensure(&c.big_ents, c.cap_big, big_cap))) return rc;
+        if ((rc = ensure(&c.big_prefix, c.cap_big_prefix, big_cap + 1))) return rc;
+    }
     for (int j = 0; j < nj; j++) {
         const BlobHdr &h = ins[j]->hdr;
         const uint64_t ns = 1ull << h.n_rels;
@@ -1432,7 +1686,11 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, c.jobs, nj,
                        c.bprefix);
     uint32_t *task_prefix = c.bprefix + (nj + 1);        /* [33][nj+1] after the chunk prefix */
-    hipLaunchKernelGGL(k_batch_plan, dim3(1), dim3(64), 0, s, c.jobs, nj, kmax, task_prefix);
+    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, task_prefix);
+    /* the level's task prefix goes to LDS after the evaluation area when it fits */
+    const uint32_t tp_lds = (lds + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
+    const size_t lds_eval_total = tp_lds ? lds + 4 * (size_t)(nj + 1) : lds;
+    uint32_t *big_hdr = c.results + 50;                  /* job 0's counters[50..51] */
     for (int k = 2; k <= kmax; k++) {
         /* grid: enough workgroups for the largest level, capped */
         uint64_t ub = 0;
@@ -1442,7 +1700,14 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             ub += binom64(h.n_rels, k);
         }
         uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
-        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds, s, c.jobs, nj, k, task_prefix);
+        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_eval_total, s, c.jobs, nj, k, task_prefix,
+                           tp_lds, big_cap ? c.big_ents : nullptr, (uint32_t)big_cap, big_hdr);
+        if (big_cap && k >= 15 && k <= big_kmax) {
+            hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, big_hdr, (uint32_t)big_cap,
+                               c.big_prefix);
+            hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s, c.jobs, c.big_ents, big_hdr,
+                               c.big_prefix);
+        }
     }
     hipLaunchKernelGGL(k_batch_tree, dim3((nj + 63) / 64), dim3(64), 0, s, c.jobs, nj);
     HIPCHK(hipGetLastError());
@@ -1645,6 +1910,31 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
     }
     HIPCHK(hipMemcpyAsync(c.lvl_off, off_h, sizeof(off_h), hipMemcpyHostToDevice, s));
+    /* MPDP on a graph with cycles of 15 or more relations: sets with a big
+     * block are deferred to the chunk-parallel big-set kernels, which see
+     * this query as a batch of one. */
+    const bool use_big = algo == DP_BICC && !h.is_tree && n >= 15;
+    BatchJob sj;                       /* lives until the stream is synchronised */
+    memset(&sj, 0, sizeof(sj));
+    if (use_big) {
+        if ((rc = ensure(&c.big_ents, c.cap_big, maxC))) return rc;
+        if ((rc = ensure(&c.big_prefix, c.cap_big_prefix, maxC + 1))) return rc;
+        if ((rc = ensure(&c.sjob, c.cap_sjob, 1))) return rc;
+        sj.h = h;
+        sj.blob = c.b.blob;
+        sj.vals = c.b.vals;
+        sj.left = c.b.left;
+        sj.conn = c.b.conn;
+        sj.lvl_sets = c.b.lvl_sets;
+        sj.counters = c.b.counters;
+        sj.lvl_best = c.b.lvl_best;
+        sj.lvl_first = c.b.lvl_first;
+        sj.nodes = c.b.nodes;
+        memcpy(sj.lvl_off, off_h, sizeof(off_h));
+        sj.conn_words = conn_words;
+        sj.bicc = 1;
+        HIPCHK(hipMemcpyAsync(c.sjob, &sj, sizeof(sj), hipMemcpyHostToDevice, s));
+    }
     const uint64_t cand_total = nsets_all - (uint64_t)n - 1ull;
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[0], s));
     {
@@ -1673,12 +1963,31 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
+            BigList bl;
+            bl.ents = nullptr;
+            bl.count = nullptr;
+            bl.cap = 0;
+            bl.job = 0;
+            bl.gidx0 = 0;
             if (algo == DP_BICC) {
+                const bool big = use_big && k >= 15;
+                if (big) {
+                    bl.ents = c.big_ents;
+                    bl.count = c.b.counters + 50;
+                    bl.cap = (uint32_t)maxC;
+                    bl.gidx0 = off_h[k];
+                }
                 hipLaunchKernelGGL(k_eval_wg<true>, dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k, B, 0,
-                                   sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters);
+                                   sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters, bl);
+                if (big) {
+                    hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, c.b.counters + 50,
+                                       (uint32_t)maxC, c.big_prefix);
+                    hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, c.sjob, c.big_ents,
+                                       c.b.counters + 50, c.big_prefix);
+                }
             } else {
                 hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, c.b.blob, k, B,
-                                   conn_lds, sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters);
+                                   conn_lds, sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters, bl);
             }
         } else {
             uint32_t ipl = 16;

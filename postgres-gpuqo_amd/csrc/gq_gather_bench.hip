@@ -73,3 +73,38 @@ extern "C" int gq_gather_bench(uint64_t footprint, uint32_t record_bytes, uint64
     *ms_out = ms;
     return 0;
 }
+
+/* Plain streaming copy of `bytes` (a multiple of 16): the calibration point
+ * for hardware byte counters -- its algorithmic traffic is exactly `bytes`
+ * read and `bytes` written per launch. */
+__global__ void __launch_bounds__(256) k_copy(const uint4 *src, uint4 *dst, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int gq_copy_bench(uint64_t bytes, int reps, double *gbps, double *ms_out) {
+    if (bytes < 16 || (bytes & 15) || reps < 1) return 1;
+    const uint64_t n = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess) return 3;
+    if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return 3; }
+    (void)hipMemset(a, 0x3c, bytes);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const uint32_t blocks = 4096;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);   /* warm-up */
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    *ms_out = ms / reps;
+    *gbps = 2.0 * (double)bytes / (*ms_out * 1e-3) / 1e9;
+    return 0;
+}

@@ -255,6 +255,35 @@ def cycle(n, seed):
     return chain(n, seed, cycle=True)
 
 
+def graph(n, edges, seed, shape="graph"):
+    """Arbitrary join graph over the given (u, v) edges: rows logU[1e3,1e7],
+    selectivity logU[1e-5,1e-1] per edge, in edge order."""
+    r = Rng(seed)
+    g = QueryGraph(n, shape)
+    for i in range(n):
+        g.rows[i] = rows_of(r.logu(1e3, 1e7))
+    for (u, v) in edges:
+        g.add_generic(u, v, r.logu(1e-5, 1e-1))
+    return g.build()
+
+
+def cycle_with_tail(c, t, seed):
+    """A cycle of c relations with a path of t more hanging off relation 0."""
+    edges = [(i, i + 1) for i in range(c - 1)] + [(c - 1, 0)]
+    prev = 0
+    for i in range(c, c + t):
+        edges.append((prev, i))
+        prev = i
+    return graph(c + t, edges, seed, "cycle+tail")
+
+
+def chorded_cycle(n, step, seed):
+    """A cycle with chords (i, i + step): one biconnected block."""
+    edges = [(i, (i + 1) % n) for i in range(n)]
+    edges += [(i, i + step) for i in range(0, n - step, 2)]
+    return graph(n, edges, seed, "chorded-cycle")
+
+
 SEED_BASE = 0x6770713000000000
 
 

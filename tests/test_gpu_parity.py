@@ -139,3 +139,34 @@ def test_fused_and_per_level_paths_agree():
             want, _ = oracle_bits(g, c)
             got, _ = gpu_bits(g, c)
             assert_same_plan(got, want, f"{g.shape}{n}")
+
+
+def _big_block_graphs():
+    return [("cycle18", qgen.cycle(18, 0x1801)), ("cycle22", qgen.cycle(22, 0x2201)),
+            ("cycle16+tail4", qgen.cycle_with_tail(16, 4, 0x1604)),
+            ("chorded15", qgen.chorded_cycle(15, 5, 0x1505)), ("chorded17", qgen.chorded_cycle(17, 3, 0x1717))]
+
+
+@pytest.mark.parametrize("name,g", _big_block_graphs(), ids=lambda x: x if isinstance(x, str) else "")
+def test_mpdp_big_blocks(name, g):
+    """MPDP sets whose biconnected block has 15+ relations take the
+    chunk-parallel big-set path."""
+    c = cfg("dpsub", 1)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, name)
+    assert gst.pairs == ost.pairs
+    assert gst.sets == ost.sets
+
+
+def test_batch_with_big_blocks():
+    """Big-block MPDP sets of several queries in one batch."""
+    import gpuqo_amd
+    qs = [qgen.cycle(16, 0x1610), qgen.star(12, 0x1211), qgen.cycle_with_tail(16, 3, 0x1612),
+          qgen.cycle(20, 0x2013), qgen.snowflake(14, 0x1414), qgen.chain(17, 0x1715), qgen.cycle(15, 0x1516)]
+    c = cfg("dpsub", 1)
+    res = gpuqo_amd.plan_batch(qs, c)
+    for g, (nodes, st) in zip(qs, res):
+        want, ost = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"batch {g.shape}{g.n}")
+        assert st.pairs == ost.pairs, f"{g.shape}{g.n}"
