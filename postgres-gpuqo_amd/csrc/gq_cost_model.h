@@ -163,6 +163,14 @@ GQ_HD float page_size(float tuples, int width) {            /* :832-840 */
 
 struct HashSize { int nbuckets, nbatches; };
 
+/* a / b for long operands; the same quotient through a 32-bit unsigned
+ * divide when both are in [0, 2^32) (a 64-bit divide is a long software
+ * sequence on the GPU). */
+GQ_HD long div_l(long a, long b) {
+    if (a >= 0 && b > 0 && a <= 0xffffffffL && b <= 0xffffffffL) return (long)((uint32_t)a / (uint32_t)b);
+    return a / b;
+}
+
 /* ExecChooseHashTableSize with NTUP_PER_BUCKET 1, SKEW_WORK_MEM_PERCENT 2,
  * SKEW_OVERHEAD 84, TUPLE_OVERHEAD 32, MaxAllocSize 0x3fffffff and 8-byte
  * pointers (cost_postgres.cuh:472-618).  min/max on float are fminf/fmaxf. */
@@ -175,7 +183,7 @@ GQ_HD HashSize hash_table_size(float ntuples, int tupwidth, int work_mem) {
     long hash_table_bytes = (long)work_mem * 1024L;
     float space_allowed = (float)hash_table_bytes;
     long skew_table_bytes = hash_table_bytes * 2 / 100;
-    long num_skew_mcvs = skew_table_bytes / (long)(tupsize + 84);
+    long num_skew_mcvs = div_l(skew_table_bytes, (long)(tupsize + 84));
     if ((int)num_skew_mcvs > 0) hash_table_bytes -= skew_table_bytes;
     long max_pointers = (long)(space_allowed / 8.0f);
     const long alloc_ptrs = 0x3fffffffL / 8;
@@ -191,7 +199,7 @@ GQ_HD HashSize hash_table_size(float ntuples, int tupwidth, int work_mem) {
     long bucket_bytes = 8L * (long)nb;
     if (inner_rel_bytes + (float)bucket_bytes > (float)hash_table_bytes) {
         long bucket_size = (long)tupsize + 8L;
-        long lbuckets = 1L << ceil_log2_l(hash_table_bytes / bucket_size);
+        long lbuckets = 1L << ceil_log2_l(div_l(hash_table_bytes, bucket_size));
         if (max_pointers < lbuckets) lbuckets = max_pointers;
         nb = (int)lbuckets;
         nb = 1 << ceil_log2_l(nb);
@@ -207,24 +215,25 @@ GQ_HD HashSize hash_table_size(float ntuples, int tupwidth, int work_mem) {
 }
 
 /* Match mask of the ECs with members on both sides. */
-template <class S>
+template <int ECW = 0, class S>
 GQ_HD int match_mask(const View<S> &q, const S &l, const S &r, uint64_t *m) {
+    const int nw = ECW ? ECW : q.ecw;        /* ECW: class-mask words fixed at compile time (0: q.ecw) */
     uint64_t ml[GQ_MAX_ECW], mr[GQ_MAX_ECW];
-    for (int w = 0; w < q.ecw; w++) { ml[w] = 0; mr[w] = 0; }
+    for (int w = 0; w < nw; w++) { ml[w] = 0; mr[w] = 0; }
     S t = l;
     while (!empty(t)) {
         int i = ctz(t);
         t = t - lowbit(t);
-        for (int w = 0; w < q.ecw; w++) ml[w] |= q.inc[i * q.ecw + w];
+        for (int w = 0; w < nw; w++) ml[w] |= q.inc[i * nw + w];
     }
     t = r;
     while (!empty(t)) {
         int i = ctz(t);
         t = t - lowbit(t);
-        for (int w = 0; w < q.ecw; w++) mr[w] |= q.inc[i * q.ecw + w];
+        for (int w = 0; w < nw; w++) mr[w] |= q.inc[i * nw + w];
     }
     int n = 0;
-    for (int w = 0; w < q.ecw; w++) { m[w] = ml[w] & mr[w]; n += popc(m[w]); }
+    for (int w = 0; w < nw; w++) { m[w] = ml[w] & mr[w]; n += popc(m[w]); }
     return n;
 }
 
@@ -259,9 +268,10 @@ GQ_HD float ec_selectivity(const View<S> &q, int e, const S &l, const S &r) {
 }
 
 /* Visit the classes of mask `m` in ascending index. */
-template <class S, class F>
+template <int ECW = 0, class S, class F>
 GQ_HD void for_each_ec(const View<S> &q, const uint64_t *m, F f) {
-    for (int w = 0; w < q.ecw; w++) {
+    const int nw = ECW ? ECW : q.ecw;
+    for (int w = 0; w < nw; w++) {
         uint64_t b = m[w];
         while (b) {
             int e = w * 64 + __builtin_ctzll(b);
@@ -272,10 +282,10 @@ GQ_HD void for_each_ec(const View<S> &q, const uint64_t *m, F f) {
 }
 
 /* estimate_join_selectivity (row_estimation.cuh:90-117) */
-template <class S>
+template <int ECW = 0, class S>
 GQ_HD float join_selectivity(const View<S> &q, const S &l, const S &r, const uint64_t *m) {
     float sel = 1.0f;
-    for_each_ec(q, m, [&](int e) { sel *= ec_selectivity(q, e, l, r); });
+    for_each_ec<ECW>(q, m, [&](int e) { sel *= ec_selectivity(q, e, l, r); });
     return sel;
 }
 
@@ -347,7 +357,7 @@ GQ_HD float bucketsize(const View<S> &q, int mi, int nbuckets) {
 }
 
 /* cost_hashjoin (cost_postgres.cuh:287-470) */
-template <class S>
+template <int ECW = 0, class S>
 GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const Rel &inner, const Extra &ex,
                     int n_quals, float qual_per_tuple, const uint64_t *m, float *o_startup, float *o_total) {
     float startup_cost = 0.0f, run_cost = 0.0f;
@@ -369,7 +379,7 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
     int nbk = f2i_x86(virtualbuckets);
     /* estimate_hash_bucket_stats (:620-668) */
     float bsize = 1.0f, mcvfreq = 1.0f;
-    for_each_ec(q, m, [&](int e) {
+    for_each_ec<ECW>(q, m, [&](int e) {
         S ec = q.ec_relids[e];
         S mr = ec & inner_id;
         while (!empty(mr)) {
@@ -405,19 +415,30 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
 
 /* One join: rows (estimate_join_rows) and cost (calc_join_cost,
  * gpuqo_cost.cuh:76-158) of outer `l` joined with inner `r`. */
-template <class S>
+template <int ECW = 0, class S>
+GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const uint64_t *m,
+                    int n_quals);
+
+template <int ECW = 0, class S>
 GQ_HD Rel join(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R) {
     uint64_t m[GQ_MAX_ECW];
-    int n_quals = match_mask(q, l, r, m);
+    int n_quals = match_mask<ECW>(q, l, r, m);
+    return join_mask<ECW>(q, l, L, r, R, m, n_quals);
+}
+
+/* The same join with the crossing-class mask `m` (n_quals classes) given. */
+template <int ECW, class S>
+GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const uint64_t *m,
+                    int n_quals) {
     Rel out;
-    out.rows = join_rows(join_selectivity(q, l, r, m), L.rows, R.rows);
+    out.rows = join_rows(join_selectivity<ECW>(q, l, r, m), L.rows, R.rows);
     out.width = L.width + R.width;
     float qual_pt = q.qualtab[n_quals];
     bool single = popc(r) == 1;
     Extra ex;
     ex.inner_unique = false;
     if (single) {                                   /* is_inner_unique :43-74 */
-        for_each_ec(q, m, [&](int e) {
+        for_each_ec<ECW>(q, m, [&](int e) {
             int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
             if ((q.mem_flags[mi] & (GQ_MF_INDEX | GQ_MF_UNIQUE)) == (GQ_MF_INDEX | GQ_MF_UNIQUE))
                 ex.inner_unique = true;
@@ -431,12 +452,12 @@ GQ_HD Rel join(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel
     nestloop(q, L, R.rows, R.startup, R.total, ex, qual_pt, &bs, &bt);
     if (q.p.enable_hashjoin) {
         float hs, ht;
-        hashjoin(q, r, L, R, ex, n_quals, qual_pt, m, &hs, &ht);
+        hashjoin<ECW>(q, r, L, R, ex, n_quals, qual_pt, m, &hs, &ht);
         if (ht < bt) { bs = hs; bt = ht; }
     }
     if (single && !bit(q.composite, ctz(r))) {       /* index nested loop :116-147 */
         ex.indexed = true;
-        for_each_ec(q, m, [&](int e) {
+        for_each_ec<ECW>(q, m, [&](int e) {
             int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
             if (q.mem_flags[mi] & GQ_MF_INDEX) {
                 float is, it;

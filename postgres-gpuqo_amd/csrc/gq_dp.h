@@ -29,6 +29,10 @@ struct BlobHdr {
      * holds each vertex's descendants (itself included), o_lglob the vertices
      * of the subtrees of smaller siblings of v and of its ancestors. */
     int is_tree, o_desc, o_lglob;
+    /* Trees whose every class of two or more members is one tree edge: the
+     * classes crossing the split at edge (parent(c), c) are exactly those on
+     * that edge; o_ecm holds their mask (ecw 64-bit words) per vertex c. */
+    int tree_ecm, o_ecm;
 };
 
 struct DPInput {

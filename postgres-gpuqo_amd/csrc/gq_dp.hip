@@ -503,21 +503,20 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
  * and the first minimum of the packed (cost, rank) key with shuffles; the
  * winning lane writes its own result to the memo.  Waves w0, w0 + wstride,
  * ... cover the level. */
+template <int ECW>
 __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
                                     bool bicc, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
                                     uint32_t *left, uint64_t w0, uint64_t wstride) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t per = 2u * (k - 1);
-    uint32_t P = 1;
-    while (P < per) P <<= 1;
-    const uint32_t spw = WAVE / P;
-    const uint32_t seg = lane / P, t = lane % P;
+    const uint32_t spw = WAVE / per;                    /* sets per wave, per consecutive lanes each */
+    const uint32_t seg = lane / per, t = lane % per;
     const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
     const uint64_t nwaves = (count + spw - 1) / spw;
     uint32_t npairs = 0;
     for (uint64_t w = w0; w < nwaves; w += wstride) {
         const uint64_t idx = w * spw + seg;
-        const bool active = idx < count && t < per;
+        const bool active = idx < count && seg < spw;
         uint32_t first = ~0u;
         unsigned long long key = ~0ull;
         Rel jr;
@@ -531,20 +530,39 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
             L = o ? sub : (S ^ sub);
             const uint32_t R = S ^ L;
             const uint32_t rank = bicc ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
-            jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
+            uint64_t m[GQ_MAX_ECW];
+            int nq = 0;
+            const int nw = ECW ? ECW : h.ecw;
+            if (h.tree_ecm) {
+                const uint64_t *em = (const uint64_t *)(smem + h.o_ecm) + c * nw;
+                for (int w = 0; w < nw; w++) {
+                    m[w] = em[w];
+                    nq += __popcll(m[w]);
+                }
+            } else {
+                nq = match_mask<ECW>(q, L, R, m);
+            }
+            jr = join_mask<ECW>(q, L, load_rel(vals, L), R, load_rel(vals, R), m, nq);
             npairs++;
             const bool nan = jr.total != jr.total;
             first = (rank << 1) | (nan ? 1u : 0u);
             if (!nan) key = ((unsigned long long)ord_key(jr.total) << 32) | rank;
         }
+        /* segmented min over the set's lanes: after the step with offset
+         * `off`, lane t holds the minimum over [t, min(t + 2 off, per)) */
         uint32_t sf = first;
         unsigned long long sk = key;
-        for (uint32_t off = P >> 1; off > 0; off >>= 1) {
-            uint32_t f2 = __shfl_xor(sf, (int)off);
-            unsigned long long k2 = __shfl_xor(sk, (int)off);
-            sf = f2 < sf ? f2 : sf;
-            sk = k2 < sk ? k2 : sk;
+        for (uint32_t off = 1; off < per; off <<= 1) {
+            const uint32_t f2 = __shfl_down(sf, off);
+            const unsigned long long k2 = __shfl_down(sk, off);
+            if (t + off < per) {
+                sf = f2 < sf ? f2 : sf;
+                sk = k2 < sk ? k2 : sk;
+            }
         }
+        const int head = (int)(seg * per);               /* the set's first lane holds its minimum */
+        sf = __shfl(sf, head < WAVE ? head : 0);
+        sk = __shfl(sk, head < WAVE ? head : 0);
         if (active) {
             const bool win = (sf & 1u) ? (first == sf) : (key == sk);
             if (win) {
@@ -556,6 +574,7 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
     return npairs;
 }
 
+template <int ECW>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, float4 *vals, uint32_t *left,
                                                       uint32_t *counters) {
@@ -564,7 +583,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    uint32_t np = eval_level_tree(h, smem, q, k, bicc != 0, lvl_sets, counters[k], vals, left, w0,
+    uint32_t np = eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, counters[k], vals, left, w0,
                                   (uint64_t)gridDim.x * (BLOCK / WAVE));
     add_pairs(counters, np);
 }
@@ -1006,11 +1025,7 @@ __host__ __device__ constexpr size_t big_lds(int words) {
 
 /* sets per task of level k */
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
-    if (J.h.is_tree) {
-        uint32_t per = 2u * (k - 1), P = 1;
-        while (P < per) P <<= 1;
-        return (BLOCK / WAVE) * (WAVE / P) * TREE_ITERS;
-    }
+    if (J.h.is_tree) return (BLOCK / WAVE) * (WAVE / (2u * (k - 1))) * TREE_ITERS;
     uint64_t items = J.bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
     uint64_t B = (768 + items - 1) / items;
     return (uint32_t)(B < 1 ? 1 : (B > BMAX ? BMAX : B));
@@ -1129,8 +1144,12 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
         uint32_t np;
         if (h.is_tree) {
-            np = eval_level_tree(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
-                                 BLOCK / WAVE);
+            if (h.ecw == 1)
+                np = eval_level_tree<1>(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                                        BLOCK / WAVE);
+            else
+                np = eval_level_tree<0>(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                                        BLOCK / WAVE);
         } else if (J.bicc) {
             BigList bl;
             bl.ents = big;
@@ -1157,46 +1176,92 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
     }
 }
 
-/* Pre-order plan walk from `root_set` (dpsub_buildQueryTree,
- * gpuqo_query_tree.cuh:46-72); returns the node count or ~0 on an
- * inconsistent memo. */
-__device__ uint32_t walk_tree(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes, int cap) {
-    uint32_t st_set[66];
-    int st_parent[66], st_side[66];
-    int sp = 0, n = 0;
-    st_set[sp] = root_set; st_parent[sp] = -1; st_side[sp] = 0; sp++;
-    while (sp > 0) {
-        sp--;
-        uint32_t S = st_set[sp];
-        int par = st_parent[sp], side = st_side[sp];
-        if (n >= cap) return ~0u;
-        int me = n++;
-        float4 v = vals[S];
-        DPNode nd;
-        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
-        nd.left = -1; nd.right = -1; nd.pad = 0;
-        nodes[me] = nd;
-        if (par >= 0) {
-            if (side == 0) nodes[par].left = me;
-            else nodes[par].right = me;
+/* Plan walk from `root_set` (dpsub_buildQueryTree, gpuqo_query_tree.cuh:46-72)
+ * by one wave: the tree is expanded breadth-first, one round per tree level
+ * (the only dependent loads are left[S] of the round's inner nodes), then
+ * lane 0 numbers the nodes in pre-order -- node, left subtree, right
+ * subtree, the order of the reference's tree -- and the lanes write the
+ * nodes with their values.  Returns the node count, or ~0 when the memo is
+ * inconsistent (a left side that is empty, not a proper subset, or a tree of
+ * more than `cap` nodes).  cap <= 63.  Called by all 64 lanes of a
+ * workgroup of one wave. */
+__device__ uint32_t walk_tree_wave(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes,
+                                   int cap) {
+    __shared__ uint32_t s_set[64];
+    __shared__ int s_l[64], s_r[64], s_pre[64], s_stack[64];
+    __shared__ int s_bad;
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        s_set[0] = root_set;
+        s_bad = cap < 1 || cap > 63;
+    }
+    __syncthreads();
+    int f0 = 0, f1 = 1;
+    while (f0 < f1 && !s_bad) {
+        const int i = f0 + lane;
+        bool inner = false;
+        uint32_t S = 0, l = 0;
+        if (i < f1) {
+            S = s_set[i];
+            s_l[i] = -1;
+            s_r[i] = -1;
+            if (__builtin_popcount(S) > 1) {
+                l = left[S];
+                if (l == 0 || (l & ~S) != 0 || l == S) s_bad = 1;
+                else inner = true;
+            }
         }
-        if (__builtin_popcount(S) > 1) {
-            uint32_t l = left[S], r = S ^ l;
-            if (l == 0 || (l & ~S) != 0 || l == S || sp + 2 > 64) return ~0u;
-            st_set[sp] = r; st_parent[sp] = me; st_side[sp] = 1; sp++;
-            st_set[sp] = l; st_parent[sp] = me; st_side[sp] = 0; sp++;
+        const unsigned long long m = __ballot(inner);
+        const int newn = f1 + 2 * (int)__popcll(m);
+        if (newn > cap) {
+            if (lane == 0) s_bad = 1;
+        } else if (inner) {
+            const int at = f1 + 2 * (int)__popcll(m & ((1ull << lane) - 1ull));
+            s_set[at] = l;
+            s_set[at + 1] = S ^ l;
+            s_l[i] = at;
+            s_r[i] = at + 1;
+        }
+        __syncthreads();
+        f0 = f1;
+        f1 = newn;
+    }
+    const int total = f0;
+    if (s_bad) return ~0u;
+    if (lane == 0) {
+        int sp = 0, cnt = 0;
+        s_stack[sp++] = 0;
+        while (sp > 0) {
+            const int x = s_stack[--sp];
+            s_pre[x] = cnt++;
+            if (s_l[x] >= 0) {
+                s_stack[sp++] = s_r[x];
+                s_stack[sp++] = s_l[x];
+            }
         }
     }
-    return (uint32_t)n;
+    __syncthreads();
+    if (lane < total) {
+        const uint32_t S = s_set[lane];
+        const float4 v = vals[S];
+        DPNode nd;
+        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
+        nd.left = s_l[lane] >= 0 ? s_pre[s_l[lane]] : -1;
+        nd.right = s_r[lane] >= 0 ? s_pre[s_r[lane]] : -1;
+        nd.pad = 0;
+        nodes[s_pre[lane]] = nd;
+    }
+    return (uint32_t)total;
 }
 
-__global__ void k_batch_tree(const BatchJob *jobs, int nj) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(WAVE) k_batch_tree(const BatchJob *jobs, int nj) {
+    const int j = blockIdx.x;
     if (j >= nj) return;
     const BatchJob &J = jobs[j];
     const int n = J.h.n_rels;
-    uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    J.counters[48] = walk_tree(J.vals, J.left, full, J.nodes, 2 * n - 1);
+    const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    const uint32_t r = walk_tree_wave(J.vals, J.left, full, J.nodes, 2 * n - 1);
+    if (threadIdx.x == 0) J.counters[48] = r;
 }
 
 /* ================================================================ DPsize
@@ -1462,38 +1527,10 @@ __global__ void __launch_bounds__(BLOCK) k_ds_finalize(BlobHdr h, const uint32_t
 
 /* Pre-order walk of the plan from `root_set` (dpsub_buildQueryTree,
  * gpuqo_query_tree.cuh:46-72, done on the device in one pass). */
-__global__ void k_tree(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes, int cap,
-                       uint32_t *status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint32_t st_set[66];
-    int st_parent[66], st_side[66];
-    int sp = 0, n = 0;
-    uint32_t bad = 0;
-    st_set[sp] = root_set; st_parent[sp] = -1; st_side[sp] = 0; sp++;
-    while (sp > 0) {
-        sp--;
-        uint32_t S = st_set[sp];
-        int par = st_parent[sp], side = st_side[sp];
-        if (n >= cap) { bad = 1; break; }
-        int me = n++;
-        float4 v = vals[S];
-        DPNode nd;
-        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
-        nd.left = -1; nd.right = -1; nd.pad = 0;
-        nodes[me] = nd;
-        if (par >= 0) {
-            if (side == 0) nodes[par].left = me;
-            else nodes[par].right = me;
-        }
-        if (__builtin_popcount(S) > 1) {
-            uint32_t l = left[S], r = S ^ l;
-            /* the left child must be a proper non-empty subset */
-            if (l == 0 || (l & ~S) != 0 || l == S || sp + 2 > 64) { bad = 1; break; }
-            st_set[sp] = r; st_parent[sp] = me; st_side[sp] = 1; sp++;
-            st_set[sp] = l; st_parent[sp] = me; st_side[sp] = 0; sp++;
-        }
-    }
-    status[0] = bad ? 0xffffffffu : (uint32_t)n;
+__global__ void __launch_bounds__(WAVE) k_tree(const float4 *vals, const uint32_t *left, uint32_t root_set,
+                                               DPNode *nodes, int cap, uint32_t *status) {
+    const uint32_t r = walk_tree_wave(vals, left, root_set, nodes, cap);
+    if (threadIdx.x == 0) status[0] = r;
 }
 
 /* ------------------------------------------------------------ host side */
@@ -1504,6 +1541,8 @@ static uint64_t binom64(int n, int k) {
     for (int i = 1; i <= k; i++) r = r * (uint64_t)(n - k + i) / (uint64_t)i;
     return r;
 }
+
+#define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
 
 struct DeviceCtx {
     int device = -1;
@@ -1529,6 +1568,9 @@ struct DeviceCtx {
     uint32_t *big_prefix = nullptr;
     BatchJob *sjob = nullptr;
     size_t cap_big = 0, cap_big_prefix = 0, cap_sjob = 0;
+    /* pinned host staging: query blob + level offsets in, counters + plan out */
+    uint32_t *h_stage = nullptr, *h_res = nullptr;
+    size_t cap_h_stage = 0;
     size_t cap_ds_best = 0, cap_ds_first = 0, cap_ds_lset = 0, cap_ds_lnbr = 0, cap_ds_raw = 0, cap_ds_tmp = 0,
            cap_ds_rc = 0, cap_ds_rf = 0, cap_ds_ro = 0, cap_ds_sz = 0, cap_l1 = 0;
     std::mutex mu;
@@ -1573,9 +1615,26 @@ static int ctx_init(DeviceCtx &c, int device) {
     HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tab, sizeof(tab)));
     c.device = device;
     c.binom_ready = true;
-    int rc = ensure(&c.b.counters, c.cap_counters, 64);
+    /* counters[0..63] and the plan nodes right after them: one read-back */
+    int rc = ensure(&c.b.counters, c.cap_counters, RES_WORDS);
     if (rc) return rc;
-    return ensure(&c.b.nodes, c.cap_nodes, 2 * 32);
+    c.b.nodes = (DPNode *)(c.b.counters + 64);
+    HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocDefault));
+    return GQ_OK;
+}
+
+/* pinned staging buffer of at least `words` words */
+static int ensure_stage(DeviceCtx &c, size_t words) {
+    if (c.h_stage && c.cap_h_stage >= words) return GQ_OK;
+    if (c.h_stage) (void)hipHostFree(c.h_stage);
+    c.h_stage = nullptr;
+    c.cap_h_stage = 0;
+    if (hipHostMalloc((void **)&c.h_stage, words * 4, hipHostMallocDefault) != hipSuccess) {
+        set_error("pinned staging allocation of %zu bytes failed", words * 4);
+        return GQ_ERR_NOMEM;
+    }
+    c.cap_h_stage = words;
+    return GQ_OK;
 }
 
 int device_init(int device) {
@@ -1586,7 +1645,6 @@ int device_init(int device) {
 
 /* ------------------------------------------------------------ batches */
 
-#define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 nodes per job */
 
 bool fused_eligible(const DPInput &in, DPAlgo algo) {
     const BlobHdr &h = in.hdr;
@@ -1709,7 +1767,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
                                c.big_prefix);
         }
     }
-    hipLaunchKernelGGL(k_batch_tree, dim3((nj + 63) / 64), dim3(64), 0, s, c.jobs, nj);
+    hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, c.jobs, nj);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
     std::vector<uint32_t> res((size_t)nj * RES_WORDS);
@@ -1786,9 +1844,12 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, nsets_all))) return rc;
     if ((rc = ensure(&c.b.lvl_best, c.cap_best, nsets_all))) return rc;
     if ((rc = ensure(&c.b.lvl_first, c.cap_first, nsets_all))) return rc;
-    if ((rc = ensure(&c.lvl_off, c.cap_lvl_off, 33))) return rc;
-    if ((rc = ensure(&c.b.blob, c.cap_blob, (size_t)h.words))) return rc;
-    if ((rc = ensure(&c.b.nodes, c.cap_nodes, (size_t)2 * n))) return rc;
+    /* device blob area: the query blob, then (at a 4-word boundary) the 33
+     * level-list offsets; both go up in one copy */
+    const size_t W4h = ((size_t)h.words + 3) & ~(size_t)3;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33))) return rc;
+    if ((rc = ensure_stage(c, W4h + 33))) return rc;
+    c.lvl_off = c.b.blob + W4h;
 
     /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
     if (profile && (int)c.prof_ev.size() < 3 * 33) {
@@ -1799,7 +1860,20 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
     }
     hipStream_t s = c.stream;
-    HIPCHK(hipMemcpyAsync(c.b.blob, in.blob.data(), blob_bytes, hipMemcpyHostToDevice, s));
+    /* level lists: level k at offset sum_{2<=j<k} C(n,j) */
+    uint32_t off_h[33];
+    memset(off_h, 0, sizeof(off_h));
+    {
+        uint64_t acc = 0;
+        for (int k = 2; k <= n; k++) {
+            off_h[k] = (uint32_t)acc;
+            acc += binom64(n, k);
+        }
+    }
+    memcpy(c.h_stage, in.blob.data(), blob_bytes);
+    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
+    memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
+    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(c.ev0, s));
     uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
     {
@@ -1869,10 +1943,11 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c.ev1, s));
         tree.resize(2 * n - 1);
-        uint32_t cnt[64];
-        HIPCHK(hipMemcpyAsync(tree.data(), c.b.nodes, sizeof(DPNode) * (2 * n - 1), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(cnt, c.b.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        const uint32_t *cnt = c.h_res;
+        HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+                              hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
         if (cnt[48] != (uint32_t)(2 * n - 1)) {
             set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
             return GQ_ERR_DEVICE;
@@ -1899,17 +1974,6 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         st.memo_kind = 1;
         return GQ_OK;
     }
-    /* level lists: level k at offset sum_{2<=j<k} C(n,j) */
-    uint32_t off_h[33];
-    memset(off_h, 0, sizeof(off_h));
-    {
-        uint64_t acc = 0;
-        for (int k = 2; k <= n; k++) {
-            off_h[k] = (uint32_t)acc;
-            acc += binom64(n, k);
-        }
-    }
-    HIPCHK(hipMemcpyAsync(c.lvl_off, off_h, sizeof(off_h), hipMemcpyHostToDevice, s));
     /* MPDP on a graph with cycles of 15 or more relations: sets with a big
      * block are deferred to the chunk-parallel big-set kernels, which see
      * this query as a batch of one. */
@@ -1953,12 +2017,15 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         const uint64_t T = (1ull << k) - 2;
         if (h.is_tree) {
-            uint64_t per = 2ull * (k - 1), P = 1;
-            while (P < per) P <<= 1;
-            uint64_t waves = (Ck + (WAVE / P) - 1) / (WAVE / P);
+            const uint64_t spw = WAVE / (2ull * (k - 1));
+            uint64_t waves = (Ck + spw - 1) / spw;
             uint32_t tblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
-            hipLaunchKernelGGL(k_eval_tree, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                               algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
+            if (h.ecw == 1)
+                hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                                   algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
+            else
+                hipLaunchKernelGGL(k_eval_tree<0>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
+                                   algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
@@ -2008,10 +2075,11 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(2 * n - 1);
-    uint32_t cnt[64];
-    HIPCHK(hipMemcpyAsync(tree.data(), c.b.nodes, sizeof(DPNode) * (2 * n - 1), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(cnt, c.b.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    const uint32_t *cnt = c.h_res;
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+                          hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
     if (cnt[49] != 0) {
         set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");
         return GQ_ERR_UNSUPPORTED;

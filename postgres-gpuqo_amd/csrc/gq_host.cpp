@@ -220,6 +220,8 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
     h.o_desc = take(32);
     h.o_lglob = take(32);
+    h.tree_ecm = 0;
+    h.o_ecm = 0;
     h.is_tree = n >= 2 ? 1 : 0;
     for (int v = 1; v < n; v++)
         if (popc((uint32_t)q.adj[v].w[0] & ((1u << v) - 1u)) != 1) h.is_tree = 0;
@@ -240,6 +242,29 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
         for (int v = 0; v < n; v++) {
             b[h.o_desc + v] = desc[v];
             b[h.o_lglob + v] = lg[v];
+        }
+        /* crossing classes per tree edge, when every class is on one edge */
+        std::vector<uint64_t> ecm((size_t)n * ecw, 0ull);
+        bool ok = true;
+        for (int e = 0; e < n_ec && ok; e++) {
+            const uint32_t rel = (uint32_t)q.ec_relids[e].w[0];
+            const int pc = popc(rel);
+            if (pc < 2) continue;
+            if (pc != 2) { ok = false; break; }
+            const int u = ctz(rel), v = 31 - __builtin_clz(rel);   /* u < v */
+            if (parent[v] != u) { ok = false; break; }
+            ecm[(size_t)v * ecw + e / 64] |= 1ull << (e % 64);
+        }
+        if (ok) {
+            align2();
+            h.o_ecm = take(2 * n * ecw);
+            for (int v = 0; v < n; v++)
+                for (int w = 0; w < ecw; w++) {
+                    const uint64_t x = ecm[(size_t)v * ecw + w];
+                    b[h.o_ecm + 2 * (v * ecw + w)] = (uint32_t)x;
+                    b[h.o_ecm + 2 * (v * ecw + w) + 1] = (uint32_t)(x >> 32);
+                }
+            h.tree_ecm = 1;
         }
     }
     h.o_base = take(4 * n);
