@@ -521,12 +521,20 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
             gq::set_error("null output nodes for query %d", i);
             return GQ_ERR_INVALID;
         }
-        if (!plain_cfg || qs[i]->n_rels > 30 || qs[i]->n_rels < 2) continue;
+    }
+    /* host preparation of the plain DPs, in parallel over queries */
+    std::vector<char> ok(n_queries, 0);
+    gq::parallel_for(n_queries, [&](int i) {
+        if (!plain_cfg || qs[i]->n_rels > 30 || qs[i]->n_rels < 2) return;
         gq::HQuery<1> h;
         gq::convert(qs[i], cfg, h);
-        if (gq::prepare_plain(h, preps[i])) continue;
-        if (gq::build_dp_input(preps[i].ri, ins[i])) continue;
-        if (!gq::fused_eligible(ins[i], algo)) continue;
+        if (gq::prepare_plain(h, preps[i])) return;
+        if (gq::build_dp_input(preps[i].ri, ins[i])) return;
+        if (!gq::fused_eligible(ins[i], algo)) return;
+        ok[i] = 1;
+    });
+    for (int i = 0; i < n_queries; i++) {
+        if (!ok[i]) continue;
         batch.push_back(&ins[i]);
         algos.push_back(algo);
         where.push_back(i);
@@ -539,14 +547,20 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
         if ((rc = gq::run_dp_batch(batch, algos, trees, sts, &ms))) return (gq_status)rc;
         auto t1 = std::chrono::steady_clock::now();
         double wall = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        for (size_t b = 0; b < batch.size(); b++) {
+        std::vector<char> cap_err(batch.size(), 0);
+        gq::parallel_for((int)batch.size(), [&](int b) {
             int i = where[b];
             gq::RunCtx ctx(cfg);
             gq::QT<1> *t = gq::finish_plain(ctx, preps[i], trees[b]);
             gq_plan_out *out = &outs[i];
             out->n_nodes = 0;
             memset(&out->stats, 0, sizeof(out->stats));
-            if (gq::emit(t, out) == -2) {
+            if (gq::emit(t, out) == -2) cap_err[b] = 1;
+        });
+        for (size_t b = 0; b < batch.size(); b++) {
+            int i = where[b];
+            gq_plan_out *out = &outs[i];
+            if (cap_err[b]) {
                 gq::set_error("output capacity %d < %d nodes", out->capacity, 2 * qs[i]->n_rels - 1);
                 return GQ_ERR_CAPACITY;
             }

@@ -8,7 +8,10 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+#include <atomic>
 #include <list>
+#include <thread>
 #include <vector>
 
 #include "gq_cost_model.h"
@@ -16,6 +19,31 @@
 #include "gq_internal.h"
 
 namespace gq {
+
+/* f(0) .. f(n-1) on up to 16 host threads (inline below 32 items); f must
+ * be safe to run concurrently for different indices. */
+template <class F>
+void parallel_for(int n, F f) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<unsigned>(hw ? hw : 1u, 16u);
+    nt = std::min(nt, n / 16);
+    if (n < 32 || nt <= 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<int> next(0);
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n) break;
+            f(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
 
 template <int W>
 using Set = SetN<W>;

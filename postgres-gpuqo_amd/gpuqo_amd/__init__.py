@@ -84,6 +84,34 @@ def plan(query, cfg):
     return plan_tuples(out), out.stats
 
 
+def plan_raw(query, cfg):
+    """gq_plan without converting the result (for timing the C ABI call);
+    returns the gq_plan_out and its node buffer."""
+    cap = 2 * query.n
+    nodes = (abi.PlanNode * cap)()
+    out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
+    _check(lib().gq_plan(C.byref(query.to_c()), C.byref(cfg), C.byref(out)))
+    return out, nodes
+
+
+class BatchArgs:
+    """Prebuilt gq_plan_batch arguments for a fixed list of queries."""
+
+    def __init__(self, queries):
+        k = len(queries)
+        self.k = k
+        self.qptrs = (C.POINTER(abi.Query) * k)(*[C.pointer(q.to_c()) for q in queries])
+        self.bufs = [(abi.PlanNode * (2 * q.n))() for q in queries]
+        self.outs = (abi.PlanOut * k)()
+        for i, q in enumerate(queries):
+            self.outs[i].nodes = C.cast(self.bufs[i], C.POINTER(abi.PlanNode))
+            self.outs[i].capacity = 2 * q.n
+
+    def run(self, cfg):
+        _check(lib().gq_plan_batch(self.k, self.qptrs, C.byref(cfg), self.outs))
+        return self.outs
+
+
 def plan_batch(queries, cfg):
     """Plan many queries in one batched device pass; returns [(nodes, stats)]."""
     k = len(queries)
