@@ -36,6 +36,27 @@ WORKLOADS = {
 }
 
 
+TRAFFIC_PROFILES = {"mpdp-snowflake20": "k_eval_tree_mpdp-snowflake20.md"}
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by the copy calibration),
+    or None when no summary covers the workload."""
+    name = TRAFFIC_PROFILES.get(workload)
+    if not name:
+        return None, None
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
+
+
 def gather_roofline():
     """Measured random-access rate (GB/s of record bytes) per regime."""
     path = os.path.join(ROOT, "postgres-gpuqo_amd", "lib", "libgq_bench.so")
@@ -210,8 +231,10 @@ def main():
     per_launch_bytes = kbytes * prof_steps / kn
     avg_launch_ms = kms / kn
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args.workload) if kern == eval_kernel else (None, None)
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kern,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+            "kernel": kern,
             "algorithmic_bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_launch_ms,
             "launches_per_step": kn // prof_steps, "profiled_steps": prof_steps,
             "phase_ms_per_step": {"filter": f_ms / prof_steps, "eval": e_ms / prof_steps}}
