@@ -238,9 +238,15 @@ typedef struct gq_server gq_server;
 gq_status gq_server_create(const gq_config *cfg, uint64_t memo_budget_bytes,
                            int32_t max_inflight, gq_server **out);
 gq_status gq_server_submit(gq_server *s, const gq_query *q, uint64_t tag);
+/* Submit with a per-request configuration (NULL: the server's); its device
+ * must be the server's.  Lets one server mix plain DPs and UnionDP/IDP2. */
+gq_status gq_server_submit_cfg(gq_server *s, const gq_query *q, const gq_config *cfg, uint64_t tag);
 /* Poll one finished request; returns GQ_OK with *done=1 when one was
- * written, *done=0 when none is ready.  out->nodes must hold 2n-1 nodes of
- * the largest query submitted. */
+ * written, *done=0 when none is ready.  wait: 0 returns at once, > 0 waits
+ * up to that many milliseconds, < 0 waits while requests are outstanding.
+ * out->nodes must hold 2n-1 nodes of the largest query submitted (else
+ * GQ_ERR_CAPACITY, and the request stays queued).  out->stats.wall_ms is the
+ * request's latency from submission to completion. */
 gq_status gq_server_poll(gq_server *s, int32_t wait, int32_t *done,
                          uint64_t *tag, gq_status *req_status,
                          gq_plan_out *out);

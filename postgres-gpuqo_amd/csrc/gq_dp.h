@@ -68,6 +68,38 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
 /* Run several DPs in one fused launch (one workgroup per DP).  Every input
  * must satisfy fused_eligible(). */
 bool fused_eligible(const DPInput &in, DPAlgo algo);
+
+/* Step engine (continuous batching).  Admitted DPs live in slices of one
+ * device arena; every engine_step advances each active DP by one level, all
+ * of them in one launch, and returns the DPs that finished.  DPs join and
+ * leave between steps.  Only DPs that satisfy fused_eligible() and run to
+ * their full size are admitted.  Not thread-safe: one engine thread drives
+ * it. */
+struct StepEngine;
+struct EngineDone {
+    uint64_t key;
+    int rc;
+    std::vector<DPNode> tree;
+    DPStats st;
+};
+int engine_create(int device, uint64_t arena_bytes, StepEngine **out);
+void engine_destroy(StepEngine *e);
+/* arena bytes one DP of n relations takes (blob allowance included) */
+uint64_t engine_slice_bytes(int n_rels);
+/* admits the DP if its slice fits now (the blob is copied at admission;
+ * `in` must stay alive until the DP finishes); false otherwise */
+bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key);
+int engine_step(StepEngine *e, std::vector<EngineDone> &done);
+int engine_active(const StepEngine *e);
+
+/* Where run_many sends its GPU DPs when set on the calling thread (the
+ * planning server's request threads); unset: the library's own paths. */
+struct DPExecutor {
+    virtual ~DPExecutor() {}
+    virtual int run(const std::vector<const DPInput *> &ins, const std::vector<DPAlgo> &algos,
+                    std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &sts) = 0;
+};
+DPExecutor *&thread_executor();
 int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAlgo> &algos,
                  std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &stats, double *gpu_ms);
 

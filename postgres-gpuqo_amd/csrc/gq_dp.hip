@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -601,6 +602,7 @@ struct BatchJob {
     uint32_t lvl_off[33];
     uint32_t conn_words;
     int bicc;
+    int cur_k;           /* step engine: the level this job evaluates next */
 };
 
 /* MPDP sets whose block decomposition yields more than BIG_SPLITS splits
@@ -1074,7 +1076,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_filter(const BatchJob *jobs, in
 }
 
 __device__ __forceinline__ uint32_t job_tasks(const BatchJob &J, int k) {
-    if (k > J.h.n_rels) return 0;
+    if (k < 2 || k > J.h.n_rels) return 0;
     const uint32_t spt = task_sets(J, k);
     return (J.counters[k] + spt - 1) / spt;
 }
@@ -1098,18 +1100,36 @@ __global__ void __launch_bounds__(1024) k_batch_plan(const BatchJob *jobs, int n
     if (threadIdx.x == 0) tp[nj] = total;
 }
 
-/* Level k of every job.  The task prefix of this level is staged in LDS at
+/* Step engine: task_prefix[j] = first task of job j at its own level
+ * J.cur_k (jobs past their last level have none); one workgroup. */
+__global__ void __launch_bounds__(1024) k_step_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix) {
+    __shared__ uint32_t part[17];
+    const uint32_t per = ((uint32_t)nj + blockDim.x - 1) / blockDim.x;
+    const uint32_t j0 = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], jobs[j].cur_k);
+    uint32_t total;
+    uint32_t acc = block_scan_excl(sum, part, &total);
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) {
+        task_prefix[j] = acc;
+        acc += job_tasks(jobs[j], jobs[j].cur_k);
+    }
+    if (threadIdx.x == 0) task_prefix[nj] = total;
+}
+
+/* One DP level of every job: level k_fixed of all of them (a batch run in
+ * lockstep), or, with k_fixed = 0, each job's own level J.cur_k (the step
+ * engine).  `tpg` is the task prefix of this launch; it is staged in LDS at
  * word `tp_lds` (0: read from global memory).  Non-tree MPDP sets with more
  * than BIG_SPLITS splits go to the big-set list (`big`, `big_hdr`). */
-__global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int nj, int k,
-                                                       const uint32_t *task_prefix, uint32_t tp_lds, BigEnt *big,
+__global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int nj, int k_fixed,
+                                                       const uint32_t *tpg, uint32_t tp_lds, BigEnt *big,
                                                        uint32_t big_cap, uint32_t *big_hdr) {
     /* The dynamic LDS must be the kernel's only LDS: a static __shared__
      * variable would move its base off the 8-byte alignment that the 64-bit
      * LDS atomics of eval_level need.  Each thread therefore finds its job
      * itself (the search is uniform across the workgroup). */
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
-    const uint32_t *tpg = task_prefix + (size_t)k * (nj + 1);
     const uint32_t total = tpg[nj];
     if (blockIdx.x >= total) return;
     const uint32_t *tp = tpg;
@@ -1130,6 +1150,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         const int j = lo;
         const BatchJob &J = jobs[j];
         const BlobHdr &h = J.h;
+        const int k = k_fixed > 0 ? k_fixed : J.cur_k;
         if (j != have) {
             __syncthreads();
             for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = J.blob[i];
@@ -1758,8 +1779,9 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             ub += binom64(h.n_rels, k);
         }
         uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
-        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_eval_total, s, c.jobs, nj, k, task_prefix,
-                           tp_lds, big_cap ? c.big_ents : nullptr, (uint32_t)big_cap, big_hdr);
+        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_eval_total, s, c.jobs, nj, k,
+                           task_prefix + (size_t)k * (nj + 1), tp_lds, big_cap ? c.big_ents : nullptr,
+                           (uint32_t)big_cap, big_hdr);
         if (big_cap && k >= 15 && k <= big_kmax) {
             hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, big_hdr, (uint32_t)big_cap,
                                c.big_prefix);
@@ -2109,6 +2131,371 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
     st.memo_bytes = nsets_all * (sizeof(float4) + 4);
     st.memo_kind = 1;
+    return GQ_OK;
+}
+
+
+/* ================================================================ step engine
+ *
+ * Continuous batching over DP levels.  An admitted DP gets one slice of the
+ * engine's device arena (256-byte aligned):
+ *     [result record: 64 counters + 64 plan nodes][query blob]
+ *     [vals][left][conn][lvl_sets][lvl_best][lvl_first]
+ * On its first step the DP is initialised and filtered; each step then runs
+ * one level of every active DP in one k_batch_eval launch (each at its own
+ * level J.cur_k) plus the big-block kernels when a DP needs them; a DP past
+ * its last level is walked and read back, and its slice is freed.  The host
+ * knows every DP's level, so no device round trip decides the schedule; one
+ * stream synchronisation per step returns the finished plans. */
+
+DPExecutor *&thread_executor() {
+    static thread_local DPExecutor *ex = nullptr;
+    return ex;
+}
+
+namespace {
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+const size_t ENGINE_BLOB_MAX = 64 * 1024;     /* blob bytes allowed per DP (the LDS bound) */
+
+size_t slice_layout(int n, int words, size_t *o_blob, size_t *o_vals, size_t *o_left, size_t *o_conn,
+                    size_t *o_sets, size_t *o_best, size_t *o_first) {
+    const uint64_t ns = 1ull << n;
+    size_t o = al256(RES_WORDS * 4);
+    *o_blob = o;
+    o += al256((size_t)words * 4);
+    *o_vals = o;
+    o += al256(ns * 16);
+    *o_left = o;
+    o += al256(ns * 4);
+    *o_conn = o;
+    o += al256(std::max<uint64_t>(1, ns / 32) * 4);
+    *o_sets = o;
+    o += al256(ns * 4);
+    *o_best = o;
+    o += al256(ns * 8);
+    *o_first = o;
+    o += al256(ns * 4);
+    return o;
+}
+}  // namespace
+
+struct EJob {
+    uint64_t key;
+    int n, cur_k;
+    DPAlgo algo;
+    size_t off, bytes;
+    BatchJob J;
+    int words;
+};
+
+struct StepEngine {
+    int device = 0;
+    hipStream_t s = nullptr;
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    std::map<size_t, size_t> free_;              /* offset -> size, coalesced */
+    std::vector<EJob *> active, incoming;
+    BatchJob *d_jobs = nullptr, *d_new = nullptr, *d_fin = nullptr;
+    uint32_t *d_tp = nullptr, *d_chunk = nullptr, *big_prefix = nullptr, *big_hdr = nullptr;
+    BigEnt *big = nullptr;
+    size_t cap_jobs = 0, cap_new = 0, cap_fin = 0, cap_tp = 0, cap_chunk = 0, cap_big = 0, cap_bigp = 0,
+           cap_hdr = 0;
+    BatchJob *h_jobs = nullptr;                  /* pinned staging */
+    uint32_t *h_res = nullptr, *h_chunk = nullptr;
+    size_t cap_h_jobs = 0, cap_h_res = 0, cap_h_chunk = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    bool alloc(size_t bytes, size_t *off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t at = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+template <class T>
+static int ensure_pinned(T **p, size_t &cap, size_t need) {
+    if (*p && cap >= need) return GQ_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    cap = 0;
+    size_t n = std::max<size_t>(need, 16);
+    if (hipHostMalloc((void **)p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        *p = nullptr;
+        set_error("pinned allocation of %zu bytes failed", n * sizeof(T));
+        return GQ_ERR_NOMEM;
+    }
+    cap = n;
+    return GQ_OK;
+}
+
+uint64_t engine_slice_bytes(int n_rels) {
+    size_t a, b, c, d, e, f, g;
+    return slice_layout(n_rels, (int)(ENGINE_BLOB_MAX / 4), &a, &b, &c, &d, &e, &f, &g);
+}
+
+int engine_create(int device, uint64_t arena_bytes, StepEngine **out) {
+    *out = nullptr;
+    StepEngine *e = new StepEngine();
+    e->device = device;
+    auto fail = [&](int rc) {
+        engine_destroy(e);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) {
+        set_error("hipSetDevice(%d) failed", device);
+        return fail(GQ_ERR_DEVICE);
+    }
+    {
+        /* the binomial table and device context of the library */
+        int rc = device_init(device);
+        if (rc) return fail(rc);
+    }
+    if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&e->e0) != hipSuccess || hipEventCreate(&e->e1) != hipSuccess) {
+        set_error("engine stream/event creation failed");
+        return fail(GQ_ERR_DEVICE);
+    }
+    e->arena_bytes = al256(arena_bytes);
+    if (hipMalloc((void **)&e->arena, e->arena_bytes) != hipSuccess) {
+        e->arena = nullptr;
+        set_error("engine arena of %llu bytes could not be allocated", (unsigned long long)e->arena_bytes);
+        return fail(GQ_ERR_NOMEM);
+    }
+    e->free_[0] = e->arena_bytes;
+    int rc = ensure(&e->big_hdr, e->cap_hdr, 4);
+    if (rc) return fail(rc);
+    if (hipMemsetAsync(e->big_hdr, 0, 16, e->s) != hipSuccess || hipStreamSynchronize(e->s) != hipSuccess) {
+        set_error("engine initialisation failed");
+        return fail(GQ_ERR_DEVICE);
+    }
+    *out = e;
+    return GQ_OK;
+}
+
+void engine_destroy(StepEngine *e) {
+    if (!e) return;
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    for (EJob *j : e->active) delete j;
+    for (EJob *j : e->incoming) delete j;
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->d_jobs) (void)hipFree(e->d_jobs);
+    if (e->d_new) (void)hipFree(e->d_new);
+    if (e->d_fin) (void)hipFree(e->d_fin);
+    if (e->d_tp) (void)hipFree(e->d_tp);
+    if (e->d_chunk) (void)hipFree(e->d_chunk);
+    if (e->big) (void)hipFree(e->big);
+    if (e->big_prefix) (void)hipFree(e->big_prefix);
+    if (e->big_hdr) (void)hipFree(e->big_hdr);
+    if (e->h_jobs) (void)hipHostFree(e->h_jobs);
+    if (e->h_res) (void)hipHostFree(e->h_res);
+    if (e->h_chunk) (void)hipHostFree(e->h_chunk);
+    if (e->e0) (void)hipEventDestroy(e->e0);
+    if (e->e1) (void)hipEventDestroy(e->e1);
+    if (e->s) (void)hipStreamDestroy(e->s);
+    delete e;
+}
+
+int engine_active(const StepEngine *e) { return (int)(e->active.size() + e->incoming.size()); }
+
+bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    if (!fused_eligible(in, algo) || (size_t)h.words * 4 > ENGINE_BLOB_MAX) return false;
+    size_t o_blob, o_vals, o_left, o_conn, o_sets, o_best, o_first;
+    const size_t bytes = slice_layout(n, h.words, &o_blob, &o_vals, &o_left, &o_conn, &o_sets, &o_best, &o_first);
+    size_t off;
+    if (!e->alloc(bytes, &off)) return false;
+    char *base = e->arena + off;
+    if (hipMemcpyAsync(base + o_blob, in.blob.data(), (size_t)h.words * 4, hipMemcpyHostToDevice, e->s) !=
+        hipSuccess) {
+        e->release(off, bytes);
+        return false;
+    }
+    EJob *j = new EJob();
+    j->key = key;
+    j->n = n;
+    j->cur_k = 2;
+    j->algo = algo;
+    j->off = off;
+    j->bytes = bytes;
+    j->words = h.words;
+    BatchJob &J = j->J;
+    memset(&J, 0, sizeof(J));
+    J.h = h;
+    J.blob = (const uint32_t *)(base + o_blob);
+    J.vals = (float4 *)(base + o_vals);
+    J.left = (uint32_t *)(base + o_left);
+    J.conn = (uint32_t *)(base + o_conn);
+    J.conn_words = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
+    J.lvl_sets = (uint32_t *)(base + o_sets);
+    J.lvl_best = (unsigned long long *)(base + o_best);
+    J.lvl_first = (uint32_t *)(base + o_first);
+    J.counters = (uint32_t *)base;
+    J.nodes = (DPNode *)(J.counters + 64);
+    uint64_t acc = 0;
+    for (int k = 0; k <= 32; k++) {
+        J.lvl_off[k] = (uint32_t)acc;
+        if (k >= 2 && k <= n) acc += binom64(n, k);
+    }
+    J.bicc = algo == DP_BICC ? 1 : 0;
+    J.cur_k = 2;
+    e->incoming.push_back(j);
+    return true;
+}
+
+#define ECHK(x)                                                                            \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #x); \
+            return GQ_ERR_DEVICE;                                                          \
+        }                                                                                  \
+    } while (0)
+
+int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
+    int rc;
+    hipStream_t s = e->s;
+    ECHK(hipSetDevice(e->device));
+    ECHK(hipEventRecord(e->e0, s));
+    /* 1. DPs admitted since the last step: init + filter, then they join */
+    if (!e->incoming.empty()) {
+        const int nn = (int)e->incoming.size();
+        if ((rc = ensure_pinned(&e->h_jobs, e->cap_h_jobs, (size_t)nn))) return rc;
+        if ((rc = ensure_pinned(&e->h_chunk, e->cap_h_chunk, (size_t)nn + 1))) return rc;
+        if ((rc = ensure(&e->d_new, e->cap_new, (size_t)nn))) return rc;
+        if ((rc = ensure(&e->d_chunk, e->cap_chunk, (size_t)nn + 1))) return rc;
+        e->h_chunk[0] = 0;
+        for (int i = 0; i < nn; i++) {
+            e->h_jobs[i] = e->incoming[i]->J;
+            const uint64_t ns = 1ull << e->incoming[i]->n;
+            e->h_chunk[i + 1] = e->h_chunk[i] + (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK);
+        }
+        ECHK(hipMemcpyAsync(e->d_new, e->h_jobs, sizeof(BatchJob) * nn, hipMemcpyHostToDevice, s));
+        ECHK(hipMemcpyAsync(e->d_chunk, e->h_chunk, sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_batch_init, dim3(std::min(nn, 1024)), dim3(64), 0, s, e->d_new, nn);
+        const uint32_t fgrid = std::min<uint32_t>(e->h_chunk[nn], 4096);
+        hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, e->d_new, nn,
+                           e->d_chunk);
+        for (EJob *j : e->incoming) e->active.push_back(j);
+        e->incoming.clear();
+        /* the pinned staging is reused below: the copies above must have
+         * read it first */
+        ECHK(hipStreamSynchronize(s));
+    }
+    const int na = (int)e->active.size();
+    if (na == 0) return GQ_OK;
+    /* 2. one level of every active DP */
+    if ((rc = ensure_pinned(&e->h_jobs, e->cap_h_jobs, (size_t)na))) return rc;
+    if ((rc = ensure(&e->d_jobs, e->cap_jobs, (size_t)na))) return rc;
+    if ((rc = ensure(&e->d_tp, e->cap_tp, (size_t)na + 1))) return rc;
+    size_t lds = 0, big_lds_max = 0;
+    uint64_t ub = 0, big_cap = 0;
+    for (int i = 0; i < na; i++) {
+        EJob *j = e->active[i];
+        j->J.cur_k = j->cur_k;
+        e->h_jobs[i] = j->J;
+        const BlobHdr &h = j->J.h;
+        lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
+        ub += binom64(j->n, j->cur_k);
+        if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
+            big_cap += binom64(j->n, j->cur_k);
+            big_lds_max = std::max(big_lds_max, big_lds(h.words));
+        }
+    }
+    if (big_cap > 0xffffffffull) {
+        set_error("step too large for the big-set list");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (big_cap) {
+        if ((rc = ensure(&e->big, e->cap_big, big_cap))) return rc;
+        if ((rc = ensure(&e->big_prefix, e->cap_bigp, big_cap + 1))) return rc;
+    }
+    ECHK(hipMemcpyAsync(e->d_jobs, e->h_jobs, sizeof(BatchJob) * na, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, e->d_tp);
+    const uint32_t tp_lds = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
+    const size_t lds_total = tp_lds ? lds + 4 * (size_t)(na + 1) : lds;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
+    hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_total, s, e->d_jobs, na, 0, e->d_tp, tp_lds,
+                       big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
+    if (big_cap) {
+        hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, e->big, e->big_hdr, (uint32_t)big_cap,
+                           e->big_prefix);
+        hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s, e->d_jobs, e->big, e->big_hdr,
+                           e->big_prefix);
+    }
+    /* 3. DPs past their last level: plan walk and read-back */
+    std::vector<EJob *> fin, keep;
+    for (EJob *j : e->active) {
+        j->cur_k++;
+        if (j->cur_k > j->n) fin.push_back(j);
+        else keep.push_back(j);
+    }
+    const int nf = (int)fin.size();
+    std::vector<BatchJob> fj(nf);             /* alive until the synchronisation below */
+    if (nf) {
+        if ((rc = ensure(&e->d_fin, e->cap_fin, (size_t)nf))) return rc;
+        if ((rc = ensure_pinned(&e->h_res, e->cap_h_res, (size_t)nf * RES_WORDS))) return rc;
+        for (int i = 0; i < nf; i++) fj[i] = fin[i]->J;
+        ECHK(hipMemcpyAsync(e->d_fin, fj.data(), sizeof(BatchJob) * nf, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_batch_tree, dim3(nf), dim3(WAVE), 0, s, e->d_fin, nf);
+        for (int i = 0; i < nf; i++)
+            ECHK(hipMemcpyAsync(e->h_res + (size_t)i * RES_WORDS, fin[i]->J.counters, RES_WORDS * 4,
+                                hipMemcpyDeviceToHost, s));
+    }
+    ECHK(hipGetLastError());
+    ECHK(hipEventRecord(e->e1, s));
+    ECHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    ECHK(hipEventElapsedTime(&ms, e->e0, e->e1));
+    e->active.swap(keep);
+    for (int i = 0; i < nf; i++) {
+        EJob *j = fin[i];
+        const uint32_t *cnt = e->h_res + (size_t)i * RES_WORDS;
+        EngineDone d;
+        d.key = j->key;
+        d.rc = GQ_OK;
+        const int n = j->n;
+        if (cnt[49]) {
+            d.rc = GQ_ERR_UNSUPPORTED;
+        } else if (cnt[48] != (uint32_t)(2 * n - 1)) {
+            d.rc = GQ_ERR_DEVICE;
+        } else {
+            const DPNode *nodes = (const DPNode *)(cnt + 64);
+            d.tree.assign(nodes, nodes + 2 * n - 1);
+            d.st.candidates = (1ull << n) - (uint64_t)n - 1ull;
+            for (int k = 2; k <= n; k++) d.st.sets += cnt[k];
+            d.st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+            d.st.memo_bytes = j->bytes;
+            d.st.memo_kind = 1;
+            d.st.gpu_ms = ms;
+        }
+        e->release(j->off, j->bytes);
+        delete j;
+        done.push_back(std::move(d));
+    }
     return GQ_OK;
 }
 

@@ -278,6 +278,7 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
         return GQ_ERR_INTERRUPTED;
     }
     const DPAlgo algo = dp_algo(ctx);
+    DPExecutor *ex = thread_executor();
     std::vector<DPInput> ins(hs.size());
     std::vector<const DPInput *> batch;
     std::vector<DPAlgo> algos;
@@ -289,7 +290,7 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
         }
         int rc = build_dp_input(*hs[i], ins[i]);
         if (rc) return rc;
-        if (hs.size() > 1 && hs[i]->n_iters == hs[i]->n && fused_eligible(ins[i], algo)) {
+        if ((ex || hs.size() > 1) && hs[i]->n_iters == hs[i]->n && fused_eligible(ins[i], algo)) {
             batch.push_back(&ins[i]);
             algos.push_back(algo);
             where.push_back(i);
@@ -307,7 +308,7 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
         std::vector<std::vector<DPNode>> trees;
         std::vector<DPStats> sts;
         double ms = 0.0;
-        int rc = run_dp_batch(batch, algos, trees, sts, &ms);
+        int rc = ex ? ex->run(batch, algos, trees, sts) : run_dp_batch(batch, algos, trees, sts, &ms);
         if (rc) return rc;
         for (size_t b = 0; b < batch.size(); b++) {
             ctx.dp_calls++;

@@ -585,21 +585,7 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
     return GQ_OK;
 }
 
-struct gq_server {
-    int dummy;
-};
 
-gq_status gq_server_create(const gq_config *, uint64_t, int32_t, gq_server **out) {
-    if (out) *out = nullptr;
-    gq::set_error("server not available");
-    return GQ_ERR_UNSUPPORTED;
-}
-gq_status gq_server_submit(gq_server *, const gq_query *, uint64_t) { return GQ_ERR_UNSUPPORTED; }
-gq_status gq_server_poll(gq_server *, int32_t, int32_t *done, uint64_t *, gq_status *, gq_plan_out *) {
-    if (done) *done = 0;
-    return GQ_ERR_UNSUPPORTED;
-}
-gq_status gq_server_destroy(gq_server *) { return GQ_OK; }
 
 void gq_interrupt(int32_t on) { gq::g_interrupt.store(on ? 1 : 0); }
 

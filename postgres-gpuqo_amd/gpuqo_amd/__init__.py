@@ -39,6 +39,7 @@ def lib():
         L.gq_interrupt.restype = None
         L.gq_server_create.argtypes = [C.POINTER(abi.Config), C.c_uint64, C.c_int32, C.POINTER(C.c_void_p)]
         L.gq_server_submit.argtypes = [C.c_void_p, C.POINTER(abi.Query), C.c_uint64]
+        L.gq_server_submit_cfg.argtypes = [C.c_void_p, C.POINTER(abi.Query), C.POINTER(abi.Config), C.c_uint64]
         L.gq_server_poll.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int32), C.POINTER(abi.PlanOut)]
         L.gq_server_destroy.argtypes = [C.c_void_p]
@@ -123,3 +124,45 @@ def plan_batch(queries, cfg):
         outs[i].capacity = 2 * q.n
     _check(lib().gq_plan_batch(k, qptrs, C.byref(cfg), outs))
     return [(plan_tuples(outs[i]), outs[i].stats) for i in range(k)]
+
+
+class Server:
+    """The continuous-batching planning server (gq_server_*)."""
+
+    def __init__(self, cfg, memo_budget_bytes, max_inflight=8):
+        self.cfg = cfg
+        h = C.c_void_p()
+        _check(lib().gq_server_create(C.byref(cfg), memo_budget_bytes, max_inflight, C.byref(h)))
+        self.h = h
+        cap = 2 * abi.GQ_MAX_RELS
+        self._buf = (abi.PlanNode * cap)()
+        self._out = abi.PlanOut(C.cast(self._buf, C.POINTER(abi.PlanNode)), cap, 0)
+
+    def submit(self, query, tag, cfg=None):
+        """Queue one query (copied); `cfg` overrides the server's config."""
+        if cfg is None:
+            _check(lib().gq_server_submit(self.h, C.byref(query.to_c()), tag))
+        else:
+            _check(lib().gq_server_submit_cfg(self.h, C.byref(query.to_c()), C.byref(cfg), tag))
+
+    def poll(self, wait_ms=-1):
+        """One finished request as (tag, status, nodes, stats), or None."""
+        done, tag, st = C.c_int32(0), C.c_uint64(0), C.c_int32(0)
+        _check(lib().gq_server_poll(self.h, wait_ms, C.byref(done), C.byref(tag), C.byref(st),
+                                    C.byref(self._out)))
+        if not done.value:
+            return None
+        stats = abi.PlanStats()
+        C.pointer(stats)[0] = self._out.stats
+        return tag.value, st.value, plan_tuples(self._out), stats
+
+    def close(self):
+        if self.h:
+            _check(lib().gq_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -24,7 +24,7 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for f in ("gq_plan", "gq_plan_batch", "gq_config_default", "gq_server_create", "gq_server_submit",
-              "gq_server_poll", "gq_server_destroy", "gq_last_error", "gq_init"):
+              "gq_server_submit_cfg", "gq_server_poll", "gq_server_destroy", "gq_last_error", "gq_init"):
         assert f in names
 
 

@@ -1,0 +1,81 @@
+"""The continuous-batching planning server: every request's plan equals the
+same query planned alone (gq_plan) and the oracle's, whatever else shares the
+device with it."""
+import time
+
+import pytest
+
+from helpers import assert_same_plan, cfg, gpu_bits, oracle_bits
+from gpuqo_amd import qgen
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    direct = cfg("dpsub", 1)
+    union = cfg("dpsub", 1, idp_type="union", idp_n_iters=25)
+    reqs = [(qgen.snowflake(n, 5000 + n), direct) for n in (10, 12, 14, 16, 18, 20)]
+    reqs += [(qgen.snowflake(70, 77, depth=4, fanout=5), union), (qgen.cycle(16, 99), direct),
+             (qgen.star(13, 98), direct), (qgen.snowflake(64, 78, depth=4, fanout=5), union),
+             (qgen.chain(15, 97), direct), (qgen.snowflake(22, 96), direct)]
+    return direct, reqs
+
+
+def _run(srv, reqs, gap_s):
+    for i, (g, c) in enumerate(reqs):
+        srv.submit(g, i, c)
+        if gap_s:
+            time.sleep(gap_s)                     # later requests arrive while earlier ones run
+    got = {}
+    while len(got) < len(reqs):
+        r = srv.poll(wait_ms=60000)
+        assert r is not None, f"server returned nothing after {len(got)} of {len(reqs)}"
+        tag, st, nodes, stats = r
+        assert st == 0, f"request {tag}: status {st}"
+        assert tag not in got
+        assert stats.wall_ms > 0.0
+        got[tag] = nodes
+    return got
+
+
+def test_server_mix_equals_alone_and_oracle():
+    import gpuqo_amd
+    direct, reqs = _mix()
+    with gpuqo_amd.Server(direct, 4 << 30, max_inflight=6) as srv:
+        got = _run(srv, reqs, 0.002)
+        assert srv.poll(wait_ms=0) is None
+    for i, (g, c) in enumerate(reqs):
+        plan = gpuqo_amd.plan_bits(got[i])
+        alone, _ = gpu_bits(g, c)
+        assert_same_plan(plan, alone, f"request {i} ({g.shape}{g.n}) vs alone")
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(plan, want, f"request {i} ({g.shape}{g.n}) vs oracle")
+
+
+def test_server_tight_budget_admits_one_at_a_time():
+    """A budget that holds one 20-relation memo: requests are admitted as
+    memory frees up, and all finish with the same plans."""
+    import gpuqo_amd
+    direct = cfg("dpsub", 1)
+    reqs = [(qgen.snowflake(n, 6100 + n), direct) for n in (20, 18, 19, 20, 16)]
+    # one 20-relation DP takes 2^20 x 36 B plus its query blob: about 38 MB
+    with gpuqo_amd.Server(direct, 40 << 20, max_inflight=4) as srv:
+        got = _run(srv, reqs, 0.0)
+    for i, (g, c) in enumerate(reqs):
+        alone, _ = gpu_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(got[i]), alone, f"request {i}")
+
+
+def test_server_rejects_a_dp_larger_than_its_budget():
+    import gpuqo_amd
+    direct = cfg("dpsub", 1)
+    with gpuqo_amd.Server(direct, 8 << 20, max_inflight=2) as srv:
+        srv.submit(qgen.snowflake(20, 6200), 1)
+        srv.submit(qgen.snowflake(8, 6201), 2)
+        res = {}
+        for _ in range(2):
+            r = srv.poll(wait_ms=60000)
+            assert r is not None
+            res[r[0]] = r[1]
+    assert res[1] == 3            # GQ_ERR_NOMEM: 2^20 memo slots do not fit 8 MiB
+    assert res[2] == 0
