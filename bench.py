@@ -116,6 +116,85 @@ def batched_section(gpuqo_amd, abi, qgen, reps=5):
             "batch_equals_single_all": same}
 
 
+def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=64):
+    """Config 5: `count` seeded requests (70% snowflakes of 10-30 relations
+    planned directly by MPDP, 30% snowflakes of 60-200 relations, depth 4,
+    fanout 1-5, by UnionDP with k=25 and MPDP per partition) arriving as a
+    Poisson process of `rate` requests/s at the continuous-batching server.
+    Latency is submission to completion (measured by the library); every
+    served plan is compared with the same request planned alone by gq_plan,
+    which also gives the one-at-a-time throughput."""
+    import random
+    import threading
+    mix = qgen.serving_mix(count, qgen.config_seed(5))
+    direct = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+    union = abi.default_config(algorithm="dpsub", dpsub_bicc=1, idp_type="union", idp_n_iters=25)
+    rng = random.Random(qgen.config_seed(5))
+    arrivals, t = [], 0.0
+    for _ in mix:
+        t += rng.expovariate(rate)
+        arrivals.append(t)
+    res = {}
+    with gpuqo_amd.Server(direct, budget_gb << 30, max_inflight=inflight) as srv:
+        srv.submit(qgen.snowflake(12, 1), count + 1)       # warm-up
+        srv.poll(60000)
+        # capacity: every request submitted at once
+        t0 = time.perf_counter()
+        for i, (g, large) in enumerate(mix):
+            srv.submit(g, i, union if large else direct)
+        sat = 0
+        while sat < count:
+            r = srv.poll(120000)
+            if r is None:
+                break
+            sat += 1
+        sat_s = time.perf_counter() - t0
+
+        submitted = [0]
+
+        def collect():
+            deadline = time.perf_counter() + 300.0
+            while len(res) < count and time.perf_counter() < deadline:
+                r = srv.poll(100)                  # returns at once while nothing is outstanding
+                if r is None:
+                    if submitted[0] == 0:
+                        time.sleep(0.0002)
+                    continue
+                res[r[0]] = (r[1], gpuqo_amd.plan_bits(r[2]), r[3].wall_ms, time.perf_counter())
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        for i, ((g, large), at) in enumerate(zip(mix, arrivals)):
+            d = t0 + at - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            srv.submit(g, i, union if large else direct)
+            submitted[0] += 1
+        th.join()
+    ok = len(res) == count and all(v[0] == 0 for v in res.values())
+    lat = sorted(v[2] for v in res.values())
+    span = max(v[3] for v in res.values()) - t0 if res else float("nan")
+    # the same requests one at a time through gq_plan: throughput and parity
+    single_s, same = 0.0, ok
+    for i, (g, large) in enumerate(mix):
+        c = union if large else direct
+        t1 = time.perf_counter()
+        out, _buf = gpuqo_amd.plan_raw(g, c)
+        single_s += time.perf_counter() - t1
+        same = same and res.get(i, (None, None))[1] == gpuqo_amd.plan_bits(gpuqo_amd.plan_tuples(out))
+    q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))] if lat else float("nan")
+    return {"workload": "serve-config5", "requests": count, "arrival_rate_per_s": rate,
+            "large_uniondp_requests": sum(1 for _, l in mix if l),
+            "capacity_queries_per_s": count / sat_s if sat == count else float("nan"),
+            "served_queries_per_s": count / span, "latency_p50_ms": q(0.50), "latency_p99_ms": q(0.99),
+            "latency_max_ms": lat[-1] if lat else float("nan"), "memo_budget_gb": budget_gb,
+            "max_inflight": inflight, "all_ok": ok,
+            "single_queries_per_s": count / single_s,
+            "capacity_vs_single": (count / sat_s) / (count / single_s) if sat == count else float("nan"),
+            "served_equals_alone_all": same}
+
+
 def cpu_baseline(g, algo, bicc, budget_s=12.0):
     """The oracle's CPU restatement timed on this host (bounded sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -271,6 +350,7 @@ def main():
     }
     if not args.no_batch and args.workload == "mpdp-snowflake20":
         out["batched"] = batched_section(gpuqo_amd, abi, qgen)
+        out["served"] = serving_section(gpuqo_amd, abi, qgen)
     if not args.no_cpu:
         cb = cpu_baseline(g, algo, bicc)
         seq = cb["seq"]

@@ -42,7 +42,17 @@ struct DPInput {
     std::vector<Rel> base;
     /* DPsize: order of the level-1 list (relation indices) */
     std::vector<int> level1;
+    /* tree graphs: number of connected sets of each size k (index k = 0..n),
+     * from the subtree-count polynomial; empty for other graphs */
+    std::vector<uint64_t> tree_counts;
+    /* memo layout chosen for this DP: GQ_MEMO_DENSE or GQ_MEMO_HASH (trees) */
+    int memo = 1;
 };
+
+/* GQ_MEMO_AUTO resolves to the open-addressing memo for tree graphs whose
+ * dense memo (36 B x 2^n) would exceed max_memo_mb; GQ_MEMO_HASH applies to
+ * tree graphs only (other graphs keep the dense memo). */
+int resolve_memo(const DPInput &in, int requested, int max_memo_mb);
 
 struct DPNode {                 /* plan node in DP numbering, pre-order */
     uint32_t set;
@@ -84,8 +94,10 @@ struct EngineDone {
 };
 int engine_create(int device, uint64_t arena_bytes, StepEngine **out);
 void engine_destroy(StepEngine *e);
-/* arena bytes one DP of n relations takes (blob allowance included) */
+/* arena bytes one DP of n relations takes (blob allowance included): dense
+ * memo, or open-addressing memo over `total_sets` connected sets */
 uint64_t engine_slice_bytes(int n_rels);
+uint64_t engine_hash_slice_bytes(uint64_t total_sets);
 /* admits the DP if its slice fits now (the blob is copied at admission;
  * `in` must stay alive until the DP finishes); false otherwise */
 bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key);

@@ -118,6 +118,61 @@ __device__ __forceinline__ void store_rel(float4 *vals, uint32_t x, const Rel &r
     vals[x] = make_float4(r.rows, r.startup, r.total, __int_as_float(r.width));
 }
 
+/* The memo of one DP.  Dense (keys == nullptr): the slot of set S is S.
+ * Open addressing (tree graphs): keys[slot] holds S (0 = empty), linear
+ * probing from hash(S); every connected set is inserted before its values
+ * are read, so a probe ends at its key.  A probe that meets an empty slot or
+ * runs over the whole table sets bit 2 of *err and returns slot 0. */
+struct MemoRef {
+    float4 *vals;
+    uint32_t *left;
+    const uint32_t *keys;
+    uint32_t mask;
+    uint32_t *err;
+};
+
+__device__ __forceinline__ uint32_t memo_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x85ebca6bu;
+    x ^= x >> 13;
+    x *= 0xc2b2ae35u;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t memo_slot(const MemoRef &m, uint32_t S) {
+    if (!m.keys) return S;
+    uint32_t h = memo_hash(S) & m.mask;
+    for (uint32_t i = 0; i <= m.mask; i++) {
+        const uint32_t key = m.keys[h];
+        if (key == S) return h;
+        if (key == 0u) break;
+        h = (h + 1u) & m.mask;
+    }
+    atomicOr(m.err, 4u);
+    return 0u;
+}
+
+/* Inserts S (absent or present); returns its slot and whether this call
+ * inserted it, or ~0u when the table is full. */
+__device__ __forceinline__ uint32_t memo_insert(uint32_t *keys, uint32_t mask, uint32_t S, bool *fresh) {
+    uint32_t h = memo_hash(S) & mask;
+    for (uint32_t i = 0; i <= mask; i++) {
+        const uint32_t old = atomicCAS(&keys[h], 0u, S);
+        if (old == 0u) {
+            *fresh = true;
+            return h;
+        }
+        if (old == S) {
+            *fresh = false;
+            return h;
+        }
+        h = (h + 1u) & mask;
+    }
+    *fresh = false;
+    return ~0u;
+}
+
 /* closure of `from` inside `subset` over the adjacency table (gpuqo_filter.cuh:113-146) */
 __device__ __forceinline__ uint32_t grow(uint32_t from, uint32_t subset, const uint32_t *adj) {
     uint32_t V = 0, N = from;
@@ -506,8 +561,8 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
  * ... cover the level. */
 template <int ECW>
 __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
-                                    bool bicc, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
-                                    uint32_t *left, uint64_t w0, uint64_t wstride) {
+                                    bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot,
+                                    uint32_t count, const MemoRef &mr, uint64_t w0, uint64_t wstride) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t per = 2u * (k - 1);
     const uint32_t spw = WAVE / per;                    /* sets per wave, per consecutive lanes each */
@@ -543,7 +598,8 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
             } else {
                 nq = match_mask<ECW>(q, L, R, m);
             }
-            jr = join_mask<ECW>(q, L, load_rel(vals, L), R, load_rel(vals, R), m, nq);
+            jr = join_mask<ECW>(q, L, load_rel(mr.vals, memo_slot(mr, L)), R, load_rel(mr.vals, memo_slot(mr, R)),
+                                m, nq);
             npairs++;
             const bool nan = jr.total != jr.total;
             first = (rank << 1) | (nan ? 1u : 0u);
@@ -567,8 +623,9 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
         if (active) {
             const bool win = (sf & 1u) ? (first == sf) : (key == sk);
             if (win) {
-                store_rel(vals, S, jr);
-                left[S] = L;
+                const uint32_t slot = lvl_slot ? lvl_slot[idx] : S;
+                store_rel(mr.vals, slot, jr);
+                mr.left[slot] = L;
             }
         }
     }
@@ -577,14 +634,14 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
 
 template <int ECW>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
-                                                      const uint32_t *lvl_sets, float4 *vals, uint32_t *left,
-                                                      uint32_t *counters) {
+                                                      const uint32_t *lvl_sets, const uint32_t *lvl_slot,
+                                                      MemoRef mr, uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    uint32_t np = eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, counters[k], vals, left, w0,
+    uint32_t np = eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
                                   (uint64_t)gridDim.x * (BLOCK / WAVE));
     add_pairs(counters, np);
 }
@@ -603,7 +660,22 @@ struct BatchJob {
     uint32_t conn_words;
     int bicc;
     int cur_k;           /* step engine: the level this job evaluates next */
+    /* open-addressing memo (tree graphs): keys, mask = capacity - 1, and the
+     * slot of each level-list entry; keys == nullptr: dense memo */
+    uint32_t *keys;
+    uint32_t hmask;
+    uint32_t *lvl_slot;
 };
+
+__device__ __forceinline__ MemoRef job_memo(const BatchJob &J) {
+    MemoRef m;
+    m.vals = J.vals;
+    m.left = J.left;
+    m.keys = J.keys;
+    m.mask = J.hmask;
+    m.err = J.counters + 49;
+    return m;
+}
 
 /* MPDP sets whose block decomposition yields more than BIG_SPLITS splits
  * (a biconnected block of 15 or more relations, e.g. a whole cycle) are not
@@ -1038,11 +1110,21 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj) {
         const BatchJob &J = jobs[j];
         const int tid = threadIdx.x;
         if (tid < 64) J.counters[tid] = 0;
+        __syncthreads();
         if (tid < J.h.n_rels) {
             const float *b = (const float *)(J.blob + J.h.o_base) + 4 * tid;
-            uint32_t s1 = 1u << tid;
-            J.vals[s1] = make_float4(b[0], b[1], b[2], b[3]);
-            J.left[s1] = 0;
+            const uint32_t s1 = 1u << tid;
+            uint32_t slot = s1;
+            if (J.keys) {                      /* keys cleared beforehand */
+                bool fresh;
+                slot = memo_insert(J.keys, J.hmask, s1, &fresh);
+                if (slot == ~0u) {
+                    atomicOr(&J.counters[49], 8u);
+                    continue;
+                }
+            }
+            J.vals[slot] = make_float4(b[0], b[1], b[2], b[3]);
+            J.left[slot] = 0;
         }
     }
 }
@@ -1165,11 +1247,13 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
         uint32_t np;
         if (h.is_tree) {
+            const MemoRef mr = job_memo(J);
+            const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
             if (h.ecw == 1)
-                np = eval_level_tree<1>(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                np = eval_level_tree<1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
                                         BLOCK / WAVE);
             else
-                np = eval_level_tree<0>(h, smem, q, k, J.bicc != 0, sets, m, J.vals, J.left, threadIdx.x / WAVE,
+                np = eval_level_tree<0>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
                                         BLOCK / WAVE);
         } else if (J.bicc) {
             BigList bl;
@@ -1206,8 +1290,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
  * inconsistent (a left side that is empty, not a proper subset, or a tree of
  * more than `cap` nodes).  cap <= 63.  Called by all 64 lanes of a
  * workgroup of one wave. */
-__device__ uint32_t walk_tree_wave(const float4 *vals, const uint32_t *left, uint32_t root_set, DPNode *nodes,
-                                   int cap) {
+__device__ uint32_t walk_tree_wave(const MemoRef &mr, uint32_t root_set, DPNode *nodes, int cap) {
     __shared__ uint32_t s_set[64];
     __shared__ int s_l[64], s_r[64], s_pre[64], s_stack[64];
     __shared__ int s_bad;
@@ -1227,7 +1310,7 @@ __device__ uint32_t walk_tree_wave(const float4 *vals, const uint32_t *left, uin
             s_l[i] = -1;
             s_r[i] = -1;
             if (__builtin_popcount(S) > 1) {
-                l = left[S];
+                l = mr.left[memo_slot(mr, S)];
                 if (l == 0 || (l & ~S) != 0 || l == S) s_bad = 1;
                 else inner = true;
             }
@@ -1264,7 +1347,7 @@ __device__ uint32_t walk_tree_wave(const float4 *vals, const uint32_t *left, uin
     __syncthreads();
     if (lane < total) {
         const uint32_t S = s_set[lane];
-        const float4 v = vals[S];
+        const float4 v = mr.vals[memo_slot(mr, S)];
         DPNode nd;
         nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
         nd.left = s_l[lane] >= 0 ? s_pre[s_l[lane]] : -1;
@@ -1281,8 +1364,66 @@ __global__ void __launch_bounds__(WAVE) k_batch_tree(const BatchJob *jobs, int n
     const BatchJob &J = jobs[j];
     const int n = J.h.n_rels;
     const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    const uint32_t r = walk_tree_wave(J.vals, J.left, full, J.nodes, 2 * n - 1);
+    const uint32_t r = walk_tree_wave(job_memo(J), full, J.nodes, 2 * n - 1);
     if (threadIdx.x == 0) J.counters[48] = r;
+}
+
+/* Fills n words with v (clears an open-addressing key array). */
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+/* Open-addressing memo, level list k from level k-1 (tree graphs): every
+ * connected set of k relations is a connected set of k-1 plus one neighbour.
+ * Each thread takes one set S of level k-1 (the singletons for k = 2) and
+ * inserts S + v for every neighbour v outside S; the insertion that claims
+ * the key appends the set and its slot to level k.  Level k is level k_fixed
+ * of every job, or each job's J.cur_k when k_fixed = 0; `prefix[j]` = first
+ * task (BLOCK source sets) of job j, host-computed from the exact level
+ * sizes.  Jobs without keys have no tasks. */
+__global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
+                                                   const uint32_t *prefix) {
+    const uint32_t total = prefix[nj];
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = nj;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (prefix[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const BatchJob &J = jobs[lo];
+        const int k = k_fixed > 0 ? k_fixed : J.cur_k;
+        const int n = J.h.n_rels;
+        const uint32_t src_count = (k - 1 == 1) ? (uint32_t)n : J.counters[k - 1];
+        const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
+        if (i >= src_count || !J.keys) continue;
+        const uint32_t S = (k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i];
+        const uint32_t *adj = J.blob + J.h.o_adj;
+        uint32_t nb = 0, rest = S;
+        while (rest) {
+            const int v = __builtin_ctz(rest);
+            rest &= rest - 1u;
+            nb |= adj[v];
+        }
+        nb &= ~S;
+        while (nb) {
+            const int v = __builtin_ctz(nb);
+            nb &= nb - 1u;
+            const uint32_t T = S | (1u << v);
+            bool fresh;
+            const uint32_t slot = memo_insert(J.keys, J.hmask, T, &fresh);
+            if (slot == ~0u) {
+                atomicOr(&J.counters[49], 8u);
+                continue;
+            }
+            if (fresh) {
+                const uint32_t pos = atomicAdd(&J.counters[k], 1u);
+                J.lvl_sets[J.lvl_off[k] + pos] = T;
+                J.lvl_slot[J.lvl_off[k] + pos] = slot;
+            }
+        }
+    }
 }
 
 /* ================================================================ DPsize
@@ -1548,9 +1689,9 @@ __global__ void __launch_bounds__(BLOCK) k_ds_finalize(BlobHdr h, const uint32_t
 
 /* Pre-order walk of the plan from `root_set` (dpsub_buildQueryTree,
  * gpuqo_query_tree.cuh:46-72, done on the device in one pass). */
-__global__ void __launch_bounds__(WAVE) k_tree(const float4 *vals, const uint32_t *left, uint32_t root_set,
-                                               DPNode *nodes, int cap, uint32_t *status) {
-    const uint32_t r = walk_tree_wave(vals, left, root_set, nodes, cap);
+__global__ void __launch_bounds__(WAVE) k_tree(MemoRef mr, uint32_t root_set, DPNode *nodes, int cap,
+                                               uint32_t *status) {
+    const uint32_t r = walk_tree_wave(mr, root_set, nodes, cap);
     if (threadIdx.x == 0) status[0] = r;
 }
 
@@ -1592,6 +1733,9 @@ struct DeviceCtx {
     /* pinned host staging: query blob + level offsets in, counters + plan out */
     uint32_t *h_stage = nullptr, *h_res = nullptr;
     size_t cap_h_stage = 0;
+    /* open-addressing memo of the single-query path */
+    uint32_t *hkeys = nullptr, *hslot = nullptr;
+    size_t cap_hkeys = 0, cap_hslot = 0;
     size_t cap_ds_best = 0, cap_ds_first = 0, cap_ds_lset = 0, cap_ds_lnbr = 0, cap_ds_raw = 0, cap_ds_tmp = 0,
            cap_ds_rc = 0, cap_ds_rf = 0, cap_ds_ro = 0, cap_ds_sz = 0, cap_l1 = 0;
     std::mutex mu;
@@ -1830,6 +1974,170 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
     return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
 }
 
+static MemoRef dense_memo(DeviceCtx &c) {
+    MemoRef m;
+    m.vals = c.b.vals;
+    m.left = c.b.left;
+    m.keys = nullptr;
+    m.mask = 0;
+    m.err = c.b.counters + 49;
+    return m;
+}
+
+/* Tree graphs with the open-addressing memo: the level lists are generated
+ * by expansion (k_expand) instead of filtering all 2^n integers, and the
+ * memo holds only the connected sets (capacity: the next power of two of
+ * twice their number, so probes stay short). */
+static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profile, std::vector<DPNode> &tree,
+                       DPStats &st) {
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    int rc;
+    if (!h.is_tree || (int)in.tree_counts.size() != n + 1 || algo == DP_DPSIZE) {
+        set_error("the open-addressing memo serves DPsub/MPDP on tree graphs");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    uint64_t total = 0, list = 0;
+    uint32_t off_h[33], tasks[33];
+    memset(off_h, 0, sizeof(off_h));
+    memset(tasks, 0, sizeof(tasks));
+    for (int k = 1; k <= n; k++) {
+        total += in.tree_counts[k];
+        if (k >= 2) {
+            off_h[k] = (uint32_t)list;
+            list += in.tree_counts[k];
+            tasks[k] = (uint32_t)((in.tree_counts[k - 1] + BLOCK - 1) / BLOCK);
+        }
+    }
+    if (2 * total > (1ull << 31)) {
+        set_error("too many connected sets for the open-addressing memo (%llu)", (unsigned long long)total);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    uint64_t cap = 64;
+    while (cap < 2 * total) cap <<= 1;
+    const size_t W4h = ((size_t)h.words + 3) & ~(size_t)3;
+    const size_t lds_eval = W4h * 4;
+    if ((rc = ensure(&c.hkeys, c.cap_hkeys, cap))) return rc;
+    if ((rc = ensure(&c.b.vals, c.cap_memo, cap))) return rc;
+    if ((rc = ensure(&c.b.left, c.cap_left, cap))) return rc;
+    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, std::max<uint64_t>(list, 1)))) return rc;
+    if ((rc = ensure(&c.hslot, c.cap_hslot, std::max<uint64_t>(list, 1)))) return rc;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33 + 2 * 33))) return rc;
+    if ((rc = ensure_stage(c, W4h + 33 + 2 * 33))) return rc;
+    if ((rc = ensure(&c.sjob, c.cap_sjob, 1))) return rc;
+    c.lvl_off = c.b.blob + W4h;
+    uint32_t *d_prefix = c.b.blob + W4h + 33;           /* [k] -> {0, tasks[k]} */
+    hipStream_t s = c.stream;
+    memcpy(c.h_stage, in.blob.data(), (size_t)h.words * 4);
+    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
+    memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
+    for (int k = 0; k < 33; k++) {
+        c.h_stage[W4h + 33 + 2 * k] = 0;
+        c.h_stage[W4h + 33 + 2 * k + 1] = tasks[k];
+    }
+    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33 + 66) * 4, hipMemcpyHostToDevice, s));
+    BatchJob sj;                                       /* alive until the synchronisation below */
+    memset(&sj, 0, sizeof(sj));
+    sj.h = h;
+    sj.blob = c.b.blob;
+    sj.vals = c.b.vals;
+    sj.left = c.b.left;
+    sj.lvl_sets = c.b.lvl_sets;
+    sj.lvl_slot = c.hslot;
+    sj.keys = c.hkeys;
+    sj.hmask = (uint32_t)(cap - 1);
+    sj.counters = c.b.counters;
+    sj.nodes = c.b.nodes;
+    memcpy(sj.lvl_off, off_h, sizeof(off_h));
+    sj.bicc = algo == DP_BICC ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(c.sjob, &sj, sizeof(sj), hipMemcpyHostToDevice, s));
+    MemoRef mr;
+    mr.vals = c.b.vals;
+    mr.left = c.b.left;
+    mr.keys = c.hkeys;
+    mr.mask = (uint32_t)(cap - 1);
+    mr.err = c.b.counters + 49;
+    if (profile && (int)c.prof_ev.size() < 3 * 33) {
+        while ((int)c.prof_ev.size() < 3 * 33) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.prof_ev.push_back(e);
+        }
+    }
+    HIPCHK(hipEventRecord(c.ev0, s));
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
+                       0, s, c.hkeys, cap, 0u);
+    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, c.sjob, 1);
+    for (int k = 2; k <= n; k++) {
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
+        hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
+                           s, c.sjob, 1, k, d_prefix + 2 * k);
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
+        const uint64_t spw = WAVE / (2ull * (k - 1));
+        const uint64_t waves = (in.tree_counts[k] + spw - 1) / spw;
+        const uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
+        if (h.ecw == 1)
+            hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sj.bicc,
+                               c.b.lvl_sets + off_h[k], (const uint32_t *)(c.hslot + off_h[k]), mr, c.b.counters);
+        else
+            hipLaunchKernelGGL(k_eval_tree<0>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sj.bicc,
+                               c.b.lvl_sets + off_h[k], (const uint32_t *)(c.hslot + off_h[k]), mr, c.b.counters);
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+    }
+    const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, mr, full, c.b.nodes, 2 * n - 1, c.b.counters + 48);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, s));
+    tree.resize(2 * n - 1);
+    const uint32_t *cnt = c.h_res;
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (cnt[49]) {
+        set_error("open-addressing memo error (flags %u)", cnt[49]);
+        return (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
+    }
+    for (int k = 2; k <= n; k++)
+        if (cnt[k] != in.tree_counts[k]) {
+            set_error("level %d holds %u sets, expected %llu", k, cnt[k], (unsigned long long)in.tree_counts[k]);
+            return GQ_ERR_DEVICE;
+        }
+    if (cnt[48] != (uint32_t)(2 * n - 1)) {
+        set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
+        return GQ_ERR_DEVICE;
+    }
+    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    st.gpu_ms = ms;
+    if (profile) {
+        for (int k = 2; k <= n; k++) {
+            float a = 0.0f, b = 0.0f;
+            HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
+            HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
+            st.filter_ms += a;
+            st.eval_ms += b;
+            st.filter_launches++;
+            st.eval_launches++;
+        }
+    }
+    st.candidates = list;            /* sets generated by expansion (no 2^n scan) */
+    st.sets = list;
+    st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+    st.memo_bytes = cap * (sizeof(float4) + 4 + 4) + list * 8;
+    st.memo_kind = GQ_MEMO_HASH;
+    return GQ_OK;
+}
+
+int resolve_memo(const DPInput &in, int requested, int max_memo_mb) {
+    const int n = in.hdr.n_rels;
+    const bool tree = in.hdr.is_tree && (int)in.tree_counts.size() == n + 1;
+    if (!tree || requested == GQ_MEMO_DENSE) return GQ_MEMO_DENSE;
+    if (requested == GQ_MEMO_HASH) return GQ_MEMO_HASH;
+    const double dense = 36.0 * (double)(1ull << n);
+    return dense > (double)max_memo_mb * 1048576.0 ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
+}
+
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
            DPStats &st) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
@@ -1846,6 +2154,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         set_error("IDP1 level cap not available on the GPU path");
         return GQ_ERR_UNSUPPORTED;
     }
+    if (in.memo == GQ_MEMO_HASH && algo != DP_DPSIZE) return run_dp_hash(c, in, algo, profile, tree, st);
     const size_t blob_bytes = (size_t)h.words * 4;
     const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
     const uint32_t conn_words_all = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
@@ -1960,7 +2269,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
         (void)cand_ub;
         uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, c.b.vals, c.b.left, full, c.b.nodes, 2 * n - 1,
+        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), full, c.b.nodes, 2 * n - 1,
                            c.b.counters + 48);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c.ev1, s));
@@ -2044,10 +2353,12 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             uint32_t tblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
             if (h.ecw == 1)
                 hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
+                                   algo == DP_BICC ? 1 : 0, sets_k, (const uint32_t *)nullptr, dense_memo(c),
+                                   c.b.counters);
             else
                 hipLaunchKernelGGL(k_eval_tree<0>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   algo == DP_BICC ? 1 : 0, sets_k, c.b.vals, c.b.left, c.b.counters);
+                                   algo == DP_BICC ? 1 : 0, sets_k, (const uint32_t *)nullptr, dense_memo(c),
+                                   c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
@@ -2092,7 +2403,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
     uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, c.b.vals, c.b.left, full, c.b.nodes, 2 * n - 1,
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), full, c.b.nodes, 2 * n - 1,
                        c.b.counters + 48);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
@@ -2186,7 +2497,40 @@ struct EJob {
     size_t off, bytes;
     BatchJob J;
     int words;
+    bool hash;
+    uint64_t cap;                        /* open-addressing capacity */
+    std::vector<uint64_t> counts;        /* hash: connected sets per size */
 };
+
+namespace {
+/* open-addressing slice: [result][blob][keys][vals][left][lvl_sets][lvl_slot] */
+size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, uint64_t *cap, size_t *o_blob,
+                         size_t *o_keys, size_t *o_vals, size_t *o_left, size_t *o_sets, size_t *o_slot) {
+    uint64_t c = 64;
+    while (c < 2 * total_sets) c <<= 1;
+    *cap = c;
+    size_t o = al256(RES_WORDS * 4);
+    *o_blob = o;
+    o += al256((size_t)words * 4);
+    *o_keys = o;
+    o += al256(c * 4);
+    *o_vals = o;
+    o += al256(c * 16);
+    *o_left = o;
+    o += al256(c * 4);
+    *o_sets = o;
+    o += al256(std::max<uint64_t>(list, 1) * 4);
+    *o_slot = o;
+    o += al256(std::max<uint64_t>(list, 1) * 4);
+    return o;
+}
+}  // namespace
+
+uint64_t engine_hash_slice_bytes(uint64_t total_sets) {
+    uint64_t cap;
+    size_t a, b, c, d, e, f;
+    return hash_slice_layout(total_sets, total_sets, (int)(ENGINE_BLOB_MAX / 4), &cap, &a, &b, &c, &d, &e, &f);
+}
 
 struct StepEngine {
     int device = 0;
@@ -2200,9 +2544,9 @@ struct StepEngine {
     BigEnt *big = nullptr;
     size_t cap_jobs = 0, cap_new = 0, cap_fin = 0, cap_tp = 0, cap_chunk = 0, cap_big = 0, cap_bigp = 0,
            cap_hdr = 0;
-    BatchJob *h_jobs = nullptr;                  /* pinned staging */
-    uint32_t *h_res = nullptr, *h_chunk = nullptr;
-    size_t cap_h_jobs = 0, cap_h_res = 0, cap_h_chunk = 0;
+    BatchJob *h_jobs = nullptr, *h_new = nullptr;    /* pinned staging */
+    uint32_t *h_res = nullptr, *h_chunk = nullptr, *h_exp = nullptr, *d_exp = nullptr;
+    size_t cap_h_jobs = 0, cap_h_new = 0, cap_h_res = 0, cap_h_chunk = 0, cap_h_exp = 0, cap_exp = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
 
     bool alloc(size_t bytes, size_t *off) {
@@ -2310,6 +2654,9 @@ void engine_destroy(StepEngine *e) {
     if (e->big_prefix) (void)hipFree(e->big_prefix);
     if (e->big_hdr) (void)hipFree(e->big_hdr);
     if (e->h_jobs) (void)hipHostFree(e->h_jobs);
+    if (e->h_new) (void)hipHostFree(e->h_new);
+    if (e->h_exp) (void)hipHostFree(e->h_exp);
+    if (e->d_exp) (void)hipFree(e->d_exp);
     if (e->h_res) (void)hipHostFree(e->h_res);
     if (e->h_chunk) (void)hipHostFree(e->h_chunk);
     if (e->e0) (void)hipEventDestroy(e->e0);
@@ -2320,10 +2667,67 @@ void engine_destroy(StepEngine *e) {
 
 int engine_active(const StepEngine *e) { return (int)(e->active.size() + e->incoming.size()); }
 
+static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    uint64_t total = 0, list = 0;
+    for (int k = 1; k <= n; k++) {
+        total += in.tree_counts[k];
+        if (k >= 2) list += in.tree_counts[k];
+    }
+    if (2 * total > (1ull << 31)) return false;
+    uint64_t cap;
+    size_t o_blob, o_keys, o_vals, o_left, o_sets, o_slot;
+    const size_t bytes = hash_slice_layout(total, list, h.words, &cap, &o_blob, &o_keys, &o_vals, &o_left, &o_sets,
+                                           &o_slot);
+    size_t off;
+    if (!e->alloc(bytes, &off)) return false;
+    char *base = e->arena + off;
+    if (hipMemcpyAsync(base + o_blob, in.blob.data(), (size_t)h.words * 4, hipMemcpyHostToDevice, e->s) !=
+        hipSuccess) {
+        e->release(off, bytes);
+        return false;
+    }
+    EJob *j = new EJob();
+    j->key = key;
+    j->n = n;
+    j->cur_k = 2;
+    j->algo = algo;
+    j->off = off;
+    j->bytes = bytes;
+    j->words = h.words;
+    j->hash = true;
+    j->cap = cap;
+    j->counts = in.tree_counts;
+    BatchJob &J = j->J;
+    memset(&J, 0, sizeof(J));
+    J.h = h;
+    J.blob = (const uint32_t *)(base + o_blob);
+    J.keys = (uint32_t *)(base + o_keys);
+    J.hmask = (uint32_t)(cap - 1);
+    J.vals = (float4 *)(base + o_vals);
+    J.left = (uint32_t *)(base + o_left);
+    J.lvl_sets = (uint32_t *)(base + o_sets);
+    J.lvl_slot = (uint32_t *)(base + o_slot);
+    J.counters = (uint32_t *)base;
+    J.nodes = (DPNode *)(J.counters + 64);
+    uint64_t acc = 0;
+    for (int k = 0; k <= 32; k++) {
+        J.lvl_off[k] = (uint32_t)acc;
+        if (k >= 2 && k <= n) acc += in.tree_counts[k];
+    }
+    J.bicc = algo == DP_BICC ? 1 : 0;
+    J.cur_k = 2;
+    e->incoming.push_back(j);
+    return true;
+}
+
 bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
     if (!fused_eligible(in, algo) || (size_t)h.words * 4 > ENGINE_BLOB_MAX) return false;
+    const bool hash = in.memo == GQ_MEMO_HASH && h.is_tree && (int)in.tree_counts.size() == n + 1;
+    if (hash) return engine_admit_hash(e, in, algo, key);
     size_t o_blob, o_vals, o_left, o_conn, o_sets, o_best, o_first;
     const size_t bytes = slice_layout(n, h.words, &o_blob, &o_vals, &o_left, &o_conn, &o_sets, &o_best, &o_first);
     size_t off;
@@ -2362,6 +2766,8 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     }
     J.bicc = algo == DP_BICC ? 1 : 0;
     J.cur_k = 2;
+    j->hash = false;
+    j->cap = 0;
     e->incoming.push_back(j);
     return true;
 }
@@ -2383,27 +2789,29 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
     /* 1. DPs admitted since the last step: init + filter, then they join */
     if (!e->incoming.empty()) {
         const int nn = (int)e->incoming.size();
-        if ((rc = ensure_pinned(&e->h_jobs, e->cap_h_jobs, (size_t)nn))) return rc;
+        if ((rc = ensure_pinned(&e->h_new, e->cap_h_new, (size_t)nn))) return rc;
         if ((rc = ensure_pinned(&e->h_chunk, e->cap_h_chunk, (size_t)nn + 1))) return rc;
         if ((rc = ensure(&e->d_new, e->cap_new, (size_t)nn))) return rc;
         if ((rc = ensure(&e->d_chunk, e->cap_chunk, (size_t)nn + 1))) return rc;
         e->h_chunk[0] = 0;
         for (int i = 0; i < nn; i++) {
-            e->h_jobs[i] = e->incoming[i]->J;
-            const uint64_t ns = 1ull << e->incoming[i]->n;
-            e->h_chunk[i + 1] = e->h_chunk[i] + (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK);
+            EJob *j = e->incoming[i];
+            e->h_new[i] = j->J;
+            const uint64_t ns = 1ull << j->n;
+            /* open-addressing jobs build their level lists by expansion */
+            e->h_chunk[i + 1] = e->h_chunk[i] + (j->hash ? 0u : (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK));
+            if (j->hash)
+                hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((j->cap + BLOCK - 1) / BLOCK, 4096)),
+                                   dim3(BLOCK), 0, s, j->J.keys, j->cap, 0u);
         }
-        ECHK(hipMemcpyAsync(e->d_new, e->h_jobs, sizeof(BatchJob) * nn, hipMemcpyHostToDevice, s));
+        ECHK(hipMemcpyAsync(e->d_new, e->h_new, sizeof(BatchJob) * nn, hipMemcpyHostToDevice, s));
         ECHK(hipMemcpyAsync(e->d_chunk, e->h_chunk, sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_batch_init, dim3(std::min(nn, 1024)), dim3(64), 0, s, e->d_new, nn);
         const uint32_t fgrid = std::min<uint32_t>(e->h_chunk[nn], 4096);
-        hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, e->d_new, nn,
-                           e->d_chunk);
+        if (fgrid)
+            hipLaunchKernelGGL(k_batch_filter, dim3(fgrid), dim3(BLOCK), 0, s, e->d_new, nn, e->d_chunk);
         for (EJob *j : e->incoming) e->active.push_back(j);
         e->incoming.clear();
-        /* the pinned staging is reused below: the copies above must have
-         * read it first */
-        ECHK(hipStreamSynchronize(s));
     }
     const int na = (int)e->active.size();
     if (na == 0) return GQ_OK;
@@ -2434,6 +2842,24 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         if ((rc = ensure(&e->big_prefix, e->cap_bigp, big_cap + 1))) return rc;
     }
     ECHK(hipMemcpyAsync(e->d_jobs, e->h_jobs, sizeof(BatchJob) * na, hipMemcpyHostToDevice, s));
+    /* open-addressing jobs: generate the level they evaluate now */
+    {
+        bool any = false;
+        for (EJob *j : e->active) any = any || j->hash;
+        if (any) {
+            if ((rc = ensure_pinned(&e->h_exp, e->cap_h_exp, (size_t)na + 1))) return rc;
+            if ((rc = ensure(&e->d_exp, e->cap_exp, (size_t)na + 1))) return rc;
+            e->h_exp[0] = 0;
+            for (int i = 0; i < na; i++) {
+                const EJob *j = e->active[i];
+                const uint64_t src = j->hash ? j->counts[j->cur_k - 1] : 0;
+                e->h_exp[i + 1] = e->h_exp[i] + (uint32_t)((src + BLOCK - 1) / BLOCK);
+            }
+            ECHK(hipMemcpyAsync(e->d_exp, e->h_exp, sizeof(uint32_t) * (na + 1), hipMemcpyHostToDevice, s));
+            const uint32_t eg = std::max<uint32_t>(1, std::min<uint32_t>(e->h_exp[na], 4096));
+            hipLaunchKernelGGL(k_expand, dim3(eg), dim3(BLOCK), 0, s, e->d_jobs, na, 0, e->d_exp);
+        }
+    }
     hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, e->d_tp);
     const uint32_t tp_lds = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
     const size_t lds_total = tp_lds ? lds + 4 * (size_t)(na + 1) : lds;
@@ -2478,18 +2904,21 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         d.key = j->key;
         d.rc = GQ_OK;
         const int n = j->n;
+        bool counts_ok = true;
+        if (j->hash)
+            for (int k = 2; k <= n; k++) counts_ok = counts_ok && cnt[k] == j->counts[k];
         if (cnt[49]) {
-            d.rc = GQ_ERR_UNSUPPORTED;
-        } else if (cnt[48] != (uint32_t)(2 * n - 1)) {
+            d.rc = (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
+        } else if (!counts_ok || cnt[48] != (uint32_t)(2 * n - 1)) {
             d.rc = GQ_ERR_DEVICE;
         } else {
             const DPNode *nodes = (const DPNode *)(cnt + 64);
             d.tree.assign(nodes, nodes + 2 * n - 1);
-            d.st.candidates = (1ull << n) - (uint64_t)n - 1ull;
             for (int k = 2; k <= n; k++) d.st.sets += cnt[k];
+            d.st.candidates = j->hash ? d.st.sets : (1ull << n) - (uint64_t)n - 1ull;
             d.st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
             d.st.memo_bytes = j->bytes;
-            d.st.memo_kind = 1;
+            d.st.memo_kind = j->hash ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
             d.st.gpu_ms = ms;
         }
         e->release(j->off, j->bytes);

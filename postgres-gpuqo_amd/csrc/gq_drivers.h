@@ -16,14 +16,15 @@ namespace gq {
 /* Per-call state: configuration, counters, and an arena owning every plan
  * node built during the call (trees share composite subtrees). */
 struct RunCtx {
-    int algorithm, bicc, idp_type, idp_n_iters, memo_kind, profile;
+    int algorithm, bicc, idp_type, idp_n_iters, memo_kind, profile, max_memo_mb;
     DPStats stats;
     int dp_calls = 0;
     std::deque<QT<1>> arena1;
     std::deque<QT<4>> arena4;
     explicit RunCtx(const gq_config *cfg)
         : algorithm(cfg->gpuqo_algorithm), bicc(cfg->gpuqo_dpsub_bicc), idp_type(cfg->gpuqo_idp_type),
-          idp_n_iters(cfg->gpuqo_idp_n_iters), memo_kind(cfg->memo_kind), profile(cfg->profile_kernels) {}
+          idp_n_iters(cfg->gpuqo_idp_n_iters), memo_kind(cfg->memo_kind), profile(cfg->profile_kernels),
+          max_memo_mb(cfg->gpuqo_max_memo_size_mb) {}
     template <int W>
     QT<W> *node();
     void add(const DPStats &s) {

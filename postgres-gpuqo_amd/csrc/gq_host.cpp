@@ -218,6 +218,8 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     for (size_t i = 0; i < q.sels.size(); i++) put_f(h.o_sels, (int)i, q.sels[i]);
     h.o_qualtab = take(n_ec + 1);
     for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
+    in.tree_counts.clear();
+    in.memo = GQ_MEMO_DENSE;
     h.o_desc = take(32);
     h.o_lglob = take(32);
     h.tree_ecm = 0;
@@ -254,6 +256,30 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
             const int u = ctz(rel), v = 31 - __builtin_clz(rel);   /* u < v */
             if (parent[v] != u) { ok = false; break; }
             ecm[(size_t)v * ecw + e / 64] |= 1ull << (e % 64);
+        }
+        /* connected sets per size: P_v(x) = x * prod_{c child of v} (1 + P_c(x)),
+         * summed over v (each subtree counted at its top vertex) */
+        {
+            std::vector<std::vector<uint64_t>> P(n);
+            std::vector<uint64_t> tot(n + 1, 0ull);
+            for (int v = n - 1; v >= 0; v--) {
+                std::vector<uint64_t> poly(n + 1, 0ull);
+                poly[1] = 1;
+                for (int c = v + 1; c < n; c++) {
+                    if (parent[c] != v) continue;
+                    std::vector<uint64_t> q(P[c]);
+                    q[0] += 1;
+                    std::vector<uint64_t> r(n + 1, 0ull);
+                    for (int a = 0; a <= n; a++) {
+                        if (!poly[a]) continue;
+                        for (int b = 0; a + b <= n; b++) r[a + b] += poly[a] * q[b];
+                    }
+                    poly.swap(r);
+                }
+                P[v] = poly;
+                for (int k = 0; k <= n; k++) tot[k] += poly[k];
+            }
+            in.tree_counts = tot;
         }
         if (ok) {
             align2();
@@ -530,6 +556,8 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
         gq::convert(qs[i], cfg, h);
         if (gq::prepare_plain(h, preps[i])) return;
         if (gq::build_dp_input(preps[i].ri, ins[i])) return;
+        /* the lockstep batch keeps dense memos; others are planned alone */
+        if (gq::resolve_memo(ins[i], cfg->memo_kind, cfg->gpuqo_max_memo_size_mb) != GQ_MEMO_DENSE) return;
         if (!gq::fused_eligible(ins[i], algo)) return;
         ok[i] = 1;
     });

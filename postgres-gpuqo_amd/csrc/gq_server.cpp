@@ -291,14 +291,63 @@ int ServerExecutor::run(const std::vector<const DPInput *> &ins, const std::vect
     return w.rc;
 }
 
+/* Connected sets of a tree-shaped query graph (0 when the graph is not a
+ * tree): sum over vertices v of P_v(1), P_v(x) = x prod_c (1 + P_c(x)). */
+static uint64_t tree_set_count(const gq_query *q) {
+    const int n = q->n_rels, nw = q->n_words;
+    if (n < 1 || n > 32) return 0;
+    std::vector<uint32_t> adj(n, 0u);
+    uint64_t deg = 0;
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++)
+            if (j != i && ((q->edge_table[(size_t)i * nw + j / 64] >> (j % 64)) & 1ull)) adj[i] |= 1u << j;
+        deg += (uint64_t)__builtin_popcount(adj[i]);
+    }
+    if (deg != 2ull * (uint64_t)(n - 1)) return 0;
+    std::vector<int> order(1, 0), parent(n, -1);
+    uint32_t seen = 1u;
+    for (size_t x = 0; x < order.size(); x++) {
+        const int v = order[x];
+        uint32_t nb = adj[v] & ~seen;
+        while (nb) {
+            const int u = __builtin_ctz(nb);
+            nb &= nb - 1u;
+            seen |= 1u << u;
+            parent[u] = v;
+            order.push_back(u);
+        }
+    }
+    if ((int)order.size() != n) return 0;
+    std::vector<uint64_t> f(n, 1ull);          /* f(v) = P_v(1) */
+    uint64_t total = 0;
+    for (int x = n - 1; x >= 0; x--) {
+        const int v = order[x];
+        total += f[v];
+        if (parent[v] >= 0) f[parent[v]] *= 1ull + f[v];
+    }
+    return total;
+}
+
 /* Memo bytes a request will hold at most at one time on the engine. */
 static uint64_t projected_bytes(const gq_query *q, const gq_config *c) {
     const int n = q->n_rels;
     const int k = c->gpuqo_idp_n_iters;
-    if (k <= 1 || k >= n) return engine_slice_bytes(std::min(n, GQ_MAX_DP_RELS - 2));
-    /* IDP2 / UnionDP: DPs of at most k relations, plus a level of small
-     * UnionDP partitions (fewer than 16 relations each) batched together */
-    return engine_slice_bytes(std::min(k, GQ_MAX_DP_RELS - 2)) + 32 * engine_slice_bytes(15);
+    const double max_memo = (double)c->gpuqo_max_memo_size_mb * 1048576.0;
+    if (k <= 1 || k >= n) {
+        const int nd = std::min(n, GQ_MAX_DP_RELS - 2);
+        const uint64_t dense = engine_slice_bytes(nd);
+        const uint64_t sets = tree_set_count(q);
+        const bool hash = sets && c->gpuqo_algorithm == GQ_ALGO_DPSUB &&
+                          (c->memo_kind == GQ_MEMO_HASH ||
+                           (c->memo_kind == GQ_MEMO_AUTO && 36.0 * (double)(1ull << nd) > max_memo));
+        return hash ? engine_hash_slice_bytes(sets) : dense;
+    }
+    /* IDP2 / UnionDP: DPs of at most k relations (dense up to the memo
+     * limit, open-addressing above it), plus a level of small UnionDP
+     * partitions (fewer than 16 relations each) batched together */
+    const uint64_t dp = std::min<uint64_t>(engine_slice_bytes(std::min(k, GQ_MAX_DP_RELS - 2)),
+                                           (uint64_t)max_memo + (64ull << 20));
+    return dp + 32 * engine_slice_bytes(15);
 }
 
 }  // namespace gq
