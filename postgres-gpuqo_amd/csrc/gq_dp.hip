@@ -1374,17 +1374,34 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-/* Open-addressing memo, level list k from level k-1 (tree graphs): every
- * connected set of k relations is a connected set of k-1 plus one neighbour.
- * Each thread takes one set S of level k-1 (the singletons for k = 2) and
- * inserts S + v for every neighbour v outside S; the insertion that claims
- * the key appends the set and its slot to level k.  Level k is level k_fixed
- * of every job, or each job's J.cur_k when k_fixed = 0; `prefix[j]` = first
- * task (BLOCK source sets) of job j, host-computed from the exact level
- * sizes.  Jobs without keys have no tasks. */
+/* Open-addressing memo, level list k from level k-1 (tree graphs).  Every
+ * connected set T of k >= 2 relations is generated exactly once: from
+ * S = T - v where v is the highest-numbered leaf of T (removing a leaf keeps
+ * a tree connected).  Each thread takes one set S of level k-1 (the
+ * singletons for k = 2) and emits S + v for each neighbour v outside S that
+ * is that leaf, i.e. no member u > v of S has at most one neighbour in
+ * S + v.  A wave reserves its list positions with one atomic, then each
+ * emitted set claims its key (a key already present would be a duplicate:
+ * error bit 16).  Level k is level k_fixed of every job, or each job's
+ * J.cur_k when k_fixed = 0; `prefix[j]` = first task (BLOCK source sets) of
+ * job j, host-computed from the exact level sizes; jobs without keys have
+ * no tasks.  Order within a level list is immaterial: each set's best split
+ * is computed independently. */
+__device__ __forceinline__ bool is_top_leaf(const uint32_t *adj, uint32_t S, int v) {
+    const uint32_t T = S | (1u << v);
+    uint32_t above = S & ~((2u << v) - 1u);       /* members of S numbered above v */
+    while (above) {
+        const int u = __builtin_ctz(above);
+        above &= above - 1u;
+        if (__builtin_popcount(adj[u] & T) <= 1) return false;
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
                                                    const uint32_t *prefix) {
     const uint32_t total = prefix[nj];
+    const int lane = threadIdx.x & (WAVE - 1);
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nj;
         while (hi - lo > 1) {
@@ -1397,31 +1414,58 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
         const int n = J.h.n_rels;
         const uint32_t src_count = (k - 1 == 1) ? (uint32_t)n : J.counters[k - 1];
         const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
-        if (i >= src_count || !J.keys) continue;
-        const uint32_t S = (k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i];
+        const bool have = i < src_count && J.keys != nullptr;
         const uint32_t *adj = J.blob + J.h.o_adj;
-        uint32_t nb = 0, rest = S;
-        while (rest) {
-            const int v = __builtin_ctz(rest);
-            rest &= rest - 1u;
-            nb |= adj[v];
+        uint32_t S = 0, nb = 0, emit = 0;
+        if (have) {
+            S = (k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i];
+            uint32_t rest = S;
+            while (rest) {
+                const int v = __builtin_ctz(rest);
+                rest &= rest - 1u;
+                nb |= adj[v];
+            }
+            nb &= ~S;
+            uint32_t c = nb;
+            while (c) {
+                const int v = __builtin_ctz(c);
+                c &= c - 1u;
+                if (is_top_leaf(adj, S, v)) emit |= 1u << v;
+            }
         }
-        nb &= ~S;
-        while (nb) {
-            const int v = __builtin_ctz(nb);
-            nb &= nb - 1u;
+        /* wave-aggregated reservation of list positions */
+        const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
+        uint32_t incl = cnt;
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        const uint32_t wave_total = __shfl(incl, WAVE - 1);
+        /* lanes of one wave may belong to different jobs only at a job
+         * boundary of the task range; reserve per (wave, job) */
+        const int first_lane_job = __shfl(lo, 0);
+        const bool uniform = __all(lo == first_lane_job);
+        uint32_t base = 0;
+        if (uniform) {
+            if (lane == 0 && wave_total) base = atomicAdd(&J.counters[k], wave_total);
+            base = __shfl(base, 0) + incl - cnt;
+        } else if (cnt) {
+            base = atomicAdd(&J.counters[k], cnt);
+        }
+        uint32_t pos = base;
+        while (emit) {
+            const int v = __builtin_ctz(emit);
+            emit &= emit - 1u;
             const uint32_t T = S | (1u << v);
             bool fresh;
             const uint32_t slot = memo_insert(J.keys, J.hmask, T, &fresh);
-            if (slot == ~0u) {
-                atomicOr(&J.counters[49], 8u);
+            if (slot == ~0u || !fresh) {
+                atomicOr(&J.counters[49], slot == ~0u ? 8u : 16u);
                 continue;
             }
-            if (fresh) {
-                const uint32_t pos = atomicAdd(&J.counters[k], 1u);
-                J.lvl_sets[J.lvl_off[k] + pos] = T;
-                J.lvl_slot[J.lvl_off[k] + pos] = slot;
-            }
+            J.lvl_sets[J.lvl_off[k] + pos] = T;
+            J.lvl_slot[J.lvl_off[k] + pos] = slot;
+            pos++;
         }
     }
 }
