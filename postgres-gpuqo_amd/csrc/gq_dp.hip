@@ -367,11 +367,12 @@ __global__ void __launch_bounds__(BLOCK) k_eval_dpsub_chunk(BlobHdr h, const uin
                                                              const float4 *vals, const uint32_t *conn,
                                                              uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    uint64_t nwaves = (uint64_t)counters[k] * cps;
+    if ((uint64_t)blockIdx.x * (BLOCK / WAVE) >= nwaves) return;
     stage(blob, smem, h.words);
     View<uint32_t> q = make_view(h, smem);
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t T = (1u << k) - 2u;
-    uint64_t nwaves = (uint64_t)counters[k] * cps;
     uint64_t wstride = (uint64_t)gridDim.x * (BLOCK / WAVE);
     uint32_t npairs = 0;
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; w < nwaves; w += wstride) {
@@ -422,9 +423,10 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
                                                            const uint32_t *lvl_first, float4 *vals,
                                                            uint32_t *left, const uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    uint32_t count = counters[k];
+    if ((uint64_t)blockIdx.x * blockDim.x >= count) return;
     stage(blob, smem, h.words);
     View<uint32_t> q = make_view(h, smem);
-    uint32_t count = counters[k];
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
         uint32_t S = lvl_sets[idx];
         uint32_t f = lvl_first[idx];
@@ -637,6 +639,10 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    /* the grid may be sized by a bound on the level: idle workgroups leave
+     * before staging the blob */
+    const uint32_t spw = WAVE / (2u * (k - 1));
+    if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= counters[k]) return;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
@@ -906,6 +912,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
                                                     uint32_t *left, const uint32_t *conn, uint32_t *counters,
                                                     BigList bl) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    if ((uint64_t)blockIdx.x * B >= counters[k]) return;      /* no batch for this workgroup */
     const int tid = threadIdx.x;
     const int W4 = (h.words + 3) & ~3;
     uint32_t *s_misc = smem + W4 + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
@@ -2393,8 +2400,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         const uint64_t T = (1ull << k) - 2;
         if (h.is_tree) {
             const uint64_t spw = WAVE / (2ull * (k - 1));
-            uint64_t waves = (Ck + spw - 1) / spw;
-            uint32_t tblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
+            /* exact level size of a tree (subtree polynomial), else C(n,k) */
+            const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
+            uint64_t waves = (lvl + spw - 1) / spw;
+            uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
             if (h.ecw == 1)
                 hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
                                    algo == DP_BICC ? 1 : 0, sets_k, (const uint32_t *)nullptr, dense_memo(c),
