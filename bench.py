@@ -216,7 +216,9 @@ def cpu_baseline(g, algo, bicc, budget_s=12.0):
     res["seq"] = {"algorithm": seq_algo, "runs": len(times), "median_ms": med * 1e3, "pairs": pairs,
                   "pairs_per_s": pairs / med}
     if algo == "dpsub":
-        ncpu = os.cpu_count() or 1
+        # threads: the cores this process may run on, at most 16 (the host
+        # share of one GPU)
+        ncpu = max(1, min(len(os.sched_getaffinity(0)), 16))
         par_algo = "parallel_cpu_dpsub_bicc" if bicc else "parallel_cpu_dpsub"
         pcfg = abi.default_config(algorithm=par_algo, dpe_n_threads=ncpu, cpu_dpsub_parallel_chunk_size=256)
         ptimes = []
@@ -230,6 +232,21 @@ def cpu_baseline(g, algo, bicc, budget_s=12.0):
         pm = statistics.median(ptimes)
         res["parallel"] = {"algorithm": par_algo, "threads": ncpu, "chunk": 256, "runs": len(ptimes),
                            "median_ms": pm * 1e3, "pairs_per_s": pairs / pm}
+        # dpe_dpccp: DPccp enumeration feeding the DPE worker pool
+        # (gpuqo_dpe_pairs_per_depbuf 65536, run_all_generic.sh:63)
+        dcfg = abi.default_config(algorithm="dpe_dpccp", dpe_n_threads=ncpu, dpe_pairs_per_depbuf=65536)
+        dtimes = []
+        t_start = time.time()
+        while True:
+            t0 = time.perf_counter()
+            st, nodes, stats = oracle.plan(g, dcfg)
+            dtimes.append(time.perf_counter() - t0)
+            if time.time() - t_start > budget_s / 2 or len(dtimes) >= 20:
+                break
+        dm = statistics.median(dtimes)
+        res["dpe_dpccp"] = {"algorithm": "dpe_dpccp", "threads": ncpu, "pairs_per_depbuf": 65536,
+                            "runs": len(dtimes), "median_ms": dm * 1e3, "pairs_per_s": pairs / dm,
+                            "note": "submits each pair twice in one orientation, as the reference's DPE"}
     return res
 
 
@@ -358,9 +375,11 @@ def main():
                                "sample": f"{seq['runs']} full plans of the same query with {seq['algorithm']} "
                                          f"(median {seq['median_ms']:.1f} ms)",
                                "plan_ms": seq["median_ms"], "host": platform.processor() or platform.machine(),
-                               "nproc": os.cpu_count()}
+                               "nproc": os.cpu_count(), "cores_available": len(os.sched_getaffinity(0))}
         if "parallel" in cb:
             out["cpu_baseline"]["parallel"] = cb["parallel"]
+        if "dpe_dpccp" in cb:
+            out["cpu_baseline"]["dpe_dpccp"] = cb["dpe_dpccp"]
         # parity of the last GPU plan against the oracle (outside the timed region)
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle

@@ -1214,11 +1214,303 @@ static QueryTree<W> *cpu_goo(const Info<W> &info) {
  * Dispatch and drivers: gpuqo_main_internal.cu, gpuqo_idp.cu,
  * gpuqo_dpdp_union.cu
  * ---------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------
+ * DPccp: gpuqo_cpu_dpccp.cu:27-103.  Connected subgraphs are grown from
+ * each relation, highest first; for each one its complements are grown from
+ * its neighbours, highest first.  `join(L, R)` receives every csg-cmp pair
+ * once, in the reference's order, as emit() does (64-bit sets).
+ * ---------------------------------------------------------------------- */
+template <typename J>
+struct DPccpEnum {
+    const Info<1> &info;
+    J &join;
+    static Bits<1> lower_inc(const Bits<1> &s) {          /* bits | (bits - 1) */
+        Bits<1> b;
+        b.w[0] = s.w[0] | (s.w[0] - 1);
+        return b;
+    }
+    static Bits<1> highest(const Bits<1> &s) {
+        Bits<1> b;
+        b.w[0] = s.w[0] ? (1ull << (63 - __builtin_clzll(s.w[0]))) : 0ull;
+        return b;
+    }
+    void csg_rec(const Bits<1> &S, const Bits<1> &X, const Bits<1> &cmp) {   /* :31-49 */
+        const Bits<1> N = get_neighbours(S, info.edge_table) - X;
+        if (N.empty()) return;
+        for (Bits<1> sub = N.lowest(); !sub.empty(); sub = nextSubset(sub, N)) emit(cmp, S | sub);
+        for (Bits<1> sub = N.lowest(); !sub.empty(); sub = nextSubset(sub, N)) csg_rec(S | sub, X | N, cmp);
+    }
+    void cmp(const Bits<1> &S) {                           /* :60-78 */
+        const Bits<1> X = lower_inc(S);
+        const Bits<1> N = get_neighbours(S, info.edge_table) - X;
+        Bits<1> t = N;
+        while (!t.empty()) {
+            const Bits<1> v = highest(t);
+            emit(S, v);
+            csg_rec(v, X | (lower_inc(v) & N), S);
+            t -= v;
+        }
+    }
+    void emit(const Bits<1> &l, const Bits<1> &r) {        /* :80-103 */
+        if (!l.empty() && !r.empty()) join(l, r);
+        else if (!l.empty()) cmp(l);
+        else cmp(r);
+    }
+    void run() {                                           /* :51-58 */
+        for (int i = info.n_rels; i >= 1; i--) {
+            Bits<1> s;
+            s.w[0] = 1ull << i;
+            emit(s, Bits<1>());
+            csg_rec(s, lower_inc(s), Bits<1>());
+        }
+    }
+};
+
+/* cpu_dpccp: the sequential framework with try_swap, i.e. both
+ * orientations of join(memo[R], memo[L]) (gpuqo_cpu_sequential.cu:34-58). */
+static QueryTree<1> *cpu_dpccp(const Info<1> &info, Counters &c) {
+    Memo<1> memo;
+    init_memo(info, memo);
+    auto join = [&](const Bits<1> &l, const Bits<1> &r) {
+        JR<1> &L = *memo.find(l)->second, &R = *memo.find(r)->second;
+        c.checks++;
+        JR<1> *o;
+        if (do_join(R, L, info, memo, &o, c)) c.sets++;
+        if (do_join(L, R, info, memo, &o, c)) c.sets++;
+    };
+    DPccpEnum<decltype(join)> e{info, join};
+    e.run();
+    return finish_memo(info, memo);
+}
+
+/* ------------------------------------------------------------------------
+ * dpe_dpccp: DPccp feeding the DPE framework (gpuqo_cpu_dpe.cu:55-417,
+ * gpuqo_dependency_buffer.cu:30-129).  The enumerating thread appends join
+ * jobs to the "next" dependency buffer: jobs with the same target relation
+ * form one ring, processed by one thread; targets are queued per size,
+ * those whose operands are ready at the head.  A full buffer is swapped
+ * with the "current" one once the gpuqo_dpe_n_threads - 1 workers (helped
+ * by the enumerating thread) have drained it.  A job waits until both of
+ * its operands have no pending ring.  As in the reference, a pair is
+ * submitted twice in the same orientation (try_swap, :285-288), so this is
+ * a timing baseline, not a parity oracle.
+ * ---------------------------------------------------------------------- */
+struct DpeRel {
+    JR<1> r;
+    std::atomic<int> num_entry{0};
+};
+
+struct DepNode {
+    DpeRel *target, *left, *right;
+    DepNode *next_join, *prev_join, *next_rel, *prev_rel;
+};
+
+struct DepBuf {
+    int n_rels, capacity;
+    std::vector<DepNode> nodes;
+    int next_free = 0;
+    std::vector<DepNode *> queues;
+    std::atomic<DepNode *> unified{nullptr};
+    std::unordered_map<uint64_t, DepNode *> lookup;
+    DepBuf(int n, int cap) : n_rels(n), capacity(cap), nodes((size_t)cap), queues((size_t)n + 2, nullptr) {}
+    bool full() const { return next_free >= capacity; }
+    bool empty() const { return unified.load() == nullptr; }
+    void clear() {
+        next_free = 0;
+        std::fill(queues.begin(), queues.end(), nullptr);
+        lookup.clear();
+        unified = nullptr;
+    }
+    void push(DpeRel *t, DpeRel *l, DpeRel *r) {               /* :30-95 */
+        const int index = t->r.id.size();
+        DepNode *nn = &nodes[(size_t)next_free++];
+        nn->target = t; nn->left = l; nn->right = r;
+        nn->next_rel = nn->prev_rel = nn->next_join = nn->prev_join = nn;
+        const bool ready = l->num_entry.load() == 0 && r->num_entry.load() == 0;
+        auto it = lookup.find(t->r.id.w[0]);
+        if (queues[(size_t)index] == nullptr) {
+            queues[(size_t)index] = nn;
+            lookup.emplace(t->r.id.w[0], nn);
+            t->num_entry++;
+        } else if (it == lookup.end()) {
+            DepNode *h = queues[(size_t)index];
+            nn->next_rel = h;
+            nn->prev_rel = h->prev_rel;
+            h->prev_rel->next_rel = nn;
+            h->prev_rel = nn;
+            if (ready) queues[(size_t)index] = nn;
+            lookup.emplace(t->r.id.w[0], nn);
+            t->num_entry++;
+        } else {
+            DepNode *node = it->second;
+            nn->next_join = node;
+            nn->prev_join = node->prev_join;
+            node->prev_join->next_join = nn;
+            node->prev_join = nn;
+            if (ready) {                                     /* the ready job represents the ring */
+                if (node != node->next_rel) {
+                    node->prev_rel->next_rel = nn;
+                    node->next_rel->prev_rel = nn;
+                    nn->prev_rel = node->prev_rel;
+                    nn->next_rel = node->next_rel;
+                }
+                if (queues[(size_t)index] == node) queues[(size_t)index] = nn;
+                it->second = nn;
+            }
+        }
+    }
+    void unify() {                                          /* :97-117 */
+        DepNode *node = nullptr;
+        for (size_t i = 0; i < queues.size(); i++) {
+            if (!queues[i]) continue;
+            if (node) {
+                node->prev_rel->next_rel = queues[i];
+                node = queues[i];
+            } else {
+                node = queues[i];
+                unified = node;
+            }
+        }
+        if (node) node->prev_rel->next_rel = nullptr;
+    }
+    DepNode *pop() {                                        /* :119-127 */
+        DepNode *n = unified.load();
+        while (n && !unified.compare_exchange_strong(n, n->next_rel)) {
+        }
+        return n;
+    }
+};
+
+static void dpe_process(DepBuf *buf, const Info<1> &info, std::atomic<uint64_t> &joins) {   /* :60-93 */
+    DepNode *job;
+    while ((job = buf->pop()) != nullptr) {
+        DpeRel *t = job->target;
+        DepNode *first = job;
+        do {
+            while (job->left->num_entry.load() != 0) {
+            }
+            while (job->right->num_entry.load() != 0) {
+            }
+            const JR<1> &L = job->left->r, &R = job->right->r;
+            JoinRel v = make_join(L.id, L.v, R.id, R.v, info);
+            joins++;
+            if (v.cost.total < t->r.v.cost.total) {
+                t->r.v = v;
+                t->r.left_id = L.id;
+                t->r.right_id = R.id;
+                t->r.lp = const_cast<JR<1> *>(&L);
+                t->r.rp = const_cast<JR<1> *>(&R);
+            }
+        } while ((job = job->next_join) != first);
+        t->num_entry--;
+    }
+}
+
+static QueryTree<1> *dpe_dpccp(const Info<1> &info, int n_threads, int pairs_per_depbuf, Counters &c) {
+    n_threads = std::max(1, n_threads);
+    pairs_per_depbuf = std::max(2, pairs_per_depbuf);
+    std::unordered_map<uint64_t, DpeRel *> memo;
+    for (int i = 0; i < info.n_rels; i++) {                 /* :371-386 */
+        DpeRel *d = new DpeRel();
+        d->r.id = info.base_rels[i].id;
+        d->r.lp = d->r.rp = nullptr;
+        d->r.v.cost = info.base_rels[i].cost;
+        d->r.v.width = info.base_rels[i].width;
+        d->r.v.rows = info.base_rels[i].rows;
+        d->r.edges = info.edge_table[i];
+        memo.emplace(d->r.id.w[0], d);
+    }
+    DepBuf b0(info.n_rels, pairs_per_depbuf), b1(info.n_rels, pairs_per_depbuf);
+    DepBuf *cur = &b0, *next = &b1;
+    std::mutex mu;
+    std::condition_variable avail, all_waiting;
+    int n_waiting = 0;
+    uint64_t generation = 0;
+    bool finish = false;
+    std::atomic<uint64_t> joins{0};
+    auto worker = [&]() {                                   /* :95-140 */
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                while (generation == seen && !finish) {
+                    n_waiting++;
+                    if (n_waiting == n_threads - 1) all_waiting.notify_one();
+                    avail.wait(lk, [&] { return generation != seen || finish; });
+                }
+                if (finish) break;
+                seen = generation;
+            }
+            dpe_process(cur, info, joins);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < n_threads - 1; t++) th.emplace_back(worker);
+    auto wait_and_swap = [&]() {                            /* :205-228 */
+        next->unify();
+        dpe_process(cur, info, joins);
+        std::unique_lock<std::mutex> lk(mu);
+        all_waiting.wait(lk, [&] { return n_waiting >= n_threads - 1; });
+        std::swap(cur, next);
+        n_waiting = 0;
+        generation++;
+        avail.notify_all();
+        lk.unlock();
+        next->clear();
+    };
+    auto submit = [&](DpeRel *l, DpeRel *r) {               /* :167-200 */
+        Bits<1> id = l->r.id | r->r.id;
+        DpeRel *t;
+        auto it = memo.find(id.w[0]);
+        if (it != memo.end()) {
+            t = it->second;
+        } else {
+            t = new DpeRel();
+            t->r.id = id;
+            t->r.left_id = l->r.id; t->r.right_id = r->r.id;
+            t->r.lp = &l->r; t->r.rp = &r->r;
+            t->r.edges = l->r.edges | r->r.edges;
+            t->r.v.cost.total = INFF_;
+            t->r.v.cost.startup = INFF_;
+            t->r.v.rows = INFF_;
+            t->r.v.width = 0;
+            memo.emplace(id.w[0], t);
+            c.sets++;
+        }
+        next->push(t, l, r);
+        if (next->full()) wait_and_swap();
+    };
+    auto join = [&](const Bits<1> &lid, const Bits<1> &rid) {   /* join(level, true, memo[R], memo[L]) */
+        DpeRel *L = memo.find(rid.w[0])->second, *R = memo.find(lid.w[0])->second;
+        c.checks++;
+        submit(L, R);
+        submit(L, R);                                       /* same orientation again (:285-288) */
+    };
+    DPccpEnum<decltype(join)> e{info, join};
+    e.run();
+    wait_and_swap();
+    dpe_process(cur, info, joins);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        finish = true;
+        avail.notify_all();
+    }
+    for (auto &t : th) t.join();
+    c.joins += joins.load();
+    Bits<1> full;
+    for (int i = 0; i < info.n_rels; i++) full |= info.base_rels[i].id;
+    QueryTree<1> *out = nullptr;
+    auto f = memo.find(full.w[0]);
+    if (f != memo.end()) build_query_tree(&f->second->r, &out);
+    for (auto &kv : memo) delete kv.second;
+    return out;
+}
+
 struct RunCtx {
     int algorithm;
     bool bicc;
     int idp_type, idp_n_iters;
-    int n_threads, chunk;
+    int n_threads, chunk, pairs_per_depbuf;
     Counters c;
     int dp_calls = 0;
 };
@@ -1251,6 +1543,10 @@ QueryTree<1> *run_switch<1>(RunCtx &ctx, const Info<1> &info) {     /* gpuqo_mai
         return parallel_cpu_dpsub(info, true, ctx.n_threads, ctx.chunk, ctx.c);
     case GQ_ALGO_CPU_GOO:
         return cpu_goo(info);
+    case GQ_ALGO_CPU_DPCCP:
+        return cpu_dpccp(info, ctx.c);
+    case GQ_ALGO_DPE_DPCCP:
+        return dpe_dpccp(info, ctx.n_threads, ctx.pairs_per_depbuf, ctx.c);
     default:
         return nullptr;
     }
@@ -1624,6 +1920,7 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     ctx.idp_n_iters = cfg->gpuqo_idp_n_iters;
     ctx.n_threads = cfg->gpuqo_dpe_n_threads > 0 ? cfg->gpuqo_dpe_n_threads : 1;
     ctx.chunk = cfg->gpuqo_cpu_dpsub_parallel_chunk_size > 0 ? cfg->gpuqo_cpu_dpsub_parallel_chunk_size : 64;
+    ctx.pairs_per_depbuf = cfg->gpuqo_dpe_pairs_per_depbuf > 0 ? cfg->gpuqo_dpe_pairs_per_depbuf : 128;
     Info<W> *info = convert<W>(q, cfg);
     auto t0 = std::chrono::steady_clock::now();
     QueryTree<W> *qt = gpuqo_run_t(ctx, *info);

@@ -405,3 +405,48 @@ def test_idp2_plan_is_a_full_join_tree(k):
     g = qgen.snowflake(20, 4)
     st, nodes, _ = oracle.plan(g, cfg("dpsub", dpsub_bicc=1, idp_type="idp2", idp_n_iters=k))
     assert st == 0 and len(nodes) == 39
+
+
+# ---------------------------------------------------------------- DPccp / DPE
+
+@pytest.mark.parametrize("n", [3, 6, 9])
+def test_dpccp_visits_exactly_the_csg_cmp_pairs(n):
+    """cpu_dpccp emits each csg-cmp pair once and costs both orientations:
+    its join count is the ordered pair count of the closed forms."""
+    cases = [
+        (qgen.star(n, 1), (n - 1) * 2 ** (n - 1)),
+        (qgen.clique(n, 2), 3 ** n - 2 ** (n + 1) + 1),
+        (qgen.chain(n, 3), (n ** 3 - n) // 3),
+        (qgen.cycle(n, 4), n ** 3 - 2 * n ** 2 + n),
+    ]
+    for g, want in cases:
+        st, nodes, stats = oracle.plan(g, cfg("cpu_dpccp"))
+        assert st == 0
+        assert stats.pairs == want, (g.shape, n)
+        assert stats.candidates * 2 == want, (g.shape, n)
+
+
+@pytest.mark.parametrize("name,g", [("snow12", qgen.snowflake(12, 5)), ("clique8", qgen.clique(8, 6)),
+                                    ("cycle9", qgen.cycle(9, 7)), ("star10", qgen.star(10, 8))])
+def test_dpccp_reaches_the_dpsub_optimum(name, g):
+    """Both explore every bushy plan without cross products: the optimal
+    total cost is the same float."""
+    _, want, _ = oracle.plan(g, cfg("cpu_dpsub"))
+    st, got, _ = oracle.plan(g, cfg("cpu_dpccp"))
+    assert st == 0
+    assert gpuqo_amd.fbits(got[0][3]) == gpuqo_amd.fbits(want[0][3]), name
+
+
+@pytest.mark.parametrize("threads,depbuf", [(1, 128), (4, 8), (4, 65536)])
+def test_dpe_dpccp_returns_a_full_tree(threads, depbuf):
+    """dpe_dpccp submits each pair twice in one orientation (the reference's
+    DPE try_swap), so it is a timing baseline: a full join tree whose cost is
+    never below the optimum."""
+    g = qgen.snowflake(14, 9)
+    _, best, _ = oracle.plan(g, cfg("cpu_dpsub"))
+    st, nodes, stats = oracle.plan(g, cfg("dpe_dpccp", dpe_n_threads=threads, dpe_pairs_per_depbuf=depbuf))
+    assert st == 0
+    assert len(nodes) == 2 * g.n - 1
+    assert nodes[0][3] >= best[0][3]
+    root = nodes[0][0]
+    assert sum(bin(w).count("1") for w in root) == g.n
