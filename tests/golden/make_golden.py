@@ -1,0 +1,78 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/plans_v1.json: the oracle's plans for seeded query
+graphs (qgen), one record per (graph, configuration).
+
+A record holds the generator call, a digest of the generated arrays (so a
+change of the generator shows up as a digest mismatch, not as a plan
+mismatch), the configuration, and the expected plan: per node in pre-order
+the relation-set words, rows/startup/total as IEEE-754 bit patterns, width
+and child indices; plus the pair and set counters.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "postgres-gpuqo_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import gpuqo_amd  # noqa: E402
+import oracle  # noqa: E402  (the oracle is the checker these fixtures pin)
+from gpuqo_amd import abi, qgen  # noqa: E402
+
+# (label, generator, args, kwargs, config overrides)
+CASES = [
+    ("config1-star12-dpsub", "config1", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("config2-snowflake20-mpdp", "config2", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("star8-dpsize", "star", [8, 11], {}, {"algorithm": "dpsize"}),
+    ("chain9-dpsub", "chain", [9, 22], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("chain9-mpdp", "chain", [9, 22], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("cycle8-dpsub", "cycle", [8, 33], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("cycle16-mpdp", "cycle", [16, 0x1610], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("clique7-dpsub", "clique", [7, 44], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("clique7-dpsize", "clique", [7, 44], {}, {"algorithm": "dpsize"}),
+    ("clique7-mpdp", "clique", [7, 44], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("snowflake14-dpsub", "snowflake", [14, 55], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("snowflake14-dpsize", "snowflake", [14, 55], {}, {"algorithm": "dpsize"}),
+    ("chorded15-mpdp", "chorded_cycle", [15, 5, 0x1505], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("snowflake70-uniondp25", "snowflake", [70, 77], {"depth": 4, "fanout": 5},
+     {"algorithm": "dpsub", "dpsub_bicc": 1, "idp_type": "union", "idp_n_iters": 25}),
+    ("snowflake20-idp2-k8", "snowflake", [20, 4], {},
+     {"algorithm": "dpsub", "dpsub_bicc": 1, "idp_type": "idp2", "idp_n_iters": 8}),
+]
+
+
+def digest(g):
+    h = hashlib.sha256()
+    for a in (g.base, g.edge_table, g.ec_offsets, g.ec_members, g.ec_sels, g.ec_fks, g.ec_vars):
+        h.update(a.tobytes())
+    return h.hexdigest()
+
+
+def make(label, gen, args, kwargs, over):
+    g = getattr(qgen, gen)(*args, **kwargs)
+    st, nodes, stats = oracle.plan(g, abi.default_config(**over))
+    assert st == 0, (label, st)
+    plan = [[list(r), a, b, c, w, l, rr] for (r, a, b, c, w, l, rr) in gpuqo_amd.plan_bits(nodes)]
+    return {"label": label, "generator": gen, "args": args, "kwargs": kwargs, "n": g.n,
+            "input_sha256": digest(g), "config": over, "plan": plan,
+            "pairs": stats.pairs, "sets": stats.sets}
+
+
+def main():
+    out = {"format": "postgres-gpuqo_amd golden plans v1",
+           "float_encoding": "IEEE-754 binary32 bit patterns",
+           "records": [make(*c) for c in CASES]}
+    path = os.path.join(HERE, "plans_v1.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(f"wrote {len(out['records'])} records to {path}")
+
+
+if __name__ == "__main__":
+    main()
