@@ -647,9 +647,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    uint32_t np = eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
-                                  (uint64_t)gridDim.x * (BLOCK / WAVE));
-    add_pairs(counters, np);
+    (void)eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
+                               (uint64_t)gridDim.x * (BLOCK / WAVE));
+    /* no pair counter: every split of a tree set is valid, so the host
+     * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
+     * address would serialise the level */
 }
 
 /* One DP's device state, as the batch kernels and the big-set kernels see it
@@ -1284,7 +1286,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
             uint32_t *misc = smem + ((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
             if (!h.is_tree && misc[1]) atomicOr(&J.counters[49], 1u);
         }
-        add_pairs(J.counters, np);
+        if (!h.is_tree) add_pairs(J.counters, np);       /* trees: counted on the host */
     }
 }
 
@@ -1757,6 +1759,14 @@ static uint64_t binom64(int n, int k) {
 
 #define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
 
+/* Ordered csg-cmp pairs of a tree query: 2(k-1) per connected set of k. */
+static uint64_t tree_pairs(const std::vector<uint64_t> &counts) {
+    uint64_t p = 0;
+    for (size_t k = 2; k < counts.size(); k++) p += counts[k] * 2ull * (k - 1);
+    return p;
+}
+
+
 struct DeviceCtx {
     int device = -1;
     hipStream_t stream = nullptr;
@@ -2010,6 +2020,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         st.candidates = (1ull << n) - (uint64_t)n - 1ull;
         for (int k = 2; k <= n; k++) st.sets += cnt[k];
         st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+        if (ins[j]->hdr.is_tree && (int)ins[j]->tree_counts.size() == n + 1) st.pairs = tree_pairs(ins[j]->tree_counts);
         st.memo_bytes = (1ull << n) * 40;
         st.memo_kind = 1;
         st.gpu_ms = ms / nj;
@@ -2174,7 +2185,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
     }
     st.candidates = list;            /* sets generated by expansion (no 2^n scan) */
     st.sets = list;
-    st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+    st.pairs = tree_pairs(in.tree_counts);
     st.memo_bytes = cap * (sizeof(float4) + 4 + 4) + list * 8;
     st.memo_kind = GQ_MEMO_HASH;
     return GQ_OK;
@@ -2493,6 +2504,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     st.sets = 0;
     for (int k = 2; k <= n; k++) st.sets += cnt[k];
     st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+    if (h.is_tree && (int)in.tree_counts.size() == n + 1) st.pairs = tree_pairs(in.tree_counts);
     st.memo_bytes = nsets_all * (sizeof(float4) + 4);
     st.memo_kind = 1;
     return GQ_OK;
@@ -2821,6 +2833,7 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     J.cur_k = 2;
     j->hash = false;
     j->cap = 0;
+    j->counts = in.tree_counts;
     e->incoming.push_back(j);
     return true;
 }
@@ -2970,6 +2983,7 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
             for (int k = 2; k <= n; k++) d.st.sets += cnt[k];
             d.st.candidates = j->hash ? d.st.sets : (1ull << n) - (uint64_t)n - 1ull;
             d.st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
+            if (j->J.h.is_tree && (int)j->counts.size() == n + 1) d.st.pairs = tree_pairs(j->counts);
             d.st.memo_bytes = j->bytes;
             d.st.memo_kind = j->hash ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
             d.st.gpu_ms = ms;
