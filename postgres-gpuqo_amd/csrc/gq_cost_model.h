@@ -356,10 +356,11 @@ GQ_HD float bucketsize(const View<S> &q, int mi, int nbuckets) {
     return est;
 }
 
-/* cost_hashjoin (cost_postgres.cuh:287-470) */
-template <int ECW = 0, class S>
+/* cost_hashjoin (cost_postgres.cuh:287-470).  `pv` lists the member slots of
+ * the inner side in the crossing classes (see join_prov). */
+template <class S, class Prov>
 GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const Rel &inner, const Extra &ex,
-                    int n_quals, float qual_per_tuple, const uint64_t *m, float *o_startup, float *o_total) {
+                    int n_quals, float qual_per_tuple, const Prov &pv, float *o_startup, float *o_total) {
     float startup_cost = 0.0f, run_cost = 0.0f;
     float outer_path_rows = outer.rows, inner_path_rows = inner.rows;
     if (!q.p.enable_hashjoin) startup_cost += q.p.disable_cost;
@@ -379,17 +380,10 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
     int nbk = f2i_x86(virtualbuckets);
     /* estimate_hash_bucket_stats (:620-668) */
     float bsize = 1.0f, mcvfreq = 1.0f;
-    for_each_ec<ECW>(q, m, [&](int e) {
-        S ec = q.ec_relids[e];
-        S mr = ec & inner_id;
-        while (!empty(mr)) {
-            S in_id = lowbit(mr);
-            int mi = q.ec_moff[e] + rank_below(ec, in_id);
-            float t = bucketsize(q, mi, nbk);
-            if (bsize > t) bsize = t;
-            if (mcvfreq > q.mem_mcv[mi]) mcvfreq = q.mem_mcv[mi];
-            mr = mr - in_id;
-        }
+    pv.for_each_inner(q, inner_id, [&](int mi) {
+        float t = bucketsize(q, mi, nbk);
+        if (bsize > t) bsize = t;
+        if (mcvfreq > q.mem_mcv[mi]) mcvfreq = q.mem_mcv[mi];
     });
     if (relation_byte_size(clamp_row_est(inner_path_rows * mcvfreq), inner.width) >
         (float)((long)q.p.work_mem * 1024L))
@@ -415,31 +409,66 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
 
 /* One join: rows (estimate_join_rows) and cost (calc_join_cost,
  * gpuqo_cost.cuh:76-158) of outer `l` joined with inner `r`. */
-template <int ECW = 0, class S>
-GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const uint64_t *m,
-                    int n_quals);
+/* The crossing classes of one join, as join_prov sees them.
+ *
+ * MaskProv: the classes of mask `m` (ascending index); the selectivity is
+ * estimate_join_selectivity over them and the inner members are found by
+ * rank within each class.
+ *
+ * EdgeProv: a split of a tree query at edge (p, c): its crossing classes
+ * are those on that edge, so the selectivity and the inner member slots
+ * depend only on (c, orientation) and are precomputed on the host with this
+ * same code (ec_selectivity on the one-member sides, the product taken in
+ * ascending class order). */
+template <int ECW>
+struct MaskProv {
+    const uint64_t *m;
+    int nq;
+    template <class S>
+    GQ_HD float selectivity(const View<S> &q, const S &l, const S &r) const { return join_selectivity<ECW>(q, l, r, m); }
+    GQ_HD int n_quals() const { return nq; }
+    /* f(member slot) for each member of `inner` in each class, in class order */
+    template <class S, class F>
+    GQ_HD void for_each_inner(const View<S> &q, const S &inner, F f) const {
+        for_each_ec<ECW>(q, m, [&](int e) {
+            S ec = q.ec_relids[e];
+            S mr = ec & inner;
+            while (!empty(mr)) {
+                S in_id = lowbit(mr);
+                f(q.ec_moff[e] + rank_below(ec, in_id));
+                mr = mr - in_id;
+            }
+        });
+    }
+};
 
-template <int ECW = 0, class S>
-GQ_HD Rel join(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R) {
-    uint64_t m[GQ_MAX_ECW];
-    int n_quals = match_mask<ECW>(q, l, r, m);
-    return join_mask<ECW>(q, l, L, r, R, m, n_quals);
-}
+struct EdgeProv {
+    float sel;
+    int nq, n_mi;
+    const uint16_t *mi;
+    template <class S>
+    GQ_HD float selectivity(const View<S> &, const S &, const S &) const { return sel; }
+    GQ_HD int n_quals() const { return nq; }
+    template <class S, class F>
+    GQ_HD void for_each_inner(const View<S> &, const S &, F f) const {
+        for (int i = 0; i < n_mi; i++) f((int)mi[i]);
+    }
+};
 
-/* The same join with the crossing-class mask `m` (n_quals classes) given. */
-template <int ECW, class S>
-GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const uint64_t *m,
-                    int n_quals) {
+/* One join: rows (estimate_join_rows) and cost (calc_join_cost,
+ * gpuqo_cost.cuh:76-158) of outer `l` joined with inner `r`. */
+template <class S, class Prov>
+GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const Prov &pv) {
     Rel out;
-    out.rows = join_rows(join_selectivity<ECW>(q, l, r, m), L.rows, R.rows);
+    out.rows = join_rows(pv.selectivity(q, l, r), L.rows, R.rows);
     out.width = L.width + R.width;
+    const int n_quals = pv.n_quals();
     float qual_pt = q.qualtab[n_quals];
     bool single = popc(r) == 1;
     Extra ex;
     ex.inner_unique = false;
     if (single) {                                   /* is_inner_unique :43-74 */
-        for_each_ec<ECW>(q, m, [&](int e) {
-            int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
+        pv.for_each_inner(q, r, [&](int mi) {
             if ((q.mem_flags[mi] & (GQ_MF_INDEX | GQ_MF_UNIQUE)) == (GQ_MF_INDEX | GQ_MF_UNIQUE))
                 ex.inner_unique = true;
         });
@@ -452,13 +481,12 @@ GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     nestloop(q, L, R.rows, R.startup, R.total, ex, qual_pt, &bs, &bt);
     if (q.p.enable_hashjoin) {
         float hs, ht;
-        hashjoin<ECW>(q, r, L, R, ex, n_quals, qual_pt, m, &hs, &ht);
+        hashjoin(q, r, L, R, ex, n_quals, qual_pt, pv, &hs, &ht);
         if (ht < bt) { bs = hs; bt = ht; }
     }
     if (single && !bit(q.composite, ctz(r))) {       /* index nested loop :116-147 */
         ex.indexed = true;
-        for_each_ec<ECW>(q, m, [&](int e) {
-            int mi = q.ec_moff[e] + rank_below(q.ec_relids[e], r);
+        pv.for_each_inner(q, r, [&](int mi) {
             if (q.mem_flags[mi] & GQ_MF_INDEX) {
                 float is, it;
                 nestloop(q, L, q.mem_idx_rows[mi], q.mem_idx_startup[mi], q.mem_idx_total[mi], ex, qual_pt,
@@ -470,6 +498,23 @@ GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     out.startup = bs;
     out.total = bt;
     return out;
+}
+
+/* The same join with the crossing-class mask `m` (n_quals classes) given. */
+template <int ECW = 0, class S>
+GQ_HD Rel join_mask(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const uint64_t *m,
+                    int n_quals) {
+    MaskProv<ECW> pv;
+    pv.m = m;
+    pv.nq = n_quals;
+    return join_prov(q, l, L, r, R, pv);
+}
+
+template <int ECW = 0, class S>
+GQ_HD Rel join(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R) {
+    uint64_t m[GQ_MAX_ECW];
+    int n_quals = match_mask<ECW>(q, l, r, m);
+    return join_mask<ECW>(q, l, L, r, R, m, n_quals);
 }
 
 }  // namespace gq

@@ -33,6 +33,12 @@ struct BlobHdr {
      * classes crossing the split at edge (parent(c), c) are exactly those on
      * that edge; o_ecm holds their mask (ecw 64-bit words) per vertex c. */
     int tree_ecm, o_ecm;
+    /* Tree graphs with at most 4 classes per edge: 8 words per vertex c
+     * describing the split at edge (parent(c), c) in both orientations:
+     * [0] selectivity (o = 0: c on the inner side), [1] selectivity (o = 1),
+     * [2] n_quals | n_classes << 8, [4..5] inner member slots for o = 0,
+     * [6..7] for o = 1 (u16 each, class order). */
+    int tree_fast, o_tedge;
 };
 
 struct DPInput {

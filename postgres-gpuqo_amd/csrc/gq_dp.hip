@@ -561,7 +561,7 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
  * and the first minimum of the packed (cost, rank) key with shuffles; the
  * winning lane writes its own result to the memo.  Waves w0, w0 + wstride,
  * ... cover the level. */
-template <int ECW>
+template <int ECW, bool FAST>
 __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
                                     bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                     uint32_t count, const MemoRef &mr, uint64_t w0, uint64_t wstride) {
@@ -588,6 +588,16 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
             L = o ? sub : (S ^ sub);
             const uint32_t R = S ^ L;
             const uint32_t rank = bicc ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
+            if (FAST) {                               /* per-edge split table (BlobHdr::tree_fast) */
+                const uint32_t *te = smem + h.o_tedge + 8 * c;
+                EdgeProv pv;
+                pv.sel = __uint_as_float(te[o]);
+                pv.nq = (int)(te[2] & 0xffu);
+                pv.n_mi = (int)((te[2] >> 8) & 0xffu);
+                pv.mi = (const uint16_t *)(te + 4 + 2 * o);
+                jr = join_prov(q, L, load_rel(mr.vals, memo_slot(mr, L)), R, load_rel(mr.vals, memo_slot(mr, R)),
+                               pv);
+            } else {
             uint64_t m[GQ_MAX_ECW];
             int nq = 0;
             const int nw = ECW ? ECW : h.ecw;
@@ -602,6 +612,7 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
             }
             jr = join_mask<ECW>(q, L, load_rel(mr.vals, memo_slot(mr, L)), R, load_rel(mr.vals, memo_slot(mr, R)),
                                 m, nq);
+            }
             npairs++;
             const bool nan = jr.total != jr.total;
             first = (rank << 1) | (nan ? 1u : 0u);
@@ -634,7 +645,7 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
     return npairs;
 }
 
-template <int ECW>
+template <int ECW, bool FAST>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters) {
@@ -647,7 +658,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    (void)eval_level_tree<ECW>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
+    (void)eval_level_tree<ECW, FAST>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
                                (uint64_t)gridDim.x * (BLOCK / WAVE));
     /* no pair counter: every split of a tree set is valid, so the host
      * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
@@ -1166,44 +1177,53 @@ __global__ void __launch_bounds__(BLOCK) k_batch_filter(const BatchJob *jobs, in
     }
 }
 
-__device__ __forceinline__ uint32_t job_tasks(const BatchJob &J, int k) {
+/* Job classes of a level launch: tree jobs with the per-edge split table
+ * run in k_batch_eval_tree (few registers, high occupancy), every other job
+ * in k_batch_eval. */
+#define JOBS_TREE 1
+#define JOBS_GENERAL 2
+
+__host__ __device__ __forceinline__ bool fast_tree_job(const BlobHdr &h) { return h.is_tree && h.tree_fast; }
+
+__device__ __forceinline__ uint32_t job_tasks(const BatchJob &J, int k, int cls) {
     if (k < 2 || k > J.h.n_rels) return 0;
+    if ((cls == JOBS_TREE) != fast_tree_job(J.h)) return 0;
     const uint32_t spt = task_sets(J, k);
     return (J.counters[k] + spt - 1) / spt;
 }
 
 /* task_prefix[k * (nj + 1) + j] = first task of job j at level k; one
  * workgroup per level k = 2 + blockIdx.x. */
-__global__ void __launch_bounds__(1024) k_batch_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix) {
+__global__ void __launch_bounds__(1024) k_batch_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix, int cls) {
     __shared__ uint32_t part[17];
     const int k = 2 + (int)blockIdx.x;
     uint32_t *tp = task_prefix + (size_t)k * (nj + 1);
     const uint32_t per = ((uint32_t)nj + blockDim.x - 1) / blockDim.x;
     const uint32_t j0 = threadIdx.x * per;
     uint32_t sum = 0;
-    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], k);
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], k, cls);
     uint32_t total;
     uint32_t acc = block_scan_excl(sum, part, &total);
     for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) {
         tp[j] = acc;
-        acc += job_tasks(jobs[j], k);
+        acc += job_tasks(jobs[j], k, cls);
     }
     if (threadIdx.x == 0) tp[nj] = total;
 }
 
 /* Step engine: task_prefix[j] = first task of job j at its own level
  * J.cur_k (jobs past their last level have none); one workgroup. */
-__global__ void __launch_bounds__(1024) k_step_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix) {
+__global__ void __launch_bounds__(1024) k_step_plan(const BatchJob *jobs, int nj, uint32_t *task_prefix, int cls) {
     __shared__ uint32_t part[17];
     const uint32_t per = ((uint32_t)nj + blockDim.x - 1) / blockDim.x;
     const uint32_t j0 = threadIdx.x * per;
     uint32_t sum = 0;
-    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], jobs[j].cur_k);
+    for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) sum += job_tasks(jobs[j], jobs[j].cur_k, cls);
     uint32_t total;
     uint32_t acc = block_scan_excl(sum, part, &total);
     for (uint32_t j = j0; j < j0 + per && j < (uint32_t)nj; j++) {
         task_prefix[j] = acc;
-        acc += job_tasks(jobs[j], jobs[j].cur_k);
+        acc += job_tasks(jobs[j], jobs[j].cur_k, cls);
     }
     if (threadIdx.x == 0) task_prefix[nj] = total;
 }
@@ -1255,15 +1275,16 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         const uint32_t m = cnt - first < spt ? cnt - first : spt;
         const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
         uint32_t np;
-        if (h.is_tree) {
+        if (h.is_tree) {                       /* trees without the per-edge table */
             const MemoRef mr = job_memo(J);
             const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
             if (h.ecw == 1)
-                np = eval_level_tree<1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
-                                        BLOCK / WAVE);
+                (void)eval_level_tree<1, false>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                                                BLOCK / WAVE);
             else
-                np = eval_level_tree<0>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
-                                        BLOCK / WAVE);
+                (void)eval_level_tree<0, false>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                                                BLOCK / WAVE);
+            np = 0;                            /* counted on the host */
         } else if (J.bicc) {
             BigList bl;
             bl.ents = big;
@@ -1284,9 +1305,56 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t *misc = smem + ((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
-            if (!h.is_tree && misc[1]) atomicOr(&J.counters[49], 1u);
+            if (misc[1]) atomicOr(&J.counters[49], 1u);
         }
-        if (!h.is_tree) add_pairs(J.counters, np);       /* trees: counted on the host */
+        add_pairs(J.counters, np);
+    }
+}
+
+/* The tree jobs with a per-edge split table (k_batch_eval's task scheme;
+ * k_batch_plan with JOBS_TREE): a kernel of its own, with only that
+ * evaluator, so that it keeps few registers and many waves per SIMD.  LDS holds the job's blob
+ * and, at word `tp_lds`, the task prefix.  Pairs are not counted: every
+ * split of a tree set is valid (the host derives the count). */
+__global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs, int nj, int k_fixed,
+                                                            const uint32_t *tpg, uint32_t tp_lds) {
+    extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    const uint32_t total = tpg[nj];
+    if (blockIdx.x >= total) return;
+    const uint32_t *tp = tpg;
+    if (tp_lds) {
+        uint32_t *tl = smem + tp_lds;
+        for (int i = threadIdx.x; i <= nj; i += BLOCK) tl[i] = tpg[i];
+        __syncthreads();
+        tp = tl;
+    }
+    int have = -1;
+    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = nj;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (tp[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const int j = lo;
+        const BatchJob &J = jobs[j];
+        const BlobHdr &h = J.h;
+        const int k = k_fixed > 0 ? k_fixed : J.cur_k;
+        if (j != have) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = J.blob[i];
+            __syncthreads();
+            have = j;
+        }
+        View<uint32_t> q = make_view(h, smem);
+        const uint32_t spt = task_sets(J, k);
+        const uint32_t first = (t - tp[j]) * spt, cnt = J.counters[k];
+        const uint32_t m = cnt - first < spt ? cnt - first : spt;
+        const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
+        const MemoRef mr = job_memo(J);
+        const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
+        (void)eval_level_tree<1, true>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                                       BLOCK / WAVE);
     }
 }
 
@@ -1770,6 +1838,10 @@ static uint64_t tree_pairs(const std::vector<uint64_t> &counts) {
 struct DeviceCtx {
     int device = -1;
     hipStream_t stream = nullptr;
+    /* second stream of the batch (general-graph jobs beside the tree jobs;
+     * default priority) and the per-level events that order the two */
+    hipStream_t stream2 = nullptr;
+    hipEvent_t lvl_ev[2][34] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBufs b{};
     size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
@@ -1833,6 +1905,9 @@ static int ctx_init(DeviceCtx &c, int device) {
     if (c.device == device && c.stream) return GQ_OK;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
+    for (int a = 0; a < 2; a++)
+        for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.lvl_ev[a][k], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c.ev0));
     HIPCHK(hipEventCreate(&c.ev1));
     uint32_t tab[33 * 33];
@@ -1892,7 +1967,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     std::vector<BatchJob> jobs(nj);
     std::vector<uint32_t> blobs, chunk_prefix(nj + 1, 0);
     std::vector<size_t> ws_off(nj), blob_off(nj);
-    size_t ws = 0, lds = 0, big_lds_max = 0;
+    size_t ws = 0, lds = 0, lds_tree = 0, big_lds_max = 0;
     uint64_t big_cap = 0;
     int kmax = 2, big_kmax = 0;
     for (int j = 0; j < nj; j++) {
@@ -1908,7 +1983,8 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         blob_off[j] = blobs.size();
         blobs.insert(blobs.end(), ins[j]->blob.begin(), ins[j]->blob.end());
         while (blobs.size() & 63) blobs.push_back(0u);
-        lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
+        if (fast_tree_job(h)) lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
+        else lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
         if (algos[j] == DP_BICC && !h.is_tree && h.n_rels >= 15) {
             uint64_t mc = 0;
             for (int k = 2; k <= h.n_rels; k++) mc = std::max(mc, binom64(h.n_rels, k));
@@ -1923,7 +1999,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     if ((rc = ensure(&c.bblob, c.cap_bblob, blobs.size()))) return rc;
     if ((rc = ensure(&c.jobs, c.cap_jobs, (size_t)nj))) return rc;
     if ((rc = ensure(&c.results, c.cap_results, (size_t)nj * RES_WORDS))) return rc;
-    if ((rc = ensure(&c.bprefix, c.cap_bprefix, (size_t)(nj + 1) * 34))) return rc;
+    if ((rc = ensure(&c.bprefix, c.cap_bprefix, (size_t)(nj + 1) * 69))) return rc;
     if (big_cap > 0xffffffffull) {
         set_error("batch too large for the big-set list");
         return GQ_ERR_UNSUPPORTED;
@@ -1969,30 +2045,52 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     const uint32_t fgrid = std::min<uint32_t>(chunk_prefix[nj], 4096);
     hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, c.jobs, nj,
                        c.bprefix);
-    uint32_t *task_prefix = c.bprefix + (nj + 1);        /* [33][nj+1] after the chunk prefix */
-    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, task_prefix);
-    /* the level's task prefix goes to LDS after the evaluation area when it fits */
-    const uint32_t tp_lds = (lds + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
-    const size_t lds_eval_total = tp_lds ? lds + 4 * (size_t)(nj + 1) : lds;
+    /* task prefixes [33][nj+1] after the chunk prefix: tree jobs, then the
+     * others.  Tree jobs run in k_batch_eval_tree on stream s, the others in
+     * k_batch_eval on stream2; level k on either waits for level k-1 on both. */
+    uint32_t *tp_tree = c.bprefix + (nj + 1), *tp_gen = c.bprefix + (size_t)(nj + 1) * 35;
+    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_tree, JOBS_TREE);
+    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_gen, JOBS_GENERAL);
+    const uint32_t tpl_tree = (lds_tree + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds_tree / 4) : 0u;
+    const size_t lds_tree_total = tpl_tree ? lds_tree + 4 * (size_t)(nj + 1) : std::max<size_t>(lds_tree, 16);
+    const uint32_t tpl_gen = (lds + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
+    const size_t lds_gen_total = tpl_gen ? lds + 4 * (size_t)(nj + 1) : lds;
     uint32_t *big_hdr = c.results + 50;                  /* job 0's counters[50..51] */
+    hipStream_t s2 = c.stream2;
+    HIPCHK(hipEventRecord(c.lvl_ev[0][1], s));
+    HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][1], 0));
     for (int k = 2; k <= kmax; k++) {
-        /* grid: enough workgroups for the largest level, capped */
-        uint64_t ub = 0;
+        /* grids: enough workgroups for the largest level, capped */
+        uint64_t ub_tree = 0, ub_gen = 0;
         for (int j = 0; j < nj; j++) {
             const BlobHdr &h = ins[j]->hdr;
             if (k > h.n_rels) continue;
-            ub += binom64(h.n_rels, k);
+            const uint64_t lvl = (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1)
+                                     ? ins[j]->tree_counts[k] : binom64(h.n_rels, k);
+            if (fast_tree_job(h)) ub_tree += lvl;
+            else ub_gen += lvl;
         }
-        uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
-        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_eval_total, s, c.jobs, nj, k,
-                           task_prefix + (size_t)k * (nj + 1), tp_lds, big_cap ? c.big_ents : nullptr,
-                           (uint32_t)big_cap, big_hdr);
-        if (big_cap && k >= 15 && k <= big_kmax) {
-            hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, big_hdr, (uint32_t)big_cap,
-                               c.big_prefix);
-            hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s, c.jobs, c.big_ents, big_hdr,
-                               c.big_prefix);
+        if (ub_tree) {
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_tree / 16, 1), 4096);
+            hipLaunchKernelGGL(k_batch_eval_tree, dim3(grid), dim3(BLOCK), lds_tree_total, s, c.jobs, nj, k,
+                               tp_tree + (size_t)k * (nj + 1), tpl_tree);
         }
+        if (ub_gen) {
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen / 16, 1), 2048);
+            hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, c.jobs, nj, k,
+                               tp_gen + (size_t)k * (nj + 1), tpl_gen, big_cap ? c.big_ents : nullptr,
+                               (uint32_t)big_cap, big_hdr);
+            if (big_cap && k >= 15 && k <= big_kmax) {
+                hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s2, c.big_ents, big_hdr, (uint32_t)big_cap,
+                                   c.big_prefix);
+                hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s2, c.jobs, c.big_ents,
+                                   big_hdr, c.big_prefix);
+            }
+        }
+        HIPCHK(hipEventRecord(c.lvl_ev[0][k], s));
+        HIPCHK(hipEventRecord(c.lvl_ev[1][k], s2));
+        HIPCHK(hipStreamWaitEvent(s, c.lvl_ev[1][k], 0));
+        HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][k], 0));
     }
     hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, c.jobs, nj);
     HIPCHK(hipGetLastError());
@@ -2034,6 +2132,21 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
                  std::vector<std::vector<DPNode>> &trees, std::vector<DPStats> &stats, double *gpu_ms) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
     return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
+}
+
+/* One level of a tree query: the per-edge-table kernel when the blob has
+ * the table, else the class-mask kernel for its mask width. */
+static void launch_eval_tree(const BlobHdr &h, dim3 grid, size_t lds, hipStream_t s, const uint32_t *blob, int k,
+                             int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr, uint32_t *counters) {
+    if (h.tree_fast)
+        hipLaunchKernelGGL((k_eval_tree<1, true>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
+                           counters);
+    else if (h.ecw == 1)
+        hipLaunchKernelGGL((k_eval_tree<1, false>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
+                           counters);
+    else
+        hipLaunchKernelGGL((k_eval_tree<0, false>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
+                           counters);
 }
 
 static MemoRef dense_memo(DeviceCtx &c) {
@@ -2138,12 +2251,8 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
         const uint64_t spw = WAVE / (2ull * (k - 1));
         const uint64_t waves = (in.tree_counts[k] + spw - 1) / spw;
         const uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
-        if (h.ecw == 1)
-            hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sj.bicc,
-                               c.b.lvl_sets + off_h[k], (const uint32_t *)(c.hslot + off_h[k]), mr, c.b.counters);
-        else
-            hipLaunchKernelGGL(k_eval_tree<0>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sj.bicc,
-                               c.b.lvl_sets + off_h[k], (const uint32_t *)(c.hslot + off_h[k]), mr, c.b.counters);
+        launch_eval_tree(h, dim3(tblocks), lds_eval, s, c.b.blob, k, sj.bicc, c.b.lvl_sets + off_h[k],
+                         c.hslot + off_h[k], mr, c.b.counters);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
     const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
@@ -2415,14 +2524,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
             uint64_t waves = (lvl + spw - 1) / spw;
             uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
-            if (h.ecw == 1)
-                hipLaunchKernelGGL(k_eval_tree<1>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   algo == DP_BICC ? 1 : 0, sets_k, (const uint32_t *)nullptr, dense_memo(c),
-                                   c.b.counters);
-            else
-                hipLaunchKernelGGL(k_eval_tree<0>, dim3(tblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k,
-                                   algo == DP_BICC ? 1 : 0, sets_k, (const uint32_t *)nullptr, dense_memo(c),
-                                   c.b.counters);
+            launch_eval_tree(h, dim3(tblocks), lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
+                             dense_memo(c), c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
@@ -2599,7 +2702,8 @@ uint64_t engine_hash_slice_bytes(uint64_t total_sets) {
 
 struct StepEngine {
     int device = 0;
-    hipStream_t s = nullptr;
+    hipStream_t s = nullptr, s2 = nullptr;       /* tree jobs / general jobs (default priority) */
+    hipEvent_t fork = nullptr, join = nullptr;
     char *arena = nullptr;
     size_t arena_bytes = 0;
     std::map<size_t, size_t> free_;              /* offset -> size, coalesced */
@@ -2683,6 +2787,9 @@ int engine_create(int device, uint64_t arena_bytes, StepEngine **out) {
         if (rc) return fail(rc);
     }
     if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join, hipEventDisableTiming) != hipSuccess ||
         hipEventCreate(&e->e0) != hipSuccess || hipEventCreate(&e->e1) != hipSuccess) {
         set_error("engine stream/event creation failed");
         return fail(GQ_ERR_DEVICE);
@@ -2707,6 +2814,7 @@ int engine_create(int device, uint64_t arena_bytes, StepEngine **out) {
 void engine_destroy(StepEngine *e) {
     if (!e) return;
     if (e->s) (void)hipStreamSynchronize(e->s);
+    if (e->s2) (void)hipStreamSynchronize(e->s2);
     for (EJob *j : e->active) delete j;
     for (EJob *j : e->incoming) delete j;
     if (e->arena) (void)hipFree(e->arena);
@@ -2726,6 +2834,9 @@ void engine_destroy(StepEngine *e) {
     if (e->h_chunk) (void)hipHostFree(e->h_chunk);
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->join) (void)hipEventDestroy(e->join);
+    if (e->s2) (void)hipStreamDestroy(e->s2);
     if (e->s) (void)hipStreamDestroy(e->s);
     delete e;
 }
@@ -2884,16 +2995,22 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
     /* 2. one level of every active DP */
     if ((rc = ensure_pinned(&e->h_jobs, e->cap_h_jobs, (size_t)na))) return rc;
     if ((rc = ensure(&e->d_jobs, e->cap_jobs, (size_t)na))) return rc;
-    if ((rc = ensure(&e->d_tp, e->cap_tp, (size_t)na + 1))) return rc;
-    size_t lds = 0, big_lds_max = 0;
-    uint64_t ub = 0, big_cap = 0;
+    if ((rc = ensure(&e->d_tp, e->cap_tp, 2 * ((size_t)na + 1)))) return rc;
+    size_t lds = 0, lds_tree = 0, big_lds_max = 0;
+    uint64_t ub = 0, ub_tree = 0, big_cap = 0;
     for (int i = 0; i < na; i++) {
         EJob *j = e->active[i];
         j->J.cur_k = j->cur_k;
         e->h_jobs[i] = j->J;
         const BlobHdr &h = j->J.h;
-        lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
-        ub += binom64(j->n, j->cur_k);
+        const uint64_t lvl = (int)j->counts.size() == j->n + 1 ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
+        if (fast_tree_job(h)) {
+            lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
+            ub_tree += lvl;
+        } else {
+            lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
+            ub += lvl;
+        }
         if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
             big_cap += binom64(j->n, j->cur_k);
             big_lds_max = std::max(big_lds_max, big_lds(h.words));
@@ -2926,18 +3043,33 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
             hipLaunchKernelGGL(k_expand, dim3(eg), dim3(BLOCK), 0, s, e->d_jobs, na, 0, e->d_exp);
         }
     }
-    hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, e->d_tp);
-    const uint32_t tp_lds = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
-    const size_t lds_total = tp_lds ? lds + 4 * (size_t)(na + 1) : lds;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
-    hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_total, s, e->d_jobs, na, 0, e->d_tp, tp_lds,
-                       big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
-    if (big_cap) {
-        hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, e->big, e->big_hdr, (uint32_t)big_cap,
-                           e->big_prefix);
-        hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s, e->d_jobs, e->big, e->big_hdr,
-                           e->big_prefix);
+    /* tree jobs on s (k_batch_eval_tree), the others on s2 (k_batch_eval) */
+    uint32_t *tp_tree = e->d_tp, *tp_gen = e->d_tp + (na + 1);
+    if (ub_tree) hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, tp_tree, JOBS_TREE);
+    if (ub) hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, tp_gen, JOBS_GENERAL);
+    if (ub) {
+        ECHK(hipEventRecord(e->fork, s));
+        ECHK(hipStreamWaitEvent(e->s2, e->fork, 0));
+        const uint32_t tpl = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
+        const size_t lt = tpl ? lds + 4 * (size_t)(na + 1) : lds;
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
+        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, e->d_jobs, na, 0, tp_gen, tpl,
+                           big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
+        if (big_cap) {
+            hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, e->s2, e->big, e->big_hdr, (uint32_t)big_cap,
+                               e->big_prefix);
+            hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, e->s2, e->d_jobs, e->big,
+                               e->big_hdr, e->big_prefix);
+        }
+        ECHK(hipEventRecord(e->join, e->s2));
     }
+    if (ub_tree) {
+        const uint32_t tpl = (lds_tree + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds_tree / 4) : 0u;
+        const size_t lt = tpl ? lds_tree + 4 * (size_t)(na + 1) : std::max<size_t>(lds_tree, 16);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_tree / 16, 1), 4096);
+        hipLaunchKernelGGL(k_batch_eval_tree, dim3(grid), dim3(BLOCK), lt, s, e->d_jobs, na, 0, tp_tree, tpl);
+    }
+    if (ub) ECHK(hipStreamWaitEvent(s, e->join, 0));
     /* 3. DPs past their last level: plan walk and read-back */
     std::vector<EJob *> fin, keep;
     for (EJob *j : e->active) {

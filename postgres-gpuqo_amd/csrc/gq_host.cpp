@@ -140,6 +140,75 @@ template struct HostView<4>;
 
 /* ------------------------------------------------------------ device blob */
 
+/* The device's view of a blob (make_view in gq_dp.hip), built on the host
+ * over the same words so that host-side precomputation runs the same code
+ * on the same data. */
+static View<uint32_t> host_view(const BlobHdr &h, const uint32_t *s) {
+    View<uint32_t> v;
+    v.n_rels = h.n_rels;
+    v.n_ec = h.n_ec;
+    v.ecw = h.ecw;
+    v.p = h.p;
+    v.rel_rows = (const float *)(s + h.o_rel_rows);
+    v.rel_fksel = (const float *)(s + h.o_rel_fksel);
+    v.composite = h.composite;
+    v.inc = (const uint64_t *)(s + h.o_inc);
+    v.ec_relids = s + h.o_ec_relids;
+    v.ec_moff = (const int *)(s + h.o_ec_moff);
+    v.ec_soff = (const int *)(s + h.o_ec_soff);
+    v.ec_hasfk = (const uint8_t *)(s + h.o_ec_hasfk);
+    v.mem_fks = s + h.o_mem_fks;
+    v.mem_nd = (const float *)(s + h.o_mem_nd);
+    v.mem_avgfreq = (const float *)(s + h.o_mem_avgfreq);
+    v.mem_mcv = (const float *)(s + h.o_mem_mcv);
+    v.mem_flags = (const uint8_t *)(s + h.o_mem_flags);
+    v.mem_idx_rows = (const float *)(s + h.o_mem_idx_rows);
+    v.mem_idx_startup = (const float *)(s + h.o_mem_idx_startup);
+    v.mem_idx_total = (const float *)(s + h.o_mem_idx_total);
+    v.sels = (const float *)(s + h.o_sels);
+    v.qualtab = (const float *)(s + h.o_qualtab);
+    return v;
+}
+
+/* Per-edge split table of a tree query (BlobHdr::tree_fast): for the split
+ * at edge (p, c), the crossing classes are those on that edge (o_ecm), so
+ * estimate_join_selectivity reduces to the product, in ascending class
+ * order, of ec_selectivity on the one-member sides, and the inner member
+ * slots to those of c (o = 0, c on the inner side) or p (o = 1). */
+static void fill_tree_edges(const BlobHdr &h, std::vector<uint32_t> &b) {
+    const View<uint32_t> q = host_view(h, b.data());
+    for (int c = 1; c < h.n_rels; c++) {
+        /* the parent: the one lower-numbered neighbour (BlobHdr::is_tree) */
+        const int p = __builtin_ctz(b[h.o_adj + c] & ((1u << c) - 1u));
+        const uint64_t *em = (const uint64_t *)(b.data() + h.o_ecm) + (size_t)c * h.ecw;
+        const uint32_t sc = 1u << c, sp = 1u << p;
+        float sel0 = 1.0f, sel1 = 1.0f;
+        uint16_t mi0[4] = {0, 0, 0, 0}, mi1[4] = {0, 0, 0, 0};
+        int k = 0, nq = 0;
+        for (int w = 0; w < h.ecw; w++) {
+            uint64_t m = em[w];
+            while (m) {
+                const int e = w * 64 + __builtin_ctzll(m);
+                m &= m - 1;
+                sel0 *= ec_selectivity(q, e, sp, sc);
+                sel1 *= ec_selectivity(q, e, sc, sp);
+                const uint32_t ec = q.ec_relids[e];
+                mi0[k] = (uint16_t)(q.ec_moff[e] + rank_below(ec, sc));
+                mi1[k] = (uint16_t)(q.ec_moff[e] + rank_below(ec, sp));
+                k++;
+                nq++;
+            }
+        }
+        uint32_t *t = b.data() + h.o_tedge + 8 * c;
+        memcpy(&t[0], &sel0, 4);
+        memcpy(&t[1], &sel1, 4);
+        t[2] = (uint32_t)nq | ((uint32_t)k << 8);
+        t[3] = 0;
+        memcpy(&t[4], mi0, 8);
+        memcpy(&t[6], mi1, 8);
+    }
+}
+
 int build_dp_input(const HQuery<1> &q, DPInput &in) {
     const int n = q.n;
     if (n > 32) {
@@ -293,6 +362,22 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
             h.tree_ecm = 1;
         }
     }
+    /* per-edge split table of tree graphs (see BlobHdr::tree_fast) */
+    h.tree_fast = 0;
+    h.o_tedge = 0;
+    if (h.is_tree && h.tree_ecm) {
+        bool fits = true;
+        for (int v = 1; v < n && fits; v++) {
+            int cnt = 0;
+            for (int w = 0; w < ecw; w++)
+                cnt += __builtin_popcountll(((const uint64_t *)&b[h.o_ecm])[v * ecw + w]);
+            fits = cnt <= 4;
+        }
+        if (fits) {
+            h.o_tedge = take(8 * n);
+            h.tree_fast = 1;
+        }
+    }
     h.o_base = take(4 * n);
     in.base.resize(n);
     for (int i = 0; i < n; i++) {
@@ -304,6 +389,7 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     }
     while (b.size() & 3) b.push_back(0u);
     h.words = (int)b.size();
+    if (h.tree_fast) fill_tree_edges(h, b);
     /* Level-1 list of cpu_dpsize: iteration order of a libstdc++
      * unordered_map holding the base relations, inserted in index order, keyed
      * by the reference's relation ids (relation i -> bit i+1), whose hash is
