@@ -1312,50 +1312,38 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
 }
 
 /* The tree jobs with a per-edge split table (k_batch_eval's task scheme;
- * k_batch_plan with JOBS_TREE): a kernel of its own, with only that
- * evaluator, so that it keeps few registers and many waves per SIMD.  LDS holds the job's blob
- * and, at word `tp_lds`, the task prefix.  Pairs are not counted: every
- * split of a tree set is valid (the host derives the count). */
+ * k_batch_plan / k_step_plan with JOBS_TREE), one task per workgroup: the
+ * grid is the exact task count, which the host derives from the exact level
+ * sizes of tree queries (a device total above the grid would leave tasks
+ * undone and raises error bit 32 of job 0).  A kernel of its own with only
+ * this evaluator and no task loop keeps few registers and many waves per
+ * SIMD.  Pairs are not counted: every split of a tree set is valid. */
 __global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs, int nj, int k_fixed,
-                                                            const uint32_t *tpg, uint32_t tp_lds) {
+                                                            const uint32_t *tpg) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
+    const uint32_t t = blockIdx.x;
     const uint32_t total = tpg[nj];
-    if (blockIdx.x >= total) return;
-    const uint32_t *tp = tpg;
-    if (tp_lds) {
-        uint32_t *tl = smem + tp_lds;
-        for (int i = threadIdx.x; i <= nj; i += BLOCK) tl[i] = tpg[i];
-        __syncthreads();
-        tp = tl;
+    if (t == 0 && total > gridDim.x) atomicOr(&jobs[0].counters[49], 32u);
+    if (t >= total) return;
+    int lo = 0, hi = nj;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (tpg[mid] <= t) lo = mid;
+        else hi = mid;
     }
-    int have = -1;
-    for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
-        int lo = 0, hi = nj;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (tp[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        const int j = lo;
-        const BatchJob &J = jobs[j];
-        const BlobHdr &h = J.h;
-        const int k = k_fixed > 0 ? k_fixed : J.cur_k;
-        if (j != have) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = J.blob[i];
-            __syncthreads();
-            have = j;
-        }
-        View<uint32_t> q = make_view(h, smem);
-        const uint32_t spt = task_sets(J, k);
-        const uint32_t first = (t - tp[j]) * spt, cnt = J.counters[k];
-        const uint32_t m = cnt - first < spt ? cnt - first : spt;
-        const uint32_t *sets = J.lvl_sets + J.lvl_off[k] + first;
-        const MemoRef mr = job_memo(J);
-        const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
-        (void)eval_level_tree<1, true>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
-                                       BLOCK / WAVE);
-    }
+    const BatchJob &J = jobs[lo];
+    const BlobHdr &h = J.h;
+    const int k = k_fixed > 0 ? k_fixed : J.cur_k;
+    for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = J.blob[i];
+    __syncthreads();
+    View<uint32_t> q = make_view(h, smem);
+    const uint32_t spt = task_sets(J, k);
+    const uint32_t first = (t - tpg[lo]) * spt, cnt = J.counters[k];
+    const uint32_t m = cnt - first < spt ? cnt - first : spt;
+    const MemoRef mr = job_memo(J);
+    (void)eval_level_tree<1, true>(h, smem, q, k, J.bicc != 0, J.lvl_sets + J.lvl_off[k] + first,
+                                   J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr, m, mr,
+                                   threadIdx.x / WAVE, BLOCK / WAVE);
 }
 
 /* Plan walk from `root_set` (dpsub_buildQueryTree, gpuqo_query_tree.cuh:46-72)
@@ -1827,6 +1815,12 @@ static uint64_t binom64(int n, int k) {
 
 #define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
 
+/* Tasks of a tree job at level k (task_sets on the device). */
+static uint64_t tree_tasks(uint64_t sets, int k) {
+    const uint64_t spt = (uint64_t)(BLOCK / WAVE) * (WAVE / (2 * (k - 1))) * TREE_ITERS;
+    return (sets + spt - 1) / spt;
+}
+
 /* Ordered csg-cmp pairs of a tree query: 2(k-1) per connected set of k. */
 static uint64_t tree_pairs(const std::vector<uint64_t> &counts) {
     uint64_t p = 0;
@@ -1983,6 +1977,10 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         blob_off[j] = blobs.size();
         blobs.insert(blobs.end(), ins[j]->blob.begin(), ins[j]->blob.end());
         while (blobs.size() & 63) blobs.push_back(0u);
+        if (fast_tree_job(h) && (int)ins[j]->tree_counts.size() != h.n_rels + 1) {
+            set_error("tree job %d without level sizes", j);
+            return GQ_ERR_INVALID;
+        }
         if (fast_tree_job(h)) lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
         else lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
         if (algos[j] == DP_BICC && !h.is_tree && h.n_rels >= 15) {
@@ -2051,8 +2049,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     uint32_t *tp_tree = c.bprefix + (nj + 1), *tp_gen = c.bprefix + (size_t)(nj + 1) * 35;
     hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_tree, JOBS_TREE);
     hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_gen, JOBS_GENERAL);
-    const uint32_t tpl_tree = (lds_tree + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds_tree / 4) : 0u;
-    const size_t lds_tree_total = tpl_tree ? lds_tree + 4 * (size_t)(nj + 1) : std::max<size_t>(lds_tree, 16);
+    const size_t lds_tree_total = std::max<size_t>(lds_tree, 16);
     const uint32_t tpl_gen = (lds + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
     const size_t lds_gen_total = tpl_gen ? lds + 4 * (size_t)(nj + 1) : lds;
     uint32_t *big_hdr = c.results + 50;                  /* job 0's counters[50..51] */
@@ -2061,19 +2058,21 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][1], 0));
     for (int k = 2; k <= kmax; k++) {
         /* grids: enough workgroups for the largest level, capped */
-        uint64_t ub_tree = 0, ub_gen = 0;
+        uint64_t tasks_tree = 0, ub_gen = 0;
         for (int j = 0; j < nj; j++) {
             const BlobHdr &h = ins[j]->hdr;
             if (k > h.n_rels) continue;
-            const uint64_t lvl = (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1)
-                                     ? ins[j]->tree_counts[k] : binom64(h.n_rels, k);
-            if (fast_tree_job(h)) ub_tree += lvl;
-            else ub_gen += lvl;
+            if (fast_tree_job(h)) tasks_tree += tree_tasks(ins[j]->tree_counts[k], k);
+            else ub_gen += (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1) ? ins[j]->tree_counts[k]
+                                                                                         : binom64(h.n_rels, k);
         }
-        if (ub_tree) {
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_tree / 16, 1), 4096);
-            hipLaunchKernelGGL(k_batch_eval_tree, dim3(grid), dim3(BLOCK), lds_tree_total, s, c.jobs, nj, k,
-                               tp_tree + (size_t)k * (nj + 1), tpl_tree);
+        if (tasks_tree) {
+            if (tasks_tree > 0x7fffffffull) {
+                set_error("too many tasks in one level");
+                return GQ_ERR_UNSUPPORTED;
+            }
+            hipLaunchKernelGGL(k_batch_eval_tree, dim3((uint32_t)tasks_tree), dim3(BLOCK), lds_tree_total, s, c.jobs,
+                               nj, k, (const uint32_t *)(tp_tree + (size_t)k * (nj + 1)));
         }
         if (ub_gen) {
             const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen / 16, 1), 2048);
@@ -3004,9 +3003,9 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         e->h_jobs[i] = j->J;
         const BlobHdr &h = j->J.h;
         const uint64_t lvl = (int)j->counts.size() == j->n + 1 ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
-        if (fast_tree_job(h)) {
+        if (fast_tree_job(h) && (int)j->counts.size() == j->n + 1) {
             lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
-            ub_tree += lvl;
+            ub_tree += tree_tasks(lvl, j->cur_k);
         } else {
             lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
             ub += lvl;
@@ -3063,11 +3062,13 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         }
         ECHK(hipEventRecord(e->join, e->s2));
     }
-    if (ub_tree) {
-        const uint32_t tpl = (lds_tree + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds_tree / 4) : 0u;
-        const size_t lt = tpl ? lds_tree + 4 * (size_t)(na + 1) : std::max<size_t>(lds_tree, 16);
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_tree / 16, 1), 4096);
-        hipLaunchKernelGGL(k_batch_eval_tree, dim3(grid), dim3(BLOCK), lt, s, e->d_jobs, na, 0, tp_tree, tpl);
+    if (ub_tree) {                                /* ub_tree: exact task count */
+        if (ub_tree > 0x7fffffffull) {
+            set_error("too many tasks in one step");
+            return GQ_ERR_UNSUPPORTED;
+        }
+        hipLaunchKernelGGL(k_batch_eval_tree, dim3((uint32_t)ub_tree), dim3(BLOCK), std::max<size_t>(lds_tree, 16),
+                           s, e->d_jobs, na, 0, (const uint32_t *)tp_tree);
     }
     if (ub) ECHK(hipStreamWaitEvent(s, e->join, 0));
     /* 3. DPs past their last level: plan walk and read-back */
