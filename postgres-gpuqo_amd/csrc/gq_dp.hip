@@ -702,7 +702,7 @@ __device__ __forceinline__ MemoRef job_memo(const BatchJob &J) {
  * list and their split range is spread over many workgroups, BIG_CHUNK
  * splits each (k_big_scan, k_big_eval). */
 #define BIG_SPLITS 16384u
-#define BIG_CHUNK 16384u
+#define BIG_CHUNK 2048u
 
 struct BigEnt {
     uint32_t job, gidx, cnt, done;   /* gidx: index into the job's level arrays */
