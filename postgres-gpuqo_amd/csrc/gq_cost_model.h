@@ -183,8 +183,12 @@ GQ_HD HashSize hash_table_size(float ntuples, int tupwidth, int work_mem) {
     long hash_table_bytes = (long)work_mem * 1024L;
     float space_allowed = (float)hash_table_bytes;
     long skew_table_bytes = hash_table_bytes * 2 / 100;
-    long num_skew_mcvs = div_l(skew_table_bytes, (long)(tupsize + 84));
-    if ((int)num_skew_mcvs > 0) hash_table_bytes -= skew_table_bytes;
+    /* (int)(skew_table_bytes / (tupsize + 84)) > 0.  For tupwidth >= 0 the
+     * divisor is at least 116 and, work_mem being an int, the dividend below
+     * 2^36, so the quotient fits an int and the test is a comparison. */
+    const bool skew = tupwidth >= 0 ? skew_table_bytes >= (long)(tupsize + 84)
+                                    : (int)div_l(skew_table_bytes, (long)(tupsize + 84)) > 0;
+    if (skew) hash_table_bytes -= skew_table_bytes;
     long max_pointers = (long)(space_allowed / 8.0f);
     const long alloc_ptrs = 0x3fffffffL / 8;
     if (alloc_ptrs < max_pointers) max_pointers = alloc_ptrs;

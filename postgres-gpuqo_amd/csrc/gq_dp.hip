@@ -186,6 +186,21 @@ __device__ __forceinline__ uint32_t grow(uint32_t from, uint32_t subset, const u
     return V;
 }
 
+/* index of the member of x that has n members below it (n < popcount(x)):
+ * a binary search on popcounts of halves, 5 steps whatever the size of x */
+__device__ __forceinline__ int select_bit(uint32_t x, uint32_t n) {
+    int p = 0;
+    uint32_t c = __builtin_popcount(x & 0xffffu);
+    if (n >= c) { n -= c; x >>= 16; p += 16; }
+    c = __builtin_popcount(x & 0xffu);
+    if (n >= c) { n -= c; x >>= 8; p += 8; }
+    c = __builtin_popcount(x & 0xfu);
+    if (n >= c) { n -= c; x >>= 4; p += 4; }
+    c = __builtin_popcount(x & 0x3u);
+    if (n >= c) { n -= c; x >>= 2; p += 2; }
+    return (n >= (x & 1u)) ? p + 1 : p;
+}
+
 __device__ __forceinline__ uint32_t pdep32(uint32_t j, uint32_t mask) {
     uint32_t res = 0;
     while (mask) {
@@ -555,68 +570,88 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
  * gpuqo_cpu_dpsub.cu:32-79); only pairs whose sides are both connected are
  * costed, gathered first into an LDS queue so that every lane of a wave
  * costs a valid pair. */
-/* Tree graphs, wave per set(s): a set of level k has per = 2(k-1) splits;
- * a segment of P = nextpow2(per) lanes owns one set and lane t evaluates
- * split t.  The segment reduces the first valid split (rank and NaN flag)
- * and the first minimum of the packed (cost, rank) key with shuffles; the
- * winning lane writes its own result to the memo.  Waves w0, w0 + wstride,
- * ... cover the level. */
-template <int ECW, bool FAST>
-__device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
-                                    bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot,
-                                    uint32_t count, const MemoRef &mr, uint64_t w0, uint64_t wstride) {
+/* sets of level k per wave of the tree evaluator: (k-1) * lpe lanes each */
+__host__ __device__ constexpr uint32_t tree_spw(int k, int lpe = 1) { return WAVE / ((uint32_t)(k - 1) * lpe); }
+
+/* Tree graphs, wave per set(s): a set of level k has k-1 edges and 2(k-1)
+ * splits, (S - sub, sub) and (sub, S - sub) for the subtree `sub` below each
+ * edge.  A segment of (k-1) * LPE consecutive lanes owns one set.  LPE 1:
+ * lane e costs both splits of edge e, which share their two memo entries
+ * (fewer lanes per split: the form for large levels); LPE 2: lanes 2e and
+ * 2e+1 cost one split each (half the latency: the form for levels too
+ * small to fill the GPU).  The segment reduces the first valid split (rank
+ * and NaN flag) and the first minimum of the packed (cost, rank) key with
+ * shuffles, and the winning lane writes its split to the memo.  Waves w0,
+ * w0 + wstride, ... cover the level. */
+template <int ECW, bool FAST, int LPE>
+__device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
+                                bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot, uint32_t count,
+                                const MemoRef &mr, uint64_t w0, uint64_t wstride) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const uint32_t per = 2u * (k - 1);
-    const uint32_t spw = WAVE / per;                    /* sets per wave, per consecutive lanes each */
-    const uint32_t seg = lane / per, t = lane % per;
+    const uint32_t per = (uint32_t)(k - 1) * LPE;
+    const uint32_t spw = tree_spw(k, LPE);              /* sets per wave, per consecutive lanes each */
+    const uint32_t seg = lane / per, t = lane % per, e = LPE == 1 ? t : t >> 1;
+    const int o_lo = LPE == 1 ? 0 : (int)(t & 1u), o_hi = LPE == 1 ? 2 : o_lo + 1;
     const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
     const uint64_t nwaves = (count + spw - 1) / spw;
-    uint32_t npairs = 0;
     for (uint64_t w = w0; w < nwaves; w += wstride) {
         const uint64_t idx = w * spw + seg;
         const bool active = idx < count && seg < spw;
-        uint32_t first = ~0u;
-        unsigned long long key = ~0ull;
-        Rel jr;
-        uint32_t S = 0, L = 0;
+        uint32_t first = ~0u, f[2] = {~0u, ~0u};
+        unsigned long long key = ~0ull, kk[2] = {~0ull, ~0ull};
+        Rel jr[2];
+        uint32_t S = 0, sub = 0;
         if (active) {
             S = lvl_sets[idx];
-            const uint32_t cb = pdep32(2u << (t >> 1), S);   /* skip the root (lowest member) */
-            const int c = __builtin_ctz(cb);
-            const uint32_t sub = S & desc[c];
-            const uint32_t o = t & 1u;
-            L = o ? sub : (S ^ sub);
-            const uint32_t R = S ^ L;
-            const uint32_t rank = bicc ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
-            if (FAST) {                               /* per-edge split table (BlobHdr::tree_fast) */
-                const uint32_t *te = smem + h.o_tedge + 8 * c;
-                EdgeProv pv;
-                pv.sel = __uint_as_float(te[o]);
-                pv.nq = (int)(te[2] & 0xffu);
-                pv.n_mi = (int)((te[2] >> 8) & 0xffu);
-                pv.mi = (const uint16_t *)(te + 4 + 2 * o);
-                jr = join_prov(q, L, load_rel(mr.vals, memo_slot(mr, L)), R, load_rel(mr.vals, memo_slot(mr, R)),
-                               pv);
-            } else {
+            const int c = select_bit(S, e + 1u);        /* member e+1: the root (lowest member) is skipped */
+            sub = S & desc[c];
+            const uint32_t rest = S ^ sub;
+            const Rel Rs = load_rel(mr.vals, memo_slot(mr, sub)), Rr = load_rel(mr.vals, memo_slot(mr, rest));
+            uint32_t rk[2];
+            if (bicc) {
+                rk[0] = 2u * tree_post(S, c, desc, lglob);
+                rk[1] = rk[0] + 1u;
+            } else {                                    /* pext(S - sub, S) is the complement of pext(sub, S) */
+                rk[1] = pext32(sub, S);
+                rk[0] = rk[1] ^ (0xffffffffu >> (32 - k));
+            }
             uint64_t m[GQ_MAX_ECW];
             int nq = 0;
-            const int nw = ECW ? ECW : h.ecw;
-            if (h.tree_ecm) {
-                const uint64_t *em = (const uint64_t *)(smem + h.o_ecm) + c * nw;
-                for (int w = 0; w < nw; w++) {
-                    m[w] = em[w];
-                    nq += __popcll(m[w]);
+            const uint32_t *te = smem + h.o_tedge + 8 * c;
+            if (!FAST) {
+                const int nw = ECW ? ECW : h.ecw;
+                if (h.tree_ecm) {                       /* the crossing classes of edge c, both orientations */
+                    const uint64_t *em = (const uint64_t *)(smem + h.o_ecm) + c * nw;
+                    for (int i = 0; i < nw; i++) {
+                        m[i] = em[i];
+                        nq += __popcll(m[i]);
+                    }
+                } else {
+                    nq = match_mask<ECW>(q, rest, sub, m);
                 }
-            } else {
-                nq = match_mask<ECW>(q, L, R, m);
             }
-            jr = join_mask<ECW>(q, L, load_rel(mr.vals, memo_slot(mr, L)), R, load_rel(mr.vals, memo_slot(mr, R)),
-                                m, nq);
+#pragma unroll 1
+            for (int o = o_lo; o < o_hi; o++) {        /* o = 0: (S - sub, sub); o = 1: (sub, S - sub) */
+                const uint32_t L = o ? sub : rest, R = o ? rest : sub;
+                const Rel &RL = o ? Rs : Rr, &RR = o ? Rr : Rs;
+                Rel r;
+                if (FAST) {                             /* per-edge split table (BlobHdr::tree_fast) */
+                    EdgeProv pv;
+                    pv.sel = __uint_as_float(te[o]);
+                    pv.nq = (int)(te[2] & 0xffu);
+                    pv.n_mi = (int)((te[2] >> 8) & 0xffu);
+                    pv.mi = (const uint16_t *)(te + 4 + 2 * o);
+                    r = join_prov(q, L, RL, R, RR, pv);
+                } else {
+                    r = join_mask<ECW>(q, L, RL, R, RR, m, nq);
+                }
+                const bool nan = r.total != r.total;
+                f[o] = (rk[o] << 1) | (nan ? 1u : 0u);
+                if (!nan) kk[o] = ((unsigned long long)ord_key(r.total) << 32) | rk[o];
+                jr[o] = r;
             }
-            npairs++;
-            const bool nan = jr.total != jr.total;
-            first = (rank << 1) | (nan ? 1u : 0u);
-            if (!nan) key = ((unsigned long long)ord_key(jr.total) << 32) | rank;
+            first = f[0] < f[1] ? f[0] : f[1];
+            key = kk[0] < kk[1] ? kk[0] : kk[1];
         }
         /* segmented min over the set's lanes: after the step with offset
          * `off`, lane t holds the minimum over [t, min(t + 2 off, per)) */
@@ -634,31 +669,32 @@ __device__ uint32_t eval_level_tree(const BlobHdr &h, const uint32_t *smem, cons
         sf = __shfl(sf, head < WAVE ? head : 0);
         sk = __shfl(sk, head < WAVE ? head : 0);
         if (active) {
-            const bool win = (sf & 1u) ? (first == sf) : (key == sk);
-            if (win) {
+            const bool nanfirst = (sf & 1u) != 0;
+            const bool w0n = nanfirst ? (f[0] == sf) : (kk[0] == sk);
+            const bool w1n = nanfirst ? (f[1] == sf) : (kk[1] == sk);
+            if (w0n || w1n) {
                 const uint32_t slot = lvl_slot ? lvl_slot[idx] : S;
-                store_rel(mr.vals, slot, jr);
-                mr.left[slot] = L;
+                store_rel(mr.vals, slot, w1n ? jr[1] : jr[0]);
+                mr.left[slot] = w1n ? sub : (S ^ sub);
             }
         }
     }
-    return npairs;
 }
 
-template <int ECW, bool FAST>
+template <int ECW, bool FAST, int LPE>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     /* the grid may be sized by a bound on the level: idle workgroups leave
      * before staging the blob */
-    const uint32_t spw = WAVE / (2u * (k - 1));
+    const uint32_t spw = tree_spw(k, LPE);
     if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= counters[k]) return;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    (void)eval_level_tree<ECW, FAST>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
+    eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
                                (uint64_t)gridDim.x * (BLOCK / WAVE));
     /* no pair counter: every split of a tree set is valid, so the host
      * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
@@ -808,8 +844,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
             for (unsigned long long item = tid; item < total; item += BLOCK) {
                 const uint32_t i = (uint32_t)(item / per), local = (uint32_t)(item % per);
                 const uint32_t S = s_set[i];
-                const uint32_t cb = pdep32(2u << (local >> 1), S);     /* skips the root (lowest) */
-                const int c = __builtin_ctz(cb);
+                const int c = select_bit(S, (local >> 1) + 1u);         /* skips the root (lowest) */
                 const uint32_t sub = S & desc[c];
                 const uint32_t o = local & 1u;
                 const uint32_t L = o ? sub : (S ^ sub), R = S ^ L;
@@ -1119,7 +1154,7 @@ __host__ __device__ constexpr size_t big_lds(int words) {
 
 /* sets per task of level k */
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
-    if (J.h.is_tree) return (BLOCK / WAVE) * (WAVE / (2u * (k - 1))) * TREE_ITERS;
+    if (J.h.is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
     uint64_t items = J.bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
     uint64_t B = (768 + items - 1) / items;
     return (uint32_t)(B < 1 ? 1 : (B > BMAX ? BMAX : B));
@@ -1279,10 +1314,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
             const MemoRef mr = job_memo(J);
             const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
             if (h.ecw == 1)
-                (void)eval_level_tree<1, false>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                eval_level_tree<1, false, 1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
                                                 BLOCK / WAVE);
             else
-                (void)eval_level_tree<0, false>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                eval_level_tree<0, false, 1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
                                                 BLOCK / WAVE);
             np = 0;                            /* counted on the host */
         } else if (J.bicc) {
@@ -1341,7 +1376,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs,
     const uint32_t first = (t - tpg[lo]) * spt, cnt = J.counters[k];
     const uint32_t m = cnt - first < spt ? cnt - first : spt;
     const MemoRef mr = job_memo(J);
-    (void)eval_level_tree<1, true>(h, smem, q, k, J.bicc != 0, J.lvl_sets + J.lvl_off[k] + first,
+    eval_level_tree<1, true, 1>(h, smem, q, k, J.bicc != 0, J.lvl_sets + J.lvl_off[k] + first,
                                    J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr, m, mr,
                                    threadIdx.x / WAVE, BLOCK / WAVE);
 }
@@ -1817,7 +1852,7 @@ static uint64_t binom64(int n, int k) {
 
 /* Tasks of a tree job at level k (task_sets on the device). */
 static uint64_t tree_tasks(uint64_t sets, int k) {
-    const uint64_t spt = (uint64_t)(BLOCK / WAVE) * (WAVE / (2 * (k - 1))) * TREE_ITERS;
+    const uint64_t spt = (uint64_t)(BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
     return (sets + spt - 1) / spt;
 }
 
@@ -2133,19 +2168,36 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
     return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
 }
 
-/* One level of a tree query: the per-edge-table kernel when the blob has
- * the table, else the class-mask kernel for its mask width. */
-static void launch_eval_tree(const BlobHdr &h, dim3 grid, size_t lds, hipStream_t s, const uint32_t *blob, int k,
-                             int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr, uint32_t *counters) {
-    if (h.tree_fast)
-        hipLaunchKernelGGL((k_eval_tree<1, true>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
-                           counters);
-    else if (h.ecw == 1)
-        hipLaunchKernelGGL((k_eval_tree<1, false>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
-                           counters);
-    else
-        hipLaunchKernelGGL((k_eval_tree<0, false>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr,
-                           counters);
+/* Levels of at most this many waves at two lanes per edge leave the GPU
+ * latency-bound (a few waves per SIMD): they take the two-lane form. */
+#define TREE_LPE2_WAVES 4096u
+
+/* One level of a tree query of `sets` sets (an upper bound): the
+ * per-edge-table kernel when the blob has the table, else the class-mask
+ * kernel for its mask width; two lanes per edge when the level is small.
+ * The grid covers the level, capped at 8192 workgroups. */
+static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipStream_t s, const uint32_t *blob,
+                             int k, int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr,
+                             uint32_t *counters) {
+    const uint64_t spw2 = tree_spw(k, 2);
+    const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= TREE_LPE2_WAVES ? 2 : 1;
+    const uint64_t spw = tree_spw(k, lpe);
+    const uint64_t waves = (sets_k + spw - 1) / spw;
+    const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192)));
+#define GQ_TREE_LAUNCH(E, F, P)                                                                                \
+    hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
+                       counters)
+    if (h.tree_fast) {
+        if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
+        else GQ_TREE_LAUNCH(1, true, 1);
+    } else if (h.ecw == 1) {
+        if (lpe == 2) GQ_TREE_LAUNCH(1, false, 2);
+        else GQ_TREE_LAUNCH(1, false, 1);
+    } else {
+        if (lpe == 2) GQ_TREE_LAUNCH(0, false, 2);
+        else GQ_TREE_LAUNCH(0, false, 1);
+    }
+#undef GQ_TREE_LAUNCH
 }
 
 static MemoRef dense_memo(DeviceCtx &c) {
@@ -2247,10 +2299,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
                            s, c.sjob, 1, k, d_prefix + 2 * k);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
-        const uint64_t spw = WAVE / (2ull * (k - 1));
-        const uint64_t waves = (in.tree_counts[k] + spw - 1) / spw;
-        const uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
-        launch_eval_tree(h, dim3(tblocks), lds_eval, s, c.b.blob, k, sj.bicc, c.b.lvl_sets + off_h[k],
+        launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, sj.bicc, c.b.lvl_sets + off_h[k],
                          c.hslot + off_h[k], mr, c.b.counters);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
@@ -2518,12 +2567,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         const uint64_t T = (1ull << k) - 2;
         if (h.is_tree) {
-            const uint64_t spw = WAVE / (2ull * (k - 1));
             /* exact level size of a tree (subtree polynomial), else C(n,k) */
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
-            uint64_t waves = (lvl + spw - 1) / spw;
-            uint32_t tblocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
-            launch_eval_tree(h, dim3(tblocks), lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
+            launch_eval_tree(h, lvl, lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
                              dense_memo(c), c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;

@@ -35,6 +35,20 @@ def test_config2_snowflake20_mpdp():
     assert gst.sets == ost.sets
 
 
+@pytest.mark.parametrize("bicc", [0, 1])
+def test_star18_one_and_two_lanes_per_edge(bicc):
+    """Star-18: levels 8-12 (12,376 to 24,310 sets) exceed 4096 waves at two
+    lanes per edge and take the one-lane form, the other levels the two-lane
+    form; DPsub (bicc 0) ranks splits by pext, MPDP (bicc 1) by post-order."""
+    g = qgen.star(18, 0x1818)
+    c = cfg("dpsub", bicc)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"star-18 bicc={bicc}")
+    assert gst.pairs == ost.pairs == 17 * 2 ** 17
+    assert gst.sets == ost.sets
+
+
 def test_clique14_dpsub():
     g = qgen.clique(14, 0x77)
     c = cfg("dpsub", 0)
