@@ -514,6 +514,17 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
         u = p;
     }
 #undef DFS
+    /* blocks[31] (a set of at most 32 relations has at most 31 blocks):
+     * bit b set when block b is a simple cycle -- at least three members,
+     * each with exactly two neighbours inside the block */
+    uint32_t cyc = 0;
+    for (int b = 0; b < nblk; b++) {
+        const uint32_t blk = blocks[b];
+        bool c = __builtin_popcount(blk) >= 3;
+        for (uint32_t x = blk; c && x; x &= x - 1u) c = __builtin_popcount(adj[__builtin_ctz(x)] & blk) == 2;
+        if (c) cyc |= 1u << b;
+    }
+    blocks[31] = cyc;
     return nblk;
 }
 
@@ -554,6 +565,76 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
     uint32_t lb = pdep32(local - acc + 1u, blk), rb = blk ^ lb;
     *L = grow(lb & (0u - lb), S & ~rb, adj);
     *R = grow(rb & (0u - rb), S & ~lb, adj);
+}
+
+/* Enumerated splits of a block of m members: a simple cycle (bit b of
+ * blocks[31]) has its m(m-1) arcs, any other block its 2^m - 2 non-empty
+ * proper subsets. */
+__device__ __forceinline__ uint32_t block_items(uint32_t blk, bool cyc) {
+    const uint32_t m = (uint32_t)__builtin_popcount(blk);
+    return cyc ? m * (m - 1u) : (1u << m) - 2u;
+}
+
+/* Enumerated splits of a set: *full = sum of 2^m - 2 over its blocks (the
+ * reference's split count, which bounds the ranks), return = the splits
+ * actually enumerated. */
+__device__ __forceinline__ unsigned long long set_items(const uint32_t *blocks, int nblk,
+                                                        unsigned long long *full) {
+    unsigned long long n = 0, f = 0;
+    for (int b = 0; b < nblk; b++) {
+        const uint32_t m = (uint32_t)__builtin_popcount(blocks[b]);
+        f += (1ull << m) - 2ull;
+        n += ((blocks[31] >> b) & 1u) ? (unsigned long long)m * (m - 1u) : (1ull << m) - 2ull;
+    }
+    *full = f;
+    return n;
+}
+
+/* Enumerated split `item` of a set with blocks[0..): (L, R) and its rank
+ * `local` in bicc_split's order, which is the reference's.  A split lb of a
+ * block is valid only when lb and blk - lb are both connected inside the
+ * block (a path between two members of a block never leaves it), so in a
+ * simple cycle only the arcs can be valid and they are all that is
+ * enumerated: arc (start, len) runs len members along the cycle from its
+ * start-th member, walking from the lowest member towards its lower
+ * neighbour; its rank is pext(lb, blk) - 1 within the block. */
+__device__ __forceinline__ uint32_t bicc_item(uint32_t S, const uint32_t *blocks, uint32_t item, const uint32_t *adj,
+                                              uint32_t *L, uint32_t *R) {
+    const uint32_t cyc = blocks[31];
+    uint32_t acc_i = 0, acc_r = 0, blk;
+    bool is_cyc;
+    int b = 0;
+    while (true) {
+        blk = blocks[b];
+        is_cyc = (cyc >> b) & 1u;
+        const uint32_t c = block_items(blk, is_cyc);
+        if (item < acc_i + c) break;
+        acc_i += c;
+        acc_r += block_items(blk, false);
+        b++;
+    }
+    const uint32_t local = item - acc_i;
+    uint32_t lb, rank;
+    if (is_cyc) {
+        const uint32_t m1 = (uint32_t)__builtin_popcount(blk) - 1u;
+        const uint32_t start = local / m1, len = local % m1 + 1u;
+        uint32_t cur = blk & (0u - blk), prev = 0;
+        lb = 0;
+        for (uint32_t st = 0; st < start + len; st++) {
+            if (st >= start) lb |= cur;
+            uint32_t nb = adj[__builtin_ctz(cur)] & blk & ~prev;
+            prev = cur;
+            cur = nb & (0u - nb);
+        }
+        rank = acc_r + pext32(lb, blk) - 1u;
+    } else {
+        lb = pdep32(local + 1u, blk);
+        rank = acc_r + local;
+    }
+    const uint32_t rb = blk ^ lb;
+    *L = grow(lb & (0u - lb), S & ~rb, adj);
+    *R = grow(rb & (0u - rb), S & ~lb, adj);
+    return rank;
 }
 
 /* One level, one kernel: a workgroup takes a batch of B sets.
@@ -806,9 +887,9 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
             if (BICC && !tree) {
                 int nblk = dfs_blocks(S, adj, dfs + tid, BMAX, s_blocks + tid * 32);
                 s_nblk[tid] = nblk;
-                cnt = 0;
-                for (int b = 0; b < nblk; b++) cnt += (1ull << __builtin_popcount(s_blocks[tid * 32 + b])) - 2ull;
-                if (cnt > 0x7fffffffull) {          /* split rank would not fit 31 bits */
+                unsigned long long full;
+                cnt = set_items(s_blocks + tid * 32, nblk, &full);
+                if (full > 0x7fffffffull) {         /* split rank would not fit 31 bits */
                     s_misc[1] = 1;
                     cnt = 0;
                 } else if (bl.ents && cnt > BIG_SPLITS) {
@@ -864,15 +945,14 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                     else hi = mid;
                 }
                 const uint32_t i = lo, S = s_set[i];
-                const uint32_t local = (uint32_t)(item - s_off[i]);
                 uint32_t L, R;
-                bicc_split(S, s_blocks + i * 32, local, adj, &L, &R);
+                const uint32_t rank = bicc_item(S, s_blocks + i * 32, (uint32_t)(item - s_off[i]), adj, &L, &R);
                 if ((L | R) == S) {
                     Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
                     npairs++;
                     bool nan = jr.total != jr.total;
-                    atomicMin(&s_first[i], (local << 1) | (nan ? 1u : 0u));
-                    if (!nan) atomicMin(&s_best[i], ((unsigned long long)ord_key(jr.total) << 32) | local);
+                    atomicMin(&s_first[i], (rank << 1) | (nan ? 1u : 0u));
+                    if (!nan) atomicMin(&s_best[i], ((unsigned long long)ord_key(jr.total) << 32) | rank);
                 }
             }
         } else {
@@ -1032,7 +1112,7 @@ __global__ void __launch_bounds__(1024) k_big_scan(const BigEnt *ents, uint32_t 
 }
 
 /* Big MPDP sets, chunk-parallel: workgroup-iteration c takes BIG_CHUNK
- * consecutive split ranks of one deferred set (its block decomposition is
+ * consecutive enumerated splits (bicc_item) of one deferred set (its block decomposition is
  * rebuilt in LDS when the set changes), keeps per lane the first valid
  * split (rank, NaN flag) and the first minimum of (cost, rank), reduces them
  * per wave and merges them into the set's lvl_first / lvl_best with global
@@ -1085,16 +1165,17 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
         const uint32_t c1 = (cnt - c0) < BIG_CHUNK ? cnt : c0 + BIG_CHUNK;
         uint32_t first = ~0u, npairs = 0;
         unsigned long long best = ~0ull;
-        for (uint32_t local = c0 + (uint32_t)tid; local < c1; local += BLOCK) {
+        for (uint32_t item = c0 + (uint32_t)tid; item < c1; item += BLOCK) {
             uint32_t L, R;
-            bicc_split(S, s_blocks, local, adj, &L, &R);
+            const uint32_t rank = bicc_item(S, s_blocks, item, adj, &L, &R);
             if ((L | R) == S) {
                 Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
                 npairs++;
                 const bool nan = jr.total != jr.total;
-                if (first == ~0u) first = (local << 1) | (nan ? 1u : 0u);
+                const uint32_t f = (rank << 1) | (nan ? 1u : 0u);
+                if (f < first) first = f;
                 if (!nan) {
-                    const unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | local;
+                    const unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | rank;
                     if (key < best) best = key;
                 }
             }
