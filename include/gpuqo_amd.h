@@ -86,7 +86,7 @@ typedef struct gq_base_rel {
     float rows;
     float tuples;
     gq_path_cost cost;
-    int32_t width;
+    int32_t width;          /* >= 0; the widths of a query sum to at most 2^31-1 (GQ_ERR_INVALID otherwise) */
 } gq_base_rel;
 
 /* VarInfo = VarStats + GpuqoIndexInfo:

@@ -129,6 +129,7 @@ struct View {
 struct Rel {                /* value part of JoinRelation (planner_info.cuh:22-28) */
     float rows, startup, total;
     int width;
+    uint32_t hs;             /* 0, or hash_pack(): the relation's hash table size as an inner */
 };
 
 GQ_HD int eq_class_index(int l, int r, int size) {    /* gpuqo_common.h:71-79 */
@@ -301,6 +302,27 @@ GQ_HD float join_rows(float sel, float lrows, float rrows) {
 
 struct Extra { bool inner_unique, indexed; float joinrows, outer_match_frac, match_count; };
 
+/* ExecChooseHashTableSize depends on the inner relation alone (rows,
+ * width) and the query's work_mem, so the memo keeps it with the relation:
+ * bit 31 set, log2 nbuckets in bits 21-25, log2 nbatches in bits 26-30 and
+ * the width in bits 0-20.  0 when the width needs more than 21 bits or a
+ * size is not a power of two below 2^31 (then it is computed per join). */
+GQ_HD uint32_t hash_pack(float rows, int width, int work_mem) {
+    if (width < 0 || width >= (1 << 21)) return 0u;
+    const HashSize hs = hash_table_size(rows, width, work_mem);
+    if (hs.nbuckets <= 0 || (hs.nbuckets & (hs.nbuckets - 1)) || hs.nbatches <= 0 ||
+        (hs.nbatches & (hs.nbatches - 1)))
+        return 0u;
+    return 0x80000000u | ((uint32_t)__builtin_ctz((unsigned)hs.nbatches) << 26) |
+           ((uint32_t)__builtin_ctz((unsigned)hs.nbuckets) << 21) | (uint32_t)width;
+}
+GQ_HD HashSize hash_unpack(uint32_t hs) {
+    HashSize r;
+    r.nbuckets = 1 << ((hs >> 21) & 31u);
+    r.nbatches = 1 << ((hs >> 26) & 31u);
+    return r;
+}
+
 /* cost_nestloop (cost_postgres.cuh:110-275); qual_per_tuple is
  * cost_qual_eval's per-tuple cost (its startup is 0). */
 template <class S>
@@ -373,7 +395,7 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
     startup_cost += inner.total;
     startup_cost += (q.p.cpu_operator_cost * (float)n_quals + q.p.cpu_tuple_cost) * inner_path_rows;
     run_cost += q.p.cpu_operator_cost * (float)n_quals * outer_path_rows;
-    HashSize hs = hash_table_size(inner_path_rows, inner.width, q.p.work_mem);
+    HashSize hs = inner.hs ? hash_unpack(inner.hs) : hash_table_size(inner_path_rows, inner.width, q.p.work_mem);
     if (hs.nbatches > 1) {
         float outerpages = page_size(outer_path_rows, outer.width);
         float innerpages = page_size(inner_path_rows, inner.width);
@@ -466,6 +488,7 @@ GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     Rel out;
     out.rows = join_rows(pv.selectivity(q, l, r), L.rows, R.rows);
     out.width = L.width + R.width;
+    out.hs = 0u;
     const int n_quals = pv.n_quals();
     float qual_pt = q.qualtab[n_quals];
     bool single = popc(r) == 1;

@@ -104,18 +104,41 @@ __device__ __forceinline__ bool is_conn(const uint32_t *conn, uint32_t x) {
     return (conn[x >> 5] >> (x & 31)) & 1u;
 }
 
+/* Memo record: rows, startup, total and a width word, which is the width
+ * itself or, bit 31 set, hash_pack() of the relation (the width in its low
+ * 21 bits).  Widths are below 2^31 (base widths below 2^26, at most 32
+ * relations), so bit 31 tells the two forms apart. */
+__device__ __forceinline__ int rec_width(uint32_t w) { return (w >> 31) ? (int)(w & 0x1fffffu) : (int)w; }
+
 __device__ __forceinline__ Rel load_rel(const float4 *vals, uint32_t x) {
     float4 v = vals[x];
     Rel r;
     r.rows = v.x;
     r.startup = v.y;
     r.total = v.z;
-    r.width = __float_as_int(v.w);
+    const uint32_t w = __float_as_uint(v.w);
+    r.width = rec_width(w);
+    r.hs = (w >> 31) ? w : 0u;
     return r;
 }
 
-__device__ __forceinline__ void store_rel(float4 *vals, uint32_t x, const Rel &r) {
-    vals[x] = make_float4(r.rows, r.startup, r.total, __int_as_float(r.width));
+/* pack = false keeps the plain width (readers then size the hash table per
+ * join): for latency-bound levels, where the winning lane alone would run
+ * hash_table_size on the critical path. */
+__device__ __forceinline__ void store_rel(float4 *vals, uint32_t x, const Rel &r, int work_mem, bool pack = true) {
+    const uint32_t hs = pack ? hash_pack(r.rows, r.width, work_mem) : 0u;
+    vals[x] = make_float4(r.rows, r.startup, r.total, __uint_as_float(hs ? hs : (uint32_t)r.width));
+}
+
+/* base relation i of the blob into memo slot x */
+__device__ __forceinline__ void store_base(float4 *vals, uint32_t x, const BlobHdr &h, const uint32_t *blob, int i) {
+    const float *b = (const float *)(blob + h.o_base) + 4 * i;
+    Rel r;
+    r.rows = b[0];
+    r.startup = b[1];
+    r.total = b[2];
+    r.width = __float_as_int(b[3]);
+    store_rel(vals, x, r, h.p.work_mem);
 }
 
 /* The memo of one DP.  Dense (keys == nullptr): the slot of set S is S.
@@ -259,9 +282,8 @@ __global__ void k_clear(uint32_t *conn, uint32_t words, uint32_t *counters) {
 __global__ void k_base(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left, uint32_t *conn) {
     int i = threadIdx.x;
     if (i < h.n_rels) {
-        const float *b = (const float *)(blob + h.o_base) + 4 * i;
         uint32_t s = 1u << i;
-        vals[s] = make_float4(b[0], b[1], b[2], b[3]);
+        store_base(vals, s, h, blob, i);
         left[s] = 0;
         atomicOr(&conn[s >> 5], 1u << (s & 31));
     }
@@ -449,7 +471,7 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
         uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(lvl_best[idx] & 0xffffffffull);
         uint32_t l = pdep32(rank, S), r = S ^ l;
         Rel j = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
-        store_rel(vals, S, j);
+        store_rel(vals, S, j, q.p.work_mem);
         left[S] = l;
     }
 }
@@ -755,7 +777,7 @@ __device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const Vi
             const bool w1n = nanfirst ? (f[1] == sf) : (kk[1] == sk);
             if (w0n || w1n) {
                 const uint32_t slot = lvl_slot ? lvl_slot[idx] : S;
-                store_rel(mr.vals, slot, w1n ? jr[1] : jr[0]);
+                store_rel(mr.vals, slot, w1n ? jr[1] : jr[0], q.p.work_mem, LPE == 1);
                 mr.left[slot] = w1n ? sub : (S ^ sub);
             }
         }
@@ -1025,7 +1047,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                     R = S ^ L;
                 }
                 Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
-                store_rel(vals, S, jr);
+                store_rel(vals, S, jr, q.p.work_mem);
                 left[S] = L;
             }
         }
@@ -1210,7 +1232,7 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
                 uint32_t L, R;
                 bicc_split(S, s_blocks, rank, adj, &L, &R);
                 Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
-                store_rel(J.vals, S, jr);
+                store_rel(J.vals, S, jr, q.p.work_mem);
                 J.left[S] = L;
             }
         }
@@ -1248,7 +1270,6 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj) {
         if (tid < 64) J.counters[tid] = 0;
         __syncthreads();
         if (tid < J.h.n_rels) {
-            const float *b = (const float *)(J.blob + J.h.o_base) + 4 * tid;
             const uint32_t s1 = 1u << tid;
             uint32_t slot = s1;
             if (J.keys) {                      /* keys cleared beforehand */
@@ -1259,7 +1280,7 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj) {
                     continue;
                 }
             }
-            J.vals[slot] = make_float4(b[0], b[1], b[2], b[3]);
+            store_base(J.vals, slot, J.h, J.blob, tid);
             J.left[slot] = 0;
         }
     }
@@ -1530,7 +1551,7 @@ __device__ uint32_t walk_tree_wave(const MemoRef &mr, uint32_t root_set, DPNode 
         const uint32_t S = s_set[lane];
         const float4 v = mr.vals[memo_slot(mr, S)];
         DPNode nd;
-        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = __float_as_int(v.w);
+        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = rec_width(__float_as_uint(v.w));
         nd.left = s_l[lane] >= 0 ? s_pre[s_l[lane]] : -1;
         nd.right = s_r[lane] >= 0 ? s_pre[s_r[lane]] : -1;
         nd.pad = 0;
@@ -1688,9 +1709,8 @@ __global__ void k_ds_init(BlobHdr h, const uint32_t *blob, const int *level1, Ds
     if (tid < 64) counters[tid] = 0;
     if (tid < 256) d.sz[tid] = 0;
     if (tid < (uint64_t)h.n_rels) {
-        const float *b = (const float *)(blob + h.o_base) + 4 * tid;
         uint32_t s1 = 1u << tid;
-        vals[s1] = make_float4(b[0], b[1], b[2], b[3]);
+        store_base(vals, s1, h, blob, (int)tid);
         left[s1] = 0;
         int r = level1[tid];
         d.lset[tid] = 1u << r;
@@ -1906,7 +1926,7 @@ __global__ void __launch_bounds__(BLOCK) k_ds_finalize(BlobHdr h, const uint32_t
         int ls = k - 1 - (int)tt;
         uint32_t l = d.lset[d.sz[40 + ls] + a], r = S ^ l;
         Rel jr = join(q, l, load_rel(vals, l), r, load_rel(vals, r));
-        store_rel(vals, S, jr);
+        store_rel(vals, S, jr, q.p.work_mem);
         left[S] = l;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) d.sz[k] = cnt;

@@ -436,6 +436,20 @@ static int validate(const gq_query *q) {
         set_error("null array in query");
         return GQ_ERR_INVALID;
     }
+    /* join widths are sums of base widths in an int (planner_info.cuh:22-28);
+     * the device memo keeps them below 2^31 */
+    long long wsum = 0;
+    for (int i = 0; i < q->n_rels; i++) {
+        if (q->base_rels[i].width < 0) {
+            set_error("base relation %d: negative width", i);
+            return GQ_ERR_INVALID;
+        }
+        wsum += q->base_rels[i].width;
+    }
+    if (wsum > 0x7fffffffLL) {
+        set_error("base relation widths sum to %lld (at most 2^31-1)", wsum);
+        return GQ_ERR_INVALID;
+    }
     for (int e = 0; e < q->n_eq_classes; e++) {
         int b0 = q->ec_member_offsets[e], b1 = q->ec_member_offsets[e + 1];
         if (b1 - b0 < 1) {

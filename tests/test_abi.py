@@ -5,6 +5,8 @@ import glob
 import os
 import re
 
+import pytest
+
 import gpuqo_amd
 from gpuqo_amd import abi
 
@@ -52,3 +54,22 @@ def test_config_default_matches_mirror():
 
 def test_version_string():
     assert b"gfx950" in gpuqo_amd.lib().gq_version()
+
+
+def test_widths_are_validated_before_any_device_call():
+    """Join widths are int sums of base widths: a negative base width, or
+    base widths summing past 2^31-1, is rejected as invalid input."""
+    from gpuqo_amd import qgen
+    cfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+    g = qgen.chain(4, 1)
+    g.build()
+    g.base["width"][2] = -1
+    with pytest.raises(gpuqo_amd.GpuqoError) as e:
+        gpuqo_amd.plan(g, cfg)
+    assert e.value.status == abi.GQ_ERR_INVALID and "negative width" in str(e.value)
+    g = qgen.chain(4, 1)
+    g.build()
+    g.base["width"][:] = 0x40000000
+    with pytest.raises(gpuqo_amd.GpuqoError) as e:
+        gpuqo_amd.plan(g, cfg)
+    assert e.value.status == abi.GQ_ERR_INVALID and "widths sum" in str(e.value)
