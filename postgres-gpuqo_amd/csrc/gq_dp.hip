@@ -676,6 +676,13 @@ __device__ __forceinline__ uint32_t bicc_item(uint32_t S, const uint32_t *blocks
 /* sets of level k per wave of the tree evaluator: (k-1) * lpe lanes each */
 __host__ __device__ constexpr uint32_t tree_spw(int k, int lpe = 1) { return WAVE / ((uint32_t)(k - 1) * lpe); }
 
+/* LDS of the tree evaluator: the staged blob, then (one-lane form) a record
+ * queue of WAVE entries per wave: slot, left, rows, startup, total, width */
+#define TREE_Q_WORDS (6 * WAVE)
+__host__ __device__ constexpr size_t tree_lds(int words, int lpe) {
+    return (size_t)(((words + 3) & ~3) * 4) + (lpe == 1 ? (size_t)(BLOCK / WAVE) * TREE_Q_WORDS * 4 : 0);
+}
+
 /* Tree graphs, wave per set(s): a set of level k has k-1 edges and 2(k-1)
  * splits, (S - sub, sub) and (sub, S - sub) for the subtree `sub` below each
  * edge.  A segment of (k-1) * LPE consecutive lanes owns one set.  LPE 1:
@@ -684,10 +691,14 @@ __host__ __device__ constexpr uint32_t tree_spw(int k, int lpe = 1) { return WAV
  * 2e+1 cost one split each (half the latency: the form for levels too
  * small to fill the GPU).  The segment reduces the first valid split (rank
  * and NaN flag) and the first minimum of the packed (cost, rank) key with
- * shuffles, and the winning lane writes its split to the memo.  Waves w0,
- * w0 + wstride, ... cover the level. */
+ * shuffles.  LPE 2: the winning lane writes its record with the plain
+ * width.  LPE 1: the winners' records wait in the wave's LDS queue
+ * (tree_lds) and are written a wave at a time, one record per lane, with
+ * the hash table size packed (store_rel) -- one hash_pack pass per WAVE
+ * sets instead of one per iteration.  Waves w0, w0 + wstride, ... cover the
+ * level; every record is written before the call returns. */
 template <int ECW, bool FAST, int LPE>
-__device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const View<uint32_t> &q, int k,
+__device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k,
                                 bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot, uint32_t count,
                                 const MemoRef &mr, uint64_t w0, uint64_t wstride) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -697,12 +708,33 @@ __device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const Vi
     const int o_lo = LPE == 1 ? 0 : (int)(t & 1u), o_hi = LPE == 1 ? 2 : o_lo + 1;
     const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
     const uint64_t nwaves = (count + spw - 1) / spw;
+    uint32_t *wq = smem + ((h.words + 3) & ~3) + (threadIdx.x / WAVE) * TREE_Q_WORDS;
+    uint32_t qn = 0;                                    /* queued records (wave-uniform) */
+    auto flush = [&]() {                                /* lane i writes queued record i */
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if ((uint32_t)lane < qn) {
+            Rel r;
+            r.rows = __uint_as_float(wq[2 * WAVE + lane]);
+            r.startup = __uint_as_float(wq[3 * WAVE + lane]);
+            r.total = __uint_as_float(wq[4 * WAVE + lane]);
+            r.width = (int)wq[5 * WAVE + lane];
+            const uint32_t slot = wq[lane];
+            store_rel(mr.vals, slot, r, q.p.work_mem, true);
+            mr.left[slot] = wq[WAVE + lane];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        qn = 0;
+    };
     for (uint64_t w = w0; w < nwaves; w += wstride) {
         const uint64_t idx = w * spw + seg;
         const bool active = idx < count && seg < spw;
-        uint32_t first = ~0u, f[2] = {~0u, ~0u};
-        unsigned long long key = ~0ull, kk[2] = {~0ull, ~0ull};
-        Rel jr[2];
+        /* per orientation o: first-valid key f_o, cost key k_o, result j_o
+         * (scalars, not arrays: a loop-indexed local array goes to scratch) */
+        uint32_t first = ~0u, f0 = ~0u, f1 = ~0u;
+        unsigned long long key = ~0ull, k0 = ~0ull, k1 = ~0ull;
+        Rel j0, j1;
         uint32_t S = 0, sub = 0;
         if (active) {
             S = lvl_sets[idx];
@@ -710,13 +742,13 @@ __device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const Vi
             sub = S & desc[c];
             const uint32_t rest = S ^ sub;
             const Rel Rs = load_rel(mr.vals, memo_slot(mr, sub)), Rr = load_rel(mr.vals, memo_slot(mr, rest));
-            uint32_t rk[2];
+            uint32_t rk0, rk1;
             if (bicc) {
-                rk[0] = 2u * tree_post(S, c, desc, lglob);
-                rk[1] = rk[0] + 1u;
+                rk0 = 2u * tree_post(S, c, desc, lglob);
+                rk1 = rk0 + 1u;
             } else {                                    /* pext(S - sub, S) is the complement of pext(sub, S) */
-                rk[1] = pext32(sub, S);
-                rk[0] = rk[1] ^ (0xffffffffu >> (32 - k));
+                rk1 = pext32(sub, S);
+                rk0 = rk1 ^ (0xffffffffu >> (32 - k));
             }
             uint64_t m[GQ_MAX_ECW];
             int nq = 0;
@@ -749,12 +781,21 @@ __device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const Vi
                     r = join_mask<ECW>(q, L, RL, R, RR, m, nq);
                 }
                 const bool nan = r.total != r.total;
-                f[o] = (rk[o] << 1) | (nan ? 1u : 0u);
-                if (!nan) kk[o] = ((unsigned long long)ord_key(r.total) << 32) | rk[o];
-                jr[o] = r;
+                const uint32_t rk = o ? rk1 : rk0;
+                const uint32_t fo = (rk << 1) | (nan ? 1u : 0u);
+                const unsigned long long ko = nan ? ~0ull : (((unsigned long long)ord_key(r.total) << 32) | rk);
+                if (o == 0) {
+                    f0 = fo;
+                    k0 = ko;
+                    j0 = r;
+                } else {
+                    f1 = fo;
+                    k1 = ko;
+                    j1 = r;
+                }
             }
-            first = f[0] < f[1] ? f[0] : f[1];
-            key = kk[0] < kk[1] ? kk[0] : kk[1];
+            first = f0 < f1 ? f0 : f1;
+            key = k0 < k1 ? k0 : k1;
         }
         /* segmented min over the set's lanes: after the step with offset
          * `off`, lane t holds the minimum over [t, min(t + 2 off, per)) */
@@ -771,17 +812,39 @@ __device__ void eval_level_tree(const BlobHdr &h, const uint32_t *smem, const Vi
         const int head = (int)(seg * per);               /* the set's first lane holds its minimum */
         sf = __shfl(sf, head < WAVE ? head : 0);
         sk = __shfl(sk, head < WAVE ? head : 0);
+        bool win = false;
+        uint32_t slot = 0, lft = 0;
         if (active) {
             const bool nanfirst = (sf & 1u) != 0;
-            const bool w0n = nanfirst ? (f[0] == sf) : (kk[0] == sk);
-            const bool w1n = nanfirst ? (f[1] == sf) : (kk[1] == sk);
+            const bool w0n = nanfirst ? (f0 == sf) : (k0 == sk);
+            const bool w1n = nanfirst ? (f1 == sf) : (k1 == sk);
             if (w0n || w1n) {
-                const uint32_t slot = lvl_slot ? lvl_slot[idx] : S;
-                store_rel(mr.vals, slot, w1n ? jr[1] : jr[0], q.p.work_mem, LPE == 1);
-                mr.left[slot] = w1n ? sub : (S ^ sub);
+                win = true;
+                slot = lvl_slot ? lvl_slot[idx] : S;
+                lft = w1n ? sub : (S ^ sub);
+                if (w1n) j0 = j1;
+                if (LPE == 2) {
+                    store_rel(mr.vals, slot, j0, q.p.work_mem, false);
+                    mr.left[slot] = lft;
+                }
             }
         }
+        if (LPE == 1) {
+            const unsigned long long bal = __ballot(win);
+            if (win) {
+                const uint32_t pos = qn + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                wq[pos] = slot;
+                wq[WAVE + pos] = lft;
+                wq[2 * WAVE + pos] = __float_as_uint(j0.rows);
+                wq[3 * WAVE + pos] = __float_as_uint(j0.startup);
+                wq[4 * WAVE + pos] = __float_as_uint(j0.total);
+                wq[5 * WAVE + pos] = (uint32_t)j0.width;
+            }
+            qn += (uint32_t)__popcll(bal);
+            if (qn + spw > WAVE) flush();
+        }
     }
+    if (LPE == 1 && qn) flush();
 }
 
 template <int ECW, bool FAST, int LPE>
@@ -2117,7 +2180,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             set_error("tree job %d without level sizes", j);
             return GQ_ERR_INVALID;
         }
-        if (fast_tree_job(h)) lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
+        if (fast_tree_job(h)) lds_tree = std::max(lds_tree, tree_lds(h.words, 1));
         else lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
         if (algos[j] == DP_BICC && !h.is_tree && h.n_rels >= 15) {
             uint64_t mc = 0;
@@ -2285,6 +2348,7 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint64_t spw = tree_spw(k, lpe);
     const uint64_t waves = (sets_k + spw - 1) / spw;
     const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192)));
+    lds = std::max(lds, tree_lds(h.words, lpe));
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
                        counters)
@@ -3151,7 +3215,7 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         const BlobHdr &h = j->J.h;
         const uint64_t lvl = (int)j->counts.size() == j->n + 1 ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
         if (fast_tree_job(h) && (int)j->counts.size() == j->n + 1) {
-            lds_tree = std::max(lds_tree, (size_t)(((h.words + 3) & ~3) * 4));
+            lds_tree = std::max(lds_tree, tree_lds(h.words, 1));
             ub_tree += tree_tasks(lvl, j->cur_k);
         } else {
             lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
