@@ -1318,12 +1318,15 @@ __host__ __device__ constexpr size_t big_lds(int words) {
 
 #define TREE_ITERS 16
 
-/* sets per task of level k */
-__device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
-    if (J.h.is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
-    uint64_t items = J.bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
+/* sets per task of level k: trees, else MPDP (bicc) or DPsub sets */
+__host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bicc, int k) {
+    if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
+    uint64_t items = bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
     uint64_t B = (768 + items - 1) / items;
     return (uint32_t)(B < 1 ? 1 : (B > BMAX ? BMAX : B));
+}
+__device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
+    return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k);
 }
 
 __global__ void k_batch_init(const BatchJob *jobs, int nj) {
@@ -2262,8 +2265,12 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             const BlobHdr &h = ins[j]->hdr;
             if (k > h.n_rels) continue;
             if (fast_tree_job(h)) tasks_tree += tree_tasks(ins[j]->tree_counts[k], k);
-            else ub_gen += (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1) ? ins[j]->tree_counts[k]
-                                                                                         : binom64(h.n_rels, k);
+            else {                        /* tasks of a bound on the level size */
+                const uint64_t lvl = (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1)
+                                         ? ins[j]->tree_counts[k] : binom64(h.n_rels, k);
+                const uint64_t spt = task_sets_of(h.is_tree != 0, algos[j] == DP_BICC, k);
+                ub_gen += (lvl + spt - 1) / spt;
+            }
         }
         if (tasks_tree) {
             if (tasks_tree > 0x7fffffffull) {
@@ -2274,7 +2281,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
                                nj, k, (const uint32_t *)(tp_tree + (size_t)k * (nj + 1)));
         }
         if (ub_gen) {
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen / 16, 1), 2048);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), 2048);
             hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, c.jobs, nj, k,
                                tp_gen + (size_t)k * (nj + 1), tpl_gen, big_cap ? c.big_ents : nullptr,
                                (uint32_t)big_cap, big_hdr);
@@ -3219,7 +3226,8 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
             ub_tree += tree_tasks(lvl, j->cur_k);
         } else {
             lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
-            ub += lvl;
+            const uint64_t spt = task_sets_of(h.is_tree != 0, j->J.bicc != 0, j->cur_k);
+            ub += (lvl + spt - 1) / spt;          /* tasks of a bound on the level size */
         }
         if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
             big_cap += binom64(j->n, j->cur_k);
@@ -3262,7 +3270,7 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         ECHK(hipStreamWaitEvent(e->s2, e->fork, 0));
         const uint32_t tpl = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
         const size_t lt = tpl ? lds + 4 * (size_t)(na + 1) : lds;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub / 16, 1), 2048);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), 2048);
         hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, e->d_jobs, na, 0, tp_gen, tpl,
                            big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
         if (big_cap) {
