@@ -1318,6 +1318,12 @@ __host__ __device__ constexpr size_t big_lds(int words) {
 
 #define TREE_ITERS 16
 
+/* Workgroups of a general-job launch (k_batch_eval): its 192 VGPRs allow
+ * two workgroups per CU, 512 on the GPU; more only queue behind the tree
+ * kernel running beside it, and the launch ends when the last has been
+ * dispatched.  Workgroups loop over the launch's tasks. */
+#define GEN_GRID_MAX 512u
+
 /* sets per task of level k: trees, else MPDP (bicc) or DPsub sets */
 __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bicc, int k) {
     if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
@@ -2281,7 +2287,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
                                nj, k, (const uint32_t *)(tp_tree + (size_t)k * (nj + 1)));
         }
         if (ub_gen) {
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), 2048);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), GEN_GRID_MAX);
             hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, c.jobs, nj, k,
                                tp_gen + (size_t)k * (nj + 1), tpl_gen, big_cap ? c.big_ents : nullptr,
                                (uint32_t)big_cap, big_hdr);
@@ -3270,7 +3276,7 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         ECHK(hipStreamWaitEvent(e->s2, e->fork, 0));
         const uint32_t tpl = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
         const size_t lt = tpl ? lds + 4 * (size_t)(na + 1) : lds;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), 2048);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), GEN_GRID_MAX);
         hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, e->d_jobs, na, 0, tp_gen, tpl,
                            big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
         if (big_cap) {
