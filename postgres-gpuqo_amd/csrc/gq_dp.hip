@@ -481,15 +481,32 @@ __global__ void __launch_bounds__(BLOCK) k_finalize_dpsub(BlobHdr h, const uint3
  * (gpuqo_cpu_dpsub_bicc.cu:65-169): DFS from the lowest vertex, neighbours
  * ascending; a block is emitted when a tree edge (p,c) finishes with
  * low[c] >= disc[p]; its vertices are p plus the vertices discovered at or
- * after c still on the vertex stack.  `dfs` is this lane's column of a
- * [32][stride] LDS array (disc | low << 8 | parent << 16). */
+ * after c still on the vertex stack, i.e. c and the vertices above it on
+ * the stack.  `dfs` is this set's column of a [32][stride] LDS array
+ * (disc | low << 8 | parent << 16); `own` and `subb` are BSET_W-word rows.
+ * Besides blocks[0..nblk) it fills, for block_side:
+ *   own[v]   v not the DFS root: v and the vertices hanging off v outside
+ *            the block where v is not the root (the blocks rooted at v and,
+ *            recursively, what hangs off those); own[root] = S;
+ *   subb[b]  the union of own[v] over the members of block b other than its
+ *            root, i.e. the vertices below block b -- the block's root is
+ *            its one member outside subb[b];
+ *   blocks[31] bit b set when block b is a simple cycle (at least three
+ *            members, each with exactly two neighbours inside the block).
+ * While the DFS runs, subb[32], subb[31], ... hold the vertex stack: the
+ * stack depth plus the blocks emitted never exceeds |S| - 1 <= 31. */
+#define BSET_W 33
 __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint32_t *dfs, int stride,
-                                          uint32_t *blocks) {
+                                          uint32_t *blocks, uint32_t *own, uint32_t *subb) {
 #define DFS(v) dfs[(v) * stride]
+#define STK(i) subb[32 - (i)]
     int nblk = 0;
-    int root = __builtin_ctz(S);
-    uint32_t visited = 1u << root, onstack = 1u << root;
+    const int root = __builtin_ctz(S);
+    for (uint32_t x = S; x; x &= x - 1u) own[__builtin_ctz(x)] = 0u;
+    uint32_t visited = 1u << root;
     uint32_t t = 1;
+    int sp = 0;
+    STK(sp++) = (uint32_t)root;
     DFS(root) = 0u | (255u << 8) | (255u << 16);
     int u = root, cursor = -1;
     while (true) {
@@ -502,7 +519,7 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
                 DFS(v) = t | (255u << 8) | ((uint32_t)u << 16);
                 t++;
                 visited |= 1u << v;
-                onstack |= 1u << v;
+                STK(sp++) = (uint32_t)v;
                 u = v;
                 cursor = -1;
             } else {
@@ -519,26 +536,29 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
         if (u == root) break;
         int p = (int)(du >> 16);
         uint32_t dp = DFS(p);
-        uint32_t lowu = (du >> 8) & 0xffu, discu = du & 0xffu;
+        uint32_t lowu = (du >> 8) & 0xffu;
         if (lowu >= (dp & 0xffu)) {
-            uint32_t rest = 0, os = onstack;
-            while (os) {
-                int w = __builtin_ctz(os);
-                os &= os - 1;
-                if ((DFS(w) & 0xffu) >= discu) rest |= 1u << w;
-            }
-            onstack &= ~rest;
-            blocks[nblk++] = rest | (1u << p);
+            uint32_t rest = 0, sub = 0, v;
+            do {                                   /* pop down to u */
+                v = STK(--sp);
+                const uint32_t o = own[v] | (1u << v);
+                own[v] = o;
+                rest |= 1u << v;
+                sub |= o;
+            } while (v != (uint32_t)u);
+            blocks[nblk] = rest | (1u << p);
+            subb[nblk] = sub;
+            nblk++;
+            own[p] |= sub;
         }
         uint32_t lowp = (dp >> 8) & 0xffu;
         if (lowu < lowp) DFS(p) = (dp & 0xffff00ffu) | (lowu << 8);
         cursor = u;
         u = p;
     }
+#undef STK
 #undef DFS
-    /* blocks[31] (a set of at most 32 relations has at most 31 blocks):
-     * bit b set when block b is a simple cycle -- at least three members,
-     * each with exactly two neighbours inside the block */
+    own[root] = S;
     uint32_t cyc = 0;
     for (int b = 0; b < nblk; b++) {
         const uint32_t blk = blocks[b];
@@ -548,6 +568,16 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
     }
     blocks[31] = cyc;
     return nblk;
+}
+
+/* The reference's grow(lowbit(x), S - (blk - x)) for a part x of block blk
+ * (subb sb) that is connected inside the block: x's non-root members with
+ * what hangs off them, plus, when x holds the block's root, everything
+ * outside the block's subtree. */
+__device__ __forceinline__ uint32_t block_side(uint32_t S, uint32_t x, uint32_t sb, const uint32_t *own) {
+    uint32_t r = (x & ~sb) ? (S & ~sb) : 0u;
+    for (uint32_t y = x & sb; y; y &= y - 1u) r |= own[__builtin_ctz(y)];
+    return r;
 }
 
 __device__ __forceinline__ uint32_t pext32(uint32_t x, uint32_t mask) {
@@ -572,9 +602,10 @@ __device__ __forceinline__ uint32_t tree_post(uint32_t S, int c, const uint32_t 
 }
 
 /* Split `local` of a set whose blocks are blocks[0..nblk): block-major,
- * lb ascending over the block's non-empty proper subsets. */
-__device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, uint32_t local, const uint32_t *adj,
-                                           uint32_t *L, uint32_t *R) {
+ * lb ascending over the block's non-empty proper subsets (lb = pdep of
+ * local + 1 on the block); a valid split, so both sides are block_side. */
+__device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, const uint32_t *own,
+                                           const uint32_t *subb, uint32_t local, uint32_t *L, uint32_t *R) {
     uint32_t acc = 0, blk;
     int b = 0;
     while (true) {
@@ -584,9 +615,9 @@ __device__ __forceinline__ void bicc_split(uint32_t S, const uint32_t *blocks, u
         acc += c;
         b++;
     }
-    uint32_t lb = pdep32(local - acc + 1u, blk), rb = blk ^ lb;
-    *L = grow(lb & (0u - lb), S & ~rb, adj);
-    *R = grow(rb & (0u - rb), S & ~lb, adj);
+    const uint32_t lb = pdep32(local - acc + 1u, blk), rb = blk ^ lb, sb = subb[b];
+    *L = block_side(S, lb, sb, own);
+    *R = block_side(S, rb, sb, own);
 }
 
 /* Enumerated splits of a block of m members: a simple cycle (bit b of
@@ -614,13 +645,15 @@ __device__ __forceinline__ unsigned long long set_items(const uint32_t *blocks, 
 
 /* Enumerated split `item` of a set with blocks[0..): (L, R) and its rank
  * `local` in bicc_split's order, which is the reference's.  A split lb of a
- * block is valid only when lb and blk - lb are both connected inside the
- * block (a path between two members of a block never leaves it), so in a
- * simple cycle only the arcs can be valid and they are all that is
- * enumerated: arc (start, len) runs len members along the cycle from its
- * start-th member, walking from the lowest member towards its lower
- * neighbour; its rank is pext(lb, blk) - 1 within the block. */
-__device__ __forceinline__ uint32_t bicc_item(uint32_t S, const uint32_t *blocks, uint32_t item, const uint32_t *adj,
+ * block is valid (the reference's L | R = S) exactly when lb and blk - lb
+ * are both connected inside the block, a path between two members of a
+ * block never leaving it; then L and R are block_side.  An invalid split
+ * returns L = R = 0.  In a simple cycle only the arcs can be valid and they
+ * are all that is enumerated: arc (start, len) runs len members along the
+ * cycle from its start-th member, walking from the lowest member towards
+ * its lower neighbour; its rank is pext(lb, blk) - 1 within the block. */
+__device__ __forceinline__ uint32_t bicc_item(uint32_t S, const uint32_t *blocks, const uint32_t *own,
+                                              const uint32_t *subb, uint32_t item, const uint32_t *adj,
                                               uint32_t *L, uint32_t *R) {
     const uint32_t cyc = blocks[31];
     uint32_t acc_i = 0, acc_r = 0, blk;
@@ -654,8 +687,15 @@ __device__ __forceinline__ uint32_t bicc_item(uint32_t S, const uint32_t *blocks
         rank = acc_r + local;
     }
     const uint32_t rb = blk ^ lb;
-    *L = grow(lb & (0u - lb), S & ~rb, adj);
-    *R = grow(rb & (0u - rb), S & ~lb, adj);
+    if (!is_cyc && __builtin_popcount(blk) > 2 &&
+        (grow(lb & (0u - lb), lb, adj) != lb || grow(rb & (0u - rb), rb, adj) != rb)) {
+        *L = 0u;
+        *R = 0u;
+        return rank;
+    }
+    const uint32_t sb = subb[b];
+    *L = block_side(S, lb, sb, own);
+    *R = block_side(S, rb, sb, own);
     return rank;
 }
 
@@ -920,26 +960,31 @@ struct BigList {
 
 /* LDS layout of the level evaluation, after the staged blob (W4 words):
  *   s_best[BMAX] u64, s_off[BMAX+2] u64, s_first[BMAX], s_set[BMAX], s_misc[4]
- *   MPDP (general graphs): s_nblk[BMAX], s_blocks[BMAX][32], dfs[32][BMAX]
+ *   MPDP (general graphs), batches of at most BMAX_B sets: s_nblk[BMAX],
+ *     s_blocks[BMAX_B][32], dfs[32][BMAX_B], s_own[BMAX_B][BSET_W],
+ *     s_subb[BMAX_B][BSET_W] (dfs_blocks)
  *   DPsub: q_l, q_i, q_r [QCAP] each, then the staged connectivity bitmap. */
+#define BMAX_B 64
 __host__ __device__ constexpr size_t eval_lds_common(int words) {
     return (size_t)(((words + 3) & ~3) * 4) + BMAX * 8 + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
 }
 __host__ __device__ constexpr size_t eval_lds_bicc(int words) {
-    return eval_lds_common(words) + BMAX * 4 + (size_t)BMAX * 32 * 4 * 2;
+    return eval_lds_common(words) + BMAX * 4 + (size_t)BMAX_B * 32 * 4 * 2 + (size_t)BMAX_B * BSET_W * 4 * 2;
 }
 __host__ __device__ constexpr size_t eval_lds_dpsub(int words, int conn_words_lds) {
     return eval_lds_common(words) + 3 * QCAP * 4 + (size_t)conn_words_lds * 4;
 }
 
-/* Evaluate level k: batches bt0, bt0 + bstride, ... of B sets each.  The
- * blob is staged at smem[0..words) (and, for DPsub, `cb` points to the
- * connectivity bitmap, staged or global).  Returns the pairs costed;
- * raises *err when a set has more than 2^31 splits. */
+/* Evaluate level k: batches bt0, bt0 + bstride, ... of B sets each (at most
+ * BMAX_B for MPDP on general graphs).  The blob is staged at smem[0..words)
+ * (and, for DPsub, `cb` points to the connectivity bitmap, staged or
+ * global).  Returns the pairs costed; raises *err when a set has more than
+ * 2^31 splits. */
 template <bool BICC>
 __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k, uint32_t B,
                                const uint32_t *cb, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
                                uint32_t *left, uint32_t bt0, uint32_t bstride, const BigList &bl) {
+    if (BICC && h.is_tree == 0 && B > BMAX_B) B = BMAX_B;
     const bool tree = h.is_tree != 0;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -950,8 +995,10 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
     uint32_t *s_set = s_first + BMAX;
     uint32_t *s_misc = s_set + BMAX;                           /* [0] queue length, [1] error */
     uint32_t *s_nblk = s_misc + 4;
-    uint32_t *s_blocks = s_nblk + BMAX;                        /* BICC: [BMAX][32]            */
-    uint32_t *dfs = s_blocks + BMAX * 32;                      /* BICC: [32][BMAX]            */
+    uint32_t *s_blocks = s_nblk + BMAX;                        /* BICC: [BMAX_B][32]          */
+    uint32_t *dfs = s_blocks + BMAX_B * 32;                    /* BICC: [32][BMAX_B]          */
+    uint32_t *s_own = dfs + 32 * BMAX_B;                       /* BICC: [BMAX_B][BSET_W]      */
+    uint32_t *s_subb = s_own + BMAX_B * BSET_W;                /* BICC: [BMAX_B][BSET_W]      */
     uint32_t *q_l = s_misc + 4;                                /* DPsub: [QCAP]               */
     uint32_t *q_i = q_l + QCAP;
     uint32_t *q_r = q_i + QCAP;
@@ -970,7 +1017,8 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
             s_first[tid] = ~0u;
             unsigned long long cnt = tree ? 2ull * (k - 1) : T;
             if (BICC && !tree) {
-                int nblk = dfs_blocks(S, adj, dfs + tid, BMAX, s_blocks + tid * 32);
+                int nblk = dfs_blocks(S, adj, dfs + tid, BMAX_B, s_blocks + tid * 32, s_own + tid * BSET_W,
+                                      s_subb + tid * BSET_W);
                 s_nblk[tid] = nblk;
                 unsigned long long full;
                 cnt = set_items(s_blocks + tid * 32, nblk, &full);
@@ -1031,7 +1079,8 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                 }
                 const uint32_t i = lo, S = s_set[i];
                 uint32_t L, R;
-                const uint32_t rank = bicc_item(S, s_blocks + i * 32, (uint32_t)(item - s_off[i]), adj, &L, &R);
+                const uint32_t rank = bicc_item(S, s_blocks + i * 32, s_own + i * BSET_W, s_subb + i * BSET_W,
+                                                (uint32_t)(item - s_off[i]), adj, &L, &R);
                 if ((L | R) == S) {
                     Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
                     npairs++;
@@ -1104,7 +1153,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                     L = (rank & 1u) ? sub : (S ^ sub);
                     R = S ^ L;
                 } else if (BICC) {
-                    bicc_split(S, s_blocks + tid * 32, rank, adj, &L, &R);
+                    bicc_split(S, s_blocks + tid * 32, s_own + tid * BSET_W, s_subb + tid * BSET_W, rank, &L, &R);
                 } else {
                     L = pdep32(rank, S);
                     R = S ^ L;
@@ -1226,7 +1275,9 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
         const int W4 = (h.words + 3) & ~3;
         uint32_t *s_blocks = smem + W4;                /* [32] */
         uint32_t *s_dfs = s_blocks + 32;               /* [32] */
-        uint32_t *s_misc = s_dfs + 32;                 /* [0] nblk, [1] S, [2] last */
+        uint32_t *s_own = s_dfs + 32;                  /* [BSET_W] */
+        uint32_t *s_subb = s_own + BSET_W;             /* [BSET_W] */
+        uint32_t *s_misc = s_subb + BSET_W;            /* [0] nblk, [1] S, [2] last */
         if ((int)job != have_job) {
             __syncthreads();
             for (int i = tid; i < h.words; i += BLOCK) smem[i] = J.blob[i];
@@ -1239,7 +1290,7 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
             if (tid == 0) {
                 const uint32_t S0 = J.lvl_sets[gidx];
                 s_misc[1] = S0;
-                s_misc[0] = (uint32_t)dfs_blocks(S0, adj, s_dfs, 1, s_blocks);
+                s_misc[0] = (uint32_t)dfs_blocks(S0, adj, s_dfs, 1, s_blocks, s_own, s_subb);
             }
             __syncthreads();
             have_b = b;
@@ -1252,7 +1303,7 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
         unsigned long long best = ~0ull;
         for (uint32_t item = c0 + (uint32_t)tid; item < c1; item += BLOCK) {
             uint32_t L, R;
-            const uint32_t rank = bicc_item(S, s_blocks, item, adj, &L, &R);
+            const uint32_t rank = bicc_item(S, s_blocks, s_own, s_subb, item, adj, &L, &R);
             if ((L | R) == S) {
                 Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
                 npairs++;
@@ -1293,7 +1344,7 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
                 const unsigned long long bst = atomicOr(&J.lvl_best[gidx], 0ull);
                 const uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(bst & 0xffffffffull);
                 uint32_t L, R;
-                bicc_split(S, s_blocks, rank, adj, &L, &R);
+                bicc_split(S, s_blocks, s_own, s_subb, rank, &L, &R);
                 Rel jr = join(q, L, load_rel(J.vals, L), R, load_rel(J.vals, R));
                 store_rel(J.vals, S, jr, q.p.work_mem);
                 J.left[S] = L;
@@ -1304,7 +1355,7 @@ __global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt
 }
 
 __host__ __device__ constexpr size_t big_lds(int words) {
-    return (size_t)(((words + 3) & ~3) * 4) + (32 + 32 + 4) * 4;
+    return (size_t)(((words + 3) & ~3) * 4) + (32 + 32 + 2 * BSET_W + 4) * 4;
 }
 /* ================================================================ batches
  *
@@ -1329,7 +1380,8 @@ __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bic
     if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
     uint64_t items = bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
     uint64_t B = (768 + items - 1) / items;
-    return (uint32_t)(B < 1 ? 1 : (B > BMAX ? BMAX : B));
+    const uint64_t bmax = bicc ? BMAX_B : BMAX;
+    return (uint32_t)(B < 1 ? 1 : (B > bmax ? bmax : B));
 }
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
     return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k);
@@ -2751,7 +2803,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                              dense_memo(c), c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
-            uint32_t B = (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
+            uint32_t B = (uint32_t)std::min<uint64_t>(algo == DP_BICC ? BMAX_B : BMAX,
+                                                      std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
             BigList bl;
             bl.ents = nullptr;
