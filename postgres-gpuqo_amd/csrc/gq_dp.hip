@@ -507,27 +507,34 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
     uint32_t t = 1;
     int sp = 0;
     STK(sp++) = (uint32_t)root;
-    DFS(root) = 0u | (255u << 8) | (255u << 16);
+    /* du = DFS(u) and au = adj[u] & S stay in registers while at u */
+    uint32_t du = 0u | (255u << 8) | (255u << 16);
+    DFS(root) = du;
+    uint32_t au = adj[root] & S;
     int u = root, cursor = -1;
     while (true) {
         uint32_t gt = cursor < 0 ? 0xffffffffu : (uint32_t)(0xffffffffull << (cursor + 1));
-        uint32_t rem = adj[u] & S & gt;
-        uint32_t du = DFS(u);
+        uint32_t rem = au & gt;
         if (rem) {
             int v = __builtin_ctz(rem);
             if (!((visited >> v) & 1u)) {
-                DFS(v) = t | (255u << 8) | ((uint32_t)u << 16);
+                du = t | (255u << 8) | ((uint32_t)u << 16);
+                DFS(v) = du;
                 t++;
                 visited |= 1u << v;
                 STK(sp++) = (uint32_t)v;
+                au = adj[v] & S;
                 u = v;
                 cursor = -1;
             } else {
-                uint32_t dv = DFS(v);
-                if ((int)(du >> 16) != v && (dv & 0xffu) < (du & 0xffu)) {
-                    uint32_t low = (du >> 8) & 0xffu;
-                    uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
-                    DFS(u) = (du & 0xffff00ffu) | (nl << 8);
+                if ((int)(du >> 16) != v) {       /* not the tree edge to u's parent */
+                    uint32_t dv = DFS(v);
+                    if ((dv & 0xffu) < (du & 0xffu)) {
+                        uint32_t low = (du >> 8) & 0xffu;
+                        uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
+                        du = (du & 0xffff00ffu) | (nl << 8);
+                        DFS(u) = du;
+                    }
                 }
                 cursor = v;
             }
@@ -536,6 +543,7 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
         if (u == root) break;
         int p = (int)(du >> 16);
         uint32_t dp = DFS(p);
+        const uint32_t ap = adj[p] & S;
         uint32_t lowu = (du >> 8) & 0xffu;
         if (lowu >= (dp & 0xffu)) {
             uint32_t rest = 0, sub = 0, v;
@@ -552,9 +560,14 @@ __device__ __forceinline__ int dfs_blocks(uint32_t S, const uint32_t *adj, uint3
             own[p] |= sub;
         }
         uint32_t lowp = (dp >> 8) & 0xffu;
-        if (lowu < lowp) DFS(p) = (dp & 0xffff00ffu) | (lowu << 8);
+        if (lowu < lowp) {
+            dp = (dp & 0xffff00ffu) | (lowu << 8);
+            DFS(p) = dp;
+        }
         cursor = u;
         u = p;
+        du = dp;
+        au = ap;
     }
 #undef STK
 #undef DFS
@@ -1379,7 +1392,9 @@ __host__ __device__ constexpr size_t big_lds(int words) {
 __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bicc, int k) {
     if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
     uint64_t items = bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
-    uint64_t B = (768 + items - 1) / items;
+    /* MPDP: about one pass of the workgroup over the splits of its sets,
+     * since each split is a long dependent chain; DPsub: three */
+    uint64_t B = ((bicc ? BLOCK : 768) + items - 1) / items;
     const uint64_t bmax = bicc ? BMAX_B : BMAX;
     return (uint32_t)(B < 1 ? 1 : (B > bmax ? bmax : B));
 }
@@ -2803,8 +2818,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                              dense_memo(c), c.b.counters);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
-            uint32_t B = (uint32_t)std::min<uint64_t>(algo == DP_BICC ? BMAX_B : BMAX,
-                                                      std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
+            uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k)
+                                         : (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
             BigList bl;
             bl.ents = nullptr;
