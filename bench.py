@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates byte counters) and exit")
+    ap.add_argument("--batch-only", type=int, default=0, metavar="REPS",
+                    help="only plan the 256-query batch (one warm-up, then REPS batches) and print its times")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
@@ -278,6 +280,20 @@ def main():
 
     import gpuqo_amd
     from gpuqo_amd import abi, qgen
+    if args.batch_only:
+        qs = qgen.config4()
+        bcfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+        bargs = gpuqo_amd.BatchArgs(qs)
+        bargs.run(bcfg)
+        wall, dev = [], []
+        for _ in range(args.batch_only):
+            t0 = time.perf_counter()
+            outs = bargs.run(bcfg)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            dev.append(outs[0].stats.gpu_ms)
+        print(json.dumps({"workload": "batch256-mpdp", "queries": len(qs), "batch_ms_wall": wall,
+                          "batch_ms_device": dev}), flush=True)
+        return 0
     cfg_name, algo, bicc = WORKLOADS[args.workload]
     g = getattr(qgen, cfg_name)()
     cfg = abi.default_config(algorithm=algo, dpsub_bicc=bicc)
