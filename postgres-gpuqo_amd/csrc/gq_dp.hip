@@ -900,13 +900,69 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
     if (LPE == 1 && qn) flush();
 }
 
+/* Tree graphs: is v (a neighbour of connected S outside it, hence a leaf
+ * of S + v) the highest-numbered leaf of S + v?  Then S + v is generated
+ * from S alone: removing a leaf keeps a tree connected, so every connected
+ * set T of two or more relations comes from exactly one S = T - (its
+ * highest leaf). */
+__device__ __forceinline__ bool is_top_leaf(const uint32_t *adj, uint32_t S, int v) {
+    const uint32_t T = S | (1u << v);
+    uint32_t above = S & ~((2u << v) - 1u);       /* members of S numbered above v */
+    while (above) {
+        const int u = __builtin_ctz(above);
+        above &= above - 1u;
+        if (__builtin_popcount(adj[u] & T) <= 1) return false;
+    }
+    return true;
+}
+
+/* Dense-memo tree DP: the level list k+1 from set i of level k (is_top_leaf
+ * rule), positions reserved with one atomic per wave on *dst_count.  Called
+ * by whole waves; the order within a level list is immaterial, each set's
+ * best split being computed on its own. */
+__device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t *src, uint32_t count, uint32_t i,
+                                             uint32_t *dst, uint32_t *dst_count) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    uint32_t S = 0, emit = 0;
+    if (i < count) {
+        S = src[i];
+        uint32_t nb = 0;
+        for (uint32_t r = S; r; r &= r - 1u) nb |= adj[__builtin_ctz(r)];
+        nb &= ~S;
+        for (uint32_t c = nb; c; c &= c - 1u) {
+            const int v = __builtin_ctz(c);
+            if (is_top_leaf(adj, S, v)) emit |= 1u << v;
+        }
+    }
+    const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
+    uint32_t incl = cnt;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, WAVE - 1);
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(dst_count, total);
+    uint32_t pos = __shfl(base, 0) + incl - cnt;
+    for (; emit; emit &= emit - 1u) dst[pos++] = S | (emit & (0u - emit));
+}
+
+/* Level k of a tree DP: workgroups [0, ev_blocks) evaluate it (the grid may
+ * be sized by a bound on the level: idle ones leave before staging the
+ * blob); with `next` set, workgroups from ev_blocks on build the level k+1
+ * list there (dense memo, counters[k+1] counts it), BLOCK level-k sets
+ * each. */
 template <int ECW, bool FAST, int LPE>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
-                                                      MemoRef mr, uint32_t *counters) {
+                                                      MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
+                                                      uint32_t *next) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
-    /* the grid may be sized by a bound on the level: idle workgroups leave
-     * before staging the blob */
+    if (blockIdx.x >= ev_blocks) {
+        expand_level(blob + h.o_adj, lvl_sets, counters[k], (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next,
+                     counters + k + 1);
+        return;
+    }
     const uint32_t spw = tree_spw(k, LPE);
     if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= counters[k]) return;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
@@ -914,10 +970,35 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     View<uint32_t> q = make_view(h, smem);
     const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
     eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
-                               (uint64_t)gridDim.x * (BLOCK / WAVE));
+                                    (uint64_t)ev_blocks * (BLOCK / WAVE));
     /* no pair counter: every split of a tree set is valid, so the host
      * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
      * address would serialise the level */
+}
+
+/* Start of a dense-memo tree DP: zeroes the 64 result counters, stores the
+ * base relations and builds level 2, the query's edges, each from its lower
+ * end (the is_top_leaf rule on singletons).  One wave. */
+__global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left,
+                                                      uint32_t *lvl2, uint32_t *counters) {
+    const int i = threadIdx.x;
+    counters[i] = 0u;
+    uint32_t emit = 0;
+    if (i < h.n_rels) {
+        store_base(vals, 1u << i, h, blob, i);
+        left[1u << i] = 0u;
+        emit = blob[h.o_adj + i] & ~((2u << i) - 1u);          /* neighbours above i */
+    }
+    const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
+    uint32_t incl = cnt;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (i >= off) incl += y;
+    }
+    uint32_t pos = incl - cnt;
+    for (; emit; emit &= emit - 1u) lvl2[pos++] = (1u << i) | (emit & (0u - emit));
+    __syncthreads();
+    if (i == WAVE - 1) counters[2] = incl;
 }
 
 /* One DP's device state, as the batch kernels and the big-set kernels see it
@@ -1728,17 +1809,6 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
  * job j, host-computed from the exact level sizes; jobs without keys have
  * no tasks.  Order within a level list is immaterial: each set's best split
  * is computed independently. */
-__device__ __forceinline__ bool is_top_leaf(const uint32_t *adj, uint32_t S, int v) {
-    const uint32_t T = S | (1u << v);
-    uint32_t above = S & ~((2u << v) - 1u);       /* members of S numbered above v */
-    while (above) {
-        const int u = __builtin_ctz(above);
-        above &= above - 1u;
-        if (__builtin_popcount(adj[u] & T) <= 1) return false;
-    }
-    return true;
-}
-
 __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
                                                    const uint32_t *prefix) {
     const uint32_t total = prefix[nj];
@@ -2422,16 +2492,19 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
  * The grid covers the level, capped at 8192 workgroups. */
 static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipStream_t s, const uint32_t *blob,
                              int k, int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr,
-                             uint32_t *counters) {
+                             uint32_t *counters, uint32_t *next = nullptr) {
     const uint64_t spw2 = tree_spw(k, 2);
     const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= TREE_LPE2_WAVES ? 2 : 1;
     const uint64_t spw = tree_spw(k, lpe);
     const uint64_t waves = (sets_k + spw - 1) / spw;
-    const dim3 grid((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192)));
+    const uint32_t ev = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
+    /* with `next`: the level k+1 list from the sets_k (exact) level-k sets */
+    const uint32_t ex = next ? (uint32_t)((sets_k + BLOCK - 1) / BLOCK) : 0u;
+    const dim3 grid(ev + ex);
     lds = std::max(lds, tree_lds(h.words, lpe));
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters)
+                       counters, ev, next)
     if (h.tree_fast) {
         if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
         else GQ_TREE_LAUNCH(1, true, 1);
@@ -2671,7 +2744,14 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(c.ev0, s));
     uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
-    {
+    /* Trees with exact level sizes skip the 2^n filter: level 2 comes from
+     * k_tree_start and each level's launch builds the next level's list by
+     * expansion (no connectivity bitmap is needed). */
+    const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2;
+    if (tree_exp) {
+        hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
+                           c.b.lvl_sets + off_h[2], c.b.counters);
+    } else {
         uint32_t blocks = (uint32_t)std::min<uint64_t>((conn_words + BLOCK - 1) / BLOCK, 4096);
         hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, c.b.conn, conn_words, c.b.counters);
         hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, c.b.blob, c.b.vals, c.b.left, c.b.conn);
@@ -2796,7 +2876,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     }
     const uint64_t cand_total = nsets_all - (uint64_t)n - 1ull;
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[0], s));
-    {
+    if (!tree_exp) {
         uint64_t waves = (nsets_all + WAVE - 1) / WAVE;
         uint32_t fblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
         hipLaunchKernelGGL(k_filter_all, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, c.lvl_off, c.b.lvl_sets,
@@ -2815,7 +2895,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             /* exact level size of a tree (subtree polynomial), else C(n,k) */
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
             launch_eval_tree(h, lvl, lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
-                             dense_memo(c), c.b.counters);
+                             dense_memo(c), c.b.counters,
+                             tree_exp && k < n ? c.b.lvl_sets + off_h[k + 1] : nullptr);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k)
