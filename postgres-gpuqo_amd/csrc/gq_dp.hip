@@ -2482,6 +2482,9 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
     return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
 }
 
+/* Largest dense tree DP whose level lists are built by expansion */
+#define TREE_EXPAND_MAX_N 20
+
 /* Levels of at most this many waves at two lanes per edge leave the GPU
  * latency-bound (a few waves per SIMD): they take the two-lane form. */
 #define TREE_LPE2_WAVES 4096u
@@ -2744,10 +2747,14 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(c.ev0, s));
     uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
-    /* Trees with exact level sizes skip the 2^n filter: level 2 comes from
-     * k_tree_start and each level's launch builds the next level's list by
-     * expansion (no connectivity bitmap is needed). */
-    const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2;
+    /* Trees with exact level sizes and a memo of at most 2^20 records skip
+     * the 2^n filter: level 2 comes from k_tree_start and each level's
+     * launch builds the next level's list by expansion (no connectivity
+     * bitmap is needed).  Expansion lists are in no particular order; with
+     * larger memos the filter's increasing order keeps a wave's memo
+     * gathers on nearby lines, which outweighs the filter. */
+    const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2 &&
+                          n <= TREE_EXPAND_MAX_N;
     if (tree_exp) {
         hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
                            c.b.lvl_sets + off_h[2], c.b.counters);
