@@ -753,7 +753,7 @@ __host__ __device__ constexpr size_t tree_lds(int words, int lpe) {
 template <int ECW, bool FAST, int LPE>
 __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k,
                                 bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot, uint32_t count,
-                                const MemoRef &mr, uint64_t w0, uint64_t wstride) {
+                                const MemoRef &mr, uint64_t w0, uint64_t wstride, uint32_t s0 = 0u) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t per = (uint32_t)(k - 1) * LPE;
     const uint32_t spw = tree_spw(k, LPE);              /* sets per wave, per consecutive lanes each */
@@ -790,7 +790,7 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
         Rel j0, j1;
         uint32_t S = 0, sub = 0;
         if (active) {
-            S = lvl_sets[idx];
+            S = (w == w0 && s0) ? s0 : lvl_sets[idx];   /* s0: this lane's first set, loaded by the caller */
             const int c = select_bit(S, e + 1u);        /* member e+1: the root (lowest member) is skipped */
             sub = S & desc[c];
             const uint32_t rest = S ^ sub;
@@ -948,29 +948,35 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
 }
 
 /* Level k of a tree DP: workgroups [0, ev_blocks) evaluate it (the grid may
- * be sized by a bound on the level: idle ones leave before staging the
+ * be sized by a bound on the level: idle ones leave after staging the
  * blob); with `next` set, workgroups from ev_blocks on build the level k+1
  * list there (dense memo, counters[k+1] counts it), BLOCK level-k sets
- * each. */
+ * each.  The level's list has room for `cap` entries: each lane loads its
+ * first set before the level size is known, so that this load, the level
+ * size and the blob staging are in flight together. */
 template <int ECW, bool FAST, int LPE>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
-                                                      uint32_t *next) {
+                                                      uint32_t *next, uint32_t cap) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     if (blockIdx.x >= ev_blocks) {
         expand_level(blob + h.o_adj, lvl_sets, counters[k], (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next,
                      counters + k + 1);
         return;
     }
-    const uint32_t spw = tree_spw(k, LPE);
-    if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= counters[k]) return;
+    const uint32_t spw = tree_spw(k, LPE), per = (uint32_t)(k - 1) * LPE;
+    const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+    const uint32_t seg = (threadIdx.x & (WAVE - 1)) / per;
+    const uint64_t idx0 = w0 * spw + seg;
+    const uint32_t count = counters[k];
+    const uint32_t s_pre = (seg < spw && idx0 < cap) ? lvl_sets[idx0] : 0u;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
+    if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= count) return;
     View<uint32_t> q = make_view(h, smem);
-    const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-    eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, counters[k], mr, w0,
-                                    (uint64_t)ev_blocks * (BLOCK / WAVE));
+    eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, count, mr, w0,
+                                    (uint64_t)ev_blocks * (BLOCK / WAVE), idx0 < count ? s_pre : 0u);
     /* no pair counter: every split of a tree set is valid, so the host
      * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
      * address would serialise the level */
@@ -2505,9 +2511,10 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint32_t ex = next ? (uint32_t)((sets_k + BLOCK - 1) / BLOCK) : 0u;
     const dim3 grid(ev + ex);
     lds = std::max(lds, tree_lds(h.words, lpe));
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(sets_k, 0xffffffffull);
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters, ev, next)
+                       counters, ev, next, cap)
     if (h.tree_fast) {
         if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
         else GQ_TREE_LAUNCH(1, true, 1);
