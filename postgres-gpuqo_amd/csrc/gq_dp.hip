@@ -36,6 +36,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "gq_dp.h"
@@ -2155,6 +2157,51 @@ __global__ void __launch_bounds__(WAVE) k_tree(MemoRef mr, uint32_t root_set, DP
     if (threadIdx.x == 0) status[0] = r;
 }
 
+/* IDP1 (gpuqo_idp.cu:38-80): the DP stops at level K and its result is the
+ * cheapest set of K relations.  The reference scans its memo keeping the
+ * first set whose cost total is strictly below the best so far, starting
+ * from +inf (cpu_sequential.cu:101-112; gpuqo_dpsub_common.cu:350-358), so
+ * NaN and +inf totals never win.  out[0] = the least order key of the
+ * level-K totals below +inf (~0: none), out[1] = how many sets have it,
+ * out[2 ..] = the first IDP1_TIES of them (the host orders ties).  The
+ * level's list is list[*list_off ..] (list_off may be null) of *count sets. */
+#define IDP1_TIES 64
+__global__ void __launch_bounds__(1024) k_best_of_level(const uint32_t *list, const uint32_t *list_off,
+                                                        const uint32_t *count, MemoRef mr, uint32_t *out) {
+    __shared__ uint32_t s_min, s_n;
+    const uint32_t off = list_off ? *list_off : 0u, cnt = *count;
+    const float inf = __builtin_huge_valf();
+    if (threadIdx.x == 0) {
+        s_min = ~0u;
+        s_n = 0u;
+    }
+    __syncthreads();
+    uint32_t best = ~0u;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const float t = mr.vals[memo_slot(mr, list[off + i])].z;
+        if (t < inf) {
+            const uint32_t key = ord_key(t);
+            best = key < best ? key : best;
+        }
+    }
+    atomicMin(&s_min, best);
+    __syncthreads();
+    const uint32_t m = s_min;
+    for (uint32_t i = threadIdx.x; i < cnt && m != ~0u; i += blockDim.x) {
+        const uint32_t S = list[off + i];
+        const float t = mr.vals[memo_slot(mr, S)].z;
+        if (t < inf && ord_key(t) == m) {
+            const uint32_t pos = atomicAdd(&s_n, 1u);
+            if (pos < IDP1_TIES) out[2 + pos] = S;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = s_n;
+    }
+}
+
 /* ------------------------------------------------------------ host side */
 
 static uint64_t binom64(int n, int k) {
@@ -2173,9 +2220,9 @@ static uint64_t tree_tasks(uint64_t sets, int k) {
 }
 
 /* Ordered csg-cmp pairs of a tree query: 2(k-1) per connected set of k. */
-static uint64_t tree_pairs(const std::vector<uint64_t> &counts) {
+static uint64_t tree_pairs(const std::vector<uint64_t> &counts, size_t kmax = 64) {
     uint64_t p = 0;
-    for (size_t k = 2; k < counts.size(); k++) p += counts[k] * 2ull * (k - 1);
+    for (size_t k = 2; k < counts.size() && k <= kmax; k++) p += counts[k] * 2ull * (k - 1);
     return p;
 }
 
@@ -2538,12 +2585,98 @@ static MemoRef dense_memo(DeviceCtx &c) {
     return m;
 }
 
+/* IDP1 ties: the first of `cands` in the iteration order of the CPU DP's
+ * memo -- a libstdc++ unordered_map keyed by the reference's relids
+ * (relation i is bit i+1) with the identity hash, filled with the base
+ * relations in index order and then with every set of 2..K relations in
+ * the order the DP creates them, `created` (cpu_sequential.cu:27-129).
+ * The same container, keys and insertion sequence give the same order. */
+static uint32_t memo_order_first(int n, const std::vector<uint32_t> &created,
+                                 const std::unordered_set<uint32_t> &cands) {
+    std::unordered_map<uint64_t, char> m;
+    for (int i = 0; i < n; i++) m.emplace(1ull << (i + 1), 0);
+    for (uint32_t S : created) m.emplace((uint64_t)S << 1, 0);
+    for (const auto &kv : m)
+        if (cands.count((uint32_t)(kv.first >> 1))) return (uint32_t)(kv.first >> 1);
+    return 0u;
+}
+
+/* The sets of 2..K relations in cpu_dpsub's / cpu_dpsub_bicc's creation
+ * order under IDP: level by level, connected sets in increasing integer
+ * order (gpuqo_cpu_dpsub.cuh:26-63). */
+static std::vector<uint32_t> dpsub_creation_order(const DPInput &in, int K) {
+    const int n = in.hdr.n_rels;
+    const uint32_t *adj = in.blob.data() + in.hdr.o_adj;
+    std::vector<uint32_t> out;
+    for (int k = 2; k <= K; k++) {
+        uint64_t S = (1ull << k) - 1ull;
+        const uint64_t last = S << (n - k);
+        while (true) {
+            const uint32_t s32 = (uint32_t)S;
+            uint32_t seen = s32 & (0u - s32), front = seen;
+            while (front) {
+                uint32_t nb = 0;
+                for (uint32_t f = front; f; f &= f - 1u) nb |= adj[__builtin_ctz(f)];
+                front = nb & s32 & ~seen;
+                seen |= front;
+            }
+            if (seen == s32) out.push_back(s32);
+            if (S == last) break;
+            const uint64_t c = S & (0ull - S), r = S + c;             /* next k-subset (Gosper) */
+            S = (((r ^ S) >> 2) / c) | r;
+        }
+    }
+    return out;
+}
+
+/* Largest IDP1 memo the host replays to order tied sets */
+#define IDP1_REPLAY_MAX (1ull << 22)
+
+/* IDP1: the plan root, the cheapest set of level K (k_best_of_level) with
+ * ties ordered as the CPU DP orders them.  `created` produces the creation
+ * order when there is a tie. */
+template <class Created>
+static int idp1_root(DeviceCtx &c, hipStream_t s, const MemoRef &mr, const uint32_t *list, const uint32_t *list_off,
+                     const uint32_t *count, int n, int K, Created created, uint32_t *root) {
+    uint32_t *dout = (uint32_t *)c.b.nodes;                  /* free until the plan walk */
+    hipLaunchKernelGGL(k_best_of_level, dim3(1), dim3(1024), 0, s, list, list_off, count, mr, dout);
+    HIPCHK(hipGetLastError());
+    uint32_t *r = c.h_res + 64;
+    HIPCHK(hipMemcpyAsync(r, dout, (2 + IDP1_TIES) * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (r[0] == ~0u || r[1] == 0) {
+        set_error("IDP1: no set of %d relations with a cost below +inf", K);
+        return GQ_ERR_NO_PLAN;
+    }
+    if (r[1] == 1) {
+        *root = r[2];
+        return GQ_OK;
+    }
+    uint64_t sets = 0;
+    for (int k = 1; k <= K; k++) sets += binom64(n, k);
+    if (r[1] > IDP1_TIES || sets > IDP1_REPLAY_MAX) {
+        set_error("IDP1: %u sets of %d relations tie for the cheapest plan; ordering them needs a memo replay "
+                  "beyond this build's limit", r[1], K);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    const std::unordered_set<uint32_t> cands(r + 2, r + 2 + r[1]);
+    std::vector<uint32_t> order;
+    int rc = created(order);
+    if (rc) return rc;
+    *root = memo_order_first(n, order, cands);
+    if (!*root) {
+        set_error("IDP1: tied sets missing from the replayed memo");
+        return GQ_ERR_DEVICE;
+    }
+    return GQ_OK;
+}
+
 /* Tree graphs with the open-addressing memo: the level lists are generated
  * by expansion (k_expand) instead of filtering all 2^n integers, and the
  * memo holds only the connected sets (capacity: the next power of two of
  * twice their number, so probes stay short). */
-static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profile, std::vector<DPNode> &tree,
-                       DPStats &st) {
+static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool profile,
+                       std::vector<DPNode> &tree, DPStats &st) {
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
     int rc;
@@ -2622,7 +2755,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
                        0, s, c.hkeys, cap, 0u);
     hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, c.sjob, 1);
-    for (int k = 2; k <= n; k++) {
+    for (int k = 2; k <= K; k++) {
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
                            s, c.sjob, 1, k, d_prefix + 2 * k);
@@ -2631,34 +2764,40 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
                          c.hslot + off_h[k], mr, c.b.counters);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
-    const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, mr, full, c.b.nodes, 2 * n - 1, c.b.counters + 48);
+    uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    if (K < n) {                                   /* IDP1: the cheapest set of K relations */
+        rc = idp1_root(c, s, mr, c.b.lvl_sets + off_h[K], nullptr, c.b.counters + K, n, K,
+                       [&](std::vector<uint32_t> &o) -> int { o = dpsub_creation_order(in, K); return GQ_OK; }, &root);
+        if (rc) return rc;
+    }
+    const int nodes_n = 2 * K - 1;
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, mr, root, c.b.nodes, nodes_n, c.b.counters + 48);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
-    tree.resize(2 * n - 1);
+    tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (cnt[49]) {
         set_error("open-addressing memo error (flags %u)", cnt[49]);
         return (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
     }
-    for (int k = 2; k <= n; k++)
+    for (int k = 2; k <= K; k++)
         if (cnt[k] != in.tree_counts[k]) {
             set_error("level %d holds %u sets, expected %llu", k, cnt[k], (unsigned long long)in.tree_counts[k]);
             return GQ_ERR_DEVICE;
         }
-    if (cnt[48] != (uint32_t)(2 * n - 1)) {
+    if (cnt[48] != (uint32_t)nodes_n) {
         set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
         return GQ_ERR_DEVICE;
     }
-    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
+    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     st.gpu_ms = ms;
     if (profile) {
-        for (int k = 2; k <= n; k++) {
+        for (int k = 2; k <= K; k++) {
             float a = 0.0f, b = 0.0f;
             HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
             HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
@@ -2668,9 +2807,11 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, bool profil
             st.eval_launches++;
         }
     }
-    st.candidates = list;            /* sets generated by expansion (no 2^n scan) */
-    st.sets = list;
-    st.pairs = tree_pairs(in.tree_counts);
+    uint64_t made = 0;
+    for (int k = 2; k <= K; k++) made += in.tree_counts[k];
+    st.candidates = made;            /* sets generated by expansion (no 2^n scan) */
+    st.sets = made;
+    st.pairs = tree_pairs(in.tree_counts, (size_t)K);
     st.memo_bytes = cap * (sizeof(float4) + 4 + 4) + list * 8;
     st.memo_kind = GQ_MEMO_HASH;
     return GQ_OK;
@@ -2697,11 +2838,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         set_error("GPU DP supports 1..32 relations, got %d", n);
         return GQ_ERR_UNSUPPORTED;
     }
-    if (n_iters != n) {
-        set_error("IDP1 level cap not available on the GPU path");
-        return GQ_ERR_UNSUPPORTED;
-    }
-    if (in.memo == GQ_MEMO_HASH && algo != DP_DPSIZE) return run_dp_hash(c, in, algo, profile, tree, st);
+    /* n_iters < n: IDP1's level cap; the result is the cheapest set of
+     * n_iters relations */
+    const int K = (n_iters >= 2 && n_iters < n) ? n_iters : n;
+    if (in.memo == GQ_MEMO_HASH && algo != DP_DPSIZE) return run_dp_hash(c, in, algo, K, profile, tree, st);
     const size_t blob_bytes = (size_t)h.words * 4;
     const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
     const uint32_t conn_words_all = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
@@ -2799,7 +2939,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                                c.b.left, c.b.counters);
         }
         uint64_t cand_ub = 0;
-        for (int k = 2; k <= n; k++) {
+        for (int k = 2; k <= K; k++) {
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
             hipLaunchKernelGGL(k_ds_prep, dim3(1), dim3(64), 0, s, k, n, d);
             uint32_t rblocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK) / BLOCK, 4096);
@@ -2826,18 +2966,34 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
         }
         (void)cand_ub;
-        uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), full, c.b.nodes, 2 * n - 1,
+        uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+        if (K < n) {                               /* IDP1: the cheapest set of K relations */
+            /* creation order: cpu_dpsize's level lists (gpuqo_cpu_dpsize.cu:55-92), which d.lset reproduces */
+            auto created = [&](std::vector<uint32_t> &o) -> int {
+                std::vector<uint32_t> sz(256);
+                HIPCHK(hipMemcpy(sz.data(), d.sz, 256 * 4, hipMemcpyDeviceToHost));
+                const uint64_t total = (uint64_t)sz[40 + K] + sz[K];
+                std::vector<uint32_t> all(total);
+                if (total) HIPCHK(hipMemcpy(all.data(), d.lset, total * 4, hipMemcpyDeviceToHost));
+                o.clear();
+                for (int k = 2; k <= K; k++) o.insert(o.end(), all.begin() + sz[40 + k], all.begin() + sz[40 + k] + sz[k]);
+                return (int)GQ_OK;
+            };
+            rc = idp1_root(c, s, dense_memo(c), d.lset, d.sz + 40 + K, d.sz + K, n, K, created, &root);
+            if (rc) return rc;
+        }
+        const int nodes_n = 2 * K - 1;
+        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n,
                            c.b.counters + 48);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c.ev1, s));
-        tree.resize(2 * n - 1);
+        tree.resize(nodes_n);
         const uint32_t *cnt = c.h_res;
-        HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+        HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
                               hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
-        if (cnt[48] != (uint32_t)(2 * n - 1)) {
+        memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
+        if (cnt[48] != (uint32_t)nodes_n) {
             set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
             return GQ_ERR_DEVICE;
         }
@@ -2845,7 +3001,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
         st.gpu_ms = ms;
         if (profile) {
-            for (int k = 2; k <= n; k++) {
+            for (int k = 2; k <= K; k++) {
                 float a = 0.0f, b = 0.0f;
                 HIPCHK(hipEventElapsedTime(&a, c.prof_ev[3 * k], c.prof_ev[3 * k + 1]));
                 HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
@@ -2857,7 +3013,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
         st.candidates = (uint64_t)cnt[44] | ((uint64_t)cnt[45] << 32);
         st.sets = 0;
-        for (int k = 2; k <= n; k++) st.sets += cnt[k];
+        for (int k = 2; k <= K; k++) st.sets += cnt[k];
         st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
         st.memo_bytes = nsets_all * (sizeof(float4) + 4 + 16 + 8 + 12);
         st.memo_kind = 1;
@@ -2897,7 +3053,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                            c.b.lvl_best, c.b.lvl_first, c.b.conn, conn_words, c.b.counters);
     }
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[1], s));
-    for (int k = 2; k <= n; k++) {
+    for (int k = 2; k <= K; k++) {
         uint64_t Ck = binom64(n, k);
         uint32_t *sets_k = c.b.lvl_sets + off_h[k];
         unsigned long long *best_k = c.b.lvl_best + off_h[k];
@@ -2910,7 +3066,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
             launch_eval_tree(h, lvl, lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
                              dense_memo(c), c.b.counters,
-                             tree_exp && k < n ? c.b.lvl_sets + off_h[k + 1] : nullptr);
+                             tree_exp && k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k)
@@ -2955,22 +3111,27 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
-    uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), full, c.b.nodes, 2 * n - 1,
-                       c.b.counters + 48);
+    uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    if (K < n) {                                   /* IDP1: the cheapest set of K relations */
+        rc = idp1_root(c, s, dense_memo(c), c.b.lvl_sets + off_h[K], nullptr, c.b.counters + K, n, K,
+                       [&](std::vector<uint32_t> &o) -> int { o = dpsub_creation_order(in, K); return GQ_OK; }, &root);
+        if (rc) return rc;
+    }
+    const int nodes_n = 2 * K - 1;
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n, c.b.counters + 48);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
-    tree.resize(2 * n - 1);
+    tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (2 * n - 1) * (sizeof(DPNode) / 4)) * 4,
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
                           hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * (2 * n - 1));
+    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
     if (cnt[49] != 0) {
         set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");
         return GQ_ERR_UNSUPPORTED;
     }
-    if (cnt[48] != (uint32_t)(2 * n - 1)) {
+    if (cnt[48] != (uint32_t)nodes_n) {
         set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
         return GQ_ERR_DEVICE;
     }
@@ -2982,7 +3143,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipEventElapsedTime(&a, c.prof_ev[0], c.prof_ev[1]));
         st.filter_ms = a;
         st.filter_launches = 1;
-        for (int k = 2; k <= n; k++) {
+        for (int k = 2; k <= K; k++) {
             float b = 0.0f;
             HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
             st.eval_ms += b;
@@ -2991,9 +3152,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     }
     st.candidates = cand_total;
     st.sets = 0;
-    for (int k = 2; k <= n; k++) st.sets += cnt[k];
+    for (int k = 2; k <= K; k++) st.sets += cnt[k];
     st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
-    if (h.is_tree && (int)in.tree_counts.size() == n + 1) st.pairs = tree_pairs(in.tree_counts);
+    if (h.is_tree && (int)in.tree_counts.size() == n + 1) st.pairs = tree_pairs(in.tree_counts, (size_t)K);
     st.memo_bytes = nsets_all * (sizeof(float4) + 4);
     st.memo_kind = 1;
     return GQ_OK;

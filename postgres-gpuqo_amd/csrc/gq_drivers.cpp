@@ -484,6 +484,47 @@ static int run_idp2(RunCtx &ctx, const HQuery<W> &h, QT<W> **out) {             
     return idp2_rec<W, W>(ctx, g, h, identity_list(h), ctx.idp_n_iters, out);
 }
 
+/* ------------------------------------------------------------ IDP1 */
+
+template <int W>
+static int run_idp1(RunCtx &ctx, HQuery<W> &info, QT<W> **out);
+
+/* gpuqo_run_idp1_next (gpuqo_idp.cu:15-36) */
+template <int WO, int WI>
+static int idp1_next(RunCtx &ctx, const HQuery<WO> &info, const std::vector<RemapEl<WO>> &lst, QT<WO> **out) {
+    Remapper<WO, WI> rm(lst);
+    HQuery<WI> ni;
+    rm.remap(info, ni);
+    QT<WI> *t = nullptr;
+    int rc = run_idp1<WI>(ctx, ni, &t);
+    if (rc) return rc;
+    *out = rm.back(ctx, t);
+    return GQ_OK;
+}
+
+/* IDP1 (gpuqo_idp.cu:38-80): a DP capped at n_iters levels returns the
+ * cheapest plan of n_iters relations (run_dp), which becomes relation 0 of
+ * the next query, the remaining relations following in index order, until
+ * one DP covers the whole query. */
+template <int W>
+static int run_idp1(RunCtx &ctx, HQuery<W> &info, QT<W> **out) {
+    info.n_iters = std::min(info.n, ctx.idp_n_iters);
+    QT<W> *qt = nullptr;
+    int rc = run_switch<W>(ctx, info, &qt);
+    if (rc) return rc;
+    if (info.n_iters == info.n) {
+        *out = qt;
+        return GQ_OK;
+    }
+    std::vector<RemapEl<W>> lst;
+    lst.push_back(RemapEl<W>{qt->id, 0, qt});
+    int j = 1;
+    for (int i = 0; i < info.n; i++)
+        if (!subset_of(Set<W>::nth(i), qt->id)) lst.push_back(RemapEl<W>{Set<W>::nth(i), j++, nullptr});
+    if (W == 1 || lst.size() < 64) return idp1_next<W, 1>(ctx, info, lst, out);
+    return idp1_next<W, 4>(ctx, info, lst, out);
+}
+
 /* ------------------------------------------------------------ UnionDP */
 
 struct GraphEdge {                   /* union.cu:111-123 */
@@ -643,8 +684,7 @@ int gpuqo_run_t(RunCtx &ctx, const HQuery<W> &h, QT<W> **out) {
     } else if (ctx.idp_type == GQ_UNION) {
         rc = run_union(ctx, ri, &t);
     } else {
-        set_error("IDP1 is not available on the GPU path");
-        rc = GQ_ERR_UNSUPPORTED;
+        rc = run_idp1(ctx, ri, &t);
     }
     if (rc) return rc;
     *out = rm.back(ctx, t);

@@ -184,3 +184,56 @@ def test_batch_with_big_blocks():
         want, ost = oracle_bits(g, c)
         assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"batch {g.shape}{g.n}")
         assert st.pairs == ost.pairs, f"{g.shape}{g.n}"
+
+
+def _idp1_cases():
+    return [("snow16-mpdp", qgen.snowflake(16, 0x16), "dpsub", 1, 5),
+            ("snow16-dpsub", qgen.snowflake(16, 0x16), "dpsub", 0, 4),
+            ("snow20-mpdp", qgen.config2(), "dpsub", 1, 8),
+            ("cycle12-mpdp", qgen.cycle(12, 0x12), "dpsub", 1, 4),
+            ("chain15-mpdp", qgen.chain(15, 0x15), "dpsub", 1, 6),
+            ("clique10-dpsub", qgen.clique(10, 0x10), "dpsub", 0, 4),
+            ("star12-dpsize", qgen.star(12, 0x12), "dpsize", 0, 5)]
+
+
+@pytest.mark.parametrize("name,g,algo,bicc,k", _idp1_cases(), ids=lambda x: x if isinstance(x, str) else "")
+def test_idp1(name, g, algo, bicc, k):
+    """IDP1: DPs capped at k levels, each returning its cheapest k-relation
+    plan, which becomes one relation of the next DP."""
+    c = cfg(algo, bicc, idp_type="idp1", idp_n_iters=k)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"idp1 {name} k={k}")
+    assert gst.pairs == ost.pairs
+
+
+def _uniform(n, edges, shape):
+    """Identical relations and selectivities: many sets of one size cost the
+    same, so IDP1 must order ties as the CPU DP's memo does."""
+    g = qgen.QueryGraph(n, shape)
+    for i in range(n):
+        g.rows[i] = qgen.rows_of(1e3)
+    for (u, v) in edges:
+        g.add_generic(u, v, 1e-3)
+    return g.build()
+
+
+@pytest.mark.parametrize("algo,bicc", [("dpsub", 1), ("dpsub", 0), ("dpsize", 0)])
+def test_idp1_ties(algo, bicc):
+    star = _uniform(9, [(0, i) for i in range(1, 9)], "uniform-star")     # C(8,2) = 28 tied 3-sets
+    chain = _uniform(10, [(i, i + 1) for i in range(9)], "uniform-chain")  # 7 tied 4-paths
+    for name, g, k in (("star9", star, 3), ("chain10", chain, 4)):
+        c = cfg(algo, bicc, idp_type="idp1", idp_n_iters=k)
+        want, _ = oracle_bits(g, c)
+        got, _ = gpu_bits(g, c)
+        assert_same_plan(got, want, f"idp1 ties {name} {algo} bicc={bicc}")
+
+
+def test_idp1_open_addressing_memo():
+    """IDP1 on a tree DP that takes the open-addressing memo."""
+    g = qgen.snowflake(22, 0x22)
+    c = cfg("dpsub", 1, idp_type="idp1", idp_n_iters=8, max_memo_size_mb=16)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, "idp1 snowflake-22 open addressing")
+    assert gst.pairs == ost.pairs
