@@ -252,6 +252,39 @@ gq_status gq_server_poll(gq_server *s, int32_t wait, int32_t *done,
                          gq_plan_out *out);
 gq_status gq_server_destroy(gq_server *s);
 
+/*
+ * Launch-shape knobs of the device DP, per process.  None of them changes a
+ * plan, a cost or a count.  They take the place of the reference's
+ * hand-set gpuqo_n_parallel / gpuqo_scratchpad_size_mb /
+ * gpuqo_dpsub_filter_keys_overprovisioning GUCs (guc.c:2150-2217), which
+ * size its own kernels and have no meaning here.
+ *   tree_two_lane_waves   tree levels of at most this many waves at two
+ *                         lanes per split take the two-lane form (1..2^24)
+ *   general_grid_max      workgroups of a general-graph batch launch (1..65536)
+ *   tree_task_iters       wave-iterations of sets per batch tree task (1..256)
+ *   tree_expand_max_rels  largest dense tree DP whose level lists are built
+ *                         by expansion instead of the 2^n filter (0..32)
+ */
+typedef struct gq_tuning {
+    int32_t tree_two_lane_waves;
+    int32_t general_grid_max;
+    int32_t tree_task_iters;
+    int32_t tree_expand_max_rels;
+} gq_tuning;
+gq_status gq_tuning_get(gq_tuning *t);
+gq_status gq_tuning_set(const gq_tuning *t);
+/*
+ * Autotuner (no reference counterpart): times the sample queries under
+ * candidate knob values, one knob at a time (tree_two_lane_waves and
+ * tree_expand_max_rels by planning the queries one by one with gq_plan,
+ * tree_task_iters and general_grid_max by planning them as one
+ * gq_plan_batch), keeps the fastest value of each, leaves it set and
+ * returns it in *out.  reps >= 1 timings per candidate (median).  Must not
+ * run concurrently with other planning calls of the process.
+ */
+gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
+                      gq_tuning *out);
+
 /* Request an interrupt of the running call (CHECK_FOR_INTERRUPTS,
  * gpuqo.cuh:21-28); the call returns GQ_ERR_INTERRUPTED and frees its
  * device memory. */

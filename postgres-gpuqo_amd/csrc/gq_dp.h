@@ -60,6 +60,17 @@ struct DPInput {
  * tree graphs only (other graphs keep the dense memo). */
 int resolve_memo(const DPInput &in, int requested, int max_memo_mb);
 
+/* Launch-shape knobs of the device DP (gq_tuning in the C ABI); none of
+ * them changes a plan, a cost or a count. */
+struct Tuning {
+    uint32_t lpe2_waves = 4096;    /* tree levels of at most this many two-lane waves take the two-lane form */
+    uint32_t gen_grid_max = 512;   /* workgroups of a general-job batch launch (they loop over its tasks) */
+    uint32_t tree_iters = 16;      /* wave-iterations of sets per task of the batch tree kernel */
+    uint32_t expand_max_n = 20;    /* largest dense tree DP whose level lists are built by expansion */
+};
+Tuning tuning_get();
+int tuning_set(const Tuning &t);   /* GQ_ERR_INVALID outside the accepted ranges */
+
 struct DPNode {                 /* plan node in DP numbering, pre-order */
     uint32_t set;
     float rows, startup, total;

@@ -1028,6 +1028,7 @@ struct BatchJob {
     uint32_t *keys;
     uint32_t hmask;
     uint32_t *lvl_slot;
+    uint32_t tree_iters;  /* wave-iterations of sets per tree task (Tuning::tree_iters when the job was made) */
 };
 
 __device__ __forceinline__ MemoRef job_memo(const BatchJob &J) {
@@ -1469,17 +1470,11 @@ __host__ __device__ constexpr size_t big_lds(int words) {
  * query's cost-model description in LDS and runs the same evaluators as the
  * single-query path. */
 
-#define TREE_ITERS 16
 
-/* Workgroups of a general-job launch (k_batch_eval): its 192 VGPRs allow
- * two workgroups per CU, 512 on the GPU; more only queue behind the tree
- * kernel running beside it, and the launch ends when the last has been
- * dispatched.  Workgroups loop over the launch's tasks. */
-#define GEN_GRID_MAX 512u
 
 /* sets per task of level k: trees, else MPDP (bicc) or DPsub sets */
-__host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bicc, int k) {
-    if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
+__host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bicc, int k, uint32_t tree_iters) {
+    if (is_tree) return (BLOCK / WAVE) * tree_spw(k) * tree_iters;
     uint64_t items = bicc ? 2ull * (k - 1) : ((1ull << k) - 2ull);
     /* MPDP: about one pass of the workgroup over the splits of its sets,
      * since each split is a long dependent chain; DPsub: three */
@@ -1488,7 +1483,7 @@ __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bic
     return (uint32_t)(B < 1 ? 1 : (B > bmax ? bmax : B));
 }
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
-    return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k);
+    return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k, J.tree_iters);
 }
 
 __global__ void k_batch_init(const BatchJob *jobs, int nj) {
@@ -2213,11 +2208,18 @@ static uint64_t binom64(int n, int k) {
 
 #define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
 
-/* Tasks of a tree job at level k (task_sets on the device). */
-static uint64_t tree_tasks(uint64_t sets, int k) {
-    const uint64_t spt = (uint64_t)(BLOCK / WAVE) * tree_spw(k) * TREE_ITERS;
+/* Tasks of a tree job at level k with `iters` wave-iterations per task
+ * (task_sets on the device). */
+static uint64_t tree_tasks(uint64_t sets, int k, uint32_t iters) {
+    const uint64_t spt = (uint64_t)(BLOCK / WAVE) * tree_spw(k) * iters;
     return (sets + spt - 1) / spt;
 }
+
+/* Launch-shape knobs.  The single-query and batch paths read g_tune under
+ * the device mutex (g_ctx.mu); the step engine takes a copy under
+ * g_tune_mu; tuning_set holds both. */
+static Tuning g_tune;
+static std::mutex g_tune_mu;
 
 /* Ordered csg-cmp pairs of a tree query: 2(k-1) per connected set of k. */
 static uint64_t tree_pairs(const std::vector<uint64_t> &counts, size_t kmax = 64) {
@@ -2355,6 +2357,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     trees.assign(nj, std::vector<DPNode>());
     stats.assign(nj, DPStats());
     if (nj == 0) return GQ_OK;
+    const Tuning tn = g_tune;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     std::vector<BatchJob> jobs(nj);
     std::vector<uint32_t> blobs, chunk_prefix(nj + 1, 0);
@@ -2431,6 +2434,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             if (k >= 2 && k <= h.n_rels) acc += binom64(h.n_rels, k);
         }
         J.bicc = algos[j] == DP_BICC ? 1 : 0;
+        J.tree_iters = tn.tree_iters;
     }
     hipStream_t s = c.stream;
     HIPCHK(hipMemcpyAsync(c.bblob, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice, s));
@@ -2460,11 +2464,11 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         for (int j = 0; j < nj; j++) {
             const BlobHdr &h = ins[j]->hdr;
             if (k > h.n_rels) continue;
-            if (fast_tree_job(h)) tasks_tree += tree_tasks(ins[j]->tree_counts[k], k);
+            if (fast_tree_job(h)) tasks_tree += tree_tasks(ins[j]->tree_counts[k], k, tn.tree_iters);
             else {                        /* tasks of a bound on the level size */
                 const uint64_t lvl = (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1)
                                          ? ins[j]->tree_counts[k] : binom64(h.n_rels, k);
-                const uint64_t spt = task_sets_of(h.is_tree != 0, algos[j] == DP_BICC, k);
+                const uint64_t spt = task_sets_of(h.is_tree != 0, algos[j] == DP_BICC, k, tn.tree_iters);
                 ub_gen += (lvl + spt - 1) / spt;
             }
         }
@@ -2477,7 +2481,10 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
                                nj, k, (const uint32_t *)(tp_tree + (size_t)k * (nj + 1)));
         }
         if (ub_gen) {
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), GEN_GRID_MAX);
+            /* its 192 VGPRs allow two workgroups per CU, 512 on the GPU; more
+             * only queue behind the tree kernel running beside it, and the
+             * launch ends when the last has been dispatched */
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), tn.gen_grid_max);
             hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, c.jobs, nj, k,
                                tp_gen + (size_t)k * (nj + 1), tpl_gen, big_cap ? c.big_ents : nullptr,
                                (uint32_t)big_cap, big_hdr);
@@ -2535,12 +2542,7 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
     return run_dp_batch_impl(ins, algos, trees, stats, gpu_ms);
 }
 
-/* Largest dense tree DP whose level lists are built by expansion */
-#define TREE_EXPAND_MAX_N 20
 
-/* Levels of at most this many waves at two lanes per edge leave the GPU
- * latency-bound (a few waves per SIMD): they take the two-lane form. */
-#define TREE_LPE2_WAVES 4096u
 
 /* One level of a tree query of `sets` sets (an upper bound): the
  * per-edge-table kernel when the blob has the table, else the class-mask
@@ -2550,7 +2552,9 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
                              int k, int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr,
                              uint32_t *counters, uint32_t *next = nullptr) {
     const uint64_t spw2 = tree_spw(k, 2);
-    const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= TREE_LPE2_WAVES ? 2 : 1;
+    /* levels of at most g_tune.lpe2_waves waves at two lanes per edge leave
+     * the GPU latency-bound (a few waves per SIMD): the two-lane form */
+    const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= g_tune.lpe2_waves ? 2 : 1;
     const uint64_t spw = tree_spw(k, lpe);
     const uint64_t waves = (sets_k + spw - 1) / spw;
     const uint32_t ev = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
@@ -2817,6 +2821,24 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     return GQ_OK;
 }
 
+Tuning tuning_get() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_tune;
+}
+
+int tuning_set(const Tuning &t) {
+    if (t.lpe2_waves < 1 || t.lpe2_waves > (1u << 24) || t.gen_grid_max < 1 || t.gen_grid_max > 65536 ||
+        t.tree_iters < 1 || t.tree_iters > 256 || t.expand_max_n > 32) {
+        set_error("tuning out of range (two-lane waves 1..2^24, general grid 1..65536, tree task iterations "
+                  "1..256, expansion up to 32 relations)");
+        return GQ_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    std::lock_guard<std::mutex> g2(g_tune_mu);
+    g_tune = t;
+    return GQ_OK;
+}
+
 int resolve_memo(const DPInput &in, int requested, int max_memo_mb) {
     const int n = in.hdr.n_rels;
     const bool tree = in.hdr.is_tree && (int)in.tree_counts.size() == n + 1;
@@ -2901,7 +2923,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * larger memos the filter's increasing order keeps a wave's memo
      * gathers on nearby lines, which outweighs the filter. */
     const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2 &&
-                          n <= TREE_EXPAND_MAX_N;
+                          n <= (int)g_tune.expand_max_n;
     if (tree_exp) {
         hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
                            c.b.lvl_sets + off_h[2], c.b.counters);
@@ -3069,7 +3091,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                              tree_exp && k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
-            uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k)
+            uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k, g_tune.tree_iters)
                                          : (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
             BigList bl;
@@ -3440,6 +3462,7 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
         if (k >= 2 && k <= n) acc += in.tree_counts[k];
     }
     J.bicc = algo == DP_BICC ? 1 : 0;
+    J.tree_iters = tuning_get().tree_iters;
     J.cur_k = 2;
     e->incoming.push_back(j);
     return true;
@@ -3488,6 +3511,7 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
         if (k >= 2 && k <= n) acc += binom64(n, k);
     }
     J.bicc = algo == DP_BICC ? 1 : 0;
+    J.tree_iters = tuning_get().tree_iters;
     J.cur_k = 2;
     j->hash = false;
     j->cap = 0;
@@ -3553,10 +3577,10 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         const uint64_t lvl = (int)j->counts.size() == j->n + 1 ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
         if (fast_tree_job(h) && (int)j->counts.size() == j->n + 1) {
             lds_tree = std::max(lds_tree, tree_lds(h.words, 1));
-            ub_tree += tree_tasks(lvl, j->cur_k);
+            ub_tree += tree_tasks(lvl, j->cur_k, j->J.tree_iters);
         } else {
             lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
-            const uint64_t spt = task_sets_of(h.is_tree != 0, j->J.bicc != 0, j->cur_k);
+            const uint64_t spt = task_sets_of(h.is_tree != 0, j->J.bicc != 0, j->cur_k, j->J.tree_iters);
             ub += (lvl + spt - 1) / spt;          /* tasks of a bound on the level size */
         }
         if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
@@ -3600,7 +3624,7 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         ECHK(hipStreamWaitEvent(e->s2, e->fork, 0));
         const uint32_t tpl = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
         const size_t lt = tpl ? lds + 4 * (size_t)(na + 1) : lds;
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), GEN_GRID_MAX);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), tuning_get().gen_grid_max);
         hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, e->d_jobs, na, 0, tp_gen, tpl,
                            big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
         if (big_cap) {

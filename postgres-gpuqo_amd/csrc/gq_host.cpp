@@ -717,6 +717,101 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
 
 void gq_interrupt(int32_t on) { gq::g_interrupt.store(on ? 1 : 0); }
 
+gq_status gq_tuning_get(gq_tuning *t) {
+    if (!t) {
+        gq::set_error("null tuning");
+        return GQ_ERR_INVALID;
+    }
+    const gq::Tuning v = gq::tuning_get();
+    t->tree_two_lane_waves = (int32_t)v.lpe2_waves;
+    t->general_grid_max = (int32_t)v.gen_grid_max;
+    t->tree_task_iters = (int32_t)v.tree_iters;
+    t->tree_expand_max_rels = (int32_t)v.expand_max_n;
+    return GQ_OK;
+}
+
+gq_status gq_tuning_set(const gq_tuning *t) {
+    if (!t || t->tree_two_lane_waves < 1 || t->general_grid_max < 1 || t->tree_task_iters < 1 ||
+        t->tree_expand_max_rels < 0) {
+        gq::set_error("null or non-positive tuning");
+        return GQ_ERR_INVALID;
+    }
+    gq::Tuning v;
+    v.lpe2_waves = (uint32_t)t->tree_two_lane_waves;
+    v.gen_grid_max = (uint32_t)t->general_grid_max;
+    v.tree_iters = (uint32_t)t->tree_task_iters;
+    v.expand_max_n = (uint32_t)t->tree_expand_max_rels;
+    return (gq_status)gq::tuning_set(v);
+}
+
+gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
+                      gq_tuning *out) {
+    if (n_queries < 1 || !qs || !cfg || reps < 1 || !out) {
+        gq::set_error("bad autotune arguments");
+        return GQ_ERR_INVALID;
+    }
+    int rc = gq::device_init(cfg->device);
+    if (rc) return (gq_status)rc;
+    std::vector<std::vector<gq_plan_node>> bufs(n_queries);
+    std::vector<gq_plan_out> outs(n_queries);
+    for (int i = 0; i < n_queries; i++) {
+        if ((rc = gq::validate(qs[i]))) return (gq_status)rc;
+        bufs[i].resize(2 * (size_t)qs[i]->n_rels);
+        outs[i].nodes = bufs[i].data();
+        outs[i].capacity = (int32_t)bufs[i].size();
+    }
+    auto median_ms = [&](bool batch, double *ms) -> int {
+        std::vector<double> t;
+        for (int r = 0; r <= reps; r++) {                    /* the first run warms up */
+            const auto t0 = std::chrono::steady_clock::now();
+            if (batch) {
+                int e = gq_plan_batch(n_queries, qs, cfg, outs.data());
+                if (e) return e;
+            } else {
+                for (int i = 0; i < n_queries; i++) {
+                    int e = gq_plan(qs[i], cfg, &outs[i]);
+                    if (e) return e;
+                }
+            }
+            if (r) t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        std::sort(t.begin(), t.end());
+        *ms = t[t.size() / 2];
+        return GQ_OK;
+    };
+    gq::Tuning best = gq::tuning_get();
+    struct Knob { uint32_t gq::Tuning::*field; bool batch; std::vector<uint32_t> cands; };
+    const Knob knobs[] = {
+        {&gq::Tuning::lpe2_waves, false, {1024, 2048, 4096, 8192, 16384}},
+        {&gq::Tuning::expand_max_n, false, {0, 16, 18, 20, 22}},
+        {&gq::Tuning::tree_iters, true, {4, 8, 16, 32}},
+        {&gq::Tuning::gen_grid_max, true, {256, 512, 1024, 2048}},
+    };
+    for (const Knob &kb : knobs) {
+        uint32_t pick = best.*kb.field;
+        double pick_ms = 0.0;
+        bool first = true;
+        for (uint32_t v : kb.cands) {
+            gq::Tuning t = best;
+            t.*kb.field = v;
+            if ((rc = gq::tuning_set(t))) return (gq_status)rc;
+            double ms = 0.0;
+            if ((rc = median_ms(kb.batch, &ms))) {
+                gq::tuning_set(best);
+                return (gq_status)rc;
+            }
+            if (first || ms < pick_ms) {
+                pick = v;
+                pick_ms = ms;
+                first = false;
+            }
+        }
+        best.*kb.field = pick;
+        if ((rc = gq::tuning_set(best))) return (gq_status)rc;
+    }
+    return gq_tuning_get(out);
+}
+
 const char *gq_last_error(void) { return gq::g_err; }
 
 const char *gq_version(void) { return "gpuqo-mi355x 0.1 (gfx950)"; }

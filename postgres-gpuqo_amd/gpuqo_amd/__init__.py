@@ -43,6 +43,10 @@ def lib():
         L.gq_server_poll.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int32), C.POINTER(abi.PlanOut)]
         L.gq_server_destroy.argtypes = [C.c_void_p]
+        L.gq_tuning_get.argtypes = [C.POINTER(abi.Tuning)]
+        L.gq_tuning_set.argtypes = [C.POINTER(abi.Tuning)]
+        L.gq_autotune.argtypes = [C.c_int32, C.POINTER(C.POINTER(abi.Query)), C.POINTER(abi.Config), C.c_int32,
+                                  C.POINTER(abi.Tuning)]
         _lib = L
     return _lib
 
@@ -166,3 +170,33 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def _tuning_dict(t):
+    return {name: getattr(t, name) for name, _ in abi.Tuning._fields_}
+
+
+def tuning_get():
+    """The launch-shape knobs in force (gq_tuning) as a dict."""
+    t = abi.Tuning()
+    _check(lib().gq_tuning_get(C.byref(t)))
+    return _tuning_dict(t)
+
+
+def tuning_set(**kw):
+    """Set some launch-shape knobs (the others keep their values)."""
+    t = abi.Tuning()
+    _check(lib().gq_tuning_get(C.byref(t)))
+    for k, v in kw.items():
+        setattr(t, k, v)
+    _check(lib().gq_tuning_set(C.byref(t)))
+
+
+def autotune(queries, cfg, reps=3):
+    """Time the sample queries under candidate knob values (gq_autotune) and
+    keep the fastest; returns the chosen knobs."""
+    cq = [q.to_c() for q in queries]
+    arr = (C.POINTER(abi.Query) * len(cq))(*[C.pointer(q) for q in cq])
+    t = abi.Tuning()
+    _check(lib().gq_autotune(len(cq), arr, C.byref(cfg), reps, C.byref(t)))
+    return _tuning_dict(t)

@@ -109,6 +109,11 @@ class PlanStats(C.Structure):
                 ("filter_launches", C.c_int32), ("eval_launches", C.c_int32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tree_two_lane_waves", C.c_int32), ("general_grid_max", C.c_int32),
+                ("tree_task_iters", C.c_int32), ("tree_expand_max_rels", C.c_int32)]
+
+
 class PlanOut(C.Structure):
     _fields_ = [("nodes", C.POINTER(PlanNode)), ("capacity", C.c_int32),
                 ("n_nodes", C.c_int32), ("stats", PlanStats)]

@@ -73,3 +73,21 @@ def test_widths_are_validated_before_any_device_call():
     with pytest.raises(gpuqo_amd.GpuqoError) as e:
         gpuqo_amd.plan(g, cfg)
     assert e.value.status == abi.GQ_ERR_INVALID and "widths sum" in str(e.value)
+
+
+def test_tuning_defaults_and_validation():
+    """The launch-shape knobs start at their defaults; values outside the
+    accepted ranges are refused and leave the knobs unchanged."""
+    d = gpuqo_amd.tuning_get()
+    assert d == {"tree_two_lane_waves": 4096, "general_grid_max": 512, "tree_task_iters": 16,
+                 "tree_expand_max_rels": 20}
+    for bad in ({"tree_task_iters": 0}, {"tree_task_iters": 257}, {"general_grid_max": 0},
+                {"tree_two_lane_waves": -1}, {"tree_expand_max_rels": 33}):
+        with pytest.raises(gpuqo_amd.GpuqoError) as e:
+            gpuqo_amd.tuning_set(**bad)
+        assert e.value.status == abi.GQ_ERR_INVALID
+        assert gpuqo_amd.tuning_get() == d
+    gpuqo_amd.tuning_set(tree_task_iters=8)
+    assert gpuqo_amd.tuning_get()["tree_task_iters"] == 8
+    gpuqo_amd.tuning_set(**d)
+    assert gpuqo_amd.tuning_get() == d
