@@ -263,6 +263,8 @@ def main():
                     help="only run a plain device copy of MIB MiB (calibrates byte counters) and exit")
     ap.add_argument("--batch-only", type=int, default=0, metavar="REPS",
                     help="only plan the 256-query batch (one warm-up, then REPS batches) and print its times")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch-shape knobs instead of tuning them on the workload first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
@@ -300,6 +302,20 @@ def main():
     cfg_prof = abi.default_config(algorithm=algo, dpsub_bicc=bicc, profile_kernels=1)
     print(f"# workload {args.workload}: {g.shape}-{g.n}, algorithm {algo} bicc={bicc}", file=sys.stderr, flush=True)
 
+    tuning = None
+    if not args.no_autotune:
+        # gq_autotune before anything is timed: the single-plan knobs on the
+        # workload itself, the batch knobs on the 256-query batch; the chosen
+        # knobs hold for every section below
+        t0 = time.perf_counter()
+        with_batch = not args.no_batch and args.workload == "mpdp-snowflake20"
+        knobs = gpuqo_amd.autotune([g], cfg, reps=5, knobs=gpuqo_amd.TUNE_SINGLE)
+        if with_batch:
+            knobs = gpuqo_amd.autotune(qgen.config4(), abi.default_config(algorithm="dpsub", dpsub_bicc=1),
+                                       reps=3, knobs=gpuqo_amd.TUNE_BATCH)
+        tuning = {"knobs": knobs, "tune_s": round(time.perf_counter() - t0, 2),
+                  "sample": {"single": f"{g.shape}-{g.n}", "batch": "batch256-mpdp" if with_batch else None}}
+        print(f"# autotune {knobs} in {tuning['tune_s']} s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         gpuqo_amd.plan(g, cfg)
     # timed region: plain calls (no per-level events); device time is the
@@ -380,6 +396,7 @@ def main():
         "plan_ms_wall_median": statistics.median(wall_ms),
         "plan_ms_wall_p99": sorted(wall_ms)[min(len(wall_ms) - 1, int(0.99 * len(wall_ms)))],
         "roofline": roof,
+        "tuning": tuning,
     }
     if not args.no_batch and args.workload == "mpdp-snowflake20":
         out["batched"] = batched_section(gpuqo_amd, abi, qgen)

@@ -275,15 +275,17 @@ gq_status gq_tuning_get(gq_tuning *t);
 gq_status gq_tuning_set(const gq_tuning *t);
 /*
  * Autotuner (no reference counterpart): times the sample queries under
- * candidate knob values, one knob at a time (tree_two_lane_waves and
- * tree_expand_max_rels by planning the queries one by one with gq_plan,
- * tree_task_iters and general_grid_max by planning them as one
- * gq_plan_batch), keeps the fastest value of each, leaves it set and
- * returns it in *out.  reps >= 1 timings per candidate (median).  Must not
- * run concurrently with other planning calls of the process.
+ * candidate knob values, one knob at a time, keeps the fastest value of
+ * each, leaves it set and returns all knobs in *out.  `knobs` selects the
+ * groups: GQ_TUNE_SINGLE (tree_two_lane_waves, tree_expand_max_rels; timed
+ * by planning the queries one by one with gq_plan) and GQ_TUNE_BATCH
+ * (tree_task_iters, general_grid_max; timed by planning them as one
+ * gq_plan_batch).  reps >= 1 timings per candidate (median).  Must not run
+ * concurrently with other planning calls of the process.
  */
+enum { GQ_TUNE_SINGLE = 1, GQ_TUNE_BATCH = 2 };
 gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
-                      gq_tuning *out);
+                      int32_t knobs, gq_tuning *out);
 
 /* Request an interrupt of the running call (CHECK_FOR_INTERRUPTS,
  * gpuqo.cuh:21-28); the call returns GQ_ERR_INTERRUPTED and frees its

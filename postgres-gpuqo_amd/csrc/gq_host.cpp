@@ -745,8 +745,8 @@ gq_status gq_tuning_set(const gq_tuning *t) {
 }
 
 gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
-                      gq_tuning *out) {
-    if (n_queries < 1 || !qs || !cfg || reps < 1 || !out) {
+                      int32_t knobs, gq_tuning *out) {
+    if (n_queries < 1 || !qs || !cfg || reps < 1 || !out || (knobs & ~(GQ_TUNE_SINGLE | GQ_TUNE_BATCH))) {
         gq::set_error("bad autotune arguments");
         return GQ_ERR_INVALID;
     }
@@ -781,13 +781,14 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
     };
     gq::Tuning best = gq::tuning_get();
     struct Knob { uint32_t gq::Tuning::*field; bool batch; std::vector<uint32_t> cands; };
-    const Knob knobs[] = {
+    const Knob knob_list[] = {
         {&gq::Tuning::lpe2_waves, false, {1024, 2048, 4096, 8192, 16384}},
         {&gq::Tuning::expand_max_n, false, {0, 16, 18, 20, 22}},
         {&gq::Tuning::tree_iters, true, {4, 8, 16, 32}},
         {&gq::Tuning::gen_grid_max, true, {256, 512, 1024, 2048}},
     };
-    for (const Knob &kb : knobs) {
+    for (const Knob &kb : knob_list) {
+        if (!(knobs & (kb.batch ? GQ_TUNE_BATCH : GQ_TUNE_SINGLE))) continue;
         uint32_t pick = best.*kb.field;
         double pick_ms = 0.0;
         bool first = true;

@@ -46,7 +46,7 @@ def lib():
         L.gq_tuning_get.argtypes = [C.POINTER(abi.Tuning)]
         L.gq_tuning_set.argtypes = [C.POINTER(abi.Tuning)]
         L.gq_autotune.argtypes = [C.c_int32, C.POINTER(C.POINTER(abi.Query)), C.POINTER(abi.Config), C.c_int32,
-                                  C.POINTER(abi.Tuning)]
+                                  C.c_int32, C.POINTER(abi.Tuning)]
         _lib = L
     return _lib
 
@@ -192,11 +192,14 @@ def tuning_set(**kw):
     _check(lib().gq_tuning_set(C.byref(t)))
 
 
-def autotune(queries, cfg, reps=3):
+TUNE_SINGLE, TUNE_BATCH = 1, 2
+
+
+def autotune(queries, cfg, reps=3, knobs=TUNE_SINGLE | TUNE_BATCH):
     """Time the sample queries under candidate knob values (gq_autotune) and
-    keep the fastest; returns the chosen knobs."""
+    keep the fastest; returns all knobs after tuning."""
     cq = [q.to_c() for q in queries]
     arr = (C.POINTER(abi.Query) * len(cq))(*[C.pointer(q) for q in cq])
     t = abi.Tuning()
-    _check(lib().gq_autotune(len(cq), arr, C.byref(cfg), reps, C.byref(t)))
+    _check(lib().gq_autotune(len(cq), arr, C.byref(cfg), reps, knobs, C.byref(t)))
     return _tuning_dict(t)

@@ -63,3 +63,19 @@ def test_autotune_keeps_plans():
         assert_same_plan(a, b, f"{g.shape}{g.n} single after autotune")
     for g, a, b in zip(qs, got_b, want_b):
         assert_same_plan(a, b, f"{g.shape}{g.n} batch after autotune")
+
+
+def test_autotune_groups_leave_other_knobs():
+    qs = _sample()[:3]
+    c = cfg("dpsub", 1)
+    try:
+        gpuqo_amd.tuning_set(**DEFAULTS)
+        t = gpuqo_amd.autotune(qs, c, reps=1, knobs=gpuqo_amd.TUNE_BATCH)
+        assert t["tree_two_lane_waves"] == DEFAULTS["tree_two_lane_waves"]
+        assert t["tree_expand_max_rels"] == DEFAULTS["tree_expand_max_rels"]
+        gpuqo_amd.tuning_set(**DEFAULTS)
+        t = gpuqo_amd.autotune(qs, c, reps=1, knobs=gpuqo_amd.TUNE_SINGLE)
+        assert t["tree_task_iters"] == DEFAULTS["tree_task_iters"]
+        assert t["general_grid_max"] == DEFAULTS["general_grid_max"]
+    finally:
+        gpuqo_amd.tuning_set(**DEFAULTS)
