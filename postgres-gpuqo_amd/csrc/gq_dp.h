@@ -97,11 +97,13 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
 bool fused_eligible(const DPInput &in, DPAlgo algo);
 
 /* Step engine (continuous batching).  Admitted DPs live in slices of one
- * device arena; every engine_step advances each active DP by one level, all
- * of them in one launch, and returns the DPs that finished.  DPs join and
- * leave between steps.  Only DPs that satisfy fused_eligible() and run to
- * their full size are admitted.  Not thread-safe: one engine thread drives
- * it. */
+ * device arena; every step advances each active DP by one level, all of
+ * them in one launch.  engine_step enqueues the next step and returns the
+ * DPs that finished in the step before it (so the host prepares one step
+ * while the device runs the previous one); with no DP active it drains the
+ * last step.  DPs join and leave between steps.  Only DPs that satisfy
+ * fused_eligible() and run to their full size are admitted.  Not
+ * thread-safe: one engine thread drives it. */
 struct StepEngine;
 struct EngineDone {
     uint64_t key;
@@ -115,10 +117,13 @@ void engine_destroy(StepEngine *e);
  * memo, or open-addressing memo over `total_sets` connected sets */
 uint64_t engine_slice_bytes(int n_rels);
 uint64_t engine_hash_slice_bytes(uint64_t total_sets);
-/* admits the DP if its slice fits now (the blob is copied at admission;
- * `in` must stay alive until the DP finishes); false otherwise */
+/* admits the DP if its slice fits now (no device call: the blob is copied
+ * to the host side of the engine and reaches the device with the next
+ * step); false otherwise */
 bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key);
 int engine_step(StepEngine *e, std::vector<EngineDone> &done);
+/* DPs admitted or on the device, plus steps not yet collected: engine_step
+ * has work while this is nonzero */
 int engine_active(const StepEngine *e);
 
 /* Where run_many sends its GPU DPs when set on the calling thread (the

@@ -1482,14 +1482,25 @@ __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bic
     const uint64_t bmax = bicc ? BMAX_B : BMAX;
     return (uint32_t)(B < 1 ? 1 : (B > bmax ? bmax : B));
 }
+#define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
+
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
     return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k, J.tree_iters);
 }
 
-__global__ void k_batch_init(const BatchJob *jobs, int nj) {
+/* Zeroes each job's 64 counters and stores its base relations (inserting
+ * their keys first in an open-addressing memo, whose keys are cleared
+ * beforehand).  With `blob_src`, job j's query blob is first copied from
+ * blob_src + blob_at[j] (a staging area of this launch) into J.blob. */
+__global__ void k_batch_init(const BatchJob *jobs, int nj, const uint32_t *blob_src, const uint32_t *blob_at) {
     for (int j = blockIdx.x; j < nj; j += gridDim.x) {
         const BatchJob &J = jobs[j];
         const int tid = threadIdx.x;
+        const uint32_t *blob = J.blob;
+        if (blob_src) {
+            blob = blob_src + blob_at[j];
+            for (uint32_t i = tid; i < (uint32_t)J.h.words; i += blockDim.x) const_cast<uint32_t *>(J.blob)[i] = blob[i];
+        }
         if (tid < 64) J.counters[tid] = 0;
         __syncthreads();
         if (tid < J.h.n_rels) {
@@ -1503,9 +1514,20 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj) {
                     continue;
                 }
             }
-            store_base(J.vals, slot, J.h, J.blob, tid);
+            store_base(J.vals, slot, J.h, blob, tid);
             J.left[slot] = 0;
         }
+    }
+}
+
+/* Clears the open-addressing keys of every job that has them. */
+__global__ void __launch_bounds__(BLOCK) k_clear_keys(const BatchJob *jobs, int nj) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (int j = 0; j < nj; j++) {
+        uint32_t *keys = jobs[j].keys;
+        if (!keys) continue;
+        const uint64_t n = (uint64_t)jobs[j].hmask + 1u;
+        for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) keys[i] = 0u;
     }
 }
 
@@ -1699,6 +1721,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs,
     View<uint32_t> q = make_view(h, smem);
     const uint32_t spt = task_sets(J, k);
     const uint32_t first = (t - tpg[lo]) * spt, cnt = J.counters[k];
+    /* the task prefix may come from the host's level sizes: a level that
+     * came out shorter (an expansion error, flagged in counters[49]) leaves
+     * tasks past its end */
+    if (cnt <= first) return;
     const uint32_t m = cnt - first < spt ? cnt - first : spt;
     const MemoRef mr = job_memo(J);
     eval_level_tree<1, true, 1>(h, smem, q, k, J.bicc != 0, J.lvl_sets + J.lvl_off[k] + first,
@@ -1783,14 +1809,25 @@ __device__ uint32_t walk_tree_wave(const MemoRef &mr, uint32_t root_set, DPNode 
     return (uint32_t)total;
 }
 
-__global__ void __launch_bounds__(WAVE) k_batch_tree(const BatchJob *jobs, int nj) {
-    const int j = blockIdx.x;
-    if (j >= nj) return;
-    const BatchJob &J = jobs[j];
+/* Plan walk of finished jobs, one wave each: job jobs[idx ? idx[b] : b] for
+ * workgroup b < nj.  res == nullptr: the nodes go to J.nodes and the status
+ * to J.counters[48].  Otherwise workgroup b writes RES_WORDS words at
+ * res + b * RES_WORDS: the job's 64 counters (status in word 48), then the
+ * nodes, so one copy returns every result of the launch. */
+__global__ void __launch_bounds__(WAVE) k_batch_tree(const BatchJob *jobs, int nj, const uint32_t *idx,
+                                                      uint32_t *res) {
+    const int b = blockIdx.x;
+    if (b >= nj) return;
+    const BatchJob &J = jobs[idx ? idx[b] : (uint32_t)b];
     const int n = J.h.n_rels;
     const uint32_t full = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
-    const uint32_t r = walk_tree_wave(job_memo(J), full, J.nodes, 2 * n - 1);
-    if (threadIdx.x == 0) J.counters[48] = r;
+    uint32_t *out = res ? res + (size_t)b * RES_WORDS : nullptr;
+    const uint32_t r = walk_tree_wave(job_memo(J), full, out ? (DPNode *)(out + 64) : J.nodes, 2 * n - 1);
+    if (out) {
+        if (threadIdx.x < 64) out[threadIdx.x] = threadIdx.x == 48 ? r : J.counters[threadIdx.x];
+    } else if (threadIdx.x == 0) {
+        J.counters[48] = r;
+    }
 }
 
 /* Fills n words with v (clears an open-addressing key array). */
@@ -2206,8 +2243,6 @@ static uint64_t binom64(int n, int k) {
     return r;
 }
 
-#define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
-
 /* Tasks of a tree job at level k with `iters` wave-iterations per task
  * (task_sets on the device). */
 static uint64_t tree_tasks(uint64_t sets, int k, uint32_t iters) {
@@ -2441,7 +2476,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     HIPCHK(hipMemcpyAsync(c.jobs, jobs.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c.bprefix, chunk_prefix.data(), sizeof(uint32_t) * (nj + 1), hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(c.ev0, s));
-    hipLaunchKernelGGL(k_batch_init, dim3(std::min(nj, 1024)), dim3(64), 0, s, c.jobs, nj);
+    hipLaunchKernelGGL(k_batch_init, dim3(std::min(nj, 1024)), dim3(64), 0, s, c.jobs, nj, nullptr, nullptr);
     const uint32_t fgrid = std::min<uint32_t>(chunk_prefix[nj], 4096);
     hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, c.jobs, nj,
                        c.bprefix);
@@ -2500,7 +2535,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         HIPCHK(hipStreamWaitEvent(s, c.lvl_ev[1][k], 0));
         HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][k], 0));
     }
-    hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, c.jobs, nj);
+    hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, c.jobs, nj, nullptr, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
     std::vector<uint32_t> res((size_t)nj * RES_WORDS);
@@ -2758,7 +2793,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev0, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
                        0, s, c.hkeys, cap, 0u);
-    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, c.sjob, 1);
+    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, c.sjob, 1, nullptr, nullptr);
     for (int k = 2; k <= K; k++) {
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
@@ -3237,6 +3272,7 @@ struct EJob {
     bool hash;
     uint64_t cap;                        /* open-addressing capacity */
     std::vector<uint64_t> counts;        /* hash: connected sets per size */
+    std::vector<uint32_t> blob;          /* query blob, copied to the device by the DP's first step */
 };
 
 namespace {
@@ -3269,6 +3305,20 @@ uint64_t engine_hash_slice_bytes(uint64_t total_sets) {
     return hash_slice_layout(total_sets, total_sets, (int)(ENGINE_BLOB_MAX / 4), &cap, &a, &b, &c, &d, &e, &f);
 }
 
+/* One step's host side.  Steps alternate between two slots: the engine
+ * enqueues step i (slot i % 2) before it waits for step i - 1, so the host
+ * prepares and launches one step while the device runs the previous one.
+ * A slot's pinned packet and result buffer are rewritten only after its
+ * previous step has completed. */
+struct StepSlot {
+    uint8_t *h_pkt = nullptr;                    /* pinned: the step's single host-to-device packet */
+    uint32_t *h_res = nullptr;                   /* pinned: RES_WORDS per finishing DP */
+    size_t cap_h_pkt = 0, cap_h_res = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<EJob *> fin;                     /* DPs walked in this step, read back at its completion */
+    bool busy = false;
+};
+
 struct StepEngine {
     int device = 0;
     hipStream_t s = nullptr, s2 = nullptr;       /* tree jobs / general jobs (default priority) */
@@ -3277,15 +3327,12 @@ struct StepEngine {
     size_t arena_bytes = 0;
     std::map<size_t, size_t> free_;              /* offset -> size, coalesced */
     std::vector<EJob *> active, incoming;
-    BatchJob *d_jobs = nullptr, *d_new = nullptr, *d_fin = nullptr;
-    uint32_t *d_tp = nullptr, *d_chunk = nullptr, *big_prefix = nullptr, *big_hdr = nullptr;
+    uint8_t *d_pkt = nullptr;                    /* device copy of the current step's packet */
+    uint32_t *d_res = nullptr, *d_tp = nullptr, *big_prefix = nullptr, *big_hdr = nullptr;
     BigEnt *big = nullptr;
-    size_t cap_jobs = 0, cap_new = 0, cap_fin = 0, cap_tp = 0, cap_chunk = 0, cap_big = 0, cap_bigp = 0,
-           cap_hdr = 0;
-    BatchJob *h_jobs = nullptr, *h_new = nullptr;    /* pinned staging */
-    uint32_t *h_res = nullptr, *h_chunk = nullptr, *h_exp = nullptr, *d_exp = nullptr;
-    size_t cap_h_jobs = 0, cap_h_new = 0, cap_h_res = 0, cap_h_chunk = 0, cap_h_exp = 0, cap_exp = 0;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    size_t cap_pkt = 0, cap_res = 0, cap_tp = 0, cap_big = 0, cap_bigp = 0, cap_hdr = 0;
+    StepSlot slot[2];
+    int cur = 0;                                 /* slot of the next step */
 
     bool alloc(size_t bytes, size_t *off) {
         for (auto it = free_.begin(); it != free_.end(); ++it) {
@@ -3355,11 +3402,13 @@ int engine_create(int device, uint64_t arena_bytes, StepEngine **out) {
         int rc = device_init(device);
         if (rc) return fail(rc);
     }
-    if (hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&e->join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&e->e0) != hipSuccess || hipEventCreate(&e->e1) != hipSuccess) {
+    bool ok = hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&e->s2, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&e->join, hipEventDisableTiming) == hipSuccess;
+    for (StepSlot &sl : e->slot)
+        ok = ok && hipEventCreate(&sl.e0) == hipSuccess && hipEventCreate(&sl.e1) == hipSuccess;
+    if (!ok) {
         set_error("engine stream/event creation failed");
         return fail(GQ_ERR_DEVICE);
     }
@@ -3386,23 +3435,20 @@ void engine_destroy(StepEngine *e) {
     if (e->s2) (void)hipStreamSynchronize(e->s2);
     for (EJob *j : e->active) delete j;
     for (EJob *j : e->incoming) delete j;
+    for (StepSlot &sl : e->slot) {
+        for (EJob *j : sl.fin) delete j;
+        if (sl.h_pkt) (void)hipHostFree(sl.h_pkt);
+        if (sl.h_res) (void)hipHostFree(sl.h_res);
+        if (sl.e0) (void)hipEventDestroy(sl.e0);
+        if (sl.e1) (void)hipEventDestroy(sl.e1);
+    }
     if (e->arena) (void)hipFree(e->arena);
-    if (e->d_jobs) (void)hipFree(e->d_jobs);
-    if (e->d_new) (void)hipFree(e->d_new);
-    if (e->d_fin) (void)hipFree(e->d_fin);
+    if (e->d_pkt) (void)hipFree(e->d_pkt);
+    if (e->d_res) (void)hipFree(e->d_res);
     if (e->d_tp) (void)hipFree(e->d_tp);
-    if (e->d_chunk) (void)hipFree(e->d_chunk);
     if (e->big) (void)hipFree(e->big);
     if (e->big_prefix) (void)hipFree(e->big_prefix);
     if (e->big_hdr) (void)hipFree(e->big_hdr);
-    if (e->h_jobs) (void)hipHostFree(e->h_jobs);
-    if (e->h_new) (void)hipHostFree(e->h_new);
-    if (e->h_exp) (void)hipHostFree(e->h_exp);
-    if (e->d_exp) (void)hipFree(e->d_exp);
-    if (e->h_res) (void)hipHostFree(e->h_res);
-    if (e->h_chunk) (void)hipHostFree(e->h_chunk);
-    if (e->e0) (void)hipEventDestroy(e->e0);
-    if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->join) (void)hipEventDestroy(e->join);
     if (e->s2) (void)hipStreamDestroy(e->s2);
@@ -3410,8 +3456,12 @@ void engine_destroy(StepEngine *e) {
     delete e;
 }
 
-int engine_active(const StepEngine *e) { return (int)(e->active.size() + e->incoming.size()); }
+int engine_active(const StepEngine *e) {
+    return (int)(e->active.size() + e->incoming.size()) + (e->slot[0].busy ? 1 : 0) + (e->slot[1].busy ? 1 : 0);
+}
 
+/* Admission touches no device state: the slice is carved from the arena and
+ * the query blob kept on the host until the next step's packet carries it. */
 static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
@@ -3428,11 +3478,6 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     size_t off;
     if (!e->alloc(bytes, &off)) return false;
     char *base = e->arena + off;
-    if (hipMemcpyAsync(base + o_blob, in.blob.data(), (size_t)h.words * 4, hipMemcpyHostToDevice, e->s) !=
-        hipSuccess) {
-        e->release(off, bytes);
-        return false;
-    }
     EJob *j = new EJob();
     j->key = key;
     j->n = n;
@@ -3441,6 +3486,7 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     j->off = off;
     j->bytes = bytes;
     j->words = h.words;
+    j->blob.assign(in.blob.begin(), in.blob.begin() + h.words);
     j->hash = true;
     j->cap = cap;
     j->counts = in.tree_counts;
@@ -3471,7 +3517,8 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
 bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
-    if (!fused_eligible(in, algo) || (size_t)h.words * 4 > ENGINE_BLOB_MAX) return false;
+    if (!fused_eligible(in, algo) || (size_t)h.words * 4 > ENGINE_BLOB_MAX || in.blob.size() < (size_t)h.words)
+        return false;
     const bool hash = in.memo == GQ_MEMO_HASH && h.is_tree && (int)in.tree_counts.size() == n + 1;
     if (hash) return engine_admit_hash(e, in, algo, key);
     size_t o_blob, o_vals, o_left, o_conn, o_sets, o_best, o_first;
@@ -3479,11 +3526,6 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     size_t off;
     if (!e->alloc(bytes, &off)) return false;
     char *base = e->arena + off;
-    if (hipMemcpyAsync(base + o_blob, in.blob.data(), (size_t)h.words * 4, hipMemcpyHostToDevice, e->s) !=
-        hipSuccess) {
-        e->release(off, bytes);
-        return false;
-    }
     EJob *j = new EJob();
     j->key = key;
     j->n = n;
@@ -3492,6 +3534,7 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     j->off = off;
     j->bytes = bytes;
     j->words = h.words;
+    j->blob.assign(in.blob.begin(), in.blob.begin() + h.words);
     BatchJob &J = j->J;
     memset(&J, 0, sizeof(J));
     J.h = h;
@@ -3529,60 +3572,78 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
         }                                                                                  \
     } while (0)
 
-int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
+namespace {
+size_t al64(size_t x) { return (x + 63) & ~(size_t)63; }
+
+/* Enqueues one step: every active DP (the newly admitted ones last) advances
+ * one level.  The host writes one packet into the slot's pinned buffer
+ *   jobs[na] | expansion prefix[na+1] | tree task prefix[na+1] |
+ *   filter chunk prefix[nn+1] | blob offsets[nn] | finishing job indexes[nf] |
+ *   blobs of the new DPs
+ * and the step starts with its single copy to the device; it ends with one
+ * copy of the finishing DPs' results into the slot.  The arena slices of the
+ * finishing DPs are released at once: whatever the next steps place there
+ * runs after this step's walk on the same stream. */
+int engine_enqueue(StepEngine *e, int nn) {
     int rc;
     hipStream_t s = e->s;
-    ECHK(hipSetDevice(e->device));
-    ECHK(hipEventRecord(e->e0, s));
-    /* 1. DPs admitted since the last step: init + filter, then they join */
-    if (!e->incoming.empty()) {
-        const int nn = (int)e->incoming.size();
-        if ((rc = ensure_pinned(&e->h_new, e->cap_h_new, (size_t)nn))) return rc;
-        if ((rc = ensure_pinned(&e->h_chunk, e->cap_h_chunk, (size_t)nn + 1))) return rc;
-        if ((rc = ensure(&e->d_new, e->cap_new, (size_t)nn))) return rc;
-        if ((rc = ensure(&e->d_chunk, e->cap_chunk, (size_t)nn + 1))) return rc;
-        e->h_chunk[0] = 0;
-        for (int i = 0; i < nn; i++) {
-            EJob *j = e->incoming[i];
-            e->h_new[i] = j->J;
-            const uint64_t ns = 1ull << j->n;
-            /* open-addressing jobs build their level lists by expansion */
-            e->h_chunk[i + 1] = e->h_chunk[i] + (j->hash ? 0u : (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK));
-            if (j->hash)
-                hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((j->cap + BLOCK - 1) / BLOCK, 4096)),
-                                   dim3(BLOCK), 0, s, j->J.keys, j->cap, 0u);
-        }
-        ECHK(hipMemcpyAsync(e->d_new, e->h_new, sizeof(BatchJob) * nn, hipMemcpyHostToDevice, s));
-        ECHK(hipMemcpyAsync(e->d_chunk, e->h_chunk, sizeof(uint32_t) * (nn + 1), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_batch_init, dim3(std::min(nn, 1024)), dim3(64), 0, s, e->d_new, nn);
-        const uint32_t fgrid = std::min<uint32_t>(e->h_chunk[nn], 4096);
-        if (fgrid)
-            hipLaunchKernelGGL(k_batch_filter, dim3(fgrid), dim3(BLOCK), 0, s, e->d_new, nn, e->d_chunk);
-        for (EJob *j : e->incoming) e->active.push_back(j);
-        e->incoming.clear();
-    }
+    StepSlot &S = e->slot[e->cur];
     const int na = (int)e->active.size();
-    if (na == 0) return GQ_OK;
-    /* 2. one level of every active DP */
-    if ((rc = ensure_pinned(&e->h_jobs, e->cap_h_jobs, (size_t)na))) return rc;
-    if ((rc = ensure(&e->d_jobs, e->cap_jobs, (size_t)na))) return rc;
-    if ((rc = ensure(&e->d_tp, e->cap_tp, 2 * ((size_t)na + 1)))) return rc;
+    const int j_new = na - nn;
+    std::vector<uint32_t> fin_idx;
+    size_t blob_words = 0;
+    bool any_hash = false, new_hash = false;
+    for (int i = 0; i < na; i++) {
+        const EJob *j = e->active[i];
+        if (j->cur_k >= j->n) fin_idx.push_back((uint32_t)i);
+        any_hash = any_hash || j->hash;
+        if (i >= j_new) {
+            blob_words += j->blob.size();
+            new_hash = new_hash || j->hash;
+        }
+    }
+    const int nf = (int)fin_idx.size();
+    const size_t o_exp = al64(sizeof(BatchJob) * (size_t)na);
+    const size_t o_ttp = al64(o_exp + 4 * ((size_t)na + 1));
+    const size_t o_chunk = al64(o_ttp + 4 * ((size_t)na + 1));
+    const size_t o_bsrc = al64(o_chunk + 4 * ((size_t)nn + 1));
+    const size_t o_fin = al64(o_bsrc + 4 * (size_t)nn);
+    const size_t o_blob = al64(o_fin + 4 * (size_t)nf);
+    const size_t pkt_bytes = o_blob + 4 * blob_words;
+    if ((rc = ensure_pinned(&S.h_pkt, S.cap_h_pkt, pkt_bytes))) return rc;
+    if ((rc = ensure_pinned(&S.h_res, S.cap_h_res, std::max<size_t>((size_t)nf, 1) * RES_WORDS))) return rc;
+    uint8_t *P = S.h_pkt;
+    BatchJob *pj = (BatchJob *)P;
+    uint32_t *pexp = (uint32_t *)(P + o_exp), *pttp = (uint32_t *)(P + o_ttp), *pchunk = (uint32_t *)(P + o_chunk),
+             *pbsrc = (uint32_t *)(P + o_bsrc), *pfin = (uint32_t *)(P + o_fin), *pblob = (uint32_t *)(P + o_blob);
     size_t lds = 0, lds_tree = 0, big_lds_max = 0;
     uint64_t ub = 0, ub_tree = 0, big_cap = 0;
+    pexp[0] = 0;
+    pttp[0] = 0;
     for (int i = 0; i < na; i++) {
         EJob *j = e->active[i];
         j->J.cur_k = j->cur_k;
-        e->h_jobs[i] = j->J;
+        pj[i] = j->J;
         const BlobHdr &h = j->J.h;
-        const uint64_t lvl = (int)j->counts.size() == j->n + 1 ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
-        if (fast_tree_job(h) && (int)j->counts.size() == j->n + 1) {
+        const bool exact = (int)j->counts.size() == j->n + 1;
+        const uint64_t lvl = exact ? j->counts[j->cur_k] : binom64(j->n, j->cur_k);
+        uint64_t tt = 0;
+        if (fast_tree_job(h)) {
             lds_tree = std::max(lds_tree, tree_lds(h.words, 1));
-            ub_tree += tree_tasks(lvl, j->cur_k, j->J.tree_iters);
+            tt = tree_tasks(lvl, j->cur_k, j->J.tree_iters);
         } else {
             lds = std::max(lds, std::max(eval_lds_bicc(h.words), eval_lds_dpsub(h.words, 0)));
             const uint64_t spt = task_sets_of(h.is_tree != 0, j->J.bicc != 0, j->cur_k, j->J.tree_iters);
             ub += (lvl + spt - 1) / spt;          /* tasks of a bound on the level size */
         }
+        ub_tree += tt;
+        if (ub_tree > 0x7fffffffull) {
+            set_error("too many tasks in one step");
+            return GQ_ERR_UNSUPPORTED;
+        }
+        pttp[i + 1] = (uint32_t)ub_tree;
+        const uint64_t src = j->hash ? j->counts[j->cur_k - 1] : 0;
+        pexp[i + 1] = pexp[i] + (uint32_t)((src + BLOCK - 1) / BLOCK);
         if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
             big_cap += binom64(j->n, j->cur_k);
             big_lds_max = std::max(big_lds_max, big_lds(h.words));
@@ -3592,86 +3653,120 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
         set_error("step too large for the big-set list");
         return GQ_ERR_UNSUPPORTED;
     }
-    if (big_cap) {
-        if ((rc = ensure(&e->big, e->cap_big, big_cap))) return rc;
-        if ((rc = ensure(&e->big_prefix, e->cap_bigp, big_cap + 1))) return rc;
+    pchunk[0] = 0;
+    size_t bw = 0;
+    for (int i = 0; i < nn; i++) {
+        const EJob *j = e->active[j_new + i];
+        const uint64_t ns = 1ull << j->n;
+        /* open-addressing jobs build their level lists by expansion */
+        pchunk[i + 1] = pchunk[i] + (j->hash ? 0u : (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK));
+        pbsrc[i] = (uint32_t)bw;
+        memcpy(pblob + bw, j->blob.data(), 4 * j->blob.size());
+        bw += j->blob.size();
     }
-    ECHK(hipMemcpyAsync(e->d_jobs, e->h_jobs, sizeof(BatchJob) * na, hipMemcpyHostToDevice, s));
-    /* open-addressing jobs: generate the level they evaluate now */
-    {
-        bool any = false;
-        for (EJob *j : e->active) any = any || j->hash;
-        if (any) {
-            if ((rc = ensure_pinned(&e->h_exp, e->cap_h_exp, (size_t)na + 1))) return rc;
-            if ((rc = ensure(&e->d_exp, e->cap_exp, (size_t)na + 1))) return rc;
-            e->h_exp[0] = 0;
-            for (int i = 0; i < na; i++) {
-                const EJob *j = e->active[i];
-                const uint64_t src = j->hash ? j->counts[j->cur_k - 1] : 0;
-                e->h_exp[i + 1] = e->h_exp[i] + (uint32_t)((src + BLOCK - 1) / BLOCK);
-            }
-            ECHK(hipMemcpyAsync(e->d_exp, e->h_exp, sizeof(uint32_t) * (na + 1), hipMemcpyHostToDevice, s));
-            const uint32_t eg = std::max<uint32_t>(1, std::min<uint32_t>(e->h_exp[na], 4096));
-            hipLaunchKernelGGL(k_expand, dim3(eg), dim3(BLOCK), 0, s, e->d_jobs, na, 0, e->d_exp);
+    for (int i = 0; i < nf; i++) pfin[i] = fin_idx[i];
+    /* device buffers grow only with the previous step drained (a freed
+     * buffer may still be in use by it) */
+    const bool grow = e->cap_pkt < pkt_bytes || e->cap_tp < 2 * ((size_t)na + 1) ||
+                      e->cap_res < (size_t)std::max(nf, 1) * RES_WORDS ||
+                      (big_cap && (e->cap_big < big_cap || e->cap_bigp < big_cap + 1));
+    if (grow) {
+        ECHK(hipStreamSynchronize(s));
+        if ((rc = ensure(&e->d_pkt, e->cap_pkt, pkt_bytes))) return rc;
+        if ((rc = ensure(&e->d_tp, e->cap_tp, 2 * ((size_t)na + 1)))) return rc;
+        if ((rc = ensure(&e->d_res, e->cap_res, (size_t)std::max(nf, 1) * RES_WORDS))) return rc;
+        if (big_cap) {
+            if ((rc = ensure(&e->big, e->cap_big, big_cap))) return rc;
+            if ((rc = ensure(&e->big_prefix, e->cap_bigp, big_cap + 1))) return rc;
         }
     }
-    /* tree jobs on s (k_batch_eval_tree), the others on s2 (k_batch_eval) */
-    uint32_t *tp_tree = e->d_tp, *tp_gen = e->d_tp + (na + 1);
-    if (ub_tree) hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, tp_tree, JOBS_TREE);
-    if (ub) hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, e->d_jobs, na, tp_gen, JOBS_GENERAL);
+    uint8_t *D = e->d_pkt;
+    const BatchJob *d_jobs = (const BatchJob *)D;
+    ECHK(hipEventRecord(S.e0, s));
+    ECHK(hipMemcpyAsync(D, P, pkt_bytes, hipMemcpyHostToDevice, s));
+    /* 1. DPs admitted since the last step: keys, base relations, filter */
+    if (nn) {
+        if (new_hash) {
+            uint64_t keys = 0;
+            for (int i = j_new; i < na; i++)
+                if (e->active[i]->hash) keys += e->active[i]->cap;
+            hipLaunchKernelGGL(k_clear_keys, dim3((uint32_t)std::min<uint64_t>((keys + BLOCK - 1) / BLOCK, 4096)),
+                               dim3(BLOCK), 0, s, d_jobs + j_new, nn);
+        }
+        hipLaunchKernelGGL(k_batch_init, dim3(std::min(nn, 1024)), dim3(64), 0, s, d_jobs + j_new, nn,
+                           (const uint32_t *)(D + o_blob), (const uint32_t *)(D + o_bsrc));
+        const uint32_t fgrid = std::min<uint32_t>(pchunk[nn], 4096);
+        if (fgrid)
+            hipLaunchKernelGGL(k_batch_filter, dim3(fgrid), dim3(BLOCK), 0, s, d_jobs + j_new, nn,
+                               (const uint32_t *)(D + o_chunk));
+    }
+    /* 2. open-addressing jobs generate the level they evaluate now */
+    if (any_hash)
+        hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(pexp[na], 4096))), dim3(BLOCK), 0,
+                           s, d_jobs, na, 0, (const uint32_t *)(D + o_exp));
+    /* 3. one level of every job: tree jobs on s (k_batch_eval_tree, task
+     * prefix from the exact level sizes), the others on s2 (k_batch_eval,
+     * task prefix from the device's level counters) */
     if (ub) {
+        uint32_t *tp_gen = e->d_tp;
+        hipLaunchKernelGGL(k_step_plan, dim3(1), dim3(1024), 0, s, d_jobs, na, tp_gen, JOBS_GENERAL);
         ECHK(hipEventRecord(e->fork, s));
         ECHK(hipStreamWaitEvent(e->s2, e->fork, 0));
         const uint32_t tpl = (lds + 4 * (size_t)(na + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
         const size_t lt = tpl ? lds + 4 * (size_t)(na + 1) : lds;
         const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub, 1), tuning_get().gen_grid_max);
-        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, e->d_jobs, na, 0, tp_gen, tpl,
+        hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lt, e->s2, d_jobs, na, 0, tp_gen, tpl,
                            big_cap ? e->big : nullptr, (uint32_t)big_cap, e->big_hdr);
         if (big_cap) {
             hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, e->s2, e->big, e->big_hdr, (uint32_t)big_cap,
                                e->big_prefix);
-            hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, e->s2, e->d_jobs, e->big,
+            hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, e->s2, d_jobs, e->big,
                                e->big_hdr, e->big_prefix);
         }
         ECHK(hipEventRecord(e->join, e->s2));
     }
-    if (ub_tree) {                                /* ub_tree: exact task count */
-        if (ub_tree > 0x7fffffffull) {
-            set_error("too many tasks in one step");
-            return GQ_ERR_UNSUPPORTED;
-        }
+    if (ub_tree)
         hipLaunchKernelGGL(k_batch_eval_tree, dim3((uint32_t)ub_tree), dim3(BLOCK), std::max<size_t>(lds_tree, 16),
-                           s, e->d_jobs, na, 0, (const uint32_t *)tp_tree);
-    }
+                           s, d_jobs, na, 0, (const uint32_t *)(D + o_ttp));
     if (ub) ECHK(hipStreamWaitEvent(s, e->join, 0));
-    /* 3. DPs past their last level: plan walk and read-back */
-    std::vector<EJob *> fin, keep;
-    for (EJob *j : e->active) {
-        j->cur_k++;
-        if (j->cur_k > j->n) fin.push_back(j);
-        else keep.push_back(j);
-    }
-    const int nf = (int)fin.size();
-    std::vector<BatchJob> fj(nf);             /* alive until the synchronisation below */
+    /* 4. DPs past their last level: plan walk, one copy of the results */
     if (nf) {
-        if ((rc = ensure(&e->d_fin, e->cap_fin, (size_t)nf))) return rc;
-        if ((rc = ensure_pinned(&e->h_res, e->cap_h_res, (size_t)nf * RES_WORDS))) return rc;
-        for (int i = 0; i < nf; i++) fj[i] = fin[i]->J;
-        ECHK(hipMemcpyAsync(e->d_fin, fj.data(), sizeof(BatchJob) * nf, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_batch_tree, dim3(nf), dim3(WAVE), 0, s, e->d_fin, nf);
-        for (int i = 0; i < nf; i++)
-            ECHK(hipMemcpyAsync(e->h_res + (size_t)i * RES_WORDS, fin[i]->J.counters, RES_WORDS * 4,
-                                hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_batch_tree, dim3(nf), dim3(WAVE), 0, s, d_jobs, nf, (const uint32_t *)(D + o_fin),
+                           e->d_res);
+        ECHK(hipMemcpyAsync(S.h_res, e->d_res, (size_t)nf * RES_WORDS * 4, hipMemcpyDeviceToHost, s));
     }
     ECHK(hipGetLastError());
-    ECHK(hipEventRecord(e->e1, s));
-    ECHK(hipStreamSynchronize(s));
-    float ms = 0.0f;
-    ECHK(hipEventElapsedTime(&ms, e->e0, e->e1));
+    ECHK(hipEventRecord(S.e1, s));
+    std::vector<EJob *> keep;
+    keep.reserve(na);
+    for (int i = 0, f = 0; i < na; i++) {
+        EJob *j = e->active[i];
+        if (f < nf && fin_idx[f] == (uint32_t)i) {
+            f++;
+            e->release(j->off, j->bytes);
+            S.fin.push_back(j);
+        } else {
+            j->cur_k++;
+            keep.push_back(j);
+        }
+    }
     e->active.swap(keep);
-    for (int i = 0; i < nf; i++) {
-        EJob *j = fin[i];
-        const uint32_t *cnt = e->h_res + (size_t)i * RES_WORDS;
+    S.busy = true;
+    e->cur ^= 1;
+    return GQ_OK;
+}
+
+/* Waits for the step of slot `si` and turns its finished DPs into results. */
+int engine_collect(StepEngine *e, int si, std::vector<EngineDone> &done) {
+    StepSlot &S = e->slot[si];
+    if (!S.busy) return GQ_OK;
+    ECHK(hipEventSynchronize(S.e1));
+    S.busy = false;
+    float ms = 0.0f;
+    ECHK(hipEventElapsedTime(&ms, S.e0, S.e1));
+    for (size_t i = 0; i < S.fin.size(); i++) {
+        EJob *j = S.fin[i];
+        const uint32_t *cnt = S.h_res + i * RES_WORDS;
         EngineDone d;
         d.key = j->key;
         d.rc = GQ_OK;
@@ -3694,11 +3789,27 @@ int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
             d.st.memo_kind = j->hash ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
             d.st.gpu_ms = ms;
         }
-        e->release(j->off, j->bytes);
         delete j;
         done.push_back(std::move(d));
     }
+    S.fin.clear();
     return GQ_OK;
 }
+}  // namespace
 
+/* Enqueues the next step (when any DP is active) and returns the DPs of the
+ * step before it; with nothing to enqueue, drains the last step. */
+int engine_step(StepEngine *e, std::vector<EngineDone> &done) {
+    int rc;
+    ECHK(hipSetDevice(e->device));
+    const int nn = (int)e->incoming.size();
+    for (EJob *j : e->incoming) e->active.push_back(j);
+    e->incoming.clear();
+    if (!e->active.empty()) {
+        if ((rc = engine_enqueue(e, nn))) return rc;
+        return engine_collect(e, e->cur, done);     /* the older step: slot cur after the flip */
+    }
+    if ((rc = engine_collect(e, e->cur ^ 1, done))) return rc;
+    return engine_collect(e, e->cur, done);
+}
 }  // namespace gq
