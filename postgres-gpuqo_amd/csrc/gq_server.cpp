@@ -89,15 +89,20 @@ struct Request {
 };
 
 /* One run_many call of a request thread: its DPs are in the engine's hands
- * until `remaining` drops to zero. */
+ * until `remaining` drops to zero.  `refs` counts the queue and engine
+ * entries that still point at it; the thread waits on its own `cv` until
+ * the work is done and unreferenced, so a step wakes only the threads whose
+ * DPs it finished. */
 struct DPWork {
     const std::vector<const DPInput *> *ins;
     const std::vector<DPAlgo> *algos;
     std::vector<std::vector<DPNode>> *trees;
     std::vector<DPStats> *sts;
     size_t remaining = 0;
+    size_t refs = 0;
     int rc = GQ_OK;
     bool done = false;
+    std::condition_variable cv;
 };
 
 }  // namespace
@@ -117,7 +122,7 @@ struct Server {
     StepEngine *eng = nullptr;
 
     std::mutex mu;
-    std::condition_variable cv_engine, cv_work, cv_done, cv_dp;
+    std::condition_variable cv_engine, cv_work, cv_done;
     std::deque<Request *> waiting, admitted, finished;
     int running = 0;
     uint64_t running_bytes = 0;
@@ -151,6 +156,10 @@ struct Server {
         w->remaining = 0;
         w->done = true;
     }
+    /* with mu held: one queue or engine entry of `w` is gone */
+    static void unref(DPWork *w) {
+        if (--w->refs == 0 && w->done) w->cv.notify_one();
+    }
 
     void engine_loop() {
         (void)hipSetDevice(cfg.device);
@@ -161,13 +170,12 @@ struct Server {
                 if (stop) break;
                 continue;
             }
-            bool dropped = false;
             while (!dp_pending.empty()) {
                 DPWork *w = dp_pending.front().first;
                 const size_t i = dp_pending.front().second;
                 if (w->done) {                       /* failed earlier: drop */
                     dp_pending.pop_front();
-                    dropped = true;
+                    unref(w);
                     continue;
                 }
                 const uint64_t key = next_key++;
@@ -180,12 +188,11 @@ struct Server {
                     set_error("a DP of %d relations does not fit the memo budget", (*w->ins)[i]->hdr.n_rels);
                     fail_work(w, GQ_ERR_NOMEM);
                     dp_pending.pop_front();
-                    cv_dp.notify_all();
+                    unref(w);
                     continue;
                 }
                 break;
             }
-            if (dropped) cv_dp.notify_all();
             if (engine_active(eng) == 0) continue;
             lk.unlock();
             std::vector<EngineDone> done;
@@ -197,28 +204,29 @@ struct Server {
                 DPWork *w = it->second.first;
                 const size_t i = it->second.second;
                 in_engine.erase(it);
-                if (w->done) continue;
-                if (d.rc) {
-                    fail_work(w, d.rc);
-                    continue;
+                if (!w->done) {
+                    if (d.rc) {
+                        fail_work(w, d.rc);
+                    } else {
+                        (*w->trees)[i] = std::move(d.tree);
+                        (*w->sts)[i] = d.st;
+                        if (--w->remaining == 0) w->done = true;
+                    }
                 }
-                (*w->trees)[i] = std::move(d.tree);
-                (*w->sts)[i] = d.st;
-                if (--w->remaining == 0) w->done = true;
+                unref(w);
             }
             if (rc) {                                /* device failure: fail everything on the device */
-                for (auto &kv : in_engine) fail_work(kv.second.first, rc);
+                std::vector<DPWork *> ws;
+                for (auto &kv : in_engine) ws.push_back(kv.second.first);
+                for (auto &p : dp_pending) ws.push_back(p.first);
                 in_engine.clear();
-                for (auto &p : dp_pending) fail_work(p.first, rc);
                 dp_pending.clear();
+                for (DPWork *w : ws) fail_work(w, rc);
+                for (DPWork *w : ws) unref(w);
                 engine_destroy(eng);
                 eng = nullptr;
-                if (engine_create(cfg.device, budget, &eng)) {
-                    cv_dp.notify_all();
-                    break;
-                }
+                if (engine_create(cfg.device, budget, &eng)) break;
             }
-            cv_dp.notify_all();
         }
     }
 
@@ -269,6 +277,7 @@ int ServerExecutor::run(const std::vector<const DPInput *> &ins, const std::vect
     w.trees = &trees;
     w.sts = &sts;
     w.remaining = ins.size();
+    w.refs = ins.size();
     std::unique_lock<std::mutex> lk(srv->mu);
     if (!srv->eng) {
         set_error("planning server engine unavailable");
@@ -276,18 +285,9 @@ int ServerExecutor::run(const std::vector<const DPInput *> &ins, const std::vect
     }
     for (size_t i = 0; i < ins.size(); i++) srv->dp_pending.push_back(std::make_pair(&w, i));
     srv->cv_engine.notify_one();
-    srv->cv_dp.wait(lk, [&] { return w.done; });
     /* DPs of a failed work may still be queued or on the device: wait until
-     * the engine no longer references this stack object */
-    for (;;) {
-        bool referenced = false;
-        for (auto &p : srv->dp_pending)
-            if (p.first == &w) referenced = true;
-        for (auto &kv : srv->in_engine)
-            if (kv.second.first == &w) referenced = true;
-        if (!referenced) break;
-        srv->cv_dp.wait(lk);
-    }
+     * the engine no longer references this stack object too */
+    w.cv.wait(lk, [&] { return w.done && w.refs == 0; });
     return w.rc;
 }
 
