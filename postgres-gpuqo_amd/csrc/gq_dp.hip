@@ -902,39 +902,45 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
     if (LPE == 1 && qn) flush();
 }
 
-/* Tree graphs: is v (a neighbour of connected S outside it, hence a leaf
- * of S + v) the highest-numbered leaf of S + v?  Then S + v is generated
- * from S alone: removing a leaf keeps a tree connected, so every connected
- * set T of two or more relations comes from exactly one S = T - (its
- * highest leaf). */
-__device__ __forceinline__ bool is_top_leaf(const uint32_t *adj, uint32_t S, int v) {
-    const uint32_t T = S | (1u << v);
-    uint32_t above = S & ~((2u << v) - 1u);       /* members of S numbered above v */
-    while (above) {
-        const int u = __builtin_ctz(above);
-        above &= above - 1u;
-        if (__builtin_popcount(adj[u] & T) <= 1) return false;
+/* Tree graphs: every connected set T of two or more relations is generated
+ * from exactly one S = T - (T's highest-numbered leaf), since removing a
+ * leaf keeps a tree connected.  Returns the relations v outside connected S
+ * for which v is the highest-numbered leaf of S + v.  One pass over S gives
+ * its neighbourhood and the members with exactly one neighbour in S.  In a
+ * tree, v has exactly one neighbour p in S, whose degree rises by one: the
+ * leaves of S + v among S are those members minus p, or S itself when S is
+ * a singleton (its member then has one neighbour in S + v).  v qualifies
+ * when none of them is numbered above v. */
+__device__ __forceinline__ uint32_t top_leaf_children(const uint32_t *adj, uint32_t S) {
+    uint32_t nb = 0, l1 = 0;
+    for (uint32_t r = S; r; r &= r - 1u) {
+        const int u = __builtin_ctz(r);
+        const uint32_t a = adj[u];
+        nb |= a;
+        if (__builtin_popcount(a & S) == 1) l1 |= 1u << u;
     }
-    return true;
+    nb &= ~S;
+    const bool single = (S & (S - 1u)) == 0u;
+    uint32_t emit = 0;
+    for (uint32_t c = nb; c; c &= c - 1u) {
+        const int v = __builtin_ctz(c);
+        const uint32_t leaves = single ? S : (l1 & ~adj[v]);
+        if ((leaves & ~((2u << v) - 1u)) == 0u) emit |= 1u << v;
+    }
+    return emit;
 }
 
-/* Dense-memo tree DP: the level list k+1 from set i of level k (is_top_leaf
- * rule), positions reserved with one atomic per wave on *dst_count.  Called
- * by whole waves; the order within a level list is immaterial, each set's
- * best split being computed on its own. */
+/* Dense-memo tree DP: the level list k+1 from set i of level k
+ * (top_leaf_children), positions reserved with one atomic per wave on
+ * *dst_count.  Called by whole waves; the order within a level list is
+ * immaterial, each set's best split being computed on its own. */
 __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t *src, uint32_t count, uint32_t i,
                                              uint32_t *dst, uint32_t *dst_count) {
     const int lane = threadIdx.x & (WAVE - 1);
     uint32_t S = 0, emit = 0;
     if (i < count) {
         S = src[i];
-        uint32_t nb = 0;
-        for (uint32_t r = S; r; r &= r - 1u) nb |= adj[__builtin_ctz(r)];
-        nb &= ~S;
-        for (uint32_t c = nb; c; c &= c - 1u) {
-            const int v = __builtin_ctz(c);
-            if (is_top_leaf(adj, S, v)) emit |= 1u << v;
-        }
+        emit = top_leaf_children(adj, S);
     }
     const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
     uint32_t incl = cnt;
@@ -962,9 +968,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
                                                       uint32_t *next, uint32_t cap) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
-    if (blockIdx.x >= ev_blocks) {
-        expand_level(blob + h.o_adj, lvl_sets, counters[k], (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next,
-                     counters + k + 1);
+    if (blockIdx.x >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
+        const uint32_t count = counters[k];
+        if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
+        __syncthreads();
+        expand_level(smem, lvl_sets, count, (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1);
         return;
     }
     const uint32_t spw = tree_spw(k, LPE), per = (uint32_t)(k - 1) * LPE;
@@ -986,7 +994,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
 
 /* Start of a dense-memo tree DP: zeroes the 64 result counters, stores the
  * base relations and builds level 2, the query's edges, each from its lower
- * end (the is_top_leaf rule on singletons).  One wave. */
+ * end (top_leaf_children of a singleton).  One wave. */
 __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left,
                                                       uint32_t *lvl2, uint32_t *counters) {
     const int i = threadIdx.x;
@@ -1837,22 +1845,22 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 /* Open-addressing memo, level list k from level k-1 (tree graphs).  Every
- * connected set T of k >= 2 relations is generated exactly once: from
- * S = T - v where v is the highest-numbered leaf of T (removing a leaf keeps
- * a tree connected).  Each thread takes one set S of level k-1 (the
- * singletons for k = 2) and emits S + v for each neighbour v outside S that
- * is that leaf, i.e. no member u > v of S has at most one neighbour in
- * S + v.  A wave reserves its list positions with one atomic, then each
- * emitted set claims its key (a key already present would be a duplicate:
- * error bit 16).  Level k is level k_fixed of every job, or each job's
- * J.cur_k when k_fixed = 0; `prefix[j]` = first task (BLOCK source sets) of
- * job j, host-computed from the exact level sizes; jobs without keys have
- * no tasks.  Order within a level list is immaterial: each set's best split
- * is computed independently. */
+ * connected set T of k >= 2 relations is generated exactly once, from
+ * S = T - (T's highest-numbered leaf) (top_leaf_children).  Each thread
+ * takes one set S of level k-1 (the singletons for k = 2); the workgroup
+ * reserves its list positions with one atomic, then each emitted set claims
+ * its key (a key already present would be a duplicate: error bit 16).
+ * Level k is level k_fixed of every job, or each job's J.cur_k when
+ * k_fixed = 0; `prefix[j]` = first task (BLOCK source sets) of job j,
+ * host-computed from the exact level sizes; jobs without keys have no
+ * tasks.  A task belongs to one job, so the whole workgroup works for it.
+ * Order within a level list is immaterial: each set's best split is
+ * computed independently. */
 __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
                                                    const uint32_t *prefix) {
+    __shared__ uint32_t s_adj[32];
+    __shared__ uint32_t part[17];
     const uint32_t total = prefix[nj];
-    const int lane = threadIdx.x & (WAVE - 1);
     for (uint32_t t = blockIdx.x; t < total; t += gridDim.x) {
         int lo = 0, hi = nj;
         while (hi - lo > 1) {
@@ -1866,44 +1874,17 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
         const uint32_t src_count = (k - 1 == 1) ? (uint32_t)n : J.counters[k - 1];
         const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
         const bool have = i < src_count && J.keys != nullptr;
-        const uint32_t *adj = J.blob + J.h.o_adj;
-        uint32_t S = 0, nb = 0, emit = 0;
-        if (have) {
-            S = (k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i];
-            uint32_t rest = S;
-            while (rest) {
-                const int v = __builtin_ctz(rest);
-                rest &= rest - 1u;
-                nb |= adj[v];
-            }
-            nb &= ~S;
-            uint32_t c = nb;
-            while (c) {
-                const int v = __builtin_ctz(c);
-                c &= c - 1u;
-                if (is_top_leaf(adj, S, v)) emit |= 1u << v;
-            }
-        }
-        /* wave-aggregated reservation of list positions */
+        const uint32_t S = have ? ((k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i]) : 0u;
+        if (threadIdx.x < 32) s_adj[threadIdx.x] = (int)threadIdx.x < n ? J.blob[J.h.o_adj + threadIdx.x] : 0u;
+        __syncthreads();
+        uint32_t emit = have ? top_leaf_children(s_adj, S) : 0u;
         const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
-        uint32_t incl = cnt;
-        for (int off = 1; off < WAVE; off <<= 1) {
-            const uint32_t y = __shfl_up(incl, off);
-            if (lane >= off) incl += y;
-        }
-        const uint32_t wave_total = __shfl(incl, WAVE - 1);
-        /* lanes of one wave may belong to different jobs only at a job
-         * boundary of the task range; reserve per (wave, job) */
-        const int first_lane_job = __shfl(lo, 0);
-        const bool uniform = __all(lo == first_lane_job);
-        uint32_t base = 0;
-        if (uniform) {
-            if (lane == 0 && wave_total) base = atomicAdd(&J.counters[k], wave_total);
-            base = __shfl(base, 0) + incl - cnt;
-        } else if (cnt) {
-            base = atomicAdd(&J.counters[k], cnt);
-        }
-        uint32_t pos = base;
+        uint32_t all;
+        const uint32_t off = block_scan_excl(cnt, part, &all);
+        if (threadIdx.x == 0) part[0] = all ? atomicAdd(&J.counters[k], all) : 0u;
+        __syncthreads();
+        uint32_t pos = part[0] + off;
+        __syncthreads();                           /* part and s_adj are rewritten by the next task */
         while (emit) {
             const int v = __builtin_ctz(emit);
             emit &= emit - 1u;
