@@ -69,9 +69,11 @@ typedef enum gq_idp_type {
 } gq_idp_type;
 
 typedef enum gq_memo_kind {
-    GQ_MEMO_AUTO = 0,   /* dense when 2^n slots fit the budget, else hash  */
+    GQ_MEMO_AUTO = 0,   /* dense when 2^n slots fit the budget, else rank  */
     GQ_MEMO_DENSE = 1,  /* perfect-hash table indexed by the relation set  */
-    GQ_MEMO_HASH = 2    /* open-addressing table, linear probing           */
+    GQ_MEMO_HASH = 2,   /* open-addressing table, linear probing           */
+    GQ_MEMO_RANK = 3    /* tree graphs: one slot per connected set, the
+                           slot computed from the set (no keys, no probes) */
 } gq_memo_kind;
 
 /* PathCost, src/include/optimizer/gpuqo_common.h:81-84. */
@@ -189,7 +191,7 @@ typedef struct gq_plan_stats {
     double gpu_ms;            /* device time of the DP (HIP events)        */
     double wall_ms;           /* host wall time of the whole call          */
     int32_t n_dp_calls;       /* DPs run (1 for a plain query)             */
-    int32_t memo_kind;        /* GQ_MEMO_DENSE or GQ_MEMO_HASH used        */
+    int32_t memo_kind;        /* GQ_MEMO_DENSE, _HASH or _RANK used        */
     /* with cfg->profile_kernels: summed device time per phase */
     double filter_ms;         /* unrank + connectivity filter kernels      */
     double eval_ms;           /* split evaluation (+ finalize) kernels     */

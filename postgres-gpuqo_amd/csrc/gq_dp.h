@@ -39,6 +39,9 @@ struct BlobHdr {
      * [2] n_quals | n_classes << 8, [4..5] inner member slots for o = 0,
      * [6..7] for o = 1 (u16 each, class order). */
     int tree_fast, o_tedge;
+    /* Rank memo of a tree graph (GQ_MEMO_RANK; 0: none): n words off[t],
+     * then n x n words Q[t][u] (attach_rank_tables). */
+    int o_rank;
 };
 
 struct DPInput {
@@ -51,14 +54,18 @@ struct DPInput {
     /* tree graphs: number of connected sets of each size k (index k = 0..n),
      * from the subtree-count polynomial; empty for other graphs */
     std::vector<uint64_t> tree_counts;
-    /* memo layout chosen for this DP: GQ_MEMO_DENSE or GQ_MEMO_HASH (trees) */
+    /* memo layout chosen for this DP: GQ_MEMO_DENSE, or GQ_MEMO_HASH /
+     * GQ_MEMO_RANK (trees) */
     int memo = 1;
 };
 
-/* GQ_MEMO_AUTO resolves to the open-addressing memo for tree graphs whose
- * dense memo (36 B x 2^n) would exceed max_memo_mb; GQ_MEMO_HASH applies to
- * tree graphs only (other graphs keep the dense memo). */
+/* GQ_MEMO_AUTO resolves to the rank memo for tree graphs whose dense memo
+ * (36 B x 2^n) would exceed max_memo_mb; GQ_MEMO_HASH and GQ_MEMO_RANK
+ * apply to tree graphs only (other graphs keep the dense memo). */
 int resolve_memo(const DPInput &in, int requested, int max_memo_mb);
+/* Sets in.memo = resolve_memo(...) and, for the rank memo, appends the rank
+ * tables to the blob (BlobHdr::o_rank). */
+void prepare_memo(DPInput &in, int requested, int max_memo_mb);
 
 /* Launch-shape knobs of the device DP (gq_tuning in the C ABI); none of
  * them changes a plan, a cost or a count. */
@@ -114,9 +121,10 @@ struct EngineDone {
 int engine_create(int device, uint64_t arena_bytes, StepEngine **out);
 void engine_destroy(StepEngine *e);
 /* arena bytes one DP of n relations takes (blob allowance included): dense
- * memo, or open-addressing memo over `total_sets` connected sets */
+ * memo, or open-addressing / rank memo over `total_sets` connected sets */
 uint64_t engine_slice_bytes(int n_rels);
 uint64_t engine_hash_slice_bytes(uint64_t total_sets);
+uint64_t engine_rank_slice_bytes(uint64_t total_sets);
 /* admits the DP if its slice fits now (no device call: the blob is copied
  * to the host side of the engine and reaches the device with the next
  * step); false otherwise */

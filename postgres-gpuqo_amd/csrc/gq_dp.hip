@@ -143,18 +143,35 @@ __device__ __forceinline__ void store_base(float4 *vals, uint32_t x, const BlobH
     store_rel(vals, x, r, h.p.work_mem);
 }
 
-/* The memo of one DP.  Dense (keys == nullptr): the slot of set S is S.
- * Open addressing (tree graphs): keys[slot] holds S (0 = empty), linear
- * probing from hash(S); every connected set is inserted before its values
- * are read, so a probe ends at its key.  A probe that meets an empty slot or
- * runs over the whole table sets bit 2 of *err and returns slot 0. */
+/* The memo of one DP.  Dense (keys == nullptr, rank == nullptr): the slot
+ * of set S is S.  Open addressing (tree graphs): keys[slot] holds S (0 =
+ * empty), linear probing from hash(S); every connected set is inserted
+ * before its values are read, so a probe ends at its key.  A probe that
+ * meets an empty slot or runs over the whole table sets bit 2 of *err and
+ * returns slot 0.  Rank (tree graphs, rank != nullptr): the slot of
+ * connected set S is rank_slot(rank, rn, S), one slot per connected set. */
 struct MemoRef {
     float4 *vals;
     uint32_t *left;
     const uint32_t *keys;
     uint32_t mask;
     uint32_t *err;
+    const uint32_t *rank;
+    int rn;
 };
+
+/* Rank memo of a tree graph whose every vertex has a lower-numbered parent
+ * (BlobHdr::is_tree): connected set S with lowest member t has slot
+ *     off[t] + sum over u in S - t of Q[t][u]
+ * (rk = off[rn] then Q[rn][rn]; the tables are built by attach_rank_tables,
+ * which shows that this numbers the connected sets 0, 1, ... without gaps). */
+__host__ __device__ __forceinline__ uint32_t rank_slot(const uint32_t *rk, int rn, uint32_t S) {
+    const int t = __builtin_ctz(S);
+    const uint32_t *q = rk + rn + t * rn;
+    uint32_t r = rk[t];
+    for (uint32_t x = S & (S - 1u); x; x &= x - 1u) r += q[__builtin_ctz(x)];
+    return r;
+}
 
 __device__ __forceinline__ uint32_t memo_hash(uint32_t x) {
     x ^= x >> 16;
@@ -166,6 +183,7 @@ __device__ __forceinline__ uint32_t memo_hash(uint32_t x) {
 }
 
 __device__ __forceinline__ uint32_t memo_slot(const MemoRef &m, uint32_t S) {
+    if (m.rank) return rank_slot(m.rank, m.rn, S);
     if (!m.keys) return S;
     uint32_t h = memo_hash(S) & m.mask;
     for (uint32_t i = 0; i <= m.mask; i++) {
@@ -762,6 +780,9 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
     const uint32_t seg = lane / per, t = lane % per, e = LPE == 1 ? t : t >> 1;
     const int o_lo = LPE == 1 ? 0 : (int)(t & 1u), o_hi = LPE == 1 ? 2 : o_lo + 1;
     const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
+    /* rank memo: its tables in the staged blob */
+    const uint32_t *rk = mr.rank ? smem + h.o_rank : nullptr;
+    const int rn = h.n_rels;
     const uint64_t nwaves = (count + spw - 1) / spw;
     uint32_t *wq = smem + ((h.words + 3) & ~3) + (threadIdx.x / WAVE) * TREE_Q_WORDS;
     uint32_t qn = 0;                                    /* queued records (wave-uniform) */
@@ -790,13 +811,41 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
         uint32_t first = ~0u, f0 = ~0u, f1 = ~0u;
         unsigned long long key = ~0ull, k0 = ~0ull, k1 = ~0ull;
         Rel j0, j1;
-        uint32_t S = 0, sub = 0;
+        uint32_t S = 0, sub = 0, rS = 0;
+        int c = 0;
         if (active) {
             S = (w == w0 && s0) ? s0 : lvl_sets[idx];   /* s0: this lane's first set, loaded by the caller */
-            const int c = select_bit(S, e + 1u);        /* member e+1: the root (lowest member) is skipped */
+            c = select_bit(S, e + 1u);                  /* member e+1: the root (lowest member) is skipped */
             sub = S & desc[c];
+        }
+        if (rk) {
+            /* rank memo: every member but the lowest, t, is the lower end
+             * of one edge, so the set's slot off[t] + sum Q[t][c] is a sum
+             * over the segment (each edge counted once) */
+            const int top = __builtin_ctz(S | 0x80000000u);
+            uint32_t x = active && (LPE == 1 || (t & 1u) == 0u) ? rk[rn + top * rn + c] : 0u;
+            for (uint32_t off = 1; off < per; off <<= 1) {
+                const uint32_t y = __shfl_down(x, off);
+                if (t + off < per) x += y;
+            }
+            const int hd = (int)(seg * per);
+            x = __shfl(x, hd < WAVE ? hd : 0);
+            rS = active ? rk[top] + x : 0u;
+        }
+        if (active) {
             const uint32_t rest = S ^ sub;
-            const Rel Rs = load_rel(mr.vals, memo_slot(mr, sub)), Rr = load_rel(mr.vals, memo_slot(mr, rest));
+            uint32_t ss, sr;
+            if (rk) {                                   /* slots of sub (lowest member c) and of S - sub */
+                const uint32_t *qc = rk + rn + c * rn;
+                uint32_t sig = 0;
+                for (uint32_t y = sub & ~(1u << c); y; y &= y - 1u) sig += qc[__builtin_ctz(y)];
+                ss = rk[c] + sig;
+                sr = rS - rk[rn + __builtin_ctz(S) * rn + c] * (1u + sig);
+            } else {
+                ss = memo_slot(mr, sub);
+                sr = memo_slot(mr, rest);
+            }
+            const Rel Rs = load_rel(mr.vals, ss), Rr = load_rel(mr.vals, sr);
             uint32_t rk0, rk1;
             if (bicc) {
                 rk0 = 2u * tree_post(S, c, desc, lglob);
@@ -875,7 +924,7 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
             const bool w1n = nanfirst ? (f1 == sf) : (k1 == sk);
             if (w0n || w1n) {
                 win = true;
-                slot = lvl_slot ? lvl_slot[idx] : S;
+                slot = rk ? rS : (lvl_slot ? lvl_slot[idx] : S);
                 lft = w1n ? sub : (S ^ sub);
                 if (w1n) j0 = j1;
                 if (LPE == 2) {
@@ -992,8 +1041,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
      * address would serialise the level */
 }
 
-/* Start of a dense-memo tree DP: zeroes the 64 result counters, stores the
- * base relations and builds level 2, the query's edges, each from its lower
+/* Start of a dense- or rank-memo tree DP: zeroes the 64 result counters,
+ * stores the base relations and builds level 2, the query's edges, each from its lower
  * end (top_leaf_children of a singleton).  One wave. */
 __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left,
                                                       uint32_t *lvl2, uint32_t *counters) {
@@ -1001,8 +1050,9 @@ __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *
     counters[i] = 0u;
     uint32_t emit = 0;
     if (i < h.n_rels) {
-        store_base(vals, 1u << i, h, blob, i);
-        left[1u << i] = 0u;
+        const uint32_t slot = h.o_rank ? blob[h.o_rank + i] : (1u << i);   /* rank memo: off[i] */
+        store_base(vals, slot, h, blob, i);
+        left[slot] = 0u;
         emit = blob[h.o_adj + i] & ~((2u << i) - 1u);          /* neighbours above i */
     }
     const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
@@ -1046,6 +1096,8 @@ __device__ __forceinline__ MemoRef job_memo(const BatchJob &J) {
     m.keys = J.keys;
     m.mask = J.hmask;
     m.err = J.counters + 49;
+    m.rank = J.h.o_rank ? J.blob + J.h.o_rank : nullptr;
+    m.rn = J.h.n_rels;
     return m;
 }
 
@@ -1521,6 +1573,8 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj, const uint32_t *blob_
                     atomicOr(&J.counters[49], 8u);
                     continue;
                 }
+            } else if (J.h.o_rank) {           /* rank memo: off[tid] */
+                slot = blob[J.h.o_rank + tid];
             }
             store_base(J.vals, slot, J.h, blob, tid);
             J.left[slot] = 0;
@@ -1844,16 +1898,16 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-/* Open-addressing memo, level list k from level k-1 (tree graphs).  Every
- * connected set T of k >= 2 relations is generated exactly once, from
- * S = T - (T's highest-numbered leaf) (top_leaf_children).  Each thread
- * takes one set S of level k-1 (the singletons for k = 2); the workgroup
- * reserves its list positions with one atomic, then each emitted set claims
- * its key (a key already present would be a duplicate: error bit 16).
- * Level k is level k_fixed of every job, or each job's J.cur_k when
- * k_fixed = 0; `prefix[j]` = first task (BLOCK source sets) of job j,
- * host-computed from the exact level sizes; jobs without keys have no
- * tasks.  A task belongs to one job, so the whole workgroup works for it.
+/* Open-addressing and rank memos, level list k from level k-1 (tree
+ * graphs).  Every connected set T of k >= 2 relations is generated exactly
+ * once, from S = T - (T's highest-numbered leaf) (top_leaf_children).  Each
+ * thread takes one set S of level k-1 (the singletons for k = 2); the
+ * workgroup reserves its list positions with one atomic, then, with the
+ * open-addressing memo, each emitted set claims its key (a key already
+ * present would be a duplicate: error bit 16).  Level k is level k_fixed of
+ * every job, or each job's J.cur_k when k_fixed = 0; `prefix[j]` = first
+ * task (BLOCK source sets) of job j, host-computed from the exact level
+ * sizes; jobs with neither keys nor rank tables have no tasks.  A task belongs to one job, so the whole workgroup works for it.
  * Order within a level list is immaterial: each set's best split is
  * computed independently. */
 __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
@@ -1873,7 +1927,7 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
         const int n = J.h.n_rels;
         const uint32_t src_count = (k - 1 == 1) ? (uint32_t)n : J.counters[k - 1];
         const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
-        const bool have = i < src_count && J.keys != nullptr;
+        const bool have = i < src_count && (J.keys != nullptr || J.h.o_rank != 0);
         const uint32_t S = have ? ((k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i]) : 0u;
         if (threadIdx.x < 32) s_adj[threadIdx.x] = (int)threadIdx.x < n ? J.blob[J.h.o_adj + threadIdx.x] : 0u;
         __syncthreads();
@@ -1889,14 +1943,16 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
             const int v = __builtin_ctz(emit);
             emit &= emit - 1u;
             const uint32_t T = S | (1u << v);
-            bool fresh;
-            const uint32_t slot = memo_insert(J.keys, J.hmask, T, &fresh);
-            if (slot == ~0u || !fresh) {
-                atomicOr(&J.counters[49], slot == ~0u ? 8u : 16u);
-                continue;
+            if (J.keys) {
+                bool fresh;
+                const uint32_t slot = memo_insert(J.keys, J.hmask, T, &fresh);
+                if (slot == ~0u || !fresh) {
+                    atomicOr(&J.counters[49], slot == ~0u ? 8u : 16u);
+                    continue;
+                }
+                J.lvl_slot[J.lvl_off[k] + pos] = slot;
             }
-            J.lvl_sets[J.lvl_off[k] + pos] = T;
-            J.lvl_slot[J.lvl_off[k] + pos] = slot;
+            J.lvl_sets[J.lvl_off[k] + pos] = T;      /* rank memo: the slot follows from T */
             pos++;
         }
     }
@@ -2602,6 +2658,8 @@ static MemoRef dense_memo(DeviceCtx &c) {
     m.keys = nullptr;
     m.mask = 0;
     m.err = c.b.counters + 49;
+    m.rank = nullptr;
+    m.rn = 0;
     return m;
 }
 
@@ -2764,6 +2822,8 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     mr.keys = c.hkeys;
     mr.mask = (uint32_t)(cap - 1);
     mr.err = c.b.counters + 49;
+    mr.rank = nullptr;
+    mr.rn = 0;
     if (profile && (int)c.prof_ev.size() < 3 * 33) {
         while ((int)c.prof_ev.size() < 3 * 33) {
             hipEvent_t e;
@@ -2837,6 +2897,116 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     return GQ_OK;
 }
 
+/* Tree graphs with the rank memo: one slot per connected set, computed from
+ * the set (rank_slot), so the memo holds 20 bytes per connected set and no
+ * keys.  Level 2 comes from k_tree_start and each level's launch builds the
+ * next level's list by expansion, as in the dense-memo tree DP. */
+static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool profile,
+                       std::vector<DPNode> &tree, DPStats &st) {
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    int rc;
+    if (!h.is_tree || !h.o_rank || (int)in.tree_counts.size() != n + 1 || algo == DP_DPSIZE || n < 2) {
+        set_error("the rank memo serves DPsub/MPDP on tree graphs");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    uint64_t total = 0, list = 0;
+    uint32_t off_h[33];
+    memset(off_h, 0, sizeof(off_h));
+    for (int k = 1; k <= n; k++) {
+        total += in.tree_counts[k];
+        if (k >= 2) {
+            off_h[k] = (uint32_t)list;
+            list += in.tree_counts[k];
+        }
+    }
+    if (total >= (1ull << 31)) {
+        set_error("too many connected sets for the rank memo (%llu)", (unsigned long long)total);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    const size_t W4h = ((size_t)h.words + 3) & ~(size_t)3;
+    const size_t lds_eval = W4h * 4;
+    if ((rc = ensure(&c.b.vals, c.cap_memo, total))) return rc;
+    if ((rc = ensure(&c.b.left, c.cap_left, total))) return rc;
+    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, std::max<uint64_t>(list, 1)))) return rc;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33))) return rc;
+    if ((rc = ensure_stage(c, W4h + 33))) return rc;
+    c.lvl_off = c.b.blob + W4h;
+    hipStream_t s = c.stream;
+    memcpy(c.h_stage, in.blob.data(), (size_t)h.words * 4);
+    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
+    memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
+    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
+    MemoRef mr = dense_memo(c);
+    mr.rank = c.b.blob + h.o_rank;
+    mr.rn = n;
+    if (profile && (int)c.prof_ev.size() < 3 * 33) {
+        while ((int)c.prof_ev.size() < 3 * 33) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.prof_ev.push_back(e);
+        }
+    }
+    const int bicc = algo == DP_BICC ? 1 : 0;
+    HIPCHK(hipEventRecord(c.ev0, s));
+    hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
+                       c.b.lvl_sets + off_h[2], c.b.counters);
+    for (int k = 2; k <= K; k++) {
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
+        launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, bicc, c.b.lvl_sets + off_h[k], nullptr, mr,
+                         c.b.counters, k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr);
+        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+    }
+    uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
+    if (K < n) {                                   /* IDP1: the cheapest set of K relations */
+        rc = idp1_root(c, s, mr, c.b.lvl_sets + off_h[K], nullptr, c.b.counters + K, n, K,
+                       [&](std::vector<uint32_t> &o) -> int { o = dpsub_creation_order(in, K); return GQ_OK; }, &root);
+        if (rc) return rc;
+    }
+    const int nodes_n = 2 * K - 1;
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, mr, root, c.b.nodes, nodes_n, c.b.counters + 48);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c.ev1, s));
+    tree.resize(nodes_n);
+    const uint32_t *cnt = c.h_res;
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (cnt[49]) {
+        set_error("rank memo error (flags %u)", cnt[49]);
+        return (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
+    }
+    for (int k = 2; k <= K; k++)
+        if (cnt[k] != in.tree_counts[k]) {
+            set_error("level %d holds %u sets, expected %llu", k, cnt[k], (unsigned long long)in.tree_counts[k]);
+            return GQ_ERR_DEVICE;
+        }
+    if (cnt[48] != (uint32_t)nodes_n) {
+        set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
+        return GQ_ERR_DEVICE;
+    }
+    memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    st.gpu_ms = ms;
+    if (profile) {
+        for (int k = 2; k <= K; k++) {
+            float b = 0.0f;
+            HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
+            st.eval_ms += b;
+            st.eval_launches++;
+        }
+    }
+    uint64_t made = 0;
+    for (int k = 2; k <= K; k++) made += in.tree_counts[k];
+    st.candidates = made;            /* sets generated by expansion (no 2^n scan) */
+    st.sets = made;
+    st.pairs = tree_pairs(in.tree_counts, (size_t)K);
+    st.memo_bytes = total * (sizeof(float4) + 4) + list * 4;
+    st.memo_kind = GQ_MEMO_RANK;
+    return GQ_OK;
+}
+
 Tuning tuning_get() {
     std::lock_guard<std::mutex> g(g_tune_mu);
     return g_tune;
@@ -2860,8 +3030,72 @@ int resolve_memo(const DPInput &in, int requested, int max_memo_mb) {
     const bool tree = in.hdr.is_tree && (int)in.tree_counts.size() == n + 1;
     if (!tree || requested == GQ_MEMO_DENSE) return GQ_MEMO_DENSE;
     if (requested == GQ_MEMO_HASH) return GQ_MEMO_HASH;
+    if (requested == GQ_MEMO_RANK) return GQ_MEMO_RANK;
     const double dense = 36.0 * (double)(1ull << n);
-    return dense > (double)max_memo_mb * 1048576.0 ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
+    return dense > (double)max_memo_mb * 1048576.0 ? GQ_MEMO_RANK : GQ_MEMO_DENSE;
+}
+
+/* Rank tables of a tree graph whose vertex v > 0 has the lower-numbered
+ * parent p(v) (BlobHdr::is_tree), so a connected set's lowest member is its
+ * top.  Let N[v] be the number of connected sets with top v:
+ * N[v] = prod over children c of (1 + N[c]).  The sets with top t are
+ * numbered in mixed radix over t's children: child c (children by
+ * increasing number) has weight M[c] = prod over its lower-numbered
+ * siblings s of (1 + N[s]) and digit 0 (c absent) or 1 + the number of the
+ * part below c (a set with top c).  These digits range over exactly
+ * 1 + N[c] values, so the sets with top t get the numbers 0 .. N[t] - 1,
+ * one each.  Unrolled, the number of T is
+ *     sum over u in T - t of Q[t][u],  Q[t][u] = prod of M over the path
+ * from t (exclusive) down to u,
+ * and off[t] = sum of N[v] over v < t places the sets of each top after the
+ * lower tops: every connected set gets a slot below their total.  The
+ * blob gains off[n] and Q[n][n] (zero where t is not an ancestor of u).
+ * A split at edge (p(c), c) of set S with top t gives the part below c,
+ * sub, with slot off[c] + sig, sig = sum over u in sub - c of Q[c][u], and
+ * since Q[t][u] = Q[t][c] Q[c][u] below c, the rest has slot
+ * slot(S) - Q[t][c] (1 + sig). */
+static void attach_rank_tables(DPInput &in) {
+    BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    std::vector<uint32_t> &b = in.blob;
+    std::vector<int> parent(n, -1);
+    std::vector<uint64_t> N(n, 1), M(n, 1);
+    for (int v = 1; v < n; v++) parent[v] = __builtin_ctz(b[h.o_adj + v] & ((1u << v) - 1u));
+    for (int v = n - 1; v >= 0; v--) {                 /* children are numbered above their parent */
+        uint64_t prod = 1;
+        for (int c = v + 1; c < n; c++)
+            if (parent[c] == v) {
+                M[c] = prod;
+                prod *= 1 + N[c];
+            }
+        N[v] = prod;
+    }
+    const int o = (int)b.size();
+    b.resize((size_t)o + n + (size_t)n * n, 0u);
+    uint64_t acc = 0;
+    for (int t = 0; t < n; t++) {
+        b[o + t] = (uint32_t)acc;
+        acc += N[t];
+        uint32_t *q = b.data() + o + n + (size_t)t * n;
+        for (int u = t + 1; u < n; u++) {
+            if (parent[u] == t) q[u] = (uint32_t)M[u];
+            else if (q[parent[u]]) q[u] = (uint32_t)((uint64_t)q[parent[u]] * M[u]);
+        }
+    }
+    while (b.size() & 3) b.push_back(0u);
+    h.o_rank = o;
+    h.words = (int)b.size();
+}
+
+void prepare_memo(DPInput &in, int requested, int max_memo_mb) {
+    in.memo = resolve_memo(in, requested, max_memo_mb);
+    if (in.memo == GQ_MEMO_RANK) {
+        uint64_t total = 0;
+        for (size_t k = 1; k < in.tree_counts.size(); k++) total += in.tree_counts[k];
+        /* slots and level-list offsets are 32-bit */
+        if (total >= (1ull << 31)) in.memo = GQ_MEMO_HASH;
+        else if (!in.hdr.o_rank) attach_rank_tables(in);
+    }
 }
 
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
@@ -2880,6 +3114,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * n_iters relations */
     const int K = (n_iters >= 2 && n_iters < n) ? n_iters : n;
     if (in.memo == GQ_MEMO_HASH && algo != DP_DPSIZE) return run_dp_hash(c, in, algo, K, profile, tree, st);
+    if (in.memo == GQ_MEMO_RANK && algo != DP_DPSIZE) return run_dp_rank(c, in, algo, K, profile, tree, st);
     const size_t blob_bytes = (size_t)h.words * 4;
     const size_t lds_eval = ((blob_bytes + 15) & ~(size_t)15);
     const uint32_t conn_words_all = (uint32_t)std::max<uint64_t>(1, (1ull << n) / 32);
@@ -3254,20 +3489,24 @@ struct EJob {
     uint64_t cap;                        /* open-addressing capacity */
     std::vector<uint64_t> counts;        /* hash: connected sets per size */
     std::vector<uint32_t> blob;          /* query blob, copied to the device by the DP's first step */
+    int memo_kind = GQ_MEMO_DENSE;
 };
 
 namespace {
-/* open-addressing slice: [result][blob][keys][vals][left][lvl_sets][lvl_slot] */
-size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, uint64_t *cap, size_t *o_blob,
+/* open-addressing slice: [result][blob][keys][vals][left][lvl_sets][lvl_slot];
+ * rank slice (rank = true): [result][blob][vals][left][lvl_sets], one memo
+ * slot per connected set */
+size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, bool rank, uint64_t *cap, size_t *o_blob,
                          size_t *o_keys, size_t *o_vals, size_t *o_left, size_t *o_sets, size_t *o_slot) {
     uint64_t c = 64;
     while (c < 2 * total_sets) c <<= 1;
+    if (rank) c = std::max<uint64_t>(total_sets, 1);
     *cap = c;
     size_t o = al256(RES_WORDS * 4);
     *o_blob = o;
     o += al256((size_t)words * 4);
     *o_keys = o;
-    o += al256(c * 4);
+    if (!rank) o += al256(c * 4);
     *o_vals = o;
     o += al256(c * 16);
     *o_left = o;
@@ -3275,7 +3514,7 @@ size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, uint64_t
     *o_sets = o;
     o += al256(std::max<uint64_t>(list, 1) * 4);
     *o_slot = o;
-    o += al256(std::max<uint64_t>(list, 1) * 4);
+    if (!rank) o += al256(std::max<uint64_t>(list, 1) * 4);
     return o;
 }
 }  // namespace
@@ -3283,7 +3522,13 @@ size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, uint64_t
 uint64_t engine_hash_slice_bytes(uint64_t total_sets) {
     uint64_t cap;
     size_t a, b, c, d, e, f;
-    return hash_slice_layout(total_sets, total_sets, (int)(ENGINE_BLOB_MAX / 4), &cap, &a, &b, &c, &d, &e, &f);
+    return hash_slice_layout(total_sets, total_sets, (int)(ENGINE_BLOB_MAX / 4), false, &cap, &a, &b, &c, &d, &e, &f);
+}
+
+uint64_t engine_rank_slice_bytes(uint64_t total_sets) {
+    uint64_t cap;
+    size_t a, b, c, d, e, f;
+    return hash_slice_layout(total_sets, total_sets, (int)(ENGINE_BLOB_MAX / 4), true, &cap, &a, &b, &c, &d, &e, &f);
 }
 
 /* One step's host side.  Steps alternate between two slots: the engine
@@ -3446,6 +3691,8 @@ int engine_active(const StepEngine *e) {
 static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
+    const bool rank = in.memo == GQ_MEMO_RANK;
+    if (rank && !h.o_rank) return false;
     uint64_t total = 0, list = 0;
     for (int k = 1; k <= n; k++) {
         total += in.tree_counts[k];
@@ -3454,8 +3701,8 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     if (2 * total > (1ull << 31)) return false;
     uint64_t cap;
     size_t o_blob, o_keys, o_vals, o_left, o_sets, o_slot;
-    const size_t bytes = hash_slice_layout(total, list, h.words, &cap, &o_blob, &o_keys, &o_vals, &o_left, &o_sets,
-                                           &o_slot);
+    const size_t bytes = hash_slice_layout(total, list, h.words, rank, &cap, &o_blob, &o_keys, &o_vals, &o_left,
+                                           &o_sets, &o_slot);
     size_t off;
     if (!e->alloc(bytes, &off)) return false;
     char *base = e->arena + off;
@@ -3469,18 +3716,19 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     j->words = h.words;
     j->blob.assign(in.blob.begin(), in.blob.begin() + h.words);
     j->hash = true;
-    j->cap = cap;
+    j->memo_kind = rank ? GQ_MEMO_RANK : GQ_MEMO_HASH;
+    j->cap = rank ? 0 : cap;
     j->counts = in.tree_counts;
     BatchJob &J = j->J;
     memset(&J, 0, sizeof(J));
     J.h = h;
     J.blob = (const uint32_t *)(base + o_blob);
-    J.keys = (uint32_t *)(base + o_keys);
-    J.hmask = (uint32_t)(cap - 1);
+    J.keys = rank ? nullptr : (uint32_t *)(base + o_keys);
+    J.hmask = rank ? 0u : (uint32_t)(cap - 1);
     J.vals = (float4 *)(base + o_vals);
     J.left = (uint32_t *)(base + o_left);
     J.lvl_sets = (uint32_t *)(base + o_sets);
-    J.lvl_slot = (uint32_t *)(base + o_slot);
+    J.lvl_slot = rank ? nullptr : (uint32_t *)(base + o_slot);
     J.counters = (uint32_t *)base;
     J.nodes = (DPNode *)(J.counters + 64);
     uint64_t acc = 0;
@@ -3500,7 +3748,8 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     const int n = h.n_rels;
     if (!fused_eligible(in, algo) || (size_t)h.words * 4 > ENGINE_BLOB_MAX || in.blob.size() < (size_t)h.words)
         return false;
-    const bool hash = in.memo == GQ_MEMO_HASH && h.is_tree && (int)in.tree_counts.size() == n + 1;
+    const bool hash = (in.memo == GQ_MEMO_HASH || in.memo == GQ_MEMO_RANK) && h.is_tree &&
+                      (int)in.tree_counts.size() == n + 1;
     if (hash) return engine_admit_hash(e, in, algo, key);
     size_t o_blob, o_vals, o_left, o_conn, o_sets, o_best, o_first;
     const size_t bytes = slice_layout(n, h.words, &o_blob, &o_vals, &o_left, &o_conn, &o_sets, &o_best, &o_first);
@@ -3573,15 +3822,12 @@ int engine_enqueue(StepEngine *e, int nn) {
     const int j_new = na - nn;
     std::vector<uint32_t> fin_idx;
     size_t blob_words = 0;
-    bool any_hash = false, new_hash = false;
+    bool any_hash = false;
     for (int i = 0; i < na; i++) {
         const EJob *j = e->active[i];
         if (j->cur_k >= j->n) fin_idx.push_back((uint32_t)i);
         any_hash = any_hash || j->hash;
-        if (i >= j_new) {
-            blob_words += j->blob.size();
-            new_hash = new_hash || j->hash;
-        }
+        if (i >= j_new) blob_words += j->blob.size();
     }
     const int nf = (int)fin_idx.size();
     const size_t o_exp = al64(sizeof(BatchJob) * (size_t)na);
@@ -3667,13 +3913,11 @@ int engine_enqueue(StepEngine *e, int nn) {
     ECHK(hipMemcpyAsync(D, P, pkt_bytes, hipMemcpyHostToDevice, s));
     /* 1. DPs admitted since the last step: keys, base relations, filter */
     if (nn) {
-        if (new_hash) {
-            uint64_t keys = 0;
-            for (int i = j_new; i < na; i++)
-                if (e->active[i]->hash) keys += e->active[i]->cap;
+        uint64_t keys = 0;
+        for (int i = j_new; i < na; i++) keys += e->active[i]->cap;   /* 0 unless open addressing */
+        if (keys)
             hipLaunchKernelGGL(k_clear_keys, dim3((uint32_t)std::min<uint64_t>((keys + BLOCK - 1) / BLOCK, 4096)),
                                dim3(BLOCK), 0, s, d_jobs + j_new, nn);
-        }
         hipLaunchKernelGGL(k_batch_init, dim3(std::min(nn, 1024)), dim3(64), 0, s, d_jobs + j_new, nn,
                            (const uint32_t *)(D + o_blob), (const uint32_t *)(D + o_bsrc));
         const uint32_t fgrid = std::min<uint32_t>(pchunk[nn], 4096);
@@ -3767,7 +4011,7 @@ int engine_collect(StepEngine *e, int si, std::vector<EngineDone> &done) {
             d.st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
             if (j->J.h.is_tree && (int)j->counts.size() == n + 1) d.st.pairs = tree_pairs(j->counts);
             d.st.memo_bytes = j->bytes;
-            d.st.memo_kind = j->hash ? GQ_MEMO_HASH : GQ_MEMO_DENSE;
+            d.st.memo_kind = j->memo_kind;
             d.st.gpu_ms = ms;
         }
         delete j;

@@ -290,7 +290,7 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
         }
         int rc = build_dp_input(*hs[i], ins[i]);
         if (rc) return rc;
-        if (algo != DP_DPSIZE) ins[i].memo = resolve_memo(ins[i], ctx.memo_kind, ctx.max_memo_mb);
+        if (algo != DP_DPSIZE) prepare_memo(ins[i], ctx.memo_kind, ctx.max_memo_mb);
         const bool to_batch = ex ? true : (hs.size() > 1 && ins[i].memo == GQ_MEMO_DENSE);
         if (to_batch && hs[i]->n_iters == hs[i]->n && fused_eligible(ins[i], algo)) {
             batch.push_back(&ins[i]);

@@ -365,6 +365,7 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     /* per-edge split table of tree graphs (see BlobHdr::tree_fast) */
     h.tree_fast = 0;
     h.o_tedge = 0;
+    h.o_rank = 0;
     if (h.is_tree && h.tree_ecm) {
         bool fits = true;
         for (int v = 1; v < n && fits; v++) {

@@ -337,13 +337,15 @@ static uint64_t projected_bytes(const gq_query *q, const gq_config *c) {
         const int nd = std::min(n, GQ_MAX_DP_RELS - 2);
         const uint64_t dense = engine_slice_bytes(nd);
         const uint64_t sets = tree_set_count(q);
-        const bool hash = sets && c->gpuqo_algorithm == GQ_ALGO_DPSUB &&
-                          (c->memo_kind == GQ_MEMO_HASH ||
-                           (c->memo_kind == GQ_MEMO_AUTO && 36.0 * (double)(1ull << nd) > max_memo));
-        return hash ? engine_hash_slice_bytes(sets) : dense;
+        const bool tree = sets && c->gpuqo_algorithm == GQ_ALGO_DPSUB;
+        if (tree && c->memo_kind == GQ_MEMO_HASH) return engine_hash_slice_bytes(sets);
+        if (tree && (c->memo_kind == GQ_MEMO_RANK ||
+                     (c->memo_kind == GQ_MEMO_AUTO && 36.0 * (double)(1ull << nd) > max_memo)))
+            return engine_rank_slice_bytes(sets);
+        return dense;
     }
     /* IDP2 / UnionDP: DPs of at most k relations (dense up to the memo
-     * limit, open-addressing above it), plus a level of small UnionDP
+     * limit, rank or open-addressing above it), plus a level of small UnionDP
      * partitions (fewer than 16 relations each) batched together */
     const uint64_t dp = std::min<uint64_t>(engine_slice_bytes(std::min(k, GQ_MAX_DP_RELS - 2)),
                                            (uint64_t)max_memo + (64ull << 20));

@@ -27,7 +27,7 @@ ALGO = {
     "cpu_goo": 11,
 }
 IDP = {"idp1": 0, "idp2": 1, "union": 2}
-MEMO = {"auto": 0, "dense": 1, "hash": 2}
+MEMO = {"auto": 0, "dense": 1, "hash": 2, "rank": 3}
 
 
 class PathCost(C.Structure):

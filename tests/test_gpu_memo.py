@@ -1,5 +1,7 @@
-"""The open-addressing memo (tree graphs): the same plans as the dense memo
-and the oracle, with level lists generated by expansion."""
+"""The open-addressing and rank memos (tree graphs): the same plans as the
+dense memo and the oracle, with level lists generated by expansion.  The rank
+memo gives every connected set its own slot, computed from the set; slots
+that collided or were skipped would break the plan or the set counts."""
 import pytest
 
 from helpers import assert_same_plan, cfg, gpu_bits, oracle_bits
@@ -7,7 +9,7 @@ from gpuqo_amd import qgen
 
 pytestmark = pytest.mark.gpu
 
-GQ_MEMO_DENSE, GQ_MEMO_HASH = 1, 2
+GQ_MEMO_DENSE, GQ_MEMO_HASH, GQ_MEMO_RANK = 1, 2, 3
 
 
 def _trees():
@@ -17,23 +19,43 @@ def _trees():
             ("snow18-d4", qgen.snowflake(18, 44, depth=4, fanout=5), 1)]
 
 
+@pytest.mark.parametrize("kind", [GQ_MEMO_HASH, GQ_MEMO_RANK], ids=["hash", "rank"])
 @pytest.mark.parametrize("name,g,bicc", _trees(), ids=lambda x: x if isinstance(x, str) else "")
-def test_hash_memo_matches_oracle(name, g, bicc):
-    c = cfg("dpsub", bicc, memo_kind=GQ_MEMO_HASH)
+def test_tree_memo_matches_oracle(name, g, bicc, kind):
+    c = cfg("dpsub", bicc, memo_kind=kind)
     want, ost = oracle_bits(g, c)
     got, gst = gpu_bits(g, c)
     assert_same_plan(got, want, name)
-    assert gst.memo_kind == GQ_MEMO_HASH
+    assert gst.memo_kind == kind
     assert gst.pairs == ost.pairs
     assert gst.sets == ost.sets
+
+
+def test_rank_memo_on_stars_and_chains():
+    """Extreme shapes of the slot numbering: a star (2^(n-1) sets with the
+    centre as lowest member), a chain (n(n+1)/2 sets), and a star whose
+    centre is not relation 0."""
+    g3 = qgen.QueryGraph(9, "star-off-centre")
+    for i in range(9):
+        g3.rows[i] = qgen.rows_of(1e3 * (i + 1))
+    for i in range(9):
+        if i != 4:
+            g3.add_generic(4, i, 1e-3 * (i + 1))
+    g3.build()
+    for g in (qgen.star(16, 0x516), qgen.chain(24, 0x524), g3):
+        c = cfg("dpsub", 1, memo_kind=GQ_MEMO_RANK)
+        want, ost = oracle_bits(g, c)
+        got, gst = gpu_bits(g, c)
+        assert_same_plan(got, want, g.shape)
+        assert gst.sets == ost.sets
 
 
 def test_auto_memo_switches_above_the_memo_limit():
     g = qgen.snowflake(22, 0x22)
     want, _ = oracle_bits(g, cfg("dpsub", 1))
     got, gst = gpu_bits(g, cfg("dpsub", 1, max_memo_size_mb=16))      # dense: 36 B x 2^22 = 151 MB
-    assert gst.memo_kind == GQ_MEMO_HASH
-    assert_same_plan(got, want, "auto -> open addressing")
+    assert gst.memo_kind == GQ_MEMO_RANK
+    assert_same_plan(got, want, "auto -> rank")
     got2, gst2 = gpu_bits(g, cfg("dpsub", 1))                           # default limit 1024 MB
     assert gst2.memo_kind == GQ_MEMO_DENSE
     assert_same_plan(got2, want, "auto -> dense")
@@ -48,18 +70,25 @@ def test_hash_request_on_a_cycle_keeps_the_dense_memo():
     assert_same_plan(got, want, "cycle10")
 
 
-def test_hash_equals_dense_at_30_relations():
+def test_hash_and_rank_equal_dense_at_30_relations():
     g = qgen.snowflake(30, 0x30)
     dense, dst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_DENSE))
     hashed, hst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_HASH))
-    assert_same_plan(hashed, dense, "snowflake-30")
+    ranked, rst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_RANK))
+    assert_same_plan(hashed, dense, "snowflake-30 hash")
+    assert_same_plan(ranked, dense, "snowflake-30 rank")
     assert hst.pairs == dst.pairs and hst.sets == dst.sets
+    assert rst.pairs == dst.pairs and rst.sets == dst.sets
     assert hst.memo_bytes * 50 < dst.memo_bytes          # 2^30 dense slots vs ~4M connected sets
+    assert rst.memo_bytes < hst.memo_bytes                # no keys, one slot per connected set
 
 
-def test_server_with_open_addressing_requests():
+@pytest.mark.parametrize("kind", [0, GQ_MEMO_HASH], ids=["auto", "hash"])
+def test_server_with_tree_memo_requests(kind):
+    """Requests above the dense limit share engine steps with dense ones
+    (auto: rank memo; hash: open addressing)."""
     import gpuqo_amd
-    direct = cfg("dpsub", 1)
+    direct = cfg("dpsub", 1, memo_kind=kind)
     reqs = [qgen.snowflake(26, 0x26), qgen.snowflake(12, 0x12), qgen.snowflake(24, 0x24), qgen.star(14, 0x14),
             qgen.snowflake(28, 0x28, depth=4, fanout=5), qgen.snowflake(10, 0x10)]
     with gpuqo_amd.Server(direct, 2 << 30, max_inflight=4) as srv:

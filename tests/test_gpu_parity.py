@@ -229,11 +229,12 @@ def test_idp1_ties(algo, bicc):
         assert_same_plan(got, want, f"idp1 ties {name} {algo} bicc={bicc}")
 
 
-def test_idp1_open_addressing_memo():
-    """IDP1 on a tree DP that takes the open-addressing memo."""
+@pytest.mark.parametrize("kind", [2, 3], ids=["hash", "rank"])
+def test_idp1_tree_memos(kind):
+    """IDP1 on a tree DP with the open-addressing (2) or rank (3) memo."""
     g = qgen.snowflake(22, 0x22)
-    c = cfg("dpsub", 1, idp_type="idp1", idp_n_iters=8, max_memo_size_mb=16)
+    c = cfg("dpsub", 1, idp_type="idp1", idp_n_iters=8, max_memo_size_mb=16, memo_kind=kind)
     want, ost = oracle_bits(g, c)
     got, gst = gpu_bits(g, c)
-    assert_same_plan(got, want, "idp1 snowflake-22 open addressing")
+    assert_same_plan(got, want, f"idp1 snowflake-22 memo {kind}")
     assert gst.pairs == ost.pairs
