@@ -1582,6 +1582,13 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj, const uint32_t *blob_
     }
 }
 
+/* Copies n 16-byte words from host memory that the device can read (a
+ * step's pinned packet) into device memory: the step stays on the compute
+ * queue, with no copy-engine hand-off before its first kernel. */
+__global__ void __launch_bounds__(BLOCK) k_stage(const uint4 *src, uint4 *dst, uint32_t n) {
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) dst[i] = src[i];
+}
+
 /* Clears the open-addressing keys of every job that has them. */
 __global__ void __launch_bounds__(BLOCK) k_clear_keys(const BatchJob *jobs, int nj) {
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -3537,8 +3544,10 @@ uint64_t engine_rank_slice_bytes(uint64_t total_sets) {
  * A slot's pinned packet and result buffer are rewritten only after its
  * previous step has completed. */
 struct StepSlot {
-    uint8_t *h_pkt = nullptr;                    /* pinned: the step's single host-to-device packet */
-    uint32_t *h_res = nullptr;                   /* pinned: RES_WORDS per finishing DP */
+    /* pinned, coherent host memory that the step's kernels read and write
+     * directly: no copy engine between the host and the step */
+    uint8_t *h_pkt = nullptr;                    /* the step's packet, staged by its first kernel */
+    uint32_t *h_res = nullptr;                   /* RES_WORDS per finishing DP, written by the walk */
     size_t cap_h_pkt = 0, cap_h_res = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<EJob *> fin;                     /* DPs walked in this step, read back at its completion */
@@ -3554,9 +3563,9 @@ struct StepEngine {
     std::map<size_t, size_t> free_;              /* offset -> size, coalesced */
     std::vector<EJob *> active, incoming;
     uint8_t *d_pkt = nullptr;                    /* device copy of the current step's packet */
-    uint32_t *d_res = nullptr, *d_tp = nullptr, *big_prefix = nullptr, *big_hdr = nullptr;
+    uint32_t *d_tp = nullptr, *big_prefix = nullptr, *big_hdr = nullptr;
     BigEnt *big = nullptr;
-    size_t cap_pkt = 0, cap_res = 0, cap_tp = 0, cap_big = 0, cap_bigp = 0, cap_hdr = 0;
+    size_t cap_pkt = 0, cap_tp = 0, cap_big = 0, cap_bigp = 0, cap_hdr = 0;
     StepSlot slot[2];
     int cur = 0;                                 /* slot of the next step */
 
@@ -3591,13 +3600,13 @@ struct StepEngine {
 };
 
 template <class T>
-static int ensure_pinned(T **p, size_t &cap, size_t need) {
+static int ensure_pinned(T **p, size_t &cap, size_t need, unsigned flags = hipHostMallocDefault) {
     if (*p && cap >= need) return GQ_OK;
     if (*p) (void)hipHostFree(*p);
     *p = nullptr;
+    const size_t n = std::max<size_t>(std::max<size_t>(need, 16), 2 * cap);
     cap = 0;
-    size_t n = std::max<size_t>(need, 16);
-    if (hipHostMalloc((void **)p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)p, n * sizeof(T), flags) != hipSuccess) {
         *p = nullptr;
         set_error("pinned allocation of %zu bytes failed", n * sizeof(T));
         return GQ_ERR_NOMEM;
@@ -3670,7 +3679,6 @@ void engine_destroy(StepEngine *e) {
     }
     if (e->arena) (void)hipFree(e->arena);
     if (e->d_pkt) (void)hipFree(e->d_pkt);
-    if (e->d_res) (void)hipFree(e->d_res);
     if (e->d_tp) (void)hipFree(e->d_tp);
     if (e->big) (void)hipFree(e->big);
     if (e->big_prefix) (void)hipFree(e->big_prefix);
@@ -3810,8 +3818,8 @@ size_t al64(size_t x) { return (x + 63) & ~(size_t)63; }
  *   jobs[na] | expansion prefix[na+1] | tree task prefix[na+1] |
  *   filter chunk prefix[nn+1] | blob offsets[nn] | finishing job indexes[nf] |
  *   blobs of the new DPs
- * and the step starts with its single copy to the device; it ends with one
- * copy of the finishing DPs' results into the slot.  The arena slices of the
+ * and the step starts with k_stage copying it to the device; the walk of
+ * the finishing DPs writes their results straight into the slot.  The arena slices of the
  * finishing DPs are released at once: whatever the next steps place there
  * runs after this step's walk on the same stream. */
 int engine_enqueue(StepEngine *e, int nn) {
@@ -3836,9 +3844,11 @@ int engine_enqueue(StepEngine *e, int nn) {
     const size_t o_bsrc = al64(o_chunk + 4 * ((size_t)nn + 1));
     const size_t o_fin = al64(o_bsrc + 4 * (size_t)nn);
     const size_t o_blob = al64(o_fin + 4 * (size_t)nf);
-    const size_t pkt_bytes = o_blob + 4 * blob_words;
-    if ((rc = ensure_pinned(&S.h_pkt, S.cap_h_pkt, pkt_bytes))) return rc;
-    if ((rc = ensure_pinned(&S.h_res, S.cap_h_res, std::max<size_t>((size_t)nf, 1) * RES_WORDS))) return rc;
+    const size_t pkt_bytes = (o_blob + 4 * blob_words + 15) & ~(size_t)15;
+    const unsigned host_flags = hipHostMallocMapped | hipHostMallocCoherent;
+    if ((rc = ensure_pinned(&S.h_pkt, S.cap_h_pkt, pkt_bytes, host_flags))) return rc;
+    if ((rc = ensure_pinned(&S.h_res, S.cap_h_res, std::max<size_t>((size_t)nf, 1) * RES_WORDS, host_flags)))
+        return rc;
     uint8_t *P = S.h_pkt;
     BatchJob *pj = (BatchJob *)P;
     uint32_t *pexp = (uint32_t *)(P + o_exp), *pttp = (uint32_t *)(P + o_ttp), *pchunk = (uint32_t *)(P + o_chunk),
@@ -3895,13 +3905,11 @@ int engine_enqueue(StepEngine *e, int nn) {
     /* device buffers grow only with the previous step drained (a freed
      * buffer may still be in use by it) */
     const bool grow = e->cap_pkt < pkt_bytes || e->cap_tp < 2 * ((size_t)na + 1) ||
-                      e->cap_res < (size_t)std::max(nf, 1) * RES_WORDS ||
                       (big_cap && (e->cap_big < big_cap || e->cap_bigp < big_cap + 1));
     if (grow) {
         ECHK(hipStreamSynchronize(s));
         if ((rc = ensure(&e->d_pkt, e->cap_pkt, pkt_bytes))) return rc;
         if ((rc = ensure(&e->d_tp, e->cap_tp, 2 * ((size_t)na + 1)))) return rc;
-        if ((rc = ensure(&e->d_res, e->cap_res, (size_t)std::max(nf, 1) * RES_WORDS))) return rc;
         if (big_cap) {
             if ((rc = ensure(&e->big, e->cap_big, big_cap))) return rc;
             if ((rc = ensure(&e->big_prefix, e->cap_bigp, big_cap + 1))) return rc;
@@ -3910,7 +3918,11 @@ int engine_enqueue(StepEngine *e, int nn) {
     uint8_t *D = e->d_pkt;
     const BatchJob *d_jobs = (const BatchJob *)D;
     ECHK(hipEventRecord(S.e0, s));
-    ECHK(hipMemcpyAsync(D, P, pkt_bytes, hipMemcpyHostToDevice, s));
+    {
+        const uint32_t n16 = (uint32_t)(pkt_bytes / 16);
+        hipLaunchKernelGGL(k_stage, dim3(std::max<uint32_t>(1, std::min<uint32_t>((n16 + BLOCK - 1) / BLOCK, 64))),
+                           dim3(BLOCK), 0, s, (const uint4 *)P, (uint4 *)D, n16);
+    }
     /* 1. DPs admitted since the last step: keys, base relations, filter */
     if (nn) {
         uint64_t keys = 0;
@@ -3955,11 +3967,9 @@ int engine_enqueue(StepEngine *e, int nn) {
                            s, d_jobs, na, 0, (const uint32_t *)(D + o_ttp));
     if (ub) ECHK(hipStreamWaitEvent(s, e->join, 0));
     /* 4. DPs past their last level: plan walk, one copy of the results */
-    if (nf) {
+    if (nf)
         hipLaunchKernelGGL(k_batch_tree, dim3(nf), dim3(WAVE), 0, s, d_jobs, nf, (const uint32_t *)(D + o_fin),
-                           e->d_res);
-        ECHK(hipMemcpyAsync(S.h_res, e->d_res, (size_t)nf * RES_WORDS * 4, hipMemcpyDeviceToHost, s));
-    }
+                           S.h_res);
     ECHK(hipGetLastError());
     ECHK(hipEventRecord(S.e1, s));
     std::vector<EJob *> keep;
