@@ -2325,12 +2325,12 @@ struct DeviceCtx {
     int *l1 = nullptr;
     uint32_t *lvl_off = nullptr;
     size_t cap_lvl_off = 0;
-    /* batches */
+    /* batches: the packet (jobs, filter chunk prefix, blobs) staged from
+     * coherent pinned memory by k_stage, results written back to it */
     char *ws = nullptr;
-    uint32_t *bblob = nullptr;
-    BatchJob *jobs = nullptr;
-    uint32_t *results = nullptr, *bprefix = nullptr;
-    size_t cap_ws = 0, cap_bblob = 0, cap_jobs = 0, cap_results = 0, cap_bprefix = 0;
+    uint8_t *bpkt = nullptr, *h_bpkt = nullptr;
+    uint32_t *results = nullptr, *bprefix = nullptr, *h_bres = nullptr;
+    size_t cap_ws = 0, cap_bpkt = 0, cap_h_bpkt = 0, cap_h_bres = 0, cap_results = 0, cap_bprefix = 0;
     /* big MPDP sets (both paths) and the single-query job descriptor */
     BigEnt *big_ents = nullptr;
     uint32_t *big_prefix = nullptr;
@@ -2393,21 +2393,47 @@ static int ctx_init(DeviceCtx &c, int device) {
     int rc = ensure(&c.b.counters, c.cap_counters, RES_WORDS);
     if (rc) return rc;
     c.b.nodes = (DPNode *)(c.b.counters + 64);
-    HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocDefault));
+    /* coherent: the plan walk writes the results straight into it */
+    HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocMapped | hipHostMallocCoherent));
     return GQ_OK;
 }
 
 /* pinned staging buffer of at least `words` words */
-static int ensure_stage(DeviceCtx &c, size_t words) {
-    if (c.h_stage && c.cap_h_stage >= words) return GQ_OK;
-    if (c.h_stage) (void)hipHostFree(c.h_stage);
-    c.h_stage = nullptr;
-    c.cap_h_stage = 0;
-    if (hipHostMalloc((void **)&c.h_stage, words * 4, hipHostMallocDefault) != hipSuccess) {
-        set_error("pinned staging allocation of %zu bytes failed", words * 4);
+/* Pinned host memory that kernels read and write directly: coherent, so
+ * that no GPU cache keeps a line of it between two uses. */
+template <class T>
+static int ensure_host(T **p, size_t &cap, size_t need) {
+    if (*p && cap >= need) return GQ_OK;
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    const size_t n = std::max<size_t>(std::max<size_t>(need, 16), 2 * cap);
+    cap = 0;
+    if (hipHostMalloc((void **)p, n * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        *p = nullptr;
+        set_error("pinned allocation of %zu bytes failed", n * sizeof(T));
         return GQ_ERR_NOMEM;
     }
-    c.cap_h_stage = words;
+    cap = n;
+    return GQ_OK;
+}
+
+static int ensure_stage(DeviceCtx &c, size_t words) {
+    return ensure_host(&c.h_stage, c.cap_h_stage, words);
+}
+
+/* The staged words of the single-query path (query blob, level offsets, ...)
+ * from c.h_stage to `dst` in one copy.  (A lone DP pays one copy-engine
+ * hand-off at each end; measured, that is shorter than a staging kernel
+ * reading host memory.) */
+static int stage_up(DeviceCtx &c, hipStream_t s, uint32_t *dst, size_t words) {
+    HIPCHK(hipMemcpyAsync(dst, c.h_stage, words * 4, hipMemcpyHostToDevice, s));
+    return GQ_OK;
+}
+
+/* The DP's result (64 counters, then `nodes` plan nodes) into c.h_res. */
+static int results_down(DeviceCtx &c, hipStream_t s, int nodes) {
+    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (size_t)nodes * (sizeof(DPNode) / 4)) * 4,
+                          hipMemcpyDeviceToHost, s));
     return GQ_OK;
 }
 
@@ -2438,9 +2464,9 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     if (nj == 0) return GQ_OK;
     const Tuning tn = g_tune;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    std::vector<BatchJob> jobs(nj);
-    std::vector<uint32_t> blobs, chunk_prefix(nj + 1, 0);
+    std::vector<uint32_t> chunk_prefix(nj + 1, 0);
     std::vector<size_t> ws_off(nj), blob_off(nj);
+    size_t blob_words = 0;
     size_t ws = 0, lds = 0, lds_tree = 0, big_lds_max = 0;
     uint64_t big_cap = 0;
     int kmax = 2, big_kmax = 0;
@@ -2454,9 +2480,8 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         ws_off[j] = ws;
         ws += al(ns * 16) + al(ns * 4) + al(std::max<uint64_t>(1, ns / 32) * 4) + al(ns * 4) + al(ns * 8) +
               al(ns * 4);
-        blob_off[j] = blobs.size();
-        blobs.insert(blobs.end(), ins[j]->blob.begin(), ins[j]->blob.end());
-        while (blobs.size() & 63) blobs.push_back(0u);
+        blob_off[j] = blob_words;
+        blob_words += (ins[j]->blob.size() + 63) & ~(size_t)63;
         if (fast_tree_job(h) && (int)ins[j]->tree_counts.size() != h.n_rels + 1) {
             set_error("tree job %d without level sizes", j);
             return GQ_ERR_INVALID;
@@ -2473,10 +2498,23 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         chunk_prefix[j + 1] = chunk_prefix[j] + (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK);
         kmax = std::max(kmax, h.n_rels);
     }
+    /* packet: jobs | chunk prefix | blobs (256-byte aligned parts) */
+    const size_t o_pref = al(sizeof(BatchJob) * (size_t)nj), o_blob = al(o_pref + 4 * ((size_t)nj + 1));
+    const size_t pkt_bytes = al(o_blob + 4 * blob_words);
     if ((rc = ensure(&c.ws, c.cap_ws, ws))) return rc;
-    if ((rc = ensure(&c.bblob, c.cap_bblob, blobs.size()))) return rc;
-    if ((rc = ensure(&c.jobs, c.cap_jobs, (size_t)nj))) return rc;
+    if ((rc = ensure(&c.bpkt, c.cap_bpkt, pkt_bytes))) return rc;
     if ((rc = ensure(&c.results, c.cap_results, (size_t)nj * RES_WORDS))) return rc;
+    if ((rc = ensure_host(&c.h_bpkt, c.cap_h_bpkt, pkt_bytes))) return rc;
+    if ((rc = ensure_host(&c.h_bres, c.cap_h_bres, (size_t)nj * RES_WORDS))) return rc;
+    BatchJob *jobs = (BatchJob *)c.h_bpkt;
+    memcpy(c.h_bpkt + o_pref, chunk_prefix.data(), 4 * ((size_t)nj + 1));
+    for (int j = 0; j < nj; j++) {
+        uint32_t *dst = (uint32_t *)(c.h_bpkt + o_blob) + blob_off[j];
+        const std::vector<uint32_t> &b = ins[j]->blob;
+        memcpy(dst, b.data(), 4 * b.size());
+        memset(dst + b.size(), 0, 4 * (((b.size() + 63) & ~(size_t)63) - b.size()));
+    }
+    BatchJob *const d_jobs = (BatchJob *)c.bpkt;
     if ((rc = ensure(&c.bprefix, c.cap_bprefix, (size_t)(nj + 1) * 69))) return rc;
     if (big_cap > 0xffffffffull) {
         set_error("batch too large for the big-set list");
@@ -2490,8 +2528,9 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         const BlobHdr &h = ins[j]->hdr;
         const uint64_t ns = 1ull << h.n_rels;
         BatchJob &J = jobs[j];
+        memset(&J, 0, sizeof(J));
         J.h = h;
-        J.blob = c.bblob + blob_off[j];
+        J.blob = (const uint32_t *)(c.bpkt + o_blob) + blob_off[j];
         char *p = c.ws + ws_off[j];
         J.vals = (float4 *)p;
         p += al(ns * 16);
@@ -2516,20 +2555,22 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         J.tree_iters = tn.tree_iters;
     }
     hipStream_t s = c.stream;
-    HIPCHK(hipMemcpyAsync(c.bblob, blobs.data(), blobs.size() * 4, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c.jobs, jobs.data(), sizeof(BatchJob) * nj, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(c.bprefix, chunk_prefix.data(), sizeof(uint32_t) * (nj + 1), hipMemcpyHostToDevice, s));
+    {
+        const uint32_t n16 = (uint32_t)(pkt_bytes / 16);
+        hipLaunchKernelGGL(k_stage, dim3(std::max<uint32_t>(1, std::min<uint32_t>((n16 + BLOCK - 1) / BLOCK, 512))),
+                           dim3(BLOCK), 0, s, (const uint4 *)c.h_bpkt, (uint4 *)c.bpkt, n16);
+    }
     HIPCHK(hipEventRecord(c.ev0, s));
-    hipLaunchKernelGGL(k_batch_init, dim3(std::min(nj, 1024)), dim3(64), 0, s, c.jobs, nj, nullptr, nullptr);
+    hipLaunchKernelGGL(k_batch_init, dim3(std::min(nj, 1024)), dim3(64), 0, s, d_jobs, nj, nullptr, nullptr);
     const uint32_t fgrid = std::min<uint32_t>(chunk_prefix[nj], 4096);
-    hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, c.jobs, nj,
-                       c.bprefix);
+    hipLaunchKernelGGL(k_batch_filter, dim3(std::max<uint32_t>(fgrid, 1)), dim3(BLOCK), 0, s, d_jobs, nj,
+                       (const uint32_t *)(c.bpkt + o_pref));
     /* task prefixes [33][nj+1] after the chunk prefix: tree jobs, then the
      * others.  Tree jobs run in k_batch_eval_tree on stream s, the others in
      * k_batch_eval on stream2; level k on either waits for level k-1 on both. */
     uint32_t *tp_tree = c.bprefix + (nj + 1), *tp_gen = c.bprefix + (size_t)(nj + 1) * 35;
-    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_tree, JOBS_TREE);
-    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, c.jobs, nj, tp_gen, JOBS_GENERAL);
+    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, d_jobs, nj, tp_tree, JOBS_TREE);
+    hipLaunchKernelGGL(k_batch_plan, dim3(kmax - 1), dim3(1024), 0, s, d_jobs, nj, tp_gen, JOBS_GENERAL);
     const size_t lds_tree_total = std::max<size_t>(lds_tree, 16);
     const uint32_t tpl_gen = (lds + 4 * (size_t)(nj + 1) <= 64 * 1024) ? (uint32_t)(lds / 4) : 0u;
     const size_t lds_gen_total = tpl_gen ? lds + 4 * (size_t)(nj + 1) : lds;
@@ -2556,7 +2597,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
                 set_error("too many tasks in one level");
                 return GQ_ERR_UNSUPPORTED;
             }
-            hipLaunchKernelGGL(k_batch_eval_tree, dim3((uint32_t)tasks_tree), dim3(BLOCK), lds_tree_total, s, c.jobs,
+            hipLaunchKernelGGL(k_batch_eval_tree, dim3((uint32_t)tasks_tree), dim3(BLOCK), lds_tree_total, s, d_jobs,
                                nj, k, (const uint32_t *)(tp_tree + (size_t)k * (nj + 1)));
         }
         if (ub_gen) {
@@ -2564,13 +2605,13 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
              * only queue behind the tree kernel running beside it, and the
              * launch ends when the last has been dispatched */
             const uint32_t grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(ub_gen, 1), tn.gen_grid_max);
-            hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, c.jobs, nj, k,
+            hipLaunchKernelGGL(k_batch_eval, dim3(grid), dim3(BLOCK), lds_gen_total, s2, d_jobs, nj, k,
                                tp_gen + (size_t)k * (nj + 1), tpl_gen, big_cap ? c.big_ents : nullptr,
                                (uint32_t)big_cap, big_hdr);
             if (big_cap && k >= 15 && k <= big_kmax) {
                 hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s2, c.big_ents, big_hdr, (uint32_t)big_cap,
                                    c.big_prefix);
-                hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s2, c.jobs, c.big_ents,
+                hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds_max, s2, d_jobs, c.big_ents,
                                    big_hdr, c.big_prefix);
             }
         }
@@ -2579,18 +2620,17 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
         HIPCHK(hipStreamWaitEvent(s, c.lvl_ev[1][k], 0));
         HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][k], 0));
     }
-    hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, c.jobs, nj, nullptr, nullptr);
+    hipLaunchKernelGGL(k_batch_tree, dim3(nj), dim3(WAVE), 0, s, d_jobs, nj, nullptr, c.h_bres);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
-    std::vector<uint32_t> res((size_t)nj * RES_WORDS);
-    HIPCHK(hipMemcpyAsync(res.data(), c.results, res.size() * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    const uint32_t *const res_h = c.h_bres;
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     if (gpu_ms) *gpu_ms = ms;
     for (int j = 0; j < nj; j++) {
         const int n = ins[j]->hdr.n_rels;
-        const uint32_t *cnt = res.data() + (size_t)j * RES_WORDS;
+        const uint32_t *cnt = res_h + (size_t)j * RES_WORDS;
         if (cnt[49]) {
             set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");
             return GQ_ERR_UNSUPPORTED;
@@ -2794,22 +2834,24 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     if ((rc = ensure(&c.b.left, c.cap_left, cap))) return rc;
     if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, std::max<uint64_t>(list, 1)))) return rc;
     if ((rc = ensure(&c.hslot, c.cap_hslot, std::max<uint64_t>(list, 1)))) return rc;
-    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33 + 2 * 33))) return rc;
-    if ((rc = ensure_stage(c, W4h + 33 + 2 * 33))) return rc;
-    if ((rc = ensure(&c.sjob, c.cap_sjob, 1))) return rc;
+    /* staged area: blob | level offsets (33, padded to 36) | task prefixes
+     * (66, padded to 68) | the job descriptor */
+    const size_t o_pre = W4h + 36, o_sj = o_pre + 68;
+    const size_t stage_words = o_sj + ((sizeof(BatchJob) + 15) & ~(size_t)15) / 4;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, stage_words))) return rc;
+    if ((rc = ensure_stage(c, stage_words))) return rc;
     c.lvl_off = c.b.blob + W4h;
-    uint32_t *d_prefix = c.b.blob + W4h + 33;           /* [k] -> {0, tasks[k]} */
+    uint32_t *d_prefix = c.b.blob + o_pre;              /* [k] -> {0, tasks[k]} */
+    BatchJob *const d_sj = (BatchJob *)(c.b.blob + o_sj);
     hipStream_t s = c.stream;
+    memset(c.h_stage, 0, stage_words * 4);
     memcpy(c.h_stage, in.blob.data(), (size_t)h.words * 4);
-    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
     memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
     for (int k = 0; k < 33; k++) {
-        c.h_stage[W4h + 33 + 2 * k] = 0;
-        c.h_stage[W4h + 33 + 2 * k + 1] = tasks[k];
+        c.h_stage[o_pre + 2 * k] = 0;
+        c.h_stage[o_pre + 2 * k + 1] = tasks[k];
     }
-    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33 + 66) * 4, hipMemcpyHostToDevice, s));
-    BatchJob sj;                                       /* alive until the synchronisation below */
-    memset(&sj, 0, sizeof(sj));
+    BatchJob &sj = *(BatchJob *)(c.h_stage + o_sj);
     sj.h = h;
     sj.blob = c.b.blob;
     sj.vals = c.b.vals;
@@ -2822,7 +2864,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     sj.nodes = c.b.nodes;
     memcpy(sj.lvl_off, off_h, sizeof(off_h));
     sj.bicc = algo == DP_BICC ? 1 : 0;
-    HIPCHK(hipMemcpyAsync(c.sjob, &sj, sizeof(sj), hipMemcpyHostToDevice, s));
+    if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
     MemoRef mr;
     mr.vals = c.b.vals;
     mr.left = c.b.left;
@@ -2841,11 +2883,11 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev0, s));
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
                        0, s, c.hkeys, cap, 0u);
-    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, c.sjob, 1, nullptr, nullptr);
+    hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, d_sj, 1, nullptr, nullptr);
     for (int k = 2; k <= K; k++) {
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
-                           s, c.sjob, 1, k, d_prefix + 2 * k);
+                           s, d_sj, 1, k, d_prefix + 2 * k);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, sj.bicc, c.b.lvl_sets + off_h[k],
                          c.hslot + off_h[k], mr, c.b.counters);
@@ -2863,8 +2905,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
-                          hipMemcpyDeviceToHost, s));
+    if ((rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     if (cnt[49]) {
         set_error("open-addressing memo error (flags %u)", cnt[49]);
@@ -2936,14 +2977,14 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     if ((rc = ensure(&c.b.vals, c.cap_memo, total))) return rc;
     if ((rc = ensure(&c.b.left, c.cap_left, total))) return rc;
     if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, std::max<uint64_t>(list, 1)))) return rc;
-    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33))) return rc;
-    if ((rc = ensure_stage(c, W4h + 33))) return rc;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 36))) return rc;
+    if ((rc = ensure_stage(c, W4h + 36))) return rc;
     c.lvl_off = c.b.blob + W4h;
     hipStream_t s = c.stream;
+    memset(c.h_stage, 0, (W4h + 36) * 4);
     memcpy(c.h_stage, in.blob.data(), (size_t)h.words * 4);
-    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
     memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
-    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
+    if ((rc = stage_up(c, s, c.b.blob, W4h + 36))) return rc;
     MemoRef mr = dense_memo(c);
     mr.rank = c.b.blob + h.o_rank;
     mr.rn = n;
@@ -2976,8 +3017,7 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
-                          hipMemcpyDeviceToHost, s));
+    if ((rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     if (cnt[49]) {
         set_error("rank memo error (flags %u)", cnt[49]);
@@ -3145,8 +3185,12 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     /* device blob area: the query blob, then (at a 4-word boundary) the 33
      * level-list offsets; both go up in one copy */
     const size_t W4h = ((size_t)h.words + 3) & ~(size_t)3;
-    if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 33))) return rc;
-    if ((rc = ensure_stage(c, W4h + 33))) return rc;
+    /* staged area: blob | level offsets (33, padded to 36) | DPsize level-1
+     * order (32) | the big-set job descriptor */
+    const size_t o_l1 = W4h + 36, o_sj = o_l1 + 32;
+    const size_t stage_words = o_sj + ((sizeof(BatchJob) + 15) & ~(size_t)15) / 4;
+    if ((rc = ensure(&c.b.blob, c.cap_blob, stage_words))) return rc;
+    if ((rc = ensure_stage(c, stage_words))) return rc;
     c.lvl_off = c.b.blob + W4h;
 
     /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
@@ -3168,12 +3212,34 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             acc += binom64(n, k);
         }
     }
+    memset(c.h_stage, 0, stage_words * 4);
     memcpy(c.h_stage, in.blob.data(), blob_bytes);
-    memset(c.h_stage + h.words, 0, (W4h - h.words) * 4);
     memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
-    HIPCHK(hipMemcpyAsync(c.b.blob, c.h_stage, (W4h + 33) * 4, hipMemcpyHostToDevice, s));
+    if (algo == DP_DPSIZE) memcpy(c.h_stage + o_l1, in.level1.data(), 4 * std::min<size_t>(in.level1.size(), 32));
+    const uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
+    /* MPDP on a graph with cycles of 15 or more relations: sets with a big
+     * block are deferred to the chunk-parallel big-set kernels, which see
+     * this query as a batch of one. */
+    const bool use_big = algo == DP_BICC && !h.is_tree && n >= 15;
+    BatchJob *const d_sj = (BatchJob *)(c.b.blob + o_sj);
+    if (use_big) {
+        BatchJob &sj = *(BatchJob *)(c.h_stage + o_sj);
+        sj.h = h;
+        sj.blob = c.b.blob;
+        sj.vals = c.b.vals;
+        sj.left = c.b.left;
+        sj.conn = c.b.conn;
+        sj.lvl_sets = c.b.lvl_sets;
+        sj.counters = c.b.counters;
+        sj.lvl_best = c.b.lvl_best;
+        sj.lvl_first = c.b.lvl_first;
+        sj.nodes = c.b.nodes;
+        memcpy(sj.lvl_off, off_h, sizeof(off_h));
+        sj.conn_words = conn_words;
+        sj.bicc = 1;
+    }
+    if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
     HIPCHK(hipEventRecord(c.ev0, s));
-    uint32_t conn_words = (uint32_t)std::max<uint64_t>(1, nsets_all / 32);
     /* Trees with exact level sizes and a memo of at most 2^20 records skip
      * the 2^n filter: level 2 comes from k_tree_start and each level's
      * launch builds the next level's list by expansion (no connectivity
@@ -3210,12 +3276,12 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if ((rc = ensure(&d.rowfill, c.cap_ds_rf, nsets_all + 1))) return rc;
         if ((rc = ensure(&d.rowoff, c.cap_ds_ro, nsets_all + 1))) return rc;
         if ((rc = ensure(&d.sz, c.cap_ds_sz, 256))) return rc;
-        if ((rc = ensure(&c.l1, c.cap_l1, 32))) return rc;
-        HIPCHK(hipMemcpyAsync(c.l1, in.level1.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+
         const size_t lds_pairs = lds_eval + (DS_TA * 2 + DS_TB + DS_TA * DS_TB + 4) * 4;
         {
             uint32_t blocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK - 1) / BLOCK, 4096);
-            hipLaunchKernelGGL(k_ds_init, dim3(blocks), dim3(BLOCK), 0, s, h, c.b.blob, c.l1, d, nsets_all, c.b.vals,
+            hipLaunchKernelGGL(k_ds_init, dim3(blocks), dim3(BLOCK), 0, s, h, c.b.blob, (const int *)(c.b.blob + o_l1),
+                               d, nsets_all, c.b.vals,
                                c.b.left, c.b.counters);
         }
         uint64_t cand_ub = 0;
@@ -3269,8 +3335,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipEventRecord(c.ev1, s));
         tree.resize(nodes_n);
         const uint32_t *cnt = c.h_res;
-        HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
-                              hipMemcpyDeviceToHost, s));
+        if ((rc = results_down(c, s, nodes_n))) return rc;
         HIPCHK(hipStreamSynchronize(s));
         memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
         if (cnt[48] != (uint32_t)nodes_n) {
@@ -3299,30 +3364,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         st.memo_kind = 1;
         return GQ_OK;
     }
-    /* MPDP on a graph with cycles of 15 or more relations: sets with a big
-     * block are deferred to the chunk-parallel big-set kernels, which see
-     * this query as a batch of one. */
-    const bool use_big = algo == DP_BICC && !h.is_tree && n >= 15;
-    BatchJob sj;                       /* lives until the stream is synchronised */
-    memset(&sj, 0, sizeof(sj));
     if (use_big) {
         if ((rc = ensure(&c.big_ents, c.cap_big, maxC))) return rc;
         if ((rc = ensure(&c.big_prefix, c.cap_big_prefix, maxC + 1))) return rc;
-        if ((rc = ensure(&c.sjob, c.cap_sjob, 1))) return rc;
-        sj.h = h;
-        sj.blob = c.b.blob;
-        sj.vals = c.b.vals;
-        sj.left = c.b.left;
-        sj.conn = c.b.conn;
-        sj.lvl_sets = c.b.lvl_sets;
-        sj.counters = c.b.counters;
-        sj.lvl_best = c.b.lvl_best;
-        sj.lvl_first = c.b.lvl_first;
-        sj.nodes = c.b.nodes;
-        memcpy(sj.lvl_off, off_h, sizeof(off_h));
-        sj.conn_words = conn_words;
-        sj.bicc = 1;
-        HIPCHK(hipMemcpyAsync(c.sjob, &sj, sizeof(sj), hipMemcpyHostToDevice, s));
     }
     const uint64_t cand_total = nsets_all - (uint64_t)n - 1ull;
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[0], s));
@@ -3371,7 +3415,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                 if (big) {
                     hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, c.b.counters + 50,
                                        (uint32_t)maxC, c.big_prefix);
-                    hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, c.sjob, c.big_ents,
+                    hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, d_sj, c.big_ents,
                                        c.b.counters + 50, c.big_prefix);
                 }
             } else {
@@ -3403,8 +3447,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + nodes_n * (sizeof(DPNode) / 4)) * 4,
-                          hipMemcpyDeviceToHost, s));
+    if ((rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
     if (cnt[49] != 0) {
