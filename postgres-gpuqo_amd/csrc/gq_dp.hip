@@ -3699,6 +3699,15 @@ int engine_create(int device, uint64_t arena_bytes, StepEngine **out) {
     e->free_[0] = e->arena_bytes;
     int rc = ensure(&e->big_hdr, e->cap_hdr, 4);
     if (rc) return fail(rc);
+    /* staging sized for a few hundred DPs up front: growing pinned memory
+     * in the middle of a burst of requests costs milliseconds */
+    const unsigned host_flags = hipHostMallocMapped | hipHostMallocCoherent;
+    for (StepSlot &sl : e->slot) {
+        if ((rc = ensure_pinned(&sl.h_pkt, sl.cap_h_pkt, (size_t)1 << 20, host_flags))) return fail(rc);
+        if ((rc = ensure_pinned(&sl.h_res, sl.cap_h_res, (size_t)128 * RES_WORDS, host_flags))) return fail(rc);
+    }
+    if ((rc = ensure(&e->d_pkt, e->cap_pkt, (size_t)1 << 20))) return fail(rc);
+    if ((rc = ensure(&e->d_tp, e->cap_tp, (size_t)2 * 1025))) return fail(rc);
     if (hipMemsetAsync(e->big_hdr, 0, 16, e->s) != hipSuccess || hipStreamSynchronize(e->s) != hipSuccess) {
         set_error("engine initialisation failed");
         return fail(GQ_ERR_DEVICE);
