@@ -815,10 +815,11 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
         int c = 0;
         if (active) {
             S = (w == w0 && s0) ? s0 : lvl_sets[idx];   /* s0: this lane's first set, loaded by the caller */
+            if (rk && lvl_slot) rS = lvl_slot[idx];     /* rank memo: the slot the expansion wrote */
             c = select_bit(S, e + 1u);                  /* member e+1: the root (lowest member) is skipped */
             sub = S & desc[c];
         }
-        if (rk) {
+        if (rk && !lvl_slot) {
             /* rank memo: every member but the lowest, t, is the lower end
              * of one edge, so the set's slot off[t] + sum Q[t][c] is a sum
              * over the segment (each edge counted once) */
@@ -951,44 +952,34 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
     if (LPE == 1 && qn) flush();
 }
 
-/* Tree graphs: every connected set T of two or more relations is generated
- * from exactly one S = T - (T's highest-numbered leaf), since removing a
- * leaf keeps a tree connected.  Returns the relations v outside connected S
- * for which v is the highest-numbered leaf of S + v.  One pass over S gives
- * its neighbourhood and the members with exactly one neighbour in S.  In a
- * tree, v has exactly one neighbour p in S, whose degree rises by one: the
- * leaves of S + v among S are those members minus p, or S itself when S is
- * a singleton (its member then has one neighbour in S + v).  v qualifies
- * when none of them is numbered above v. */
+/* Tree graphs whose every vertex has its parent numbered below it
+ * (BlobHdr::is_tree): the highest-numbered member m of a connected set T of
+ * two or more relations is a leaf of T (its only possible neighbour in T is
+ * its parent: its children are numbered above m), so it is T's
+ * highest-numbered leaf, and removing it keeps T connected.  Every such T
+ * is therefore generated exactly once, from S = T - max(T): the children of
+ * connected S are S + v for the neighbours v of S numbered above max(S).
+ * Returns that mask. */
 __device__ __forceinline__ uint32_t top_leaf_children(const uint32_t *adj, uint32_t S) {
-    uint32_t nb = 0, l1 = 0;
-    for (uint32_t r = S; r; r &= r - 1u) {
-        const int u = __builtin_ctz(r);
-        const uint32_t a = adj[u];
-        nb |= a;
-        if (__builtin_popcount(a & S) == 1) l1 |= 1u << u;
-    }
-    nb &= ~S;
-    const bool single = (S & (S - 1u)) == 0u;
-    uint32_t emit = 0;
-    for (uint32_t c = nb; c; c &= c - 1u) {
-        const int v = __builtin_ctz(c);
-        const uint32_t leaves = single ? S : (l1 & ~adj[v]);
-        if ((leaves & ~((2u << v) - 1u)) == 0u) emit |= 1u << v;
-    }
-    return emit;
+    uint32_t nb = 0;
+    for (uint32_t r = S; r; r &= r - 1u) nb |= adj[__builtin_ctz(r)];
+    const int m = 31 - __builtin_clz(S);
+    return m == 31 ? 0u : nb & ~((2u << m) - 1u);
 }
 
-/* Dense-memo tree DP: the level list k+1 from set i of level k
+/* Dense- or rank-memo tree DP: the level list k+1 from set i of level k
  * (top_leaf_children), positions reserved with one atomic per wave on
  * *dst_count.  Called by whole waves; the order within a level list is
- * immaterial, each set's best split being computed on its own. */
+ * immaterial, each set's best split being computed on its own.  With
+ * dst_slot (rank memo; rk = the rank tables), each new set's slot too. */
 __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t *src, uint32_t count, uint32_t i,
-                                             uint32_t *dst, uint32_t *dst_count) {
+                                             uint32_t *dst, uint32_t *dst_count, const uint32_t *src_slot = nullptr,
+                                             uint32_t *dst_slot = nullptr, const uint32_t *rk = nullptr, int rn = 0) {
     const int lane = threadIdx.x & (WAVE - 1);
-    uint32_t S = 0, emit = 0;
+    uint32_t S = 0, emit = 0, sl = 0;
     if (i < count) {
         S = src[i];
+        if (dst_slot) sl = src_slot[i];
         emit = top_leaf_children(adj, S);
     }
     const uint32_t cnt = (uint32_t)__builtin_popcount(emit);
@@ -1001,27 +992,35 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
     uint32_t base = 0;
     if (lane == 0 && total) base = atomicAdd(dst_count, total);
     uint32_t pos = __shfl(base, 0) + incl - cnt;
-    for (; emit; emit &= emit - 1u) dst[pos++] = S | (emit & (0u - emit));
+    /* rank memo: the new member v is above every member of S, so S + v has
+     * S's top t and slot slot(S) + Q[t][v] */
+    const uint32_t *q = rk ? rk + rn + __builtin_ctz(S | 0x80000000u) * rn : nullptr;
+    for (; emit; emit &= emit - 1u, pos++) {
+        const int v = __builtin_ctz(emit);
+        dst[pos] = S | (1u << v);
+        if (dst_slot) dst_slot[pos] = sl + q[v];
+    }
 }
 
 /* Level k of a tree DP: workgroups [0, ev_blocks) evaluate it (the grid may
  * be sized by a bound on the level: idle ones leave after staging the
  * blob); with `next` set, workgroups from ev_blocks on build the level k+1
- * list there (dense memo, counters[k+1] counts it), BLOCK level-k sets
- * each.  The level's list has room for `cap` entries: each lane loads its
+ * list there (dense or rank memo, counters[k+1] counts it; rank memo: the
+ * slots go to next_slot), BLOCK level-k sets each.  The level's list has room for `cap` entries: each lane loads its
  * first set before the level size is known, so that this load, the level
  * size and the blob staging are in flight together. */
 template <int ECW, bool FAST, int LPE>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
-                                                      uint32_t *next, uint32_t cap) {
+                                                      uint32_t *next, uint32_t cap, uint32_t *next_slot) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     if (blockIdx.x >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
         const uint32_t count = counters[k];
         if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
         __syncthreads();
-        expand_level(smem, lvl_sets, count, (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1);
+        expand_level(smem, lvl_sets, count, (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1,
+                     lvl_slot, next_slot, mr.rank, mr.rn);
         return;
     }
     const uint32_t spw = tree_spw(k, LPE), per = (uint32_t)(k - 1) * LPE;
@@ -1042,10 +1041,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
 }
 
 /* Start of a dense- or rank-memo tree DP: zeroes the 64 result counters,
- * stores the base relations and builds level 2, the query's edges, each from its lower
+ * stores the base relations and builds level 2 (with slot2, the rank
+ * memo's slots of its sets), the query's edges, each from its lower
  * end (top_leaf_children of a singleton).  One wave. */
 __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *blob, float4 *vals, uint32_t *left,
-                                                      uint32_t *lvl2, uint32_t *counters) {
+                                                      uint32_t *lvl2, uint32_t *counters, uint32_t *slot2) {
     const int i = threadIdx.x;
     counters[i] = 0u;
     uint32_t emit = 0;
@@ -1062,7 +1062,11 @@ __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *
         if (i >= off) incl += y;
     }
     uint32_t pos = incl - cnt;
-    for (; emit; emit &= emit - 1u) lvl2[pos++] = (1u << i) | (emit & (0u - emit));
+    for (; emit; emit &= emit - 1u, pos++) {
+        const int j = __builtin_ctz(emit);
+        lvl2[pos] = (1u << i) | (1u << j);
+        if (slot2) slot2[pos] = blob[h.o_rank + i] + blob[h.o_rank + h.n_rels + i * h.n_rels + j];   /* off[i] + Q[i][j] */
+    }
     __syncthreads();
     if (i == WAVE - 1) counters[2] = incl;
 }
@@ -1936,6 +1940,10 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
         const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
         const bool have = i < src_count && (J.keys != nullptr || J.h.o_rank != 0);
         const uint32_t S = have ? ((k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i]) : 0u;
+        /* rank memo: the source's slot (off[i] for a singleton); each child
+         * S + v, v above every member, has slot slot(S) + Q[top(S)][v] */
+        const uint32_t *rq = J.h.o_rank ? J.blob + J.h.o_rank : nullptr;
+        const uint32_t sl = (have && rq) ? ((k - 1 == 1) ? rq[i] : J.lvl_slot[J.lvl_off[k - 1] + i]) : 0u;
         if (threadIdx.x < 32) s_adj[threadIdx.x] = (int)threadIdx.x < n ? J.blob[J.h.o_adj + threadIdx.x] : 0u;
         __syncthreads();
         uint32_t emit = have ? top_leaf_children(s_adj, S) : 0u;
@@ -1958,8 +1966,10 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, 
                     continue;
                 }
                 J.lvl_slot[J.lvl_off[k] + pos] = slot;
+            } else {
+                J.lvl_slot[J.lvl_off[k] + pos] = sl + rq[J.h.n_rels + __builtin_ctz(S) * J.h.n_rels + v];
             }
-            J.lvl_sets[J.lvl_off[k] + pos] = T;      /* rank memo: the slot follows from T */
+            J.lvl_sets[J.lvl_off[k] + pos] = T;
             pos++;
         }
     }
@@ -2669,7 +2679,7 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
  * The grid covers the level, capped at 8192 workgroups. */
 static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipStream_t s, const uint32_t *blob,
                              int k, int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr,
-                             uint32_t *counters, uint32_t *next = nullptr) {
+                             uint32_t *counters, uint32_t *next = nullptr, uint32_t *next_slot = nullptr) {
     const uint64_t spw2 = tree_spw(k, 2);
     /* levels of at most g_tune.lpe2_waves waves at two lanes per edge leave
      * the GPU latency-bound (a few waves per SIMD): the two-lane form */
@@ -2684,7 +2694,7 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint32_t cap = (uint32_t)std::min<uint64_t>(sets_k, 0xffffffffull);
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters, ev, next, cap)
+                       counters, ev, next, cap, next_slot)
     if (h.tree_fast) {
         if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
         else GQ_TREE_LAUNCH(1, true, 1);
@@ -2948,7 +2958,9 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
 /* Tree graphs with the rank memo: one slot per connected set, computed from
  * the set (rank_slot), so the memo holds 20 bytes per connected set and no
  * keys.  Level 2 comes from k_tree_start and each level's launch builds the
- * next level's list by expansion, as in the dense-memo tree DP. */
+ * next level's list by expansion, as in the dense-memo tree DP; the
+ * expansion also writes each new set's slot (its parent's slot plus one
+ * table entry), which the next level's evaluation reads with the set. */
 static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool profile,
                        std::vector<DPNode> &tree, DPStats &st) {
     const BlobHdr &h = in.hdr;
@@ -2977,6 +2989,7 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     if ((rc = ensure(&c.b.vals, c.cap_memo, total))) return rc;
     if ((rc = ensure(&c.b.left, c.cap_left, total))) return rc;
     if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, std::max<uint64_t>(list, 1)))) return rc;
+    if ((rc = ensure(&c.hslot, c.cap_hslot, std::max<uint64_t>(list, 1)))) return rc;
     if ((rc = ensure(&c.b.blob, c.cap_blob, W4h + 36))) return rc;
     if ((rc = ensure_stage(c, W4h + 36))) return rc;
     c.lvl_off = c.b.blob + W4h;
@@ -2998,11 +3011,12 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     const int bicc = algo == DP_BICC ? 1 : 0;
     HIPCHK(hipEventRecord(c.ev0, s));
     hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
-                       c.b.lvl_sets + off_h[2], c.b.counters);
+                       c.b.lvl_sets + off_h[2], c.b.counters, c.hslot + off_h[2]);
     for (int k = 2; k <= K; k++) {
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
-        launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, bicc, c.b.lvl_sets + off_h[k], nullptr, mr,
-                         c.b.counters, k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr);
+        launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, bicc, c.b.lvl_sets + off_h[k],
+                         c.hslot + off_h[k], mr, c.b.counters, k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr,
+                         k < K ? c.hslot + off_h[k + 1] : nullptr);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
@@ -3049,7 +3063,7 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     st.candidates = made;            /* sets generated by expansion (no 2^n scan) */
     st.sets = made;
     st.pairs = tree_pairs(in.tree_counts, (size_t)K);
-    st.memo_bytes = total * (sizeof(float4) + 4) + list * 4;
+    st.memo_bytes = total * (sizeof(float4) + 4) + list * 8;
     st.memo_kind = GQ_MEMO_RANK;
     return GQ_OK;
 }
@@ -3250,7 +3264,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                           n <= (int)g_tune.expand_max_n;
     if (tree_exp) {
         hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
-                           c.b.lvl_sets + off_h[2], c.b.counters);
+                           c.b.lvl_sets + off_h[2], c.b.counters, nullptr);
     } else {
         uint32_t blocks = (uint32_t)std::min<uint64_t>((conn_words + BLOCK - 1) / BLOCK, 4096);
         hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, c.b.conn, conn_words, c.b.counters);
@@ -3544,8 +3558,8 @@ struct EJob {
 
 namespace {
 /* open-addressing slice: [result][blob][keys][vals][left][lvl_sets][lvl_slot];
- * rank slice (rank = true): [result][blob][vals][left][lvl_sets], one memo
- * slot per connected set */
+ * rank slice (rank = true): [result][blob][vals][left][lvl_sets][lvl_slot],
+ * one memo slot per connected set */
 size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, bool rank, uint64_t *cap, size_t *o_blob,
                          size_t *o_keys, size_t *o_vals, size_t *o_left, size_t *o_sets, size_t *o_slot) {
     uint64_t c = 64;
@@ -3564,7 +3578,7 @@ size_t hash_slice_layout(uint64_t total_sets, uint64_t list, int words, bool ran
     *o_sets = o;
     o += al256(std::max<uint64_t>(list, 1) * 4);
     *o_slot = o;
-    if (!rank) o += al256(std::max<uint64_t>(list, 1) * 4);
+    o += al256(std::max<uint64_t>(list, 1) * 4);
     return o;
 }
 }  // namespace
@@ -3788,7 +3802,7 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     J.vals = (float4 *)(base + o_vals);
     J.left = (uint32_t *)(base + o_left);
     J.lvl_sets = (uint32_t *)(base + o_sets);
-    J.lvl_slot = rank ? nullptr : (uint32_t *)(base + o_slot);
+    J.lvl_slot = (uint32_t *)(base + o_slot);
     J.counters = (uint32_t *)base;
     J.nodes = (DPNode *)(J.counters + 64);
     uint64_t acc = 0;
