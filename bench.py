@@ -123,7 +123,8 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
     Poisson process of `rate` requests/s at the continuous-batching server.
     Latency is submission to completion (measured by the library); every
     served plan is compared with the same request planned alone by gq_plan,
-    which also gives the one-at-a-time throughput."""
+    which also gives the one-at-a-time throughput.  Both the server and the
+    one-at-a-time leg plan the mix once untimed first."""
     import random
     import threading
     mix = qgen.serving_mix(count, qgen.config_seed(5))
@@ -136,8 +137,13 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
         arrivals.append(t)
     res = {}
     with gpuqo_amd.Server(direct, budget_gb << 30, max_inflight=inflight) as srv:
-        srv.submit(qgen.snowflake(12, 1), count + 1)       # warm-up
-        srv.poll(60000)
+        # warm-up: the whole mix once, untimed (steady-state serving: the
+        # request threads' and the engine's buffers at their working sizes)
+        for i, (g, large) in enumerate(mix):
+            srv.submit(g, count + 1 + i, union if large else direct)
+        for _ in mix:
+            if srv.poll(120000) is None:
+                break
         # capacity: every request submitted at once
         t0 = time.perf_counter()
         for i, (g, large) in enumerate(mix):
@@ -176,6 +182,9 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
     lat = sorted(v[2] for v in res.values())
     span = max(v[3] for v in res.values()) - t0 if res else float("nan")
     # the same requests one at a time through gq_plan: throughput and parity
+    # (after one untimed pass, as for the server)
+    for g, large in mix:
+        gpuqo_amd.plan_raw(g, union if large else direct)
     single_s, same = 0.0, ok
     for i, (g, large) in enumerate(mix):
         c = union if large else direct
