@@ -72,7 +72,8 @@ void prepare_memo(DPInput &in, int requested, int max_memo_mb);
 struct Tuning {
     uint32_t lpe2_waves = 4096;    /* tree levels of at most this many two-lane waves take the two-lane form */
     uint32_t gen_grid_max = 512;   /* workgroups of a general-job batch launch (they loop over its tasks) */
-    uint32_t tree_iters = 16;      /* wave-iterations of sets per task of the batch tree kernel */
+    uint32_t tree_iters = 16;      /* wave-iterations of sets per task of the batch tree kernel, and per wave of
+                                      the single-query one-lane tree form */
     uint32_t expand_max_n = 20;    /* largest dense tree DP whose level lists are built by expansion */
 };
 Tuning tuning_get();

@@ -2686,7 +2686,17 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= g_tune.lpe2_waves ? 2 : 1;
     const uint64_t spw = tree_spw(k, lpe);
     const uint64_t waves = (sets_k + spw - 1) / spw;
-    const uint32_t ev = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((waves + 3) / 4, 8192));
+    /* two-lane form: one wave-iteration per wave (latency).  One-lane form:
+     * each wave queues its winners' records and writes them, hash table
+     * size packed, a wave at a time (the packing costs the same for 2 or 64
+     * records), so each wave takes about g_tune.tree_iters iterations, on
+     * at least 1024 workgroups while the level has them */
+    uint64_t evw = (waves + 3) / 4;
+    if (lpe == 1) {
+        const uint64_t it = std::max<uint32_t>(1, g_tune.tree_iters);
+        evw = std::max<uint64_t>(std::min<uint64_t>(evw, 1024), (waves + 4 * it - 1) / (4 * it));
+    }
+    const uint32_t ev = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(evw, 8192));
     /* with `next`: the level k+1 list from the sets_k (exact) level-k sets */
     const uint32_t ex = next ? (uint32_t)((sets_k + BLOCK - 1) / BLOCK) : 0u;
     const dim3 grid(ev + ex);
