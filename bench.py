@@ -116,7 +116,7 @@ def batched_section(gpuqo_amd, abi, qgen, reps=5):
             "batch_equals_single_all": same}
 
 
-def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=64):
+def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=128):
     """Config 5: `count` seeded requests (70% snowflakes of 10-30 relations
     planned directly by MPDP, 30% snowflakes of 60-200 relations, depth 4,
     fanout 1-5, by UnionDP with k=25 and MPDP per partition) arriving as a
