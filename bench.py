@@ -274,6 +274,10 @@ def main():
                     help="only plan the 256-query batch (one warm-up, then REPS batches) and print its times")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch-shape knobs instead of tuning them on the workload first")
+    ap.add_argument("--trees-only", type=int, default=0, metavar="REPS",
+                    help="only plan the served mix's 40 direct snowflakes of 25-30 relations (rank memo) one at a "
+                         "time, one warm-up pass and REPS timed passes, and print their times")
+    ap.add_argument("--serve-only", action="store_true", help="only run the served section and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build plans on one GPU; --gpus {args.gpus} is not supported "
@@ -304,6 +308,26 @@ def main():
             dev.append(outs[0].stats.gpu_ms)
         print(json.dumps({"workload": "batch256-mpdp", "queries": len(qs), "batch_ms_wall": wall,
                           "batch_ms_device": dev}), flush=True)
+        return 0
+    if args.trees_only:
+        big = [g for g, large in qgen.serving_mix(256, qgen.config_seed(5)) if not large and g.n >= 25]
+        tcfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+        runs = []
+        for rep in range(args.trees_only + 1):
+            dev, pairs, kinds = 0.0, 0, set()
+            for g in big:
+                _, st = gpuqo_amd.plan(g, tcfg)
+                dev += st.gpu_ms
+                pairs += st.pairs
+                kinds.add(st.memo_kind)
+            if rep:
+                runs.append(dev)
+        print(json.dumps({"workload": "serve-direct-snowflakes25-30", "queries": len(big), "pairs": pairs,
+                          "memo_kinds": sorted(kinds), "device_ms": runs,
+                          "pairs_per_s_device": pairs / (statistics.median(runs) / 1e3)}), flush=True)
+        return 0
+    if args.serve_only:
+        print(json.dumps(serving_section(gpuqo_amd, abi, qgen)), flush=True)
         return 0
     cfg_name, algo, bicc = WORKLOADS[args.workload]
     g = getattr(qgen, cfg_name)()
