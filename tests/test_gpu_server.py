@@ -79,3 +79,33 @@ def test_server_rejects_a_dp_larger_than_its_budget():
             res[r[0]] = r[1]
     assert res[1] == 3            # GQ_ERR_NOMEM: 2^20 memo slots do not fit 8 MiB
     assert res[2] == 0
+
+
+def test_server_burst_of_mixed_requests():
+    """A burst of 48 requests submitted at once: dense, rank-memo and
+    general-graph DPs and UnionDP requests share steps, arena slices are
+    freed and reused while later steps are in flight, and every plan equals
+    the same request planned alone."""
+    import gpuqo_amd
+    direct = cfg("dpsub", 1)
+    union = cfg("dpsub", 1, idp_type="union", idp_n_iters=25)
+    reqs = []
+    for i in range(48):
+        r = i % 6
+        if r == 0:
+            reqs.append((qgen.snowflake(10 + i % 13, 7000 + i), direct))
+        elif r == 1:
+            reqs.append((qgen.snowflake(25 + i % 3, 7100 + i), direct))       # rank memo
+        elif r == 2:
+            reqs.append((qgen.cycle(9 + i % 8, 7200 + i), direct))
+        elif r == 3:
+            reqs.append((qgen.star(8 + i % 7, 7300 + i), direct))
+        elif r == 4:
+            reqs.append((qgen.snowflake(60 + i, 7400 + i, depth=4, fanout=5), union))
+        else:
+            reqs.append((qgen.chain(6 + i % 10, 7500 + i), direct))
+    with gpuqo_amd.Server(direct, 8 << 30, max_inflight=32) as srv:
+        got = _run(srv, reqs, 0.0)
+    for i, (g, c) in enumerate(reqs):
+        alone, _ = gpu_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(got[i]), alone, f"request {i} ({g.shape}{g.n})")
