@@ -29,10 +29,10 @@ METRIC = "MPDP plan ms + csg-cmp pairs/s (1 GPU); batched queries/s and p99 ms"
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "mpdp-snowflake20": ("config2", "dpsub", 1),
-    "star12-dpsub": ("config1", "dpsub", 0),
-    "clique18-dpsub": ("config3", "dpsub", 0),
-    "clique18-dpsize": ("config3", "dpsize", 0),
+    "mpdp-snowflake20": ("query_snowflake20", "dpsub", 1),
+    "star12-dpsub": ("query_star12", "dpsub", 0),
+    "clique18-dpsub": ("query_clique18", "dpsub", 0),
+    "clique18-dpsize": ("query_clique18", "dpsize", 0),
 }
 
 
@@ -73,12 +73,12 @@ def gather_roofline():
 
 
 def batched_section(gpuqo_amd, abi, qgen, reps=5):
-    """Config 4: 256 seeded mixed queries (n in 8..22; 40% snowflake, 30%
+    """The 256-query batch: seeded mixed queries (n in 8..22; 40% snowflake, 30%
     star, 15% chain, 15% cycle), MPDP, planned as one batch (one launch per
     DP level bucket) vs one query at a time on the same GPU.  Both sides are
     timed around the C-ABI call (gq_plan_batch once / gq_plan per query),
     with the query descriptions built beforehand."""
-    qs = qgen.config4()
+    qs = qgen.queries_batch256()
     cfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
     args = gpuqo_amd.BatchArgs(qs)
     args.run(cfg)                                       # warm-up
@@ -117,7 +117,7 @@ def batched_section(gpuqo_amd, abi, qgen, reps=5):
 
 
 def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=128):
-    """Config 5: `count` seeded requests (70% snowflakes of 10-30 relations
+    """The served mix: `count` seeded requests (70% snowflakes of 10-30 relations
     planned directly by MPDP, 30% snowflakes of 60-200 relations, depth 4,
     fanout 1-5, by UnionDP with k=25 and MPDP per partition) arriving as a
     Poisson process of `rate` requests/s at the continuous-batching server.
@@ -127,10 +127,10 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
     one-at-a-time leg plan the mix once untimed first."""
     import random
     import threading
-    mix = qgen.serving_mix(count, qgen.config_seed(5))
+    mix = qgen.serving_mix(count, qgen.workload_seed("serve256"))
     direct = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
     union = abi.default_config(algorithm="dpsub", dpsub_bicc=1, idp_type="union", idp_n_iters=25)
-    rng = random.Random(qgen.config_seed(5))
+    rng = random.Random(qgen.workload_seed("serve256"))
     arrivals, t = [], 0.0
     for _ in mix:
         t += rng.expovariate(rate)
@@ -193,7 +193,7 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
         single_s += time.perf_counter() - t1
         same = same and res.get(i, (None, None))[1] == gpuqo_amd.plan_bits(gpuqo_amd.plan_tuples(out))
     q = lambda f: lat[min(len(lat) - 1, int(f * len(lat)))] if lat else float("nan")
-    return {"workload": "serve-config5", "requests": count, "arrival_rate_per_s": rate,
+    return {"workload": "serve-mix256", "requests": count, "arrival_rate_per_s": rate,
             "large_uniondp_requests": sum(1 for _, l in mix if l),
             "capacity_queries_per_s": count / sat_s if sat == count else float("nan"),
             "served_queries_per_s": count / span, "latency_p50_ms": q(0.50), "latency_p99_ms": q(0.99),
@@ -296,7 +296,7 @@ def main():
     import gpuqo_amd
     from gpuqo_amd import abi, qgen
     if args.batch_only:
-        qs = qgen.config4()
+        qs = qgen.queries_batch256()
         bcfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
         bargs = gpuqo_amd.BatchArgs(qs)
         bargs.run(bcfg)
@@ -310,7 +310,7 @@ def main():
                           "batch_ms_device": dev}), flush=True)
         return 0
     if args.trees_only:
-        big = [g for g, large in qgen.serving_mix(256, qgen.config_seed(5)) if not large and g.n >= 25]
+        big = [g for g, large in qgen.serving_mix(256, qgen.workload_seed("serve256")) if not large and g.n >= 25]
         tcfg = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
         runs = []
         for rep in range(args.trees_only + 1):
@@ -344,7 +344,7 @@ def main():
         with_batch = not args.no_batch and args.workload == "mpdp-snowflake20"
         knobs = gpuqo_amd.autotune([g], cfg, reps=5, knobs=gpuqo_amd.TUNE_SINGLE)
         if with_batch:
-            knobs = gpuqo_amd.autotune(qgen.config4(), abi.default_config(algorithm="dpsub", dpsub_bicc=1),
+            knobs = gpuqo_amd.autotune(qgen.queries_batch256(), abi.default_config(algorithm="dpsub", dpsub_bicc=1),
                                        reps=3, knobs=gpuqo_amd.TUNE_BATCH)
         tuning = {"knobs": knobs, "tune_s": round(time.perf_counter() - t0, 2),
                   "sample": {"single": f"{g.shape}-{g.n}", "batch": "batch256-mpdp" if with_batch else None}}

@@ -287,23 +287,28 @@ def chorded_cycle(n, step, seed):
 SEED_BASE = 0x6770713000000000
 
 
-def config_seed(k):
-    return SEED_BASE + k
+WORKLOAD_SEEDS = {"star12": 1, "snowflake20": 2, "clique18": 3, "batch256": 4, "serve256": 5}
 
 
-def config1():
+def workload_seed(name):
+    """Seed of a named benchmark workload (fixed: the golden fixtures and the
+    published numbers are tied to the graphs it generates)."""
+    return SEED_BASE + WORKLOAD_SEEDS[name]
+
+
+def query_star12():
     """12-relation star, plain DPsub."""
-    return star(12, config_seed(1))
+    return star(12, workload_seed("star12"))
 
 
-def config2():
+def query_snowflake20():
     """20-relation snowflake (depth 3, fanout 1..4), MPDP."""
-    return snowflake(20, config_seed(2))
+    return snowflake(20, workload_seed("snowflake20"))
 
 
-def config3():
+def query_clique18():
     """18-relation clique, plain DPsub and DPsize."""
-    return clique(18, config_seed(3))
+    return clique(18, workload_seed("clique18"))
 
 
 def mixed_batch(count, seed, nmin=8, nmax=22):
@@ -326,8 +331,9 @@ def mixed_batch(count, seed, nmin=8, nmax=22):
     return out
 
 
-def config4(count=256):
-    return mixed_batch(count, config_seed(4))
+def queries_batch256(count=256):
+    """The 256-query mixed batch (n in 8..22), MPDP."""
+    return mixed_batch(count, workload_seed("batch256"))
 
 
 def serving_mix(count, seed):

@@ -26,8 +26,8 @@ from gpuqo_amd import abi, qgen  # noqa: E402
 
 # (label, generator, args, kwargs, config overrides)
 CASES = [
-    ("config1-star12-dpsub", "config1", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
-    ("config2-snowflake20-mpdp", "config2", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
+    ("star12-dpsub", "query_star12", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("snowflake20-mpdp", "query_snowflake20", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),
     ("star8-dpsize", "star", [8, 11], {}, {"algorithm": "dpsize"}),
     ("chain9-dpsub", "chain", [9, 22], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
     ("chain9-mpdp", "chain", [9, 22], {}, {"algorithm": "dpsub", "dpsub_bicc": 1}),

@@ -13,7 +13,7 @@ GQ_MEMO_DENSE, GQ_MEMO_HASH, GQ_MEMO_RANK = 1, 2, 3
 
 
 def _trees():
-    return [("snowflake20", qgen.config2(), 1), ("star12", qgen.config1(), 0), ("chain15", qgen.chain(15, 7), 1),
+    return [("snowflake20", qgen.query_snowflake20(), 1), ("star12", qgen.query_star12(), 0), ("chain15", qgen.chain(15, 7), 1),
             ("snow16-dpsub", qgen.snowflake(16, 33), 0), ("star9", qgen.star(9, 5), 1),
             ("chain2", qgen.chain(2, 1), 1), ("snow3", qgen.snowflake(3, 2), 0),
             ("snow18-d4", qgen.snowflake(18, 44, depth=4, fanout=5), 1)]

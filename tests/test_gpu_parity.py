@@ -16,8 +16,8 @@ def test_small_graphs(name, g, bicc):
     assert_same_plan(got, want, f"{name} bicc={bicc}")
 
 
-def test_config1_star12_dpsub():
-    g = qgen.config1()
+def test_star12_dpsub():
+    g = qgen.query_star12()
     c = cfg("dpsub", 0)
     want, ost = oracle_bits(g, c)
     got, gst = gpu_bits(g, c)
@@ -25,8 +25,8 @@ def test_config1_star12_dpsub():
     assert gst.pairs == ost.pairs == 11 * 2 ** 11
 
 
-def test_config2_snowflake20_mpdp():
-    g = qgen.config2()
+def test_snowflake20_mpdp():
+    g = qgen.query_snowflake20()
     c = cfg("dpsub", 1)
     want, ost = oracle_bits(g, c)
     got, gst = gpu_bits(g, c)
@@ -77,7 +77,7 @@ def test_dpsize_small_graphs(name, g):
 
 
 def test_dpsize_star12():
-    g = qgen.config1()
+    g = qgen.query_star12()
     c = cfg("dpsize")
     want, _ = oracle_bits(g, c)
     got, _ = gpu_bits(g, c)
@@ -189,7 +189,7 @@ def test_batch_with_big_blocks():
 def _idp1_cases():
     return [("snow16-mpdp", qgen.snowflake(16, 0x16), "dpsub", 1, 5),
             ("snow16-dpsub", qgen.snowflake(16, 0x16), "dpsub", 0, 4),
-            ("snow20-mpdp", qgen.config2(), "dpsub", 1, 8),
+            ("snow20-mpdp", qgen.query_snowflake20(), "dpsub", 1, 8),
             ("cycle12-mpdp", qgen.cycle(12, 0x12), "dpsub", 1, 4),
             ("chain15-mpdp", qgen.chain(15, 0x15), "dpsub", 1, 6),
             ("clique10-dpsub", qgen.clique(10, 0x10), "dpsub", 0, 4),
