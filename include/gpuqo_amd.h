@@ -266,12 +266,17 @@ gq_status gq_server_destroy(gq_server *s);
  *   tree_task_iters       wave-iterations of sets per batch tree task (1..256)
  *   tree_expand_max_rels  largest dense tree DP whose level lists are built
  *                         by expansion instead of the 2^n filter (0..32)
+ *   tree_one_xcd_wgs      single-query tree levels of at most this many
+ *                         workgroups run on one XCD (8x the grid, every
+ *                         eighth workgroup working), so a level's writes
+ *                         stay in the L2 the next level reads (0..256; 0 off)
  */
 typedef struct gq_tuning {
     int32_t tree_two_lane_waves;
     int32_t general_grid_max;
     int32_t tree_task_iters;
     int32_t tree_expand_max_rels;
+    int32_t tree_one_xcd_wgs;
 } gq_tuning;
 gq_status gq_tuning_get(gq_tuning *t);
 gq_status gq_tuning_set(const gq_tuning *t);
@@ -279,7 +284,8 @@ gq_status gq_tuning_set(const gq_tuning *t);
  * Autotuner (no reference counterpart): times the sample queries under
  * candidate knob values, one knob at a time, keeps the fastest value of
  * each, leaves it set and returns all knobs in *out.  `knobs` selects the
- * groups: GQ_TUNE_SINGLE (tree_two_lane_waves, tree_expand_max_rels; timed
+ * groups: GQ_TUNE_SINGLE (tree_two_lane_waves, tree_expand_max_rels,
+ * tree_one_xcd_wgs; timed
  * by planning the queries one by one with gq_plan) and GQ_TUNE_BATCH
  * (tree_task_iters, general_grid_max; timed by planning them as one
  * gq_plan_batch).  reps >= 1 timings per candidate (median).  Must not run

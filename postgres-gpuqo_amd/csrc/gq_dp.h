@@ -75,6 +75,7 @@ struct Tuning {
     uint32_t tree_iters = 16;      /* wave-iterations of sets per task of the batch tree kernel, and per wave of
                                       the single-query one-lane tree form */
     uint32_t expand_max_n = 20;    /* largest dense tree DP whose level lists are built by expansion */
+    uint32_t one_xcd_wgs = 0;      /* single-query tree levels of at most this many workgroups run on one XCD */
 };
 Tuning tuning_get();
 int tuning_set(const Tuning &t);   /* GQ_ERR_INVALID outside the accepted ranges */

@@ -47,6 +47,7 @@ namespace gq {
 
 #define WAVE 64
 #define BLOCK 256
+#define XCDS 8u     /* XCDs of an MI355X; workgroups are dispatched to them round robin */
 #define BMAX 128                 /* sets per workgroup batch            */
 #define FILT_CHUNK 4096          /* candidates per filter block pass    */
 #define QCAP 1024                /* DPsub valid-pair queue per round    */
@@ -1013,25 +1014,34 @@ template <int ECW, bool FAST, int LPE>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
-                                                      uint32_t *next, uint32_t cap, uint32_t *next_slot) {
+                                                      uint32_t *next, uint32_t cap, uint32_t *next_slot,
+                                                      uint32_t xs) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
-    if (blockIdx.x >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
+    /* xs > 1: the grid is xs times the level's workgroups, dispatched round
+     * robin over the XCDs; only every xs-th works, so a small level runs on
+     * one XCD and its writes stay in the L2 that the next level reads */
+    uint32_t bid = blockIdx.x;
+    if (xs > 1) {
+        if (bid % xs) return;
+        bid /= xs;
+    }
+    if (bid >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
         const uint32_t count = counters[k];
         if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
         __syncthreads();
-        expand_level(smem, lvl_sets, count, (blockIdx.x - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1,
+        expand_level(smem, lvl_sets, count, (bid - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1,
                      lvl_slot, next_slot, mr.rank, mr.rn);
         return;
     }
     const uint32_t spw = tree_spw(k, LPE), per = (uint32_t)(k - 1) * LPE;
-    const uint64_t w0 = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+    const uint64_t w0 = (uint64_t)bid * (BLOCK / WAVE) + threadIdx.x / WAVE;
     const uint32_t seg = (threadIdx.x & (WAVE - 1)) / per;
     const uint64_t idx0 = w0 * spw + seg;
     const uint32_t count = counters[k];
     const uint32_t s_pre = (seg < spw && idx0 < cap) ? lvl_sets[idx0] : 0u;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
-    if ((uint64_t)blockIdx.x * (BLOCK / WAVE) * spw >= count) return;
+    if ((uint64_t)bid * (BLOCK / WAVE) * spw >= count) return;
     View<uint32_t> q = make_view(h, smem);
     eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, count, mr, w0,
                                     (uint64_t)ev_blocks * (BLOCK / WAVE), idx0 < count ? s_pre : 0u);
@@ -2699,12 +2709,13 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint32_t ev = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(evw, 8192));
     /* with `next`: the level k+1 list from the sets_k (exact) level-k sets */
     const uint32_t ex = next ? (uint32_t)((sets_k + BLOCK - 1) / BLOCK) : 0u;
-    const dim3 grid(ev + ex);
+    const uint32_t xs = (g_tune.one_xcd_wgs && ev + ex <= g_tune.one_xcd_wgs) ? XCDS : 1u;
+    const dim3 grid((ev + ex) * xs);
     lds = std::max(lds, tree_lds(h.words, lpe));
     const uint32_t cap = (uint32_t)std::min<uint64_t>(sets_k, 0xffffffffull);
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters, ev, next, cap, next_slot)
+                       counters, ev, next, cap, next_slot, xs)
     if (h.tree_fast) {
         if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
         else GQ_TREE_LAUNCH(1, true, 1);
@@ -3085,9 +3096,9 @@ Tuning tuning_get() {
 
 int tuning_set(const Tuning &t) {
     if (t.lpe2_waves < 1 || t.lpe2_waves > (1u << 24) || t.gen_grid_max < 1 || t.gen_grid_max > 65536 ||
-        t.tree_iters < 1 || t.tree_iters > 256 || t.expand_max_n > 32) {
+        t.tree_iters < 1 || t.tree_iters > 256 || t.expand_max_n > 32 || t.one_xcd_wgs > 256) {
         set_error("tuning out of range (two-lane waves 1..2^24, general grid 1..65536, tree task iterations "
-                  "1..256, expansion up to 32 relations)");
+                  "1..256, expansion up to 32 relations, one-XCD levels up to 256 workgroups)");
         return GQ_ERR_INVALID;
     }
     std::lock_guard<std::mutex> g(g_ctx.mu);

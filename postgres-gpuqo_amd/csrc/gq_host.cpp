@@ -728,12 +728,13 @@ gq_status gq_tuning_get(gq_tuning *t) {
     t->general_grid_max = (int32_t)v.gen_grid_max;
     t->tree_task_iters = (int32_t)v.tree_iters;
     t->tree_expand_max_rels = (int32_t)v.expand_max_n;
+    t->tree_one_xcd_wgs = (int32_t)v.one_xcd_wgs;
     return GQ_OK;
 }
 
 gq_status gq_tuning_set(const gq_tuning *t) {
     if (!t || t->tree_two_lane_waves < 1 || t->general_grid_max < 1 || t->tree_task_iters < 1 ||
-        t->tree_expand_max_rels < 0) {
+        t->tree_expand_max_rels < 0 || t->tree_one_xcd_wgs < 0) {
         gq::set_error("null or non-positive tuning");
         return GQ_ERR_INVALID;
     }
@@ -742,6 +743,7 @@ gq_status gq_tuning_set(const gq_tuning *t) {
     v.gen_grid_max = (uint32_t)t->general_grid_max;
     v.tree_iters = (uint32_t)t->tree_task_iters;
     v.expand_max_n = (uint32_t)t->tree_expand_max_rels;
+    v.one_xcd_wgs = (uint32_t)t->tree_one_xcd_wgs;
     return (gq_status)gq::tuning_set(v);
 }
 
@@ -785,6 +787,7 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
     const Knob knob_list[] = {
         {&gq::Tuning::lpe2_waves, false, {1024, 2048, 4096, 8192, 16384}},
         {&gq::Tuning::expand_max_n, false, {0, 16, 18, 20, 22}},
+        {&gq::Tuning::one_xcd_wgs, false, {0, 8, 32, 64}},
         {&gq::Tuning::tree_iters, true, {4, 8, 16, 32}},
         {&gq::Tuning::gen_grid_max, true, {256, 512, 1024, 2048}},
     };

@@ -111,7 +111,8 @@ class PlanStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tree_two_lane_waves", C.c_int32), ("general_grid_max", C.c_int32),
-                ("tree_task_iters", C.c_int32), ("tree_expand_max_rels", C.c_int32)]
+                ("tree_task_iters", C.c_int32), ("tree_expand_max_rels", C.c_int32),
+                ("tree_one_xcd_wgs", C.c_int32)]
 
 
 class PlanOut(C.Structure):

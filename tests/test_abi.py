@@ -80,9 +80,10 @@ def test_tuning_defaults_and_validation():
     accepted ranges are refused and leave the knobs unchanged."""
     d = gpuqo_amd.tuning_get()
     assert d == {"tree_two_lane_waves": 4096, "general_grid_max": 512, "tree_task_iters": 16,
-                 "tree_expand_max_rels": 20}
+                 "tree_expand_max_rels": 20, "tree_one_xcd_wgs": 0}
     for bad in ({"tree_task_iters": 0}, {"tree_task_iters": 257}, {"general_grid_max": 0},
-                {"tree_two_lane_waves": -1}, {"tree_expand_max_rels": 33}):
+                {"tree_two_lane_waves": -1}, {"tree_expand_max_rels": 33}, {"tree_one_xcd_wgs": 257},
+                {"tree_one_xcd_wgs": -1}):
         with pytest.raises(gpuqo_amd.GpuqoError) as e:
             gpuqo_amd.tuning_set(**bad)
         assert e.value.status == abi.GQ_ERR_INVALID

@@ -9,7 +9,7 @@ from gpuqo_amd import qgen
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = {"tree_two_lane_waves": 4096, "general_grid_max": 512, "tree_task_iters": 16,
-            "tree_expand_max_rels": 20}
+            "tree_expand_max_rels": 20, "tree_one_xcd_wgs": 0}
 
 
 def _sample():
@@ -26,7 +26,8 @@ def _plans(qs, c):
 @pytest.mark.parametrize("knobs", [{"tree_two_lane_waves": 1}, {"tree_two_lane_waves": 1 << 24},
                                    {"tree_task_iters": 1, "general_grid_max": 1},
                                    {"tree_task_iters": 64, "general_grid_max": 4096},
-                                   {"tree_expand_max_rels": 0}, {"tree_expand_max_rels": 32}])
+                                   {"tree_expand_max_rels": 0}, {"tree_expand_max_rels": 32},
+                                   {"tree_one_xcd_wgs": 32}, {"tree_one_xcd_wgs": 256}])
 def test_knobs_change_no_plan(knobs):
     qs = _sample()
     for bicc in (1, 0):
@@ -54,6 +55,7 @@ def test_autotune_keeps_plans():
         assert t == gpuqo_amd.tuning_get()
         assert t["tree_two_lane_waves"] in (1024, 2048, 4096, 8192, 16384)
         assert t["tree_expand_max_rels"] in (0, 16, 18, 20, 22)
+        assert t["tree_one_xcd_wgs"] in (0, 8, 32, 64)
         assert t["tree_task_iters"] in (4, 8, 16, 32)
         assert t["general_grid_max"] in (256, 512, 1024, 2048)
         got_s, got_b = _plans(qs, c)
@@ -73,6 +75,7 @@ def test_autotune_groups_leave_other_knobs():
         t = gpuqo_amd.autotune(qs, c, reps=1, knobs=gpuqo_amd.TUNE_BATCH)
         assert t["tree_two_lane_waves"] == DEFAULTS["tree_two_lane_waves"]
         assert t["tree_expand_max_rels"] == DEFAULTS["tree_expand_max_rels"]
+        assert t["tree_one_xcd_wgs"] == DEFAULTS["tree_one_xcd_wgs"]
         gpuqo_amd.tuning_set(**DEFAULTS)
         t = gpuqo_amd.autotune(qs, c, reps=1, knobs=gpuqo_amd.TUNE_SINGLE)
         assert t["tree_task_iters"] == DEFAULTS["tree_task_iters"]
