@@ -283,13 +283,14 @@ gq_status gq_tuning_set(const gq_tuning *t);
 /*
  * Autotuner (no reference counterpart): times the sample queries under
  * candidate knob values, one knob at a time, keeps the fastest value of
- * each, leaves it set and returns all knobs in *out.  `knobs` selects the
- * groups: GQ_TUNE_SINGLE (tree_two_lane_waves, tree_expand_max_rels,
- * tree_one_xcd_wgs; timed
- * by planning the queries one by one with gq_plan) and GQ_TUNE_BATCH
- * (tree_task_iters, general_grid_max; timed by planning them as one
- * gq_plan_batch).  reps >= 1 timings per candidate (median).  Must not run
- * concurrently with other planning calls of the process.
+ * each (a knob leaves its current value only for one at least 1% faster),
+ * leaves it set and returns all knobs in *out.  `knobs` selects the groups:
+ * GQ_TUNE_SINGLE (tree_two_lane_waves, tree_expand_max_rels,
+ * tree_one_xcd_wgs; timed by the device time of planning the queries one by
+ * one with gq_plan, 4 * reps timings) and GQ_TUNE_BATCH (tree_task_iters,
+ * general_grid_max; timed by the wall time of one gq_plan_batch of them,
+ * reps timings).  The median counts.  Must not run concurrently with other
+ * planning calls of the process.
  */
 enum { GQ_TUNE_SINGLE = 1, GQ_TUNE_BATCH = 2 };
 gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,

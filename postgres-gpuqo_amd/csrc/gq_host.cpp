@@ -763,10 +763,16 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
         outs[i].nodes = bufs[i].data();
         outs[i].capacity = (int32_t)bufs[i].size();
     }
+    /* batch knobs: wall time of gq_plan_batch (milliseconds, steady);
+     * single-plan knobs: device time of each gq_plan summed over the sample
+     * (a plan's wall time jitters by more than the knobs move it), with four
+     * times the repetitions */
     auto median_ms = [&](bool batch, double *ms) -> int {
         std::vector<double> t;
-        for (int r = 0; r <= reps; r++) {                    /* the first run warms up */
+        const int rr = batch ? reps : 4 * reps;
+        for (int r = 0; r <= rr; r++) {                      /* the first run warms up */
             const auto t0 = std::chrono::steady_clock::now();
+            double dev = 0.0;
             if (batch) {
                 int e = gq_plan_batch(n_queries, qs, cfg, outs.data());
                 if (e) return e;
@@ -774,9 +780,11 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
                 for (int i = 0; i < n_queries; i++) {
                     int e = gq_plan(qs[i], cfg, &outs[i]);
                     if (e) return e;
+                    dev += outs[i].stats.gpu_ms;
                 }
             }
-            if (r) t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (r) t.push_back(batch ? wall : dev);
         }
         std::sort(t.begin(), t.end());
         *ms = t[t.size() / 2];
@@ -791,12 +799,17 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
         {&gq::Tuning::tree_iters, true, {4, 8, 16, 32}},
         {&gq::Tuning::gen_grid_max, true, {256, 512, 1024, 2048}},
     };
+    /* a knob leaves its current value only for a candidate at least 1%
+     * faster, so timing noise does not move it */
     for (const Knob &kb : knob_list) {
         if (!(knobs & (kb.batch ? GQ_TUNE_BATCH : GQ_TUNE_SINGLE))) continue;
-        uint32_t pick = best.*kb.field;
-        double pick_ms = 0.0;
-        bool first = true;
+        const uint32_t cur = best.*kb.field;
+        double cur_ms = 0.0;
+        if ((rc = median_ms(kb.batch, &cur_ms))) return (gq_status)rc;
+        uint32_t pick = cur;
+        double pick_ms = cur_ms;
         for (uint32_t v : kb.cands) {
+            if (v == cur) continue;
             gq::Tuning t = best;
             t.*kb.field = v;
             if ((rc = gq::tuning_set(t))) return (gq_status)rc;
@@ -805,10 +818,9 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
                 gq::tuning_set(best);
                 return (gq_status)rc;
             }
-            if (first || ms < pick_ms) {
+            if (ms < pick_ms && ms < 0.99 * cur_ms) {
                 pick = v;
                 pick_ms = ms;
-                first = false;
             }
         }
         best.*kb.field = pick;
