@@ -105,3 +105,35 @@ def test_server_with_tree_memo_requests(kind):
         alone, ast = gpu_bits(g, direct)
         assert_same_plan(gpuqo_amd.plan_bits(got[i][0]), alone, f"request {i} ({g.shape}{g.n})")
         assert got[i][1].memo_kind == ast.memo_kind
+
+
+@pytest.mark.parametrize("idp,k,n", [("idp2", 8, 20), ("union", 25, 40), ("idp1", 6, 16)])
+def test_rank_memo_under_idp_drivers(idp, k, n):
+    """The rank memo requested for every DP of the IDP1, IDP2 and UnionDP
+    drivers (whose sub-DPs are small trees, relabelled and contracted)."""
+    g = qgen.snowflake(n, 0x3000 + n, depth=4, fanout=5) if n > 30 else qgen.snowflake(n, 0x3000 + n)
+    c = cfg("dpsub", 1, idp_type=idp, idp_n_iters=k, memo_kind=GQ_MEMO_RANK)
+    want, _ = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"{idp} k={k} snowflake-{n} rank memo")
+
+
+def test_server_with_rank_memo_everywhere():
+    """Every tree DP of a mixed request set on the rank memo, on the engine."""
+    import gpuqo_amd
+    direct = cfg("dpsub", 1, memo_kind=GQ_MEMO_RANK)
+    union = cfg("dpsub", 1, idp_type="union", idp_n_iters=25, memo_kind=GQ_MEMO_RANK)
+    reqs = [(qgen.snowflake(18, 0x3101), direct), (qgen.star(12, 0x3102), direct), (qgen.cycle(12, 0x3103), direct),
+            (qgen.snowflake(70, 0x3104, depth=4, fanout=5), union), (qgen.chain(16, 0x3105), direct)]
+    with gpuqo_amd.Server(direct, 2 << 30, max_inflight=4) as srv:
+        for i, (g, c) in enumerate(reqs):
+            srv.submit(g, i, c)
+        got = {}
+        while len(got) < len(reqs):
+            r = srv.poll(wait_ms=60000)
+            assert r is not None
+            assert r[1] == 0, f"request {r[0]}: status {r[1]}"
+            got[r[0]] = r[2]
+    for i, (g, c) in enumerate(reqs):
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(got[i]), want, f"request {i} ({g.shape}{g.n}) vs oracle")
