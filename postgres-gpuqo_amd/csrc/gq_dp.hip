@@ -1768,7 +1768,9 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
             np = eval_level<false>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1, bl);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && !h.is_tree) {
+            /* the split-count flag of eval_level; the tree evaluator sets no
+             * flag and its per-wave record queues reuse this LDS word */
             uint32_t *misc = smem + ((h.words + 3) & ~3) + BMAX * 2 + (BMAX + 2) * 2 + BMAX * 2;
             if (misc[1]) atomicOr(&J.counters[49], 1u);
         }
