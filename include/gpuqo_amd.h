@@ -142,6 +142,15 @@ typedef struct gq_config {
     int32_t gpuqo_dpsub_ccc;                      /* on                        */
     int32_t gpuqo_dpsub_csg;                      /* on                        */
     int32_t gpuqo_dpsub_csg_threshold;            /* 128                       */
+    int32_t gpuqo_dpsub_tree;                     /* off: accepted; tree sets
+                                                     are always costed per edge
+                                                     here, in CPU DPsub split
+                                                     order                     */
+    int32_t gpuqo_spanning_tree;                  /* off: plan over the minimum
+                                                     spanning tree of the join
+                                                     graph (weight: base-pair
+                                                     join rows); costs still
+                                                     use every class           */
     int32_t gpuqo_n_parallel;                     /* 10240                     */
     int32_t gpuqo_scratchpad_size_mb;             /* 10                        */
     int32_t gpuqo_min_memo_size_mb;               /* 1                         */

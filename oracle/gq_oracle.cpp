@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <condition_variable>
@@ -1810,7 +1811,7 @@ static QueryTree<W> *run_union(RunCtx &ctx, Info<W> &info) {
     return union_rec<W, W>(ctx, info, lst, ctx.idp_n_iters);
 }
 
-/* __gpuqo_run: gpuqo_main_internal.cu:81-147 (spanning tree mode off) */
+/* __gpuqo_run: gpuqo_main_internal.cu:81-147 (the spanning tree, :94-100, is taken in convert) */
 template <int W>
 static QueryTree<W> *gpuqo_run_t(RunCtx &ctx, Info<W> &info) {
     std::list<RemapEl<W>> bfs = bfs_remap_list(info);
@@ -1856,6 +1857,39 @@ static Bits<W> from_words(const uint64_t *w, int nw) {   /* relation i -> bit i+
     return b;
 }
 
+/* minimumSpanningTree: gpuqo_spanning_tree.cu:17-68.  Grows S from base
+ * relation 0; each round scans i in S ascending and j in edge_table[i] - S
+ * ascending, keeps the first strict minimum of estimate_join_rows of the two
+ * base relations against FLT_MAX, and adds that edge to an empty edge table.
+ * Only the edge table is replaced.  Where no edge passes the test (the
+ * reference then reads an unset argmin) the tree built so far is kept. */
+template <int W>
+static void minimum_spanning_tree(Info<W> &info) {
+    const int n = info.n_rels;
+    Bits<W> S = info.base_rels[0].id, all;
+    for (int i = 0; i < n; i++) all |= info.base_rels[i].id;
+    std::vector<Bits<W>> out(n);
+    while (S != all) {
+        float best = FLT_MAX;
+        int ai = -1, aj = -1;
+        for (int i = 0; i < n; i++) {
+            if (!S.isSet(i + 1)) continue;
+            Bits<W> edges = info.edge_table[i] - S;
+            for (int j = 0; j < n; j++) {
+                if (!edges.isSet(j + 1)) continue;
+                const float rows = estimate_join_rows(info.base_rels[i].id, info.base_rels[i].rows,
+                                                      info.base_rels[j].id, info.base_rels[j].rows, info);
+                if (rows < best) { best = rows; ai = i; aj = j; }
+            }
+        }
+        if (ai < 0) break;
+        S.set(aj + 1);
+        out[ai].set(aj + 1);
+        out[aj].set(ai + 1);
+    }
+    info.edge_table.swap(out);
+}
+
 template <int W>
 static Info<W> *convert(const gq_query *q, const gq_config *cfg) {   /* convertGpuqoPlannerInfo :133-209 */
     Info<W> *info = new Info<W>();
@@ -1888,6 +1922,7 @@ static Info<W> *convert(const gq_query *q, const gq_config *cfg) {   /* convertG
             info->ec_vars.push_back(q->ec_vars[m]);
         }
     }
+    if (cfg->gpuqo_spanning_tree) minimum_spanning_tree(*info);
     return info;
 }
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -467,6 +468,50 @@ static int validate(const gq_query *q) {
     return GQ_OK;
 }
 
+/* minimumSpanningTree (gpuqo_spanning_tree.cu:17-68): Prim's algorithm from
+ * relation 0 on the join graph, each edge (i, j) weighted by the estimated
+ * rows of joining base relations i and j over every class.  Each round takes
+ * the first minimum in (i, j) order over i already in the tree and j outside
+ * it, with the reference's strict `<` against FLT_MAX, so an edge whose rows
+ * are +inf or NaN is never taken.  Only the edge table changes: the classes,
+ * and so every join cost, stay those of the full graph.  A graph in which no
+ * edge can be taken keeps the tree built so far and is then rejected as
+ * disconnected. */
+template <int W>
+static void spanning_tree(HQuery<W> &h) {
+    HostView<W> hv;
+    hv.build(h);
+    std::vector<Set<W>> out(h.n);
+    Set<W> in;
+    in.set(0);
+    for (int added = 1; added < h.n; added++) {
+        float best = FLT_MAX;
+        int bi = -1, bj = -1;
+        for (int i = 0; i < h.n; i++) {
+            if (!bit(in, i)) continue;
+            for (int j = 0; j < h.n; j++) {
+                if (!bit(h.adj[i], j) || bit(in, j)) continue;
+                Set<W> l, r;
+                l.set(i);
+                r.set(j);
+                uint64_t m[GQ_MAX_ECW];
+                match_mask(hv.v, l, r, m);
+                const float rows = join_rows(join_selectivity(hv.v, l, r, m), h.rows[i], h.rows[j]);
+                if (rows < best) {
+                    best = rows;
+                    bi = i;
+                    bj = j;
+                }
+            }
+        }
+        if (bi < 0) break;
+        in.set(bj);
+        out[bi].set(bj);
+        out[bj].set(bi);
+    }
+    h.adj.swap(out);
+}
+
 template <int W>
 static void convert(const gq_query *q, const gq_config *cfg, HQuery<W> &h) {  /* convertGpuqoPlannerInfo :133-209 */
     h.n = q->n_rels;
@@ -502,6 +547,7 @@ static void convert(const gq_query *q, const gq_config *cfg, HQuery<W> &h) {  /*
             h.vars.push_back(q->ec_vars[m]);
         }
     }
+    if (cfg->gpuqo_spanning_tree) spanning_tree(h);
 }
 
 /* ------------------------------------------------------------ output */
@@ -581,6 +627,8 @@ gq_status gq_config_default(gq_config *c) {
     c->gpuqo_dpsub_ccc = 1;                              /* :1123-1131 */
     c->gpuqo_dpsub_csg = 1;                              /* :1133-1141 */
     c->gpuqo_dpsub_csg_threshold = 128;                  /* :2220-2227 */
+    c->gpuqo_dpsub_tree = 0;                             /* :1143-1151 */
+    c->gpuqo_spanning_tree = 0;                          /* :1163-1171 */
     c->gpuqo_n_parallel = 10240;                         /* :2180-2187 */
     c->gpuqo_scratchpad_size_mb = 10;                    /* :2150-2157 */
     c->gpuqo_min_memo_size_mb = 1;                       /* :2170-2177 */

@@ -68,6 +68,8 @@ class Config(C.Structure):
                 ("gpuqo_dpsub_ccc", C.c_int32),
                 ("gpuqo_dpsub_csg", C.c_int32),
                 ("gpuqo_dpsub_csg_threshold", C.c_int32),
+                ("gpuqo_dpsub_tree", C.c_int32),
+                ("gpuqo_spanning_tree", C.c_int32),
                 ("gpuqo_n_parallel", C.c_int32),
                 ("gpuqo_scratchpad_size_mb", C.c_int32),
                 ("gpuqo_min_memo_size_mb", C.c_int32),
@@ -136,6 +138,8 @@ def default_config(**overrides):
     c.gpuqo_dpsub_ccc = 1
     c.gpuqo_dpsub_csg = 1
     c.gpuqo_dpsub_csg_threshold = 128
+    c.gpuqo_dpsub_tree = 0
+    c.gpuqo_spanning_tree = 0
     c.gpuqo_n_parallel = 10240
     c.gpuqo_scratchpad_size_mb = 10
     c.gpuqo_min_memo_size_mb = 1

@@ -1,6 +1,8 @@
 """GPU DP vs the CPU oracle: same join tree, same rows/costs bit for bit."""
 import pytest
 
+import gpuqo_amd
+
 from helpers import assert_same_plan, cfg, gpu_bits, oracle_bits, small_graphs
 from gpuqo_amd import qgen
 
@@ -238,3 +240,48 @@ def test_idp1_tree_memos(kind):
     got, gst = gpu_bits(g, c)
     assert_same_plan(got, want, f"idp1 snowflake-22 memo {kind}")
     assert gst.pairs == ost.pairs
+
+
+SPANNING = [("cycle20", qgen.cycle(20, 0x5a01)), ("chorded22", qgen.chorded_cycle(22, 5, 0x5a02)),
+            ("clique12", qgen.clique(12, 0x5a03)), ("cycle_tail", qgen.cycle_with_tail(9, 8, 0x5a04))]
+
+
+@pytest.mark.parametrize("name,g", SPANNING, ids=[s[0] for s in SPANNING])
+@pytest.mark.parametrize("algo,bicc", [("dpsub", 1), ("dpsub", 0), ("dpsize", 0)], ids=["mpdp", "dpsub", "dpsize"])
+def test_spanning_tree_mode(name, g, algo, bicc):
+    """gpuqo_spanning_tree: the DP runs on the minimum spanning tree, a tree
+    graph whose classes are not all tree edges (the tree kernels take the
+    general crossing-class test)."""
+    if algo == "dpsize" and g.n > 14:
+        pytest.skip("DPsize oracle at this size takes minutes")
+    c = cfg(algo, bicc, spanning_tree=1)
+    want, _ = oracle_bits(g, c)
+    got, st = gpu_bits(g, c)
+    assert_same_plan(got, want, f"{name} {algo} spanning tree")
+
+
+@pytest.mark.parametrize("kind", [2, 3], ids=["hash", "rank"])
+def test_spanning_tree_mode_tree_memos(kind):
+    g = qgen.chorded_cycle(24, 5, 0x5a05)
+    c = cfg("dpsub", 1, spanning_tree=1, memo_kind=kind)
+    want, _ = oracle_bits(g, c)
+    got, st = gpu_bits(g, c)
+    assert st.memo_kind == kind
+    assert_same_plan(got, want, f"chorded24 spanning tree memo {kind}")
+
+
+@pytest.mark.parametrize("idp,k", [("idp2", 10), ("union", 25), ("idp1", 8)])
+def test_spanning_tree_mode_under_idp(idp, k):
+    g = qgen.chorded_cycle(40, 7, 0x5a06) if idp == "union" else qgen.chorded_cycle(22, 5, 0x5a07)
+    c = cfg("dpsub", 1, spanning_tree=1, idp_type=idp, idp_n_iters=k)
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, f"{idp} k={k} spanning tree")
+
+
+def test_spanning_tree_mode_batched():
+    qs = [g for _, g in SPANNING] + [qgen.cycle(14, 0x5a08 + i) for i in range(6)]
+    c = cfg("dpsub", 1, spanning_tree=1)
+    for g, (nodes, _) in zip(qs, gpuqo_amd.plan_batch(qs, c)):
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"{g.shape}{g.n} batched spanning tree")

@@ -109,3 +109,22 @@ def test_server_burst_of_mixed_requests():
     for i, (g, c) in enumerate(reqs):
         alone, _ = gpu_bits(g, c)
         assert_same_plan(gpuqo_amd.plan_bits(got[i]), alone, f"request {i} ({g.shape}{g.n})")
+
+
+def test_server_spanning_tree_requests_beside_others():
+    """Requests planned over their minimum spanning tree (trees whose classes
+    are not all tree edges) share engine steps with ordinary tree and cyclic
+    requests; each equals the oracle."""
+    import gpuqo_amd
+    span = cfg("dpsub", 1, spanning_tree=1)
+    span_dpsub = cfg("dpsub", 0, spanning_tree=1)
+    direct = cfg("dpsub", 1)
+    reqs = [(qgen.cycle(n, 0x5b00 + n), span) for n in (10, 14, 18)]
+    reqs += [(qgen.chorded_cycle(20, 5, 0x5b20), span), (qgen.clique(11, 0x5b11), span_dpsub),
+             (qgen.chorded_cycle(40, 7, 0x5b40), cfg("dpsub", 1, spanning_tree=1, idp_type="union", idp_n_iters=25)),
+             (qgen.snowflake(18, 0x5b18), direct), (qgen.cycle(12, 0x5b12), direct), (qgen.star(12, 0x5b0c), direct)]
+    with gpuqo_amd.Server(direct, 4 << 30, max_inflight=8) as srv:
+        got = _run(srv, reqs, 0.0)
+    for i, (g, c) in enumerate(reqs):
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(got[i]), want, f"request {i} ({g.shape}{g.n})")

@@ -3,15 +3,17 @@ its code: closed-form pair counts, brute-force enumeration, textbook graph
 algorithms, hand-derived cost-model answers and an independent restatement
 of the DP recurrence.  The reference ships no golden vectors for this path
 and cannot be compiled here, so these are the oracle's anchors."""
+import copy
 import itertools
 from collections import deque
 
+import numpy as np
 import pytest
 
 import gpuqo_amd
 import oracle
 from gpuqo_amd import abi, qgen
-from helpers import small_graphs
+from helpers import assert_same_plan, oracle_bits, small_graphs
 
 
 def cfg(algo, **kw):
@@ -450,3 +452,75 @@ def test_dpe_dpccp_returns_a_full_tree(threads, depbuf):
     assert nodes[0][3] >= best[0][3]
     root = nodes[0][0]
     assert sum(bin(w).count("1") for w in root) == g.n
+
+
+def _pair_rows(g, c, i, j):
+    """Estimated rows of joining base relations i and j (the oracle's cost model)."""
+    nw = g.n_words
+    lw, rw = [0] * nw, [0] * nw
+    lw[i // 64] |= 1 << (i % 64)
+    rw[j // 64] |= 1 << (j % 64)
+    b = g.build().base
+    return oracle.join_cost(g, c, lw, rw, (b[i]["rows"], 0.0, 0.0, 4), (b[j]["rows"], 0.0, 0.0, 4))[0]
+
+
+def _with_edges(g, edges):
+    """A copy of g whose edge table holds exactly `edges` (classes unchanged)."""
+    g2 = copy.copy(g.build())
+    et = np.zeros_like(g.edge_table)
+    for a, b in edges:
+        et[a, b // 64] |= np.uint64(1 << (b % 64))
+        et[b, a // 64] |= np.uint64(1 << (a % 64))
+    g2.edge_table = et
+    g2._c = None
+    return g2
+
+
+def _kruskal(g, c):
+    """Minimum spanning tree by Kruskal on the base-pair rows; None on a tie."""
+    w = sorted((_pair_rows(g, c, a, b), min(a, b), max(a, b)) for a, b, _, _ in g.edges)
+    if len({x[0] for x in w}) != len(w):
+        return None
+    up = list(range(g.n))
+
+    def find(x):
+        while up[x] != x:
+            x = up[x]
+        return x
+    tree = []
+    for _, a, b in w:
+        ra, rb = find(a), find(b)
+        if ra != rb:
+            up[ra] = rb
+            tree.append((a, b))
+    return tree
+
+
+@pytest.mark.parametrize("name,g", [("cycle9", qgen.cycle(9, 0x5901)), ("clique7", qgen.clique(7, 0x5904)),
+                                    ("chorded12", qgen.chorded_cycle(12, 4, 0x5903)),
+                                    ("cycle_tail", qgen.cycle_with_tail(6, 5, 0x5903))])
+@pytest.mark.parametrize("algo,bicc", [("dpsub", 0), ("dpsub", 1), ("dpsize", 0)])
+def test_spanning_tree_mode_is_the_dp_over_the_minimum_spanning_tree(name, g, algo, bicc):
+    """gpuqo_spanning_tree: with distinct base-pair rows the minimum spanning
+    tree is unique, so Prim's tree in the oracle must be Kruskal's here; the
+    plan must be the plain DP over a graph with that edge table and the same
+    classes (the costs still see every class).  The seeds are ones whose
+    tree-restricted optimum differs from the full graph's, so a mode that
+    changed nothing would fail."""
+    c = cfg(algo, dpsub_bicc=bicc)
+    tree = _kruskal(g, c)
+    assert tree is not None and len(tree) == g.n - 1, name
+    want, _ = oracle_bits(_with_edges(g, tree), c)
+    got, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc, spanning_tree=1))
+    assert_same_plan(got, want, f"{name} {algo} bicc={bicc}")
+    full, _ = oracle_bits(g, c)
+    assert full != got, f"{name}: the spanning tree did not change the plan"
+    assert full[0][3] <= got[0][3], "the full graph's optimum cannot cost more than the tree's"
+
+
+def test_spanning_tree_mode_leaves_trees_alone():
+    for g in (qgen.snowflake(14, 0x5905), qgen.chain(9, 0x5906), qgen.star(8, 0x5907)):
+        for algo, bicc in (("dpsub", 0), ("dpsub", 1), ("dpsize", 0)):
+            a, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc))
+            b, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc, spanning_tree=1))
+            assert_same_plan(b, a, f"{g.shape}{g.n} {algo}")
