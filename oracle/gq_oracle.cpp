@@ -1,8 +1,8 @@
 /*
  * gq_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
  *
- * CPU restatement of the reference's join-order search, used as the parity
- * checker for the GPU product and as the timed CPU baseline.  Only tests/,
+ * CPU restatement of the reference's join-order search: the expected plans
+ * in the GPU product's parity tests and the timed CPU baseline.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
  * product (postgres-gpuqo_amd/) never links or calls it.
  *

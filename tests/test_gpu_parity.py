@@ -285,3 +285,14 @@ def test_spanning_tree_mode_batched():
     for g, (nodes, _) in zip(qs, gpuqo_amd.plan_batch(qs, c)):
         want, _ = oracle_bits(g, c)
         assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"{g.shape}{g.n} batched spanning tree")
+
+
+def test_dpsub_tree_flag_is_accepted_and_changes_no_plan():
+    """gpuqo_dpsub_tree only selects the reference's per-edge enumeration of
+    tree sets; here tree sets are always costed per edge in the CPU DPsub
+    split order, so the flag changes no plan."""
+    for g in (qgen.snowflake(16, 0x5c01), qgen.cycle(12, 0x5c02), qgen.chorded_cycle(14, 4, 0x5c03)):
+        for bicc in (0, 1):
+            want, _ = oracle_bits(g, cfg("dpsub", bicc))
+            got, _ = gpu_bits(g, cfg("dpsub", bicc, dpsub_tree=1))
+            assert_same_plan(got, want, f"{g.shape}{g.n} bicc={bicc} dpsub_tree")
