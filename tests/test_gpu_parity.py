@@ -296,3 +296,14 @@ def test_dpsub_tree_flag_is_accepted_and_changes_no_plan():
             want, _ = oracle_bits(g, cfg("dpsub", bicc))
             got, _ = gpu_bits(g, cfg("dpsub", bicc, dpsub_tree=1))
             assert_same_plan(got, want, f"{g.shape}{g.n} bicc={bicc} dpsub_tree")
+
+
+@pytest.mark.parametrize("idp,k", [("union", 25), ("idp2", 12)])
+def test_spanning_tree_mode_past_64_relations(idp, k):
+    """80 relations: the query's sets take more than one 64-bit word on the
+    host, so the spanning tree is built over the wide set type."""
+    g = qgen.chorded_cycle(80, 9, 0x5d50)
+    c = cfg("dpsub", 1, spanning_tree=1, idp_type=idp, idp_n_iters=k)
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, f"chorded80 {idp} k={k} spanning tree")
