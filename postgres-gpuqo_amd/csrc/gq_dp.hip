@@ -3420,7 +3420,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         unsigned long long *best_k = c.b.lvl_best + off_h[k];
         uint32_t *first_k = c.b.lvl_first + off_h[k];
         uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
-        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
+        /* trees: one launch per level, timed as one span from the first
+         * level to the last (events between launches would add their own
+         * gaps to every level) */
+        if (profile && (!h.is_tree || k == 2)) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         const uint64_t T = (1ull << k) - 2;
         if (h.is_tree) {
             /* exact level size of a tree (subtree polynomial), else C(n,k) */
@@ -3470,7 +3473,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sets_k,
                                best_k, first_k, c.b.vals, c.b.left, c.b.counters);
         }
-        if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+        if (profile && (!h.is_tree || k == K)) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     if (K < n) {                                   /* IDP1: the cheapest set of K relations */
@@ -3503,11 +3506,18 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipEventElapsedTime(&a, c.prof_ev[0], c.prof_ev[1]));
         st.filter_ms = a;
         st.filter_launches = 1;
-        for (int k = 2; k <= K; k++) {
+        if (h.is_tree && K >= 2) {
             float b = 0.0f;
-            HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
-            st.eval_ms += b;
-            st.eval_launches++;
+            HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * 2 + 1], c.prof_ev[3 * K + 2]));
+            st.eval_ms = b;
+            st.eval_launches = K - 1;
+        } else {
+            for (int k = 2; k <= K; k++) {
+                float b = 0.0f;
+                HIPCHK(hipEventElapsedTime(&b, c.prof_ev[3 * k + 1], c.prof_ev[3 * k + 2]));
+                st.eval_ms += b;
+                st.eval_launches++;
+            }
         }
     }
     st.candidates = cand_total;
