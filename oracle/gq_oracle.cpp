@@ -2005,6 +2005,58 @@ int oracle_join_cost(const gq_query *q, const gq_config *cfg, const uint64_t *l_
     return GQ_OK;
 }
 
+/* One branch of the restated cost model, exposed for known-answer tests:
+ *   which 0: calc_join_cost (gpuqo_cost.cuh:76-158) with the relations in
+ *            composite_mask (bit i = relation i) marked composite;
+ *   which 1: cost_nestloop (cost_postgres.cuh:110-275) and
+ *   which 2: cost_hashjoin (:287-470) with CostExtra taken from extra_in =
+ *            {inner_unique, indexed_join_quals, joinrows, outer_match_frac,
+ *            match_count};
+ *   which 3: estimate_hash_bucket_stats (:620-668), nbuckets = extra_in[0];
+ *   which 4: is_inner_unique (gpuqo_cost.cuh:43-74).
+ * out = {startup, total} (0-2), {mcvfreq, bucketsize} (3), {flag} (4). */
+int oracle_cost_probe(const gq_query *q, const gq_config *cfg, int which, const uint64_t *l_words,
+                      const uint64_t *r_words, const float *lvals, const float *rvals, const float *extra_in,
+                      uint64_t composite_mask, float *out) {
+    if (q->n_rels >= 64) return GQ_ERR_UNSUPPORTED;
+    oracle::Info<1> *info = oracle::convert<1>(q, cfg);
+    for (int i = 0; i < q->n_rels; i++) info->base_rels[i].composite = (composite_mask >> i) & 1ULL;
+    oracle::Bits<1> l = oracle::from_words<1>(l_words, q->n_words), r = oracle::from_words<1>(r_words, q->n_words);
+    oracle::JoinRel L, R;
+    L.rows = lvals[0]; L.cost.startup = lvals[1]; L.cost.total = lvals[2]; L.width = (int)lvals[3];
+    R.rows = rvals[0]; R.cost.startup = rvals[1]; R.cost.total = rvals[2]; R.width = (int)rvals[3];
+    oracle::CostExtra ex;
+    ex.inner_unique = extra_in[0] != 0.0f; ex.indexed_join_quals = extra_in[1] != 0.0f;
+    ex.joinrows = extra_in[2]; ex.outer_match_frac = extra_in[3]; ex.match_count = extra_in[4];
+    int st = GQ_OK;
+    switch (which) {
+    case 0: {
+        oracle::PathCost c = oracle::calc_join_cost(l, L, r, R, extra_in[2], *info);
+        out[0] = c.startup; out[1] = c.total;
+        break;
+    }
+    case 1: { oracle::PathCost c = oracle::cost_nestloop(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
+    case 2: { oracle::PathCost c = oracle::cost_hashjoin(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
+    case 3: {
+        oracle::BucketStats b = oracle::estimate_hash_bucket_stats(l, r, (int)extra_in[0], *info);
+        out[0] = b.mcvfreq; out[1] = b.bucketsize;
+        break;
+    }
+    case 4: out[0] = oracle::is_inner_unique(l, r, *info) ? 1.0f : 0.0f; break;
+    default: st = GQ_ERR_INVALID;
+    }
+    delete info;
+    return st;
+}
+
+/* __estimate_hash_bucketsize (cost_postgres.cuh:701-775) of one variable of
+ * a base relation with the given rows and tuples, for known-answer tests. */
+float oracle_hash_bucketsize(const gq_var_info *v, float rows, float tuples, int nbuckets) {
+    oracle::BaseRel<1> b;
+    b.rows = rows; b.tuples = tuples; b.width = 0; b.cost.startup = 0.0f; b.cost.total = 0.0f; b.composite = false;
+    return oracle::estimate_hash_bucketsize<1>(*v, b, nbuckets);
+}
+
 /* ExecChooseHashTableSize, exposed for known-answer tests. */
 void oracle_hash_table_size(float ntuples, int width, int work_mem, int *nbuckets, int *nbatches, int *nskew) {
     oracle::HtszRes r = oracle::exec_choose_hash_table_size(ntuples, width, work_mem);

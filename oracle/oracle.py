@@ -37,6 +37,13 @@ def lib():
         L.oracle_blocks.restype = C.c_int
         L.oracle_dpsize_level1_order.argtypes = [C.c_int, C.POINTER(C.c_int)]
         L.oracle_dpsize_level1_order.restype = C.c_int
+        L.oracle_cost_probe.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.c_int,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint64,
+                                        C.POINTER(C.c_float)]
+        L.oracle_cost_probe.restype = C.c_int
+        L.oracle_hash_bucketsize.argtypes = [C.POINTER(abi.VarInfo), C.c_float, C.c_float, C.c_int]
+        L.oracle_hash_bucketsize.restype = C.c_float
         _lib = L
     return _lib
 
@@ -63,6 +70,35 @@ def join_cost(query, cfg, l_words, r_words, lvals, rvals):
     st = lib().oracle_join_cost(C.byref(query.to_c()), C.byref(cfg), lw, rw, lv, rv, ov)
     assert st == 0
     return tuple(ov)
+
+
+PROBE_JOIN, PROBE_NESTLOOP, PROBE_HASHJOIN, PROBE_BUCKET_STATS, PROBE_INNER_UNIQUE = range(5)
+
+
+def cost_probe(query, cfg, which, l_words, r_words, lvals=(0, 0, 0, 0), rvals=(0, 0, 0, 0), extra=(0, 0, 0, 0, 0),
+               composite_mask=0):
+    """One branch of the oracle's cost model (see oracle_cost_probe):
+    which = PROBE_JOIN (extra[2] = join rows), PROBE_NESTLOOP / PROBE_HASHJOIN
+    (extra = inner_unique, indexed_join_quals, joinrows, outer_match_frac,
+    match_count), PROBE_BUCKET_STATS (extra[0] = nbuckets) or
+    PROBE_INNER_UNIQUE.  Returns two floats (one for PROBE_INNER_UNIQUE)."""
+    nw = query.n_words
+    lw = (C.c_uint64 * nw)(*l_words)
+    rw = (C.c_uint64 * nw)(*r_words)
+    lv = (C.c_float * 4)(*lvals)
+    rv = (C.c_float * 4)(*rvals)
+    ex = (C.c_float * 5)(*extra)
+    out = (C.c_float * 2)()
+    st = lib().oracle_cost_probe(C.byref(query.to_c()), C.byref(cfg), which, lw, rw, lv, rv, ex, composite_mask, out)
+    assert st == 0
+    return out[0] if which == PROBE_INNER_UNIQUE else (out[0], out[1])
+
+
+def hash_bucketsize(stadistinct, stanullfrac, mcvfreq, rows, tuples, nbuckets):
+    from gpuqo_amd import abi
+    v = abi.VarInfo()
+    v.stadistinct, v.stanullfrac, v.mcvfreq = stadistinct, stanullfrac, mcvfreq
+    return lib().oracle_hash_bucketsize(C.byref(v), rows, tuples, nbuckets)
 
 
 def hash_table_size(ntuples, width, work_mem=4096):

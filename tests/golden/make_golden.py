@@ -8,7 +8,13 @@ mismatch), the configuration, and the expected plan: per node in pre-order
 the relation-set words, rows/startup/total as IEEE-754 bit patterns, width
 and child indices; plus the pair and set counters.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # plans_v1.json
+    python tests/golden/make_golden.py --large    # plans_large_v1.json
+
+The large set holds the full-size workloads that take the oracle minutes:
+cpu_dpsub and cpu_dpsize on the 18-relation clique (about 3.9e8 csg-cmp
+pairs each, 153 classes) and cpu_dpsub_bicc on every query of the
+256-query mixed batch.  Its records are written without indentation.
 """
 import hashlib
 import json
@@ -63,7 +69,49 @@ def make(label, gen, args, kwargs, over):
             "pairs": stats.pairs, "sets": stats.sets}
 
 
+MPDP = {"algorithm": "dpsub", "dpsub_bicc": 1}
+LARGE_CASES = [
+    ("clique18-dpsub", "query_clique18", [], {}, {"algorithm": "dpsub", "dpsub_bicc": 0}),
+    ("clique18-dpsize", "query_clique18", [], {}, {"algorithm": "dpsize"}),
+]
+
+
+def make_batch_member(i):
+    """Query i of the 256-query mixed batch under MPDP (the generator call is
+    recorded as queries_batch256 plus the index)."""
+    g = qgen.queries_batch256()[i]
+    st, nodes, stats = oracle.plan(g, abi.default_config(**MPDP))
+    assert st == 0, (i, st)
+    plan = [[list(r), a, b, c, w, l, rr] for (r, a, b, c, w, l, rr) in gpuqo_amd.plan_bits(nodes)]
+    return {"label": f"batch256-{i:03d}-{g.shape}{g.n}", "generator": "queries_batch256", "index": i,
+            "n": g.n, "input_sha256": digest(g), "config": MPDP, "plan": plan,
+            "pairs": stats.pairs, "sets": stats.sets}
+
+
+def _make_large(c):
+    return make(*c)
+
+
+def main_large(procs):
+    from multiprocessing import Pool
+    with Pool(procs) as pool:
+        big = pool.map_async(_make_large, LARGE_CASES, chunksize=1)
+        batch = pool.map(make_batch_member, range(256), chunksize=4)
+        recs = big.get() + batch
+    out = {"format": "postgres-gpuqo_amd golden plans v1 (full-size workloads)",
+           "float_encoding": "IEEE-754 binary32 bit patterns",
+           "records": recs}
+    path = os.path.join(HERE, "plans_large_v1.json")
+    with open(path, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+        f.write("\n")
+    print(f"wrote {len(recs)} records to {path}")
+
+
 def main():
+    if "--large" in sys.argv:
+        main_large(int(os.environ.get("GOLDEN_PROCS", "6")))
+        return
     out = {"format": "postgres-gpuqo_amd golden plans v1",
            "float_encoding": "IEEE-754 binary32 bit patterns",
            "records": [make(*c) for c in CASES]}
