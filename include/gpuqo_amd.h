@@ -42,7 +42,10 @@ typedef enum gq_status {
     GQ_ERR_CAPACITY = 5,     /* output buffer too small                    */
     GQ_ERR_NO_PLAN = 6,      /* query graph disconnected: no join tree     */
     GQ_ERR_INTERRUPTED = 7,  /* interrupt flag raised (CHECK_FOR_INTERRUPTS) */
-    GQ_ERR_BUSY = 8          /* server queue full                          */
+    GQ_ERR_BUSY = 8,         /* server queue full                          */
+    GQ_ERR_INTERNAL = 9      /* unexpected internal failure (e.g. a thread
+                                could not be started); no C++ exception
+                                ever crosses this interface              */
 } gq_status;
 
 /* gpuqo_algorithm values: src/include/optimizer/gpuqo_common.h:13-29. */

@@ -1861,6 +1861,8 @@ static Bits<W> from_words(const uint64_t *w, int nw) {   /* relation i -> bit i+
  * relation 0; each round scans i in S ascending and j in edge_table[i] - S
  * ascending, keeps the first strict minimum of estimate_join_rows of the two
  * base relations against FLT_MAX, and adds that edge to an empty edge table.
+ * estimate_join_rows clamps a NaN product to 1 row, so an edge with a NaN
+ * selectivity weighs 1; only +inf rows are never taken.
  * Only the edge table is replaced.  Where no edge passes the test (the
  * reference then reads an unset argmin) the tree built so far is kept. */
 template <int W>

@@ -1,41 +1,60 @@
 /*
- * gq_dp.hip -- DPsub / MPDP level loop on CDNA4 (gfx950, wave64).
+ * gq_dp.hip -- the device side of the join-order search on CDNA4 (gfx950,
+ * wave64): DPsub / MPDP and DPsize level loops, their batched and
+ * step-engine forms, and the plan walk.
  *
- * One DP over n <= 32 relations (relation i = bit i of a uint32_t set).
- * Per level k (k = 2..n) three kernels are enqueued back to back on one
- * stream, with no host synchronisation between levels:
+ * One DP covers n <= 32 relations (relation i = bit i of a uint32_t set).
+ * Its level lists come from one of two producers:
+ *   k_filter_all   every integer in [0, 2^n) is a candidate; a wave tests 64
+ *                  consecutive ones for connectivity (O(1) for trees, a
+ *                  closure from the lowest member otherwise), sets their
+ *                  bits in the connectivity bitmap and compacts the
+ *                  connected ones of each size through LDS into the level
+ *                  lists, one global position reservation per workgroup and
+ *                  level (lists are increasing within a workgroup's chunk;
+ *                  chunks land in completion order -- no result depends on
+ *                  the order of a level list);
+ *   expansion      tree graphs: each connected set of k+1 relations is built
+ *                  once from the set of its k lowest-numbered members
+ *                  (k_tree_start, the expansion workgroups of k_eval_tree,
+ *                  k_expand), so no 2^n pass runs.
+ * Each level is then one launch (no host synchronisation between levels):
+ *   k_eval_tree       trees: 2(k-1) splits per set, per-edge cost tables;
+ *   k_eval_wg         general graphs: MPDP block decomposition or DPsub
+ *                     subset scan, valid pairs queued in LDS;
+ *   k_big_scan/eval   MPDP sets with very large biconnected blocks;
+ *   k_eval_dpsub_chunk + k_finalize_dpsub   DPsub levels with > 4096
+ *                     candidate splits per set;
+ *   k_ds_*            DPsize: tile-parallel pair scan in the CPU's
+ *                     creation order.
+ * k_tree walks the memo from the root to the plan.
  *
- *   k_filter      unrank the C(n,k) candidate sets (combinadic rank ->
- *                 k-subset in increasing numeric order, then Gosper's next
- *                 permutation), keep the connected ones (closure over the
- *                 LDS adjacency table), compact them with one atomic per wave
- *                 and mark them in the connectivity bitmap.
- *   k_eval_*      evaluate the splits of every kept set with the cost model
- *                 staged in LDS, reading the memo of the smaller levels.
- *   k_finalize    (split-parallel mode only) rebuild the winning split.
- *
- * The memo is dense: entry S lives at index S (a perfect hash; 2^n entries
- * of 16 B plus a 4 B left child), which fits HBM up to n = 32.  Ties and NaN
- * costs follow the reference CPU oracles: per set the first valid split in
- * their enumeration order is kept unless a later one is strictly cheaper
- * (gpuqo_cpu_common.cuh:189-211), which the kernels reproduce either with a
- * sequential scan inside one lane or with an associative (first, firstmin)
- * reduction over packed (cost, rank) keys.
+ * Memos (chosen per DP): dense -- entry S at index S, 16 B of value plus a
+ * 4 B left child, up to 2^32 entries in HBM; rank -- tree graphs, one slot
+ * per connected set computed from the set; open addressing -- tree graphs,
+ * linear probing.  Ties and NaN costs follow the reference CPU oracles: per
+ * set the first valid split in their enumeration order is kept unless a
+ * later one is strictly cheaper (gpuqo_cpu_common.cuh:189-211), which the
+ * kernels reproduce with an associative (first valid, first minimum)
+ * reduction over packed (cost bits, rank) keys.
  *
  * Reference counterparts: gpuqo_dpsub_common.cu:177-392 (level loop),
  * gpuqo_dpsub_filtered_kernels.cuh:30-240 (unrank/filter/evaluate),
  * gpuqo_dpsub_enum_all_subs.cuh:21-72 (all-subsets splits),
  * gpuqo_dpsub_bicc.cuh:35-313 (MPDP), gpuqo_cpu_dpsub_bicc.cu:65-236 (the
- * block order reproduced here).
+ * block order reproduced here), gpuqo_dpsize.cu:292-671 and
+ * gpuqo_cpu_dpsize.cu:27-93 (DPsize and its creation order).
  */
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -55,7 +74,6 @@ namespace gq {
  * natural alignment; kernels using it declare no static __shared__. */
 #define GQ_LDS_ALIGN __attribute__((aligned(16)))
 
-__constant__ uint32_t c_binom[33 * 33];   /* C(n,k) at [n*33+k], n,k <= 32 */
 
 struct DevBufs {
     float4 *vals;          /* memo: rows, startup, total, width(bits) */
@@ -261,28 +279,6 @@ __device__ __forceinline__ uint32_t ord_key(float x) {
     if (x == 0.0f) x = 0.0f;
     uint32_t u = __float_as_uint(x);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-/* r-th k-subset of {0..n-1} in increasing numeric order (colex ranking) */
-__device__ __forceinline__ uint32_t unrank(uint32_t r, int k, int n) {
-    uint32_t s = 0;
-    int j = k;
-    for (int pos = n - 1; pos >= 0 && j > 0; pos--) {
-        uint32_t c = c_binom[pos * 33 + j];
-        if (c <= r) {
-            s |= 1u << pos;
-            r -= c;
-            j--;
-        }
-    }
-    return s;
-}
-
-__device__ __forceinline__ uint32_t gosper(uint32_t v) {
-    uint64_t x = v;
-    uint64_t t = x | (x - 1);
-    uint64_t nx = (t + 1) | (((~t & (t + 1)) - 1) >> (__builtin_ctzll(x) + 1));
-    return (uint32_t)nx;
 }
 
 /* Adds the wave's pair count to counters[40..41]: one atomic per wave.  All
@@ -1142,9 +1138,20 @@ struct BigList {
  *     s_subb[BMAX_B][BSET_W] (dfs_blocks)
  *   DPsub: q_l, q_i, q_r [QCAP] each, then the staged connectivity bitmap. */
 #define BMAX_B 64
+/* byte offsets of the two 64-bit slot arrays (LDS atomics on them need
+ * 8-byte alignment: the dynamic LDS base is 16-byte aligned, GQ_LDS_ALIGN) */
+__host__ __device__ constexpr size_t eval_off_best(int words) { return (size_t)(((words + 3) & ~3) * 4); }
+__host__ __device__ constexpr size_t eval_off_off(int words) { return eval_off_best(words) + BMAX * 8; }
 __host__ __device__ constexpr size_t eval_lds_common(int words) {
-    return (size_t)(((words + 3) & ~3) * 4) + BMAX * 8 + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
+    return eval_off_off(words) + (BMAX + 2) * 8 + BMAX * 4 * 2 + 16;
 }
+__host__ __device__ constexpr bool eval_slots_aligned(int words) {
+    return eval_off_best(words) % 8 == 0 && eval_off_off(words) % 8 == 0 && eval_lds_common(words) % 4 == 0;
+}
+static_assert(eval_slots_aligned(1) && eval_slots_aligned(2) && eval_slots_aligned(3) && eval_slots_aligned(4) &&
+                  eval_slots_aligned(5) && eval_slots_aligned(4093) && eval_slots_aligned(16383),
+              "64-bit LDS slots of the level evaluator must be 8-byte aligned for every blob size");
+static_assert(BMAX % 2 == 0, "s_off must start 8-byte aligned after s_best");
 __host__ __device__ constexpr size_t eval_lds_bicc(int words) {
     return eval_lds_common(words) + BMAX * 4 + (size_t)BMAX_B * 32 * 4 * 2 + (size_t)BMAX_B * BSET_W * 4 * 2;
 }
@@ -1165,9 +1172,8 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
     const bool tree = h.is_tree != 0;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
-    const int W4 = (h.words + 3) & ~3;
-    unsigned long long *s_best = (unsigned long long *)(smem + W4);
-    unsigned long long *s_off = s_best + BMAX;                 /* BMAX + 2 entries */
+    unsigned long long *s_best = (unsigned long long *)((char *)smem + eval_off_best(h.words));
+    unsigned long long *s_off = (unsigned long long *)((char *)smem + eval_off_off(h.words));   /* BMAX + 2 */
     uint32_t *s_first = (uint32_t *)(s_off + BMAX + 2);
     uint32_t *s_set = s_first + BMAX;
     uint32_t *s_misc = s_set + BMAX;                           /* [0] queue length, [1] error */
@@ -2337,11 +2343,11 @@ struct DeviceCtx {
      * default priority) and the per-level events that order the two */
     hipStream_t stream2 = nullptr;
     hipEvent_t lvl_ev[2][34] = {};
+    hipEvent_t gate_ev[34] = {};        /* level completions of a paced DP (LevelGate) */
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBufs b{};
     size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
            cap_nodes = 0, cap_counters = 0;
-    bool binom_ready = false;
     std::vector<hipEvent_t> prof_ev;
     DsBufs ds{};
     int *l1 = nullptr;
@@ -2403,14 +2409,10 @@ static int ctx_init(DeviceCtx &c, int device) {
     HIPCHK(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
     for (int a = 0; a < 2; a++)
         for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.lvl_ev[a][k], hipEventDisableTiming));
+    for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.gate_ev[k], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c.ev0));
     HIPCHK(hipEventCreate(&c.ev1));
-    uint32_t tab[33 * 33];
-    for (int n = 0; n <= 32; n++)
-        for (int k = 0; k <= 32; k++) tab[n * 33 + k] = (uint32_t)binom64(n, k);
-    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tab, sizeof(tab)));
     c.device = device;
-    c.binom_ready = true;
     /* counters[0..63] and the plan nodes right after them: one read-back */
     int rc = ensure(&c.b.counters, c.cap_counters, RES_WORDS);
     if (rc) return rc;
@@ -2418,6 +2420,54 @@ static int ctx_init(DeviceCtx &c, int device) {
     /* coherent: the plan walk writes the results straight into it */
     HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocMapped | hipHostMallocCoherent));
     return GQ_OK;
+}
+
+/* CHECK_FOR_INTERRUPTS between the levels of one DP (the reference polls it
+ * per level and per scatter: gpuqo_dpsub_common.cu:81,127,156,300;
+ * gpuqo_dpsize.cu:451,486,536,554).  The host tests the interrupt flag
+ * before it enqueues each level.  A paced DP -- one expected to run for
+ * milliseconds -- also keeps at most two levels in flight: before level k
+ * is enqueued the host waits for level k-2 to finish, testing the flag
+ * every 50 us, so an interrupt raised while the DP runs stops it within
+ * two levels.  Levels already enqueued always run to completion (no kernel
+ * is cut short); the call then returns GQ_ERR_INTERRUPTED. */
+struct LevelGate {
+    DeviceCtx &c;
+    hipStream_t s;
+    bool pace;
+    int before(int k) {
+        if (pace && k >= 4 && k - 2 < 34) {
+            for (;;) {
+                const hipError_t q = hipEventQuery(c.gate_ev[k - 2]);
+                if (q == hipSuccess) break;
+                if (q != hipErrorNotReady) {
+                    set_error("HIP error %s while waiting for level %d", hipGetErrorString(q), k - 2);
+                    return GQ_ERR_DEVICE;
+                }
+                if (interrupted()) break;
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+            }
+        }
+        if (interrupted()) {
+            (void)hipStreamSynchronize(s);
+            set_error("interrupted before level %d", k);
+            return GQ_ERR_INTERRUPTED;
+        }
+        return GQ_OK;
+    }
+    int after(int k) {
+        if (pace && k < 34) HIPCHK(hipEventRecord(c.gate_ev[k], s));
+        return GQ_OK;
+    }
+};
+
+/* DPs that run long enough to pace: general graphs of 16 or more
+ * relations (a 16-clique plans in about 5 ms), DPsize from 14, trees from
+ * 24 (tens of millions of connected sets). */
+static bool paced_dp(const BlobHdr &h, DPAlgo algo) {
+    const int n = h.n_rels;
+    if (algo == DP_DPSIZE) return n >= 14;
+    return h.is_tree ? n >= 24 : n >= 16;
 }
 
 /* pinned staging buffer of at least `words` words */
@@ -2601,6 +2651,12 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
     HIPCHK(hipEventRecord(c.lvl_ev[0][1], s));
     HIPCHK(hipStreamWaitEvent(s2, c.lvl_ev[0][1], 0));
     for (int k = 2; k <= kmax; k++) {
+        if (interrupted()) {                       /* CHECK_FOR_INTERRUPTS per level */
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamSynchronize(s2);
+            set_error("interrupted before batch level %d", k);
+            return GQ_ERR_INTERRUPTED;
+        }
         /* grids: enough workgroups for the largest level, capped */
         uint64_t tasks_tree = 0, ub_gen = 0;
         for (int j = 0; j < nj; j++) {
@@ -2917,7 +2973,9 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
                        0, s, c.hkeys, cap, 0u);
     hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, d_sj, 1, nullptr, nullptr);
+    LevelGate gate{c, s, paced_dp(h, algo)};
     for (int k = 2; k <= K; k++) {
+        if ((rc = gate.before(k))) return rc;
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(tasks[k], 4096))), dim3(BLOCK), 0,
                            s, d_sj, 1, k, d_prefix + 2 * k);
@@ -2925,6 +2983,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
         launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, sj.bicc, c.b.lvl_sets + off_h[k],
                          c.hslot + off_h[k], mr, c.b.counters);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+        if ((rc = gate.after(k))) return rc;
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     if (K < n) {                                   /* IDP1: the cheapest set of K relations */
@@ -3035,12 +3094,15 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev0, s));
     hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
                        c.b.lvl_sets + off_h[2], c.b.counters, c.hslot + off_h[2]);
+    LevelGate gate{c, s, paced_dp(h, algo)};
     for (int k = 2; k <= K; k++) {
+        if ((rc = gate.before(k))) return rc;
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
         launch_eval_tree(h, in.tree_counts[k], lds_eval, s, c.b.blob, k, bicc, c.b.lvl_sets + off_h[k],
                          c.hslot + off_h[k], mr, c.b.counters, k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr,
                          k < K ? c.hslot + off_h[k + 1] : nullptr);
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+        if ((rc = gate.after(k))) return rc;
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     if (K < n) {                                   /* IDP1: the cheapest set of K relations */
@@ -3277,12 +3339,13 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     }
     if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
     HIPCHK(hipEventRecord(c.ev0, s));
-    /* Trees with exact level sizes and a memo of at most 2^20 records skip
+    /* Trees with exact level sizes and at most expand_max_n relations skip
      * the 2^n filter: level 2 comes from k_tree_start and each level's
      * launch builds the next level's list by expansion (no connectivity
-     * bitmap is needed).  Expansion lists are in no particular order; with
-     * larger memos the filter's increasing order keeps a wave's memo
-     * gathers on nearby lines, which outweighs the filter. */
+     * bitmap is needed).  The limit is a tuning knob: the autotuner times
+     * both producers on the caller's sample (on the 20-relation snowflake
+     * expansion is faster; above the limit the filter's single pass is
+     * kept). */
     const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2 &&
                           n <= (int)g_tune.expand_max_n;
     if (tree_exp) {
@@ -3322,7 +3385,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                                c.b.left, c.b.counters);
         }
         uint64_t cand_ub = 0;
+        LevelGate gate{c, s, paced_dp(h, algo)};
         for (int k = 2; k <= K; k++) {
+            if ((rc = gate.before(k))) return rc;
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
             hipLaunchKernelGGL(k_ds_prep, dim3(1), dim3(64), 0, s, k, n, d);
             uint32_t rblocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK) / BLOCK, 4096);
@@ -3347,6 +3412,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             hipLaunchKernelGGL(k_ds_finalize, dim3(sblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, d, c.b.vals,
                                c.b.left, c.b.counters);
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+            if ((rc = gate.after(k))) return rc;
         }
         (void)cand_ub;
         uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
@@ -3414,7 +3480,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                            c.b.lvl_best, c.b.lvl_first, c.b.conn, conn_words, c.b.counters);
     }
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[1], s));
+    LevelGate gate{c, s, paced_dp(h, algo)};
     for (int k = 2; k <= K; k++) {
+        if ((rc = gate.before(k))) return rc;
         uint64_t Ck = binom64(n, k);
         uint32_t *sets_k = c.b.lvl_sets + off_h[k];
         unsigned long long *best_k = c.b.lvl_best + off_h[k];
@@ -3474,6 +3542,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                                best_k, first_k, c.b.vals, c.b.left, c.b.counters);
         }
         if (profile && (!h.is_tree || k == K)) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
+        if ((rc = gate.after(k))) return rc;
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     if (K < n) {                                   /* IDP1: the cheapest set of K relations */

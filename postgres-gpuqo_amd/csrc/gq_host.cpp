@@ -424,7 +424,7 @@ static Set<W> set_from_words(const uint64_t *w, int nw) {
     return s;
 }
 
-static int validate(const gq_query *q) {
+int validate(const gq_query *q) {
     if (!q || q->n_rels < 1 || q->n_rels > GQ_MAX_RELS) {
         set_error("n_rels must be in 1..%d", GQ_MAX_RELS);
         return GQ_ERR_INVALID;
@@ -473,7 +473,9 @@ static int validate(const gq_query *q) {
  * rows of joining base relations i and j over every class.  Each round takes
  * the first minimum in (i, j) order over i already in the tree and j outside
  * it, with the reference's strict `<` against FLT_MAX, so an edge whose rows
- * are +inf or NaN is never taken.  Only the edge table changes: the classes,
+ * are +inf is never taken.  A NaN product of selectivity and rows is
+ * clamped to 1 row by the row estimate (`rows > 1` is false for NaN), so
+ * such an edge weighs 1 and is preferred.  Only the edge table changes: the classes,
  * and so every join cost, stay those of the full graph.  A graph in which no
  * edge can be taken keeps the tree built so far and is then rejected as
  * disconnected. */
@@ -577,7 +579,8 @@ template <int W>
 static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     auto t0 = std::chrono::steady_clock::now();
     HQuery<W> h;
-    convert(q, cfg, h);
+    convert(q, cfg, h);                /* host preparation first, then the device */
+    if (int e = device_init(cfg->device)) return e;
     RunCtx ctx(cfg);
     QT<W> *t = nullptr;
     int rc = gpuqo_run_t(ctx, h, &t);
@@ -654,9 +657,11 @@ gq_status gq_config_default(gq_config *c) {
     return GQ_OK;
 }
 
-gq_status gq_init(int32_t device) { return (gq_status)gq::device_init(device); }
+gq_status gq_init(int32_t device) {
+    return gq::guarded([&] { return gq::device_init(device); });
+}
 
-gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+static gq_status plan_impl(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     int rc = gq::validate(q);
     if (rc) return (gq_status)rc;
     if (!cfg || !out || !out->nodes) {
@@ -667,12 +672,16 @@ gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
         gq::set_error("algorithm %d is not a GPU algorithm (use dpsub or dpsize)", cfg->gpuqo_algorithm);
         return GQ_ERR_UNSUPPORTED;
     }
-    if ((rc = gq::device_init(cfg->device))) return (gq_status)rc;
     if (q->n_rels <= 64) return (gq_status)gq::plan_t<1>(q, cfg, out);
     return (gq_status)gq::plan_t<4>(q, cfg, out);
 }
 
-gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, gq_plan_out *outs) {
+gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    return gq::guarded([&] { return plan_impl(q, cfg, out); });
+}
+
+static gq_status plan_batch_impl(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg,
+                                 gq_plan_out *outs) {
     if (n_queries < 0 || (n_queries > 0 && (!qs || !outs)) || !cfg) {
         gq::set_error("bad batch arguments");
         return GQ_ERR_INVALID;
@@ -756,13 +765,15 @@ gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_c
     }
     for (int i = 0; i < n_queries; i++) {
         if (done[i]) continue;
-        gq_status st = gq_plan(qs[i], cfg, &outs[i]);
+        gq_status st = plan_impl(qs[i], cfg, &outs[i]);
         if (st != GQ_OK) return st;
     }
     return GQ_OK;
 }
 
-
+gq_status gq_plan_batch(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, gq_plan_out *outs) {
+    return gq::guarded([&] { return plan_batch_impl(n_queries, qs, cfg, outs); });
+}
 
 void gq_interrupt(int32_t on) { gq::g_interrupt.store(on ? 1 : 0); }
 
@@ -780,7 +791,7 @@ gq_status gq_tuning_get(gq_tuning *t) {
     return GQ_OK;
 }
 
-gq_status gq_tuning_set(const gq_tuning *t) {
+static gq_status tuning_set_impl(const gq_tuning *t) {
     if (!t || t->tree_two_lane_waves < 1 || t->general_grid_max < 1 || t->tree_task_iters < 1 ||
         t->tree_expand_max_rels < 0 || t->tree_one_xcd_wgs < 0) {
         gq::set_error("null or non-positive tuning");
@@ -795,8 +806,12 @@ gq_status gq_tuning_set(const gq_tuning *t) {
     return (gq_status)gq::tuning_set(v);
 }
 
-gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
-                      int32_t knobs, gq_tuning *out) {
+gq_status gq_tuning_set(const gq_tuning *t) {
+    return gq::guarded([&] { return tuning_set_impl(t); });
+}
+
+static gq_status autotune_impl(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
+                               int32_t knobs, gq_tuning *out) {
     if (n_queries < 1 || !qs || !cfg || reps < 1 || !out || (knobs & ~(GQ_TUNE_SINGLE | GQ_TUNE_BATCH))) {
         gq::set_error("bad autotune arguments");
         return GQ_ERR_INVALID;
@@ -822,11 +837,11 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
             const auto t0 = std::chrono::steady_clock::now();
             double dev = 0.0;
             if (batch) {
-                int e = gq_plan_batch(n_queries, qs, cfg, outs.data());
+                int e = plan_batch_impl(n_queries, qs, cfg, outs.data());
                 if (e) return e;
             } else {
                 for (int i = 0; i < n_queries; i++) {
-                    int e = gq_plan(qs[i], cfg, &outs[i]);
+                    int e = plan_impl(qs[i], cfg, &outs[i]);
                     if (e) return e;
                     dev += outs[i].stats.gpu_ms;
                 }
@@ -875,6 +890,14 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
         if ((rc = gq::tuning_set(best))) return (gq_status)rc;
     }
     return gq_tuning_get(out);
+}
+
+gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg, int32_t reps,
+                      int32_t knobs, gq_tuning *out) {
+    const gq::Tuning before = gq::tuning_get();
+    gq_status st = gq::guarded([&] { return autotune_impl(n_queries, qs, cfg, reps, knobs, out); });
+    if (st != GQ_OK) gq::tuning_set(before);     /* a failed search leaves the knobs as they were */
+    return st;
 }
 
 const char *gq_last_error(void) { return gq::g_err; }

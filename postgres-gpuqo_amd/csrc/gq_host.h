@@ -10,7 +10,9 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <list>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -21,7 +23,9 @@
 namespace gq {
 
 /* f(0) .. f(n-1) on up to 16 host threads (inline below 32 items); f must
- * be safe to run concurrently for different indices. */
+ * be safe to run concurrently for different indices.  An exception thrown
+ * by f (or by starting a thread) stops the remaining items and is rethrown
+ * here after every started thread has been joined. */
 template <class F>
 void parallel_for(int n, F f) {
     const unsigned hw = std::thread::hardware_concurrency();
@@ -32,17 +36,32 @@ void parallel_for(int n, F f) {
         return;
     }
     std::atomic<int> next(0);
+    std::exception_ptr err;
+    std::mutex err_mu;
     auto work = [&]() {
-        for (;;) {
-            const int i = next.fetch_add(1);
-            if (i >= n) break;
-            f(i);
+        try {
+            for (;;) {
+                const int i = next.fetch_add(1);
+                if (i >= n) break;
+                f(i);
+            }
+        } catch (...) {
+            next.store(n);
+            std::lock_guard<std::mutex> g(err_mu);
+            if (!err) err = std::current_exception();
         }
     };
     std::vector<std::thread> th;
-    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    try {
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+    } catch (...) {
+        next.store(n);
+        for (auto &x : th) x.join();
+        throw;
+    }
     work();
     for (auto &x : th) x.join();
+    if (err) std::rethrow_exception(err);
 }
 
 template <int W>

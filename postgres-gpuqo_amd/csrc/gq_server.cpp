@@ -27,6 +27,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -78,6 +79,7 @@ struct OwnedQuery {
 };
 
 struct Request {
+    Request *qnext = nullptr;          /* link in the one queue holding it */
     uint64_t tag = 0;
     gq_config cfg;
     OwnedQuery oq;
@@ -86,6 +88,36 @@ struct Request {
     std::vector<gq_plan_node> nodes;
     gq_plan_stats stats;
     gq_status st = GQ_OK;
+};
+
+/* FIFO of requests linked through Request::qnext: moving a request between
+ * the waiting, admitted and finished queues allocates nothing, so a request
+ * thread can always hand back its result. */
+struct ReqQueue {
+    Request *head = nullptr, *tail = nullptr;
+    size_t n = 0;
+    bool empty() const { return n == 0; }
+    size_t size() const { return n; }
+    Request *front() const { return head; }
+    void push_back(Request *r) {
+        r->qnext = nullptr;
+        if (tail) tail->qnext = r;
+        else head = r;
+        tail = r;
+        n++;
+    }
+    void pop_front() {
+        head = head->qnext;
+        if (!head) tail = nullptr;
+        n--;
+    }
+    void delete_all() {
+        while (head) {
+            Request *r = head;
+            pop_front();
+            delete r;
+        }
+    }
 };
 
 /* One run_many call of a request thread: its DPs are in the engine's hands
@@ -123,7 +155,7 @@ struct Server {
 
     std::mutex mu;
     std::condition_variable cv_engine, cv_work, cv_done;
-    std::deque<Request *> waiting, admitted, finished;
+    ReqQueue waiting, admitted, finished;
     int running = 0;
     uint64_t running_bytes = 0;
     size_t outstanding = 0;
@@ -179,8 +211,20 @@ struct Server {
                     continue;
                 }
                 const uint64_t key = next_key++;
-                if (engine_admit(eng, *(*w->ins)[i], (*w->algos)[i], key)) {
-                    in_engine[key] = std::make_pair(w, i);
+                bool admitted_dp = false;
+                const gq_status ast = guarded([&] {
+                    in_engine.reserve(in_engine.size() + 1);
+                    admitted_dp = engine_admit(eng, *(*w->ins)[i], (*w->algos)[i], key);
+                    if (admitted_dp) in_engine[key] = std::make_pair(w, i);
+                    return (int)GQ_OK;
+                });
+                if (ast != GQ_OK) {                  /* host allocation failed: fail this DP's work */
+                    fail_work(w, ast);
+                    dp_pending.pop_front();
+                    unref(w);
+                    continue;
+                }
+                if (admitted_dp) {
                     dp_pending.pop_front();
                     continue;
                 }
@@ -196,7 +240,7 @@ struct Server {
             if (engine_active(eng) == 0) continue;
             lk.unlock();
             std::vector<EngineDone> done;
-            const int rc = engine_step(eng, done);
+            const int rc = guarded([&] { return engine_step(eng, done); });
             lk.lock();
             for (EngineDone &d : done) {
                 auto it = in_engine.find(d.key);
@@ -225,7 +269,7 @@ struct Server {
                 for (DPWork *w : ws) unref(w);
                 engine_destroy(eng);
                 eng = nullptr;
-                if (engine_create(cfg.device, budget, &eng)) break;
+                if (guarded([&] { return engine_create(cfg.device, budget, &eng); }) != GQ_OK) break;
             }
         }
     }
@@ -250,7 +294,7 @@ struct Server {
             out.n_nodes = 0;
             memset(&out.stats, 0, sizeof(out.stats));
             r->st = gq_plan(&r->oq.q, &r->cfg, &out);
-            r->nodes.resize(out.n_nodes > 0 ? out.n_nodes : 0);
+            r->nodes.resize(out.n_nodes > 0 ? out.n_nodes : 0);   /* shrinks: no allocation */
             r->stats = out.stats;
             r->stats.wall_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
             {
@@ -277,13 +321,22 @@ int ServerExecutor::run(const std::vector<const DPInput *> &ins, const std::vect
     w.trees = &trees;
     w.sts = &sts;
     w.remaining = ins.size();
-    w.refs = ins.size();
+    w.refs = 0;
     std::unique_lock<std::mutex> lk(srv->mu);
     if (!srv->eng) {
         set_error("planning server engine unavailable");
         return GQ_ERR_DEVICE;
     }
-    for (size_t i = 0; i < ins.size(); i++) srv->dp_pending.push_back(std::make_pair(&w, i));
+    /* each queued entry is a reference; if queueing fails part way, the
+     * entries already queued are dropped by the engine thread (w is done) */
+    const gq_status qst = guarded([&] {
+        for (size_t i = 0; i < ins.size(); i++) {
+            srv->dp_pending.push_back(std::make_pair(&w, i));
+            w.refs++;
+        }
+        return (int)GQ_OK;
+    });
+    if (qst != GQ_OK) Server::fail_work(&w, qst);
     srv->cv_engine.notify_one();
     /* DPs of a failed work may still be queued or on the device: wait until
      * the engine no longer references this stack object too */
@@ -358,91 +411,111 @@ struct gq_server {
     gq::Server s;
 };
 
-extern "C" {
+namespace gq {
 
-gq_status gq_server_create(const gq_config *cfg, uint64_t memo_budget_bytes, int32_t max_inflight,
-                           gq_server **out) {
+/* Stop a server's threads and free it (the threads in `started`). */
+static void server_shutdown(gq_server *srv) {
+    Server &S = srv->s;
+    {
+        std::lock_guard<std::mutex> g(S.mu);
+        S.stop = true;
+        S.outstanding -= S.waiting.size();
+        S.waiting.delete_all();
+    }
+    S.cv_work.notify_all();
+    for (auto &t : S.workers) t.join();
+    {
+        std::lock_guard<std::mutex> g(S.mu);
+        S.cv_engine.notify_all();
+    }
+    if (S.engine_th.joinable()) S.engine_th.join();
+    S.finished.delete_all();
+    S.admitted.delete_all();
+    engine_destroy(S.eng);
+    delete srv;
+}
+
+static gq_status server_create_impl(const gq_config *cfg, uint64_t memo_budget_bytes, int32_t max_inflight,
+                                    gq_server **out) {
     if (!out || !cfg || max_inflight < 1 || memo_budget_bytes == 0) {
-        gq::set_error("bad server arguments");
+        set_error("bad server arguments");
         return GQ_ERR_INVALID;
     }
     *out = nullptr;
-    gq_server *srv = new gq_server();
-    gq::Server &S = srv->s;
+    std::unique_ptr<gq_server> owner(new gq_server());
+    Server &S = owner->s;
     S.cfg = *cfg;
     S.budget = memo_budget_bytes;
     S.max_inflight = max_inflight;
-    int rc = gq::engine_create(cfg->device, memo_budget_bytes, &S.eng);
+    int rc = engine_create(cfg->device, memo_budget_bytes, &S.eng);
     if (rc) {
-        delete srv;
+        engine_destroy(S.eng);
         return (gq_status)rc;
     }
-    S.engine_th = std::thread([&S] { S.engine_loop(); });
-    for (int i = 0; i < max_inflight; i++) S.workers.emplace_back([&S] { S.worker_loop(); });
+    gq_server *srv = owner.release();
+    try {
+        S.workers.reserve((size_t)max_inflight);
+        S.engine_th = std::thread([&S] { S.engine_loop(); });
+        for (int i = 0; i < max_inflight; i++) S.workers.emplace_back([&S] { S.worker_loop(); });
+    } catch (...) {
+        server_shutdown(srv);            /* joins the threads that did start */
+        throw;
+    }
     *out = srv;
     return GQ_OK;
 }
 
-gq_status gq_server_submit_cfg(gq_server *srv, const gq_query *q, const gq_config *cfg, uint64_t tag) {
+static gq_status server_submit_impl(gq_server *srv, const gq_query *q, const gq_config *cfg, uint64_t tag) {
     if (!srv || !q) {
-        gq::set_error("bad submit arguments");
+        set_error("bad submit arguments");
         return GQ_ERR_INVALID;
     }
-    gq::Server &S = srv->s;
+    Server &S = srv->s;
     const gq_config *c = cfg ? cfg : &S.cfg;
     if (c->device != S.cfg.device) {
-        gq::set_error("request device %d differs from the server's %d", c->device, S.cfg.device);
+        set_error("request device %d differs from the server's %d", c->device, S.cfg.device);
         return GQ_ERR_INVALID;
     }
-    if (q->n_rels < 1 || q->n_rels > GQ_MAX_RELS || q->n_words < 1 || q->n_words > GQ_MAX_WORDS) {
-        gq::set_error("bad query size");
-        return GQ_ERR_INVALID;
-    }
-    gq::Request *r = new gq::Request();
+    if (int rc = validate(q)) return (gq_status)rc;
+    std::unique_ptr<Request> r(new Request());
     r->tag = tag;
     r->cfg = *c;
     r->oq.copy(q);
-    r->projected = gq::projected_bytes(q, c);
+    r->projected = projected_bytes(q, c);
     r->nodes.resize((size_t)2 * q->n_rels);
     memset(&r->stats, 0, sizeof(r->stats));
-    r->t_submit = gq::Clock::now();
+    r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(S.mu);
     if (S.stop) {
-        delete r;
-        gq::set_error("server is stopping");
+        set_error("server is stopping");
         return GQ_ERR_INVALID;
     }
     if (S.waiting.size() >= 65536) {
-        delete r;
-        gq::set_error("server queue full");
+        set_error("server queue full");
         return GQ_ERR_BUSY;
     }
-    S.waiting.push_back(r);
+    S.waiting.push_back(r.release());
     S.outstanding++;
     S.admit_waiting();
     return GQ_OK;
 }
 
-gq_status gq_server_submit(gq_server *srv, const gq_query *q, uint64_t tag) {
-    return gq_server_submit_cfg(srv, q, nullptr, tag);
-}
-
-gq_status gq_server_poll(gq_server *srv, int32_t wait, int32_t *done, uint64_t *tag, gq_status *req_status,
-                         gq_plan_out *out) {
+static gq_status server_poll_impl(gq_server *srv, int32_t wait, int32_t *done, uint64_t *tag,
+                                  gq_status *req_status, gq_plan_out *out) {
     if (!srv || !done || !out) {
-        gq::set_error("bad poll arguments");
+        set_error("bad poll arguments");
         return GQ_ERR_INVALID;
     }
-    gq::Server &S = srv->s;
+    Server &S = srv->s;
     *done = 0;
     std::unique_lock<std::mutex> lk(S.mu);
     auto ready = [&] { return !S.finished.empty() || S.outstanding == 0; };
     if (wait < 0) S.cv_done.wait(lk, ready);
     else if (wait > 0) S.cv_done.wait_for(lk, std::chrono::milliseconds(wait), ready);
     if (S.finished.empty()) return GQ_OK;
-    gq::Request *r = S.finished.front();
+    Request *r = S.finished.front();
     if (out->capacity < (int32_t)r->nodes.size() || (!r->nodes.empty() && !out->nodes)) {
-        gq::set_error("output capacity %d < %zu nodes", out->capacity, r->nodes.size());
+        set_error("output capacity %d < %zu nodes", out->capacity, r->nodes.size());
         return GQ_ERR_CAPACITY;
     }
     S.finished.pop_front();
@@ -458,27 +531,34 @@ gq_status gq_server_poll(gq_server *srv, int32_t wait, int32_t *done, uint64_t *
     return GQ_OK;
 }
 
+}  // namespace gq
+
+extern "C" {
+
+gq_status gq_server_create(const gq_config *cfg, uint64_t memo_budget_bytes, int32_t max_inflight,
+                           gq_server **out) {
+    return gq::guarded([&] { return gq::server_create_impl(cfg, memo_budget_bytes, max_inflight, out); });
+}
+
+gq_status gq_server_submit_cfg(gq_server *srv, const gq_query *q, const gq_config *cfg, uint64_t tag) {
+    return gq::guarded([&] { return gq::server_submit_impl(srv, q, cfg, tag); });
+}
+
+gq_status gq_server_submit(gq_server *srv, const gq_query *q, uint64_t tag) {
+    return gq_server_submit_cfg(srv, q, nullptr, tag);
+}
+
+gq_status gq_server_poll(gq_server *srv, int32_t wait, int32_t *done, uint64_t *tag, gq_status *req_status,
+                         gq_plan_out *out) {
+    return gq::guarded([&] { return gq::server_poll_impl(srv, wait, done, tag, req_status, out); });
+}
+
 gq_status gq_server_destroy(gq_server *srv) {
     if (!srv) return GQ_OK;
-    gq::Server &S = srv->s;
-    {
-        std::lock_guard<std::mutex> g(S.mu);
-        S.stop = true;
-        for (gq::Request *r : S.waiting) delete r;
-        S.outstanding -= S.waiting.size();
-        S.waiting.clear();
-    }
-    S.cv_work.notify_all();
-    for (auto &t : S.workers) t.join();
-    {
-        std::lock_guard<std::mutex> g(S.mu);
-        S.cv_engine.notify_all();
-    }
-    S.engine_th.join();
-    for (gq::Request *r : S.finished) delete r;
-    gq::engine_destroy(S.eng);
-    delete srv;
-    return GQ_OK;
+    return gq::guarded([&] {
+        gq::server_shutdown(srv);
+        return (int)GQ_OK;
+    });
 }
 
 }  // extern "C"

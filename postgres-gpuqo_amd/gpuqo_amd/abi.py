@@ -18,6 +18,7 @@ GQ_ERR_CAPACITY = 5
 GQ_ERR_NO_PLAN = 6
 GQ_ERR_INTERRUPTED = 7
 GQ_ERR_BUSY = 8
+GQ_ERR_INTERNAL = 9
 
 # gpuqo_algorithm (src/include/optimizer/gpuqo_common.h:13-29)
 ALGO = {

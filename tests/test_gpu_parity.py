@@ -260,6 +260,19 @@ def test_spanning_tree_mode(name, g, algo, bicc):
     assert_same_plan(got, want, f"{name} {algo} spanning tree")
 
 
+@pytest.mark.parametrize("algo,bicc", [("dpsub", 1), ("dpsub", 0), ("dpsize", 0)], ids=["mpdp", "dpsub", "dpsize"])
+def test_spanning_tree_mode_nan_selectivity_edge(algo, bicc):
+    """An edge whose selectivity is NaN weighs 1 row in Prim's tree (the row
+    estimate clamps NaN to 1) and is taken; the GPU path builds the same
+    tree and plan as the oracle."""
+    from test_oracle import nan_triangle
+    g = nan_triangle()
+    c = cfg(algo, bicc, spanning_tree=1)
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, f"nan triangle {algo} spanning tree")
+
+
 @pytest.mark.parametrize("kind", [2, 3], ids=["hash", "rank"])
 def test_spanning_tree_mode_tree_memos(kind):
     g = qgen.chorded_cycle(24, 5, 0x5a05)

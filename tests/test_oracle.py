@@ -524,3 +524,35 @@ def test_spanning_tree_mode_leaves_trees_alone():
             a, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc))
             b, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc, spanning_tree=1))
             assert_same_plan(b, a, f"{g.shape}{g.n} {algo}")
+
+
+def nan_triangle():
+    """Three relations of 100 rows in a triangle: class {0,1} selectivity
+    1/2 (5000 rows), class {0,2} selectivity NaN, class {1,2} 1/1024
+    (rows 9.765625 -> 10)."""
+    g = qgen.QueryGraph(3, "nan-triangle")
+    g.rows = [100.0, 100.0, 100.0]
+    g.add_generic(0, 1, 0.5)
+    g.add_generic(0, 2, float("nan"))
+    g.add_generic(1, 2, 1.0 / 1024)
+    return g.build()
+
+
+def test_spanning_tree_takes_an_edge_with_nan_selectivity():
+    """estimate_join_rows (gpuqo_row_estimation.cuh:120-147) returns 1 for a
+    NaN product (`rows > 1` is false), so Prim (gpuqo_spanning_tree.cu:17-68)
+    weighs edge {0,2} at 1 row and takes it first from relation 0, then
+    {2,1} (10 rows) over {0,1} (5000): the tree is 0-2-1.  Joining {0,1}
+    first is then impossible, while the full graph's plan is free to."""
+    g = nan_triangle()
+    c = cfg("dpsub")
+    assert _pair_rows(g, c, 0, 2) == 1.0
+    assert _pair_rows(g, c, 1, 2) == 10.0 and _pair_rows(g, c, 0, 1) == 5000.0
+    tree = _with_edges(g, [(0, 2), (1, 2)])
+    for algo, bicc in (("dpsub", 0), ("dpsub", 1), ("dpsize", 0)):
+        want, _ = oracle_bits(tree, cfg(algo, dpsub_bicc=bicc))
+        got, _ = oracle_bits(g, cfg(algo, dpsub_bicc=bicc, spanning_tree=1))
+        assert_same_plan(got, want, f"nan triangle {algo} bicc={bicc}")
+        assert any(nd[0][0] == 0b101 for nd in got), "the plan joins {0,2} (1 row) first"
+        assert not any(nd[0][0] == 0b011 for nd in got)
+
