@@ -22,6 +22,7 @@ def _plan_status(g, c, box):
     nodes = (abi.PlanNode * cap)()
     out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
     box["st"] = gpuqo_amd.lib().gq_plan(C.byref(g.to_c()), C.byref(c), C.byref(out))
+    box["msg"] = gpuqo_amd.lib().gq_last_error()       # the message is per thread
     box["nodes"] = gpuqo_amd.plan_tuples(out)
 
 
@@ -45,8 +46,8 @@ def test_interrupt_during_clique18_then_clean_plan(algo):
         assert not t.is_alive()
     finally:
         L.gq_interrupt(0)
-    assert box["st"] == abi.GQ_ERR_INTERRUPTED, (box["st"], L.gq_last_error())
-    assert b"interrupted" in L.gq_last_error()
+    assert box["st"] == abi.GQ_ERR_INTERRUPTED, (box["st"], box["msg"])
+    assert b"interrupted" in box["msg"]
     print(f"{algo}: interrupted after {dt * 1e3:.1f} ms of a {st0.gpu_ms:.1f} ms DP")
     again, _ = gpuqo_amd.plan(g, c)
     assert_same_plan(gpuqo_amd.plan_bits(again), gpuqo_amd.plan_bits(want), f"clique18-{algo} after interrupt")
