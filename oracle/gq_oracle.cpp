@@ -2007,6 +2007,40 @@ int oracle_join_cost(const gq_query *q, const gq_config *cfg, const uint64_t *l_
     return GQ_OK;
 }
 
+/* A bounded sample of one DP for CPU-baseline timing: the algorithm of
+ * cfg->gpuqo_algorithm runs its levels 2..max_levels only (the reference's
+ * n_iters < n_rels: gpuqo_cpu_sequential.cu:93-113), once, on the
+ * relabelled query; stats->pairs and stats->wall_ms cover that run. */
+int oracle_dp_first_levels(const gq_query *q, const gq_config *cfg, int max_levels, gq_plan_stats *stats) {
+    if (!q || !cfg || !stats || q->n_rels < 2 || q->n_rels >= 64 || max_levels < 2) return GQ_ERR_INVALID;
+    oracle::RunCtx ctx;
+    ctx.algorithm = cfg->gpuqo_algorithm;
+    ctx.bicc = cfg->gpuqo_dpsub_bicc != 0;
+    ctx.idp_type = cfg->gpuqo_idp_type;
+    ctx.idp_n_iters = 0;
+    ctx.n_threads = cfg->gpuqo_dpe_n_threads > 0 ? cfg->gpuqo_dpe_n_threads : 1;
+    ctx.chunk = cfg->gpuqo_cpu_dpsub_parallel_chunk_size > 0 ? cfg->gpuqo_cpu_dpsub_parallel_chunk_size : 64;
+    ctx.pairs_per_depbuf = cfg->gpuqo_dpe_pairs_per_depbuf > 0 ? cfg->gpuqo_dpe_pairs_per_depbuf : 128;
+    oracle::Info<1> *info = oracle::convert<1>(q, cfg);
+    std::list<oracle::RemapEl<1>> bfs = oracle::bfs_remap_list(*info);
+    oracle::Remapper<1, 1> rm(bfs);
+    oracle::Info<1> *ri = rm.remapPlannerInfo(*info);
+    ri->n_iters = std::min(ri->n_rels, max_levels);
+    auto t0 = std::chrono::steady_clock::now();
+    oracle::QueryTree<1> *qt = oracle::run_switch(ctx, *ri);
+    auto t1 = std::chrono::steady_clock::now();
+    oracle::freeQueryTree(qt);
+    delete ri;
+    delete info;
+    memset(stats, 0, sizeof(*stats));
+    stats->sets = ctx.c.sets;
+    stats->pairs = ctx.c.joins;
+    stats->candidates = ctx.c.checks;
+    stats->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats->n_dp_calls = 1;
+    return GQ_OK;
+}
+
 /* One branch of the restated cost model, exposed for known-answer tests:
  *   which 0: calc_join_cost (gpuqo_cost.cuh:76-158) with the relations in
  *            composite_mask (bit i = relation i) marked composite;

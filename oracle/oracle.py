@@ -44,6 +44,9 @@ def lib():
         L.oracle_cost_probe.restype = C.c_int
         L.oracle_hash_bucketsize.argtypes = [C.POINTER(abi.VarInfo), C.c_float, C.c_float, C.c_int]
         L.oracle_hash_bucketsize.restype = C.c_float
+        L.oracle_dp_first_levels.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.c_int,
+                                             C.POINTER(abi.PlanStats)]
+        L.oracle_dp_first_levels.restype = C.c_int
         _lib = L
     return _lib
 
@@ -70,6 +73,16 @@ def join_cost(query, cfg, l_words, r_words, lvals, rvals):
     st = lib().oracle_join_cost(C.byref(query.to_c()), C.byref(cfg), lw, rw, lv, rv, ov)
     assert st == 0
     return tuple(ov)
+
+
+def dp_first_levels(query, cfg, max_levels):
+    """Run the configured DP through levels 2..max_levels only (a bounded
+    timing sample); returns the stats (pairs, sets, wall_ms)."""
+    from gpuqo_amd import abi
+    st = abi.PlanStats()
+    rc = lib().oracle_dp_first_levels(C.byref(query.to_c()), C.byref(cfg), max_levels, C.byref(st))
+    assert rc == 0
+    return st
 
 
 PROBE_JOIN, PROBE_NESTLOOP, PROBE_HASHJOIN, PROBE_BUCKET_STATS, PROBE_INNER_UNIQUE = range(5)

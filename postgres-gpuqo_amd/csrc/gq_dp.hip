@@ -2425,12 +2425,14 @@ static int ctx_init(DeviceCtx &c, int device) {
 /* CHECK_FOR_INTERRUPTS between the levels of one DP (the reference polls it
  * per level and per scatter: gpuqo_dpsub_common.cu:81,127,156,300;
  * gpuqo_dpsize.cu:451,486,536,554).  The host tests the interrupt flag
- * before it enqueues each level.  A paced DP -- one expected to run for
- * milliseconds -- also keeps at most two levels in flight: before level k
- * is enqueued the host waits for level k-2 to finish, testing the flag
- * every 50 us, so an interrupt raised while the DP runs stops it within
- * two levels.  Levels already enqueued always run to completion (no kernel
- * is cut short); the call then returns GQ_ERR_INTERRUPTED. */
+ * before it enqueues each level.  A paced DP -- one predicted to run for
+ * milliseconds (paced_dp) -- also keeps at most two levels in flight:
+ * before level k is enqueued the host waits for level k-2 to finish,
+ * polling the completion event and the flag, so an interrupt raised while
+ * the DP runs stops it within two levels.  Levels already enqueued always
+ * run to completion (no kernel is cut short); the call then returns
+ * GQ_ERR_INTERRUPTED.  The poll yields instead of sleeping: level k-1 is
+ * still queued behind k-2, so the device stays busy while the host reacts. */
 struct LevelGate {
     DeviceCtx &c;
     hipStream_t s;
@@ -2445,7 +2447,7 @@ struct LevelGate {
                     return GQ_ERR_DEVICE;
                 }
                 if (interrupted()) break;
-                std::this_thread::sleep_for(std::chrono::microseconds(50));
+                std::this_thread::yield();
             }
         }
         if (interrupted()) {
@@ -2461,13 +2463,25 @@ struct LevelGate {
     }
 };
 
-/* DPs that run long enough to pace: general graphs of 16 or more
- * relations (a 16-clique plans in about 5 ms), DPsize from 14, trees from
- * 24 (tens of millions of connected sets). */
-static bool paced_dp(const BlobHdr &h, DPAlgo algo) {
+/* DPs predicted to run for milliseconds, which are paced: tree DPs with
+ * more than 10^8 csg-cmp pairs (exact, from the level sizes); general
+ * graphs of at least 16 relations and at least 2n join edges (a 16-clique
+ * plans in about 5 ms; sparse graphs such as cycles stay sub-millisecond);
+ * DPsize from 14 relations with at least 2n edges, or 16 relations. */
+static bool paced_dp(const DPInput &in, DPAlgo algo) {
+    const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
-    if (algo == DP_DPSIZE) return n >= 14;
-    return h.is_tree ? n >= 24 : n >= 16;
+    uint64_t deg = 0;
+    for (int v = 0; v < n; v++) deg += (uint64_t)__builtin_popcount(in.blob[(size_t)h.o_adj + v]);
+    const bool dense = deg / 2 >= 2ull * (uint64_t)n;
+    if (algo == DP_DPSIZE) return n >= 16 || (n >= 14 && dense);
+    if (h.is_tree) {
+        if ((int)in.tree_counts.size() != n + 1) return n >= 24;
+        uint64_t pairs = 0;
+        for (int k = 2; k <= n; k++) pairs += 2ull * (uint64_t)(k - 1) * in.tree_counts[k];
+        return pairs > 100000000ull;
+    }
+    return n >= 16 && dense;
 }
 
 /* pinned staging buffer of at least `words` words */
@@ -2973,7 +2987,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)std::min<uint64_t>((cap + BLOCK - 1) / BLOCK, 4096)), dim3(BLOCK),
                        0, s, c.hkeys, cap, 0u);
     hipLaunchKernelGGL(k_batch_init, dim3(1), dim3(64), 0, s, d_sj, 1, nullptr, nullptr);
-    LevelGate gate{c, s, paced_dp(h, algo)};
+    LevelGate gate{c, s, paced_dp(in, algo)};
     for (int k = 2; k <= K; k++) {
         if ((rc = gate.before(k))) return rc;
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
@@ -3094,7 +3108,7 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev0, s));
     hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
                        c.b.lvl_sets + off_h[2], c.b.counters, c.hslot + off_h[2]);
-    LevelGate gate{c, s, paced_dp(h, algo)};
+    LevelGate gate{c, s, paced_dp(in, algo)};
     for (int k = 2; k <= K; k++) {
         if ((rc = gate.before(k))) return rc;
         if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
@@ -3385,7 +3399,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                                c.b.left, c.b.counters);
         }
         uint64_t cand_ub = 0;
-        LevelGate gate{c, s, paced_dp(h, algo)};
+        LevelGate gate{c, s, paced_dp(in, algo)};
         for (int k = 2; k <= K; k++) {
             if ((rc = gate.before(k))) return rc;
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k], s));
@@ -3480,7 +3494,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                            c.b.lvl_best, c.b.lvl_first, c.b.conn, conn_words, c.b.counters);
     }
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[1], s));
-    LevelGate gate{c, s, paced_dp(h, algo)};
+    LevelGate gate{c, s, paced_dp(in, algo)};
     for (int k = 2; k <= K; k++) {
         if ((rc = gate.before(k))) return rc;
         uint64_t Ck = binom64(n, k);

@@ -287,7 +287,8 @@ def chorded_cycle(n, step, seed):
 SEED_BASE = 0x6770713000000000
 
 
-WORKLOAD_SEEDS = {"star12": 1, "snowflake20": 2, "clique18": 3, "batch256": 4, "serve256": 5}
+WORKLOAD_SEEDS = {"star12": 1, "snowflake20": 2, "clique18": 3, "batch256": 4, "serve256": 5,
+                  "tune": 6}           # tune + i, i = 0..5: the autotuner's sample queries
 
 
 def workload_seed(name):
