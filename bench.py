@@ -270,15 +270,18 @@ class PlanCall:
         self.gpuqo_amd = gpuqo_amd
 
     def run(self, steps):
-        """Wall ms and device ms of `steps` calls."""
+        """Wall ms and device ms of `steps` calls (host phases in self.host)."""
         wall, dev = [], []
+        self.host = []
         for _ in range(steps):
             t1 = time.perf_counter()
             st = self.L.gq_plan(*self.args)
             wall.append((time.perf_counter() - t1) * 1e3)
             if st != 0:
                 self.gpuqo_amd._check(st)
-            dev.append(self.out.stats.gpu_ms)
+            s = self.out.stats
+            dev.append(s.gpu_ms)
+            self.host.append((s.host_prep_ms, s.host_enqueue_ms, s.host_post_ms))
         return wall, dev
 
 
@@ -445,6 +448,8 @@ def main():
     t0 = time.perf_counter()
     wall_ms, dev_ms = call.run(args.steps)
     total_wall = time.perf_counter() - t0
+    host_phases = {k: statistics.median(x[i] for x in call.host)
+                   for i, k in enumerate(("prep_ms", "enqueue_ms", "post_ms"))}
     nodes, stats = gpuqo_amd.plan_tuples(call.out), call.out.stats
     print(f"# timed {args.steps} steps in {total_wall:.3f} s", file=sys.stderr, flush=True)
     # per-phase / per-launch times from separate profiled calls (HIP events
@@ -521,6 +526,7 @@ def main():
         "plan_ms_wall_median": statistics.median(wall_ms),
         "plan_ms_device_median": statistics.median(dev_ms),
         "pairs_per_s_device": P * args.steps / dev_total_s,
+        "host_phases_median": host_phases,
         "plan_ms_wall_p99": sorted(wall_ms)[min(len(wall_ms) - 1, int(0.99 * len(wall_ms)))],
         "roofline": roof,
         "tuning": tuning,

@@ -209,6 +209,12 @@ typedef struct gq_plan_stats {
     double eval_ms;           /* split evaluation (+ finalize) kernels     */
     int32_t filter_launches;
     int32_t eval_launches;
+    /* host phases of wall_ms: from the call's start until the first DP's
+     * first upload is issued; from then until the last DP's last launch is
+     * issued; from the last DP's results arriving until the call returns */
+    double host_prep_ms;
+    double host_enqueue_ms;
+    double host_post_ms;
 } gq_plan_stats;
 
 typedef struct gq_plan_out {

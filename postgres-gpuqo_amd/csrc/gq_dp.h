@@ -91,7 +91,13 @@ struct DPStats {
     double gpu_ms = 0.0, filter_ms = 0.0, eval_ms = 0.0;
     int filter_launches = 0, eval_launches = 0;
     int memo_kind = 1;
+    /* host clock (steady_clock, ms): the DP's first upload or launch is
+     * issued, its last launch is issued, and its results are on the host */
+    double t_first_launch = 0.0, t_last_launch = 0.0, t_synced = 0.0;
 };
+
+/* steady_clock in milliseconds (host phase stamps of DPStats) */
+double host_now_ms();
 
 enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
 

@@ -767,10 +767,128 @@ __host__ __device__ constexpr size_t tree_lds(int words, int lpe) {
  * the hash table size packed (store_rel) -- one hash_pack pass per WAVE
  * sets instead of one per iteration.  Waves w0, w0 + wstride, ... cover the
  * level; every record is written before the call returns. */
+/* LDS words of walk_tree_ws: set, left index, right index, pre-order
+ * number and DFS stack (64 each) and a flag */
+#define WALK_WS_WORDS (5 * 64 + 4)
+
+/* Plan walk by one wave: breadth-first from root_set, one round per plan
+ * level (the only dependent loads are left[S]), then lane 0 numbers the
+ * nodes in pre-order and each lane writes one node.  ws: WALK_WS_WORDS of
+ * LDS.  root_rec (optional): the root's left set and record {left, rows,
+ * startup, total, width} when the calling wave has just computed them, so
+ * that neither is read back from memory.  Returns the node count, or ~0u
+ * for an inconsistent memo or too small a capacity. */
+__device__ uint32_t walk_tree_ws(const MemoRef &mr, uint32_t root_set, DPNode *nodes, int cap, uint32_t *ws,
+                                 const uint32_t *root_rec = nullptr) {
+    uint32_t *s_set = ws;
+    int *s_l = (int *)ws + 64, *s_r = (int *)ws + 128, *s_pre = (int *)ws + 192, *s_stack = (int *)ws + 256;
+    volatile int *s_bad = (int *)ws + 320;
+    const int lane = threadIdx.x & (WAVE - 1);
+    if (lane == 0) {
+        s_set[0] = root_set;
+        *s_bad = cap < 1 || cap > 63;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    int f0 = 0, f1 = 1;
+    while (f0 < f1 && !*s_bad) {
+        const int i = f0 + lane;
+        bool inner = false;
+        uint32_t S = 0, l = 0;
+        if (i < f1) {
+            S = s_set[i];
+            s_l[i] = -1;
+            s_r[i] = -1;
+            if (__builtin_popcount(S) > 1) {
+                l = (i == 0 && root_rec) ? root_rec[0] : mr.left[memo_slot(mr, S)];
+                if (l == 0 || (l & ~S) != 0 || l == S) *s_bad = 1;
+                else inner = true;
+            }
+        }
+        const unsigned long long m = __ballot(inner);
+        const int newn = f1 + 2 * (int)__popcll(m);
+        if (newn > cap) {
+            if (lane == 0) *s_bad = 1;
+        } else if (inner) {
+            const int at = f1 + 2 * (int)__popcll(m & ((1ull << lane) - 1ull));
+            s_set[at] = l;
+            s_set[at + 1] = S ^ l;
+            s_l[i] = at;
+            s_r[i] = at + 1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        f0 = f1;
+        f1 = newn;
+    }
+    const int total = f0;
+    if (*s_bad) return ~0u;
+    if (lane == 0) {
+        int sp = 0, cnt = 0;
+        s_stack[sp++] = 0;
+        while (sp > 0) {
+            const int x = s_stack[--sp];
+            s_pre[x] = cnt++;
+            if (s_l[x] >= 0) {
+                s_stack[sp++] = s_r[x];
+                s_stack[sp++] = s_l[x];
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane < total) {
+        const uint32_t S = s_set[lane];
+        DPNode nd;
+        if (lane == 0 && root_rec) {
+            nd.rows = __uint_as_float(root_rec[1]);
+            nd.startup = __uint_as_float(root_rec[2]);
+            nd.total = __uint_as_float(root_rec[3]);
+            nd.width = (int)root_rec[4];
+        } else {
+            const float4 v = mr.vals[memo_slot(mr, S)];
+            nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = rec_width(__float_as_uint(v.w));
+        }
+        nd.set = S;
+        nd.left = s_l[lane] >= 0 ? s_pre[s_l[lane]] : -1;
+        nd.right = s_r[lane] >= 0 ? s_pre[s_r[lane]] : -1;
+        nd.pad = 0;
+        nodes[s_pre[lane]] = nd;
+    }
+    return (uint32_t)total;
+}
+
+/* walk_tree_ws with its scratch in static LDS (kernels of one wave) */
+__device__ uint32_t walk_tree_wave(const MemoRef &mr, uint32_t root_set, DPNode *nodes, int cap) {
+    __shared__ uint32_t ws[WALK_WS_WORDS];
+    return walk_tree_ws(mr, root_set, nodes, cap, ws);
+}
+
+/* Level 2 of a tree DP needs no list and no memo loads: with every vertex
+ * v > 0 having exactly one lower-numbered neighbour, its parent p(v), the
+ * level-2 sets are the edges {p(v), v}, entry v - 1 of the implicit list,
+ * and both sides of their one split are base relations, read from the
+ * staged blob with the hash-table size packed as the memo holds it. */
+__device__ __forceinline__ uint32_t edge_set(const uint32_t *adj, uint32_t i) {
+    const uint32_t v = i + 1u;
+    return (1u << __builtin_ctz(adj[v] & ((1u << v) - 1u))) | (1u << v);
+}
+__device__ __forceinline__ Rel base_rel(const BlobHdr &h, const uint32_t *blob, int i) {
+    const float *b = (const float *)(blob + h.o_base) + 4 * i;
+    Rel r;
+    r.rows = b[0];
+    r.startup = b[1];
+    r.total = b[2];
+    r.width = __float_as_int(b[3]);
+    r.hs = hash_pack(r.rows, r.width, h.p.work_mem);
+    return r;
+}
+
 template <int ECW, bool FAST, int LPE>
 __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k,
                                 bool bicc, const uint32_t *lvl_sets, const uint32_t *lvl_slot, uint32_t count,
-                                const MemoRef &mr, uint64_t w0, uint64_t wstride, uint32_t s0 = 0u) {
+                                const MemoRef &mr, uint64_t w0, uint64_t wstride, uint32_t s0 = 0u,
+                                bool implicit2 = false, uint32_t *root_rec = nullptr) {
     const int lane = threadIdx.x & (WAVE - 1);
     const uint32_t per = (uint32_t)(k - 1) * LPE;
     const uint32_t spw = tree_spw(k, LPE);              /* sets per wave, per consecutive lanes each */
@@ -811,7 +929,8 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
         uint32_t S = 0, sub = 0, rS = 0;
         int c = 0;
         if (active) {
-            S = (w == w0 && s0) ? s0 : lvl_sets[idx];   /* s0: this lane's first set, loaded by the caller */
+            S = implicit2 ? edge_set(smem + h.o_adj, (uint32_t)idx)
+                          : ((w == w0 && s0) ? s0 : lvl_sets[idx]);   /* s0: this lane's first set, loaded by the caller */
             if (rk && lvl_slot) rS = lvl_slot[idx];     /* rank memo: the slot the expansion wrote */
             c = select_bit(S, e + 1u);                  /* member e+1: the root (lowest member) is skipped */
             sub = S & desc[c];
@@ -843,7 +962,8 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
                 ss = memo_slot(mr, sub);
                 sr = memo_slot(mr, rest);
             }
-            const Rel Rs = load_rel(mr.vals, ss), Rr = load_rel(mr.vals, sr);
+            const Rel Rs = implicit2 ? base_rel(h, smem, c) : load_rel(mr.vals, ss);
+            const Rel Rr = implicit2 ? base_rel(h, smem, __builtin_ctz(S)) : load_rel(mr.vals, sr);
             uint32_t rk0, rk1;
             if (bicc) {
                 rk0 = 2u * tree_post(S, c, desc, lglob);
@@ -928,6 +1048,13 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
                 if (LPE == 2) {
                     store_rel(mr.vals, slot, j0, q.p.work_mem, false);
                     mr.left[slot] = lft;
+                    if (root_rec) {                     /* one winning lane: the caller walks from this set */
+                        root_rec[0] = lft;
+                        root_rec[1] = __float_as_uint(j0.rows);
+                        root_rec[2] = __float_as_uint(j0.startup);
+                        root_rec[3] = __float_as_uint(j0.total);
+                        root_rec[4] = (uint32_t)j0.width;
+                    }
                 }
             }
         }
@@ -975,7 +1102,7 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
     const int lane = threadIdx.x & (WAVE - 1);
     uint32_t S = 0, emit = 0, sl = 0;
     if (i < count) {
-        S = src[i];
+        S = src ? src[i] : edge_set(adj, i);       /* src == nullptr: the implicit level-2 list */
         if (dst_slot) sl = src_slot[i];
         emit = top_leaf_children(adj, S);
     }
@@ -999,6 +1126,16 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
     }
 }
 
+/* k_eval_tree flags.  TREE_START: level 2 of a dense-memo DP whose lists
+ * come by expansion -- its list is implicit (edge_set), its splits read the
+ * base relations from the blob, and workgroup 0 stores the base relations
+ * and counters[2] (the counters must be zero on entry).  TREE_WALK: the last
+ * level (one set): workgroup 0 then walks the plan from it and writes the
+ * 64 counters (word 48: the walk's status) and the plan nodes straight to
+ * res_host (coherent pinned host memory), and zeroes the device counters. */
+#define TREE_START 1u
+#define TREE_WALK 2u
+
 /* Level k of a tree DP: workgroups [0, ev_blocks) evaluate it (the grid may
  * be sized by a bound on the level: idle ones leave after staging the
  * blob); with `next` set, workgroups from ev_blocks on build the level k+1
@@ -1011,7 +1148,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
                                                       uint32_t *next, uint32_t cap, uint32_t *next_slot,
-                                                      uint32_t xs) {
+                                                      uint32_t xs, uint32_t flags, uint32_t *res_host) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     /* xs > 1: the grid is xs times the level's workgroups, dispatched round
      * robin over the XCDs; only every xs-th works, so a small level runs on
@@ -1021,26 +1158,48 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
         if (bid % xs) return;
         bid /= xs;
     }
+    const bool start = (flags & TREE_START) != 0;
     if (bid >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
-        const uint32_t count = counters[k];
+        const uint32_t count = start ? (uint32_t)(h.n_rels - 1) : counters[k];
         if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
         __syncthreads();
-        expand_level(smem, lvl_sets, count, (bid - ev_blocks) * BLOCK + threadIdx.x, next, counters + k + 1,
-                     lvl_slot, next_slot, mr.rank, mr.rn);
+        expand_level(smem, start ? nullptr : lvl_sets, count, (bid - ev_blocks) * BLOCK + threadIdx.x, next,
+                     counters + k + 1, lvl_slot, next_slot, mr.rank, mr.rn);
         return;
     }
     const uint32_t spw = tree_spw(k, LPE), per = (uint32_t)(k - 1) * LPE;
     const uint64_t w0 = (uint64_t)bid * (BLOCK / WAVE) + threadIdx.x / WAVE;
     const uint32_t seg = (threadIdx.x & (WAVE - 1)) / per;
     const uint64_t idx0 = w0 * spw + seg;
-    const uint32_t count = counters[k];
-    const uint32_t s_pre = (seg < spw && idx0 < cap) ? lvl_sets[idx0] : 0u;
+    const uint32_t count = start ? (uint32_t)(h.n_rels - 1) : counters[k];
+    const uint32_t s_pre = (!start && seg < spw && idx0 < cap) ? lvl_sets[idx0] : 0u;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
+    if (start && bid == 0) {                /* the base relations (read from the memo by later levels) */
+        if ((int)threadIdx.x < h.n_rels) {
+            store_base(mr.vals, 1u << threadIdx.x, h, smem, (int)threadIdx.x);
+            mr.left[1u << threadIdx.x] = 0u;
+        }
+        if (threadIdx.x == 0) counters[2] = (uint32_t)(h.n_rels - 1);
+    }
     if ((uint64_t)bid * (BLOCK / WAVE) * spw >= count) return;
     View<uint32_t> q = make_view(h, smem);
+    const uint32_t o_ws = (uint32_t)(tree_lds(h.words, LPE) / 4);   /* walk scratch after the queues */
+    uint32_t *root_rec = (flags & TREE_WALK) ? smem + o_ws + WALK_WS_WORDS : nullptr;
     eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, count, mr, w0,
-                                    (uint64_t)ev_blocks * (BLOCK / WAVE), idx0 < count ? s_pre : 0u);
+                                    (uint64_t)ev_blocks * (BLOCK / WAVE), idx0 < count ? s_pre : 0u, start,
+                                    root_rec);
+    if (flags & TREE_WALK) {                /* one set, one workgroup: the plan walk */
+        __syncthreads();
+        if (threadIdx.x < WAVE) {
+            const int n = h.n_rels, lane = (int)threadIdx.x;
+            const uint32_t root = n == 32 ? 0xffffffffu : ((1u << n) - 1u);
+            const uint32_t r = walk_tree_ws(mr, root, (DPNode *)(res_host + 64), 2 * n - 1, smem + o_ws, root_rec);
+            const uint32_t cv = (start && lane == 2) ? (uint32_t)(n - 1) : counters[lane];
+            res_host[lane] = lane == 48 ? r : cv;
+            counters[lane] = 0u;
+        }
+    }
     /* no pair counter: every split of a tree set is valid, so the host
      * counts sum_k |level k| * 2(k-1) -- one atomic per wave on a single
      * address would serialise the level */
@@ -1832,74 +1991,6 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs,
  * inconsistent (a left side that is empty, not a proper subset, or a tree of
  * more than `cap` nodes).  cap <= 63.  Called by all 64 lanes of a
  * workgroup of one wave. */
-__device__ uint32_t walk_tree_wave(const MemoRef &mr, uint32_t root_set, DPNode *nodes, int cap) {
-    __shared__ uint32_t s_set[64];
-    __shared__ int s_l[64], s_r[64], s_pre[64], s_stack[64];
-    __shared__ int s_bad;
-    const int lane = threadIdx.x;
-    if (lane == 0) {
-        s_set[0] = root_set;
-        s_bad = cap < 1 || cap > 63;
-    }
-    __syncthreads();
-    int f0 = 0, f1 = 1;
-    while (f0 < f1 && !s_bad) {
-        const int i = f0 + lane;
-        bool inner = false;
-        uint32_t S = 0, l = 0;
-        if (i < f1) {
-            S = s_set[i];
-            s_l[i] = -1;
-            s_r[i] = -1;
-            if (__builtin_popcount(S) > 1) {
-                l = mr.left[memo_slot(mr, S)];
-                if (l == 0 || (l & ~S) != 0 || l == S) s_bad = 1;
-                else inner = true;
-            }
-        }
-        const unsigned long long m = __ballot(inner);
-        const int newn = f1 + 2 * (int)__popcll(m);
-        if (newn > cap) {
-            if (lane == 0) s_bad = 1;
-        } else if (inner) {
-            const int at = f1 + 2 * (int)__popcll(m & ((1ull << lane) - 1ull));
-            s_set[at] = l;
-            s_set[at + 1] = S ^ l;
-            s_l[i] = at;
-            s_r[i] = at + 1;
-        }
-        __syncthreads();
-        f0 = f1;
-        f1 = newn;
-    }
-    const int total = f0;
-    if (s_bad) return ~0u;
-    if (lane == 0) {
-        int sp = 0, cnt = 0;
-        s_stack[sp++] = 0;
-        while (sp > 0) {
-            const int x = s_stack[--sp];
-            s_pre[x] = cnt++;
-            if (s_l[x] >= 0) {
-                s_stack[sp++] = s_r[x];
-                s_stack[sp++] = s_l[x];
-            }
-        }
-    }
-    __syncthreads();
-    if (lane < total) {
-        const uint32_t S = s_set[lane];
-        const float4 v = mr.vals[memo_slot(mr, S)];
-        DPNode nd;
-        nd.set = S; nd.rows = v.x; nd.startup = v.y; nd.total = v.z; nd.width = rec_width(__float_as_uint(v.w));
-        nd.left = s_l[lane] >= 0 ? s_pre[s_l[lane]] : -1;
-        nd.right = s_r[lane] >= 0 ? s_pre[s_r[lane]] : -1;
-        nd.pad = 0;
-        nodes[s_pre[lane]] = nd;
-    }
-    return (uint32_t)total;
-}
-
 /* Plan walk of finished jobs, one wave each: job jobs[idx ? idx[b] : b] for
  * workgroup b < nj.  res == nullptr: the nodes go to J.nodes and the status
  * to J.counters[48].  Otherwise workgroup b writes RES_WORDS words at
@@ -2344,6 +2435,9 @@ struct DeviceCtx {
     hipStream_t stream2 = nullptr;
     hipEvent_t lvl_ev[2][34] = {};
     hipEvent_t gate_ev[34] = {};        /* level completions of a paced DP (LevelGate) */
+    /* the 64 device counters are known to be zero (a fused tree DP's walk
+     * leaves them so): the next fused tree DP needs no clearing launch */
+    bool counters_clean = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBufs b{};
     size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
@@ -2420,6 +2514,10 @@ static int ctx_init(DeviceCtx &c, int device) {
     /* coherent: the plan walk writes the results straight into it */
     HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocMapped | hipHostMallocCoherent));
     return GQ_OK;
+}
+
+double host_now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 /* CHECK_FOR_INTERRUPTS between the levels of one DP (the reference polls it
@@ -2761,7 +2859,8 @@ int run_dp_batch(const std::vector<const DPInput *> &ins, const std::vector<DPAl
  * The grid covers the level, capped at 8192 workgroups. */
 static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipStream_t s, const uint32_t *blob,
                              int k, int bicc, const uint32_t *sets, const uint32_t *slots, MemoRef mr,
-                             uint32_t *counters, uint32_t *next = nullptr, uint32_t *next_slot = nullptr) {
+                             uint32_t *counters, uint32_t *next = nullptr, uint32_t *next_slot = nullptr,
+                             uint32_t flags = 0u, uint32_t *res_host = nullptr) {
     const uint64_t spw2 = tree_spw(k, 2);
     /* levels of at most g_tune.lpe2_waves waves at two lanes per edge leave
      * the GPU latency-bound (a few waves per SIMD): the two-lane form */
@@ -2784,10 +2883,11 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint32_t xs = (g_tune.one_xcd_wgs && ev + ex <= g_tune.one_xcd_wgs) ? XCDS : 1u;
     const dim3 grid((ev + ex) * xs);
     lds = std::max(lds, tree_lds(h.words, lpe));
+    if (flags & TREE_WALK) lds = std::max(lds, tree_lds(h.words, lpe) + (WALK_WS_WORDS + 8) * 4);
     const uint32_t cap = (uint32_t)std::min<uint64_t>(sets_k, 0xffffffffull);
 #define GQ_TREE_LAUNCH(E, F, P)                                                                                \
     hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters, ev, next, cap, next_slot, xs)
+                       counters, ev, next, cap, next_slot, xs, flags, res_host)
     if (h.tree_fast) {
         if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
         else GQ_TREE_LAUNCH(1, true, 1);
@@ -2967,6 +3067,7 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     sj.nodes = c.b.nodes;
     memcpy(sj.lvl_off, off_h, sizeof(off_h));
     sj.bicc = algo == DP_BICC ? 1 : 0;
+    st.t_first_launch = host_now_ms();
     if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
     MemoRef mr;
     mr.vals = c.b.vals;
@@ -3011,8 +3112,10 @@ static int run_dp_hash(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
+    st.t_last_launch = host_now_ms();
     if ((rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
+    st.t_synced = host_now_ms();
     if (cnt[49]) {
         set_error("open-addressing memo error (flags %u)", cnt[49]);
         return (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
@@ -3093,6 +3196,7 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     memset(c.h_stage, 0, (W4h + 36) * 4);
     memcpy(c.h_stage, in.blob.data(), (size_t)h.words * 4);
     memcpy(c.h_stage + W4h, off_h, sizeof(off_h));
+    st.t_first_launch = host_now_ms();
     if ((rc = stage_up(c, s, c.b.blob, W4h + 36))) return rc;
     MemoRef mr = dense_memo(c);
     mr.rank = c.b.blob + h.o_rank;
@@ -3130,8 +3234,10 @@ static int run_dp_rank(DeviceCtx &c, const DPInput &in, DPAlgo algo, int K, bool
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
+    st.t_last_launch = host_now_ms();
     if ((rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
+    st.t_synced = host_now_ms();
     if (cnt[49]) {
         set_error("rank memo error (flags %u)", cnt[49]);
         return (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
@@ -3264,6 +3370,8 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     DeviceCtx &c = g_ctx;
     int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
     if (rc) return rc;
+    const bool counters_were_clean = c.counters_clean;
+    c.counters_clean = false;
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
     if (n < 1 || n > 32) {
@@ -3351,6 +3459,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         sj.conn_words = conn_words;
         sj.bicc = 1;
     }
+    st.t_first_launch = host_now_ms();
     if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
     HIPCHK(hipEventRecord(c.ev0, s));
     /* Trees with exact level sizes and at most expand_max_n relations skip
@@ -3362,7 +3471,13 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * kept). */
     const bool tree_exp = h.is_tree && (int)in.tree_counts.size() == n + 1 && algo != DP_DPSIZE && n >= 2 &&
                           n <= (int)g_tune.expand_max_n;
-    if (tree_exp) {
+    /* fused: level 2 starts the DP (TREE_START) and the last level walks the
+     * plan into pinned host memory (TREE_WALK), so no start kernel, walk
+     * kernel or read-back copy runs */
+    const bool fused = tree_exp && K == n;
+    if (fused) {
+        if (!counters_were_clean) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, nullptr, 0u, c.b.counters);
+    } else if (tree_exp) {
         hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
                            c.b.lvl_sets + off_h[2], c.b.counters, nullptr);
     } else {
@@ -3452,8 +3567,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         HIPCHK(hipEventRecord(c.ev1, s));
         tree.resize(nodes_n);
         const uint32_t *cnt = c.h_res;
+        st.t_last_launch = host_now_ms();
         if ((rc = results_down(c, s, nodes_n))) return rc;
         HIPCHK(hipStreamSynchronize(s));
+        st.t_synced = host_now_ms();
         memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
         if (cnt[48] != (uint32_t)nodes_n) {
             set_error("plan reconstruction failed (memo inconsistent, status %u)", cnt[48]);
@@ -3510,9 +3627,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if (h.is_tree) {
             /* exact level size of a tree (subtree polynomial), else C(n,k) */
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
+            const uint32_t tf = fused ? ((k == 2 ? TREE_START : 0u) | (k == K ? TREE_WALK : 0u)) : 0u;
             launch_eval_tree(h, lvl, lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
                              dense_memo(c), c.b.counters,
-                             tree_exp && k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr);
+                             tree_exp && k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr, nullptr, tf, c.h_res);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k, g_tune.tree_iters)
@@ -3565,13 +3683,17 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         if (rc) return rc;
     }
     const int nodes_n = 2 * K - 1;
-    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n, c.b.counters + 48);
+    if (!fused) hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n,
+                                   c.b.counters + 48);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
-    if ((rc = results_down(c, s, nodes_n))) return rc;
+    st.t_last_launch = host_now_ms();
+    if (!fused && (rc = results_down(c, s, nodes_n))) return rc;
     HIPCHK(hipStreamSynchronize(s));
+    st.t_synced = host_now_ms();
+    if (fused) c.counters_clean = true;            /* the walk zeroed them */
     memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
     if (cnt[49] != 0) {
         set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");

@@ -38,6 +38,10 @@ struct RunCtx {
         stats.eval_launches += s.eval_launches;
         if (s.memo_bytes > stats.memo_bytes) stats.memo_bytes = s.memo_bytes;
         stats.memo_kind = s.memo_kind;
+        if (s.t_first_launch > 0.0 && (stats.t_first_launch == 0.0 || s.t_first_launch < stats.t_first_launch))
+            stats.t_first_launch = s.t_first_launch;
+        if (s.t_last_launch > stats.t_last_launch) stats.t_last_launch = s.t_last_launch;
+        if (s.t_synced > stats.t_synced) stats.t_synced = s.t_synced;
     }
 };
 template <>

@@ -584,7 +584,6 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     RunCtx ctx(cfg);
     QT<W> *t = nullptr;
     int rc = gpuqo_run_t(ctx, h, &t);
-    auto t1 = std::chrono::steady_clock::now();
     out->n_nodes = 0;
     memset(&out->stats, 0, sizeof(out->stats));
     out->stats.candidates = ctx.stats.candidates;
@@ -592,7 +591,7 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     out->stats.pairs = ctx.stats.pairs;
     out->stats.memo_bytes = ctx.stats.memo_bytes;
     out->stats.gpu_ms = ctx.stats.gpu_ms;
-    out->stats.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    out->stats.wall_ms = host_now_ms() - std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
     out->stats.n_dp_calls = ctx.dp_calls;
     out->stats.memo_kind = ctx.stats.memo_kind;
     out->stats.filter_ms = ctx.stats.filter_ms;
@@ -600,16 +599,23 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     out->stats.filter_launches = ctx.stats.filter_launches;
     out->stats.eval_launches = ctx.stats.eval_launches;
     if (rc != GQ_OK) return rc;
+    int r = GQ_OK;
     if (!t) {
         set_error("no join tree (is the join graph connected?)");
-        return GQ_ERR_NO_PLAN;
-    }
-    int r = emit(t, out);
-    if (r == -2) {
+        r = GQ_ERR_NO_PLAN;
+    } else if (emit(t, out) == -2) {
         set_error("output capacity %d < %d nodes", out->capacity, 2 * q->n_rels - 1);
-        return GQ_ERR_CAPACITY;
+        r = GQ_ERR_CAPACITY;
     }
-    return GQ_OK;
+    const double t_end = host_now_ms();
+    const double t_start = std::chrono::duration<double, std::milli>(t0.time_since_epoch()).count();
+    if (ctx.stats.t_first_launch > 0.0) {
+        out->stats.host_prep_ms = ctx.stats.t_first_launch - t_start;
+        out->stats.host_enqueue_ms = ctx.stats.t_last_launch - ctx.stats.t_first_launch;
+        out->stats.host_post_ms = t_end - ctx.stats.t_synced;
+    }
+    out->stats.wall_ms = t_end - t_start;
+    return r;
 }
 
 }  // namespace gq

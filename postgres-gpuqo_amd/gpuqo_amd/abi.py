@@ -109,7 +109,8 @@ class PlanStats(C.Structure):
                 ("gpu_ms", C.c_double), ("wall_ms", C.c_double),
                 ("n_dp_calls", C.c_int32), ("memo_kind", C.c_int32),
                 ("filter_ms", C.c_double), ("eval_ms", C.c_double),
-                ("filter_launches", C.c_int32), ("eval_launches", C.c_int32)]
+                ("filter_launches", C.c_int32), ("eval_launches", C.c_int32),
+                ("host_prep_ms", C.c_double), ("host_enqueue_ms", C.c_double), ("host_post_ms", C.c_double)]
 
 
 class Tuning(C.Structure):
