@@ -1143,12 +1143,12 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
  * slots go to next_slot), BLOCK level-k sets each.  The level's list has room for `cap` entries: each lane loads its
  * first set before the level size is known, so that this load, the level
  * size and the blob staging are in flight together. */
-template <int ECW, bool FAST, int LPE>
+template <int ECW, bool FAST, int LPE, uint32_t flags>
 __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
                                                       uint32_t *next, uint32_t cap, uint32_t *next_slot,
-                                                      uint32_t xs, uint32_t flags, uint32_t *res_host) {
+                                                      uint32_t xs, uint32_t *res_host) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     /* xs > 1: the grid is xs times the level's workgroups, dispatched round
      * robin over the XCDs; only every xs-th works, so a small level runs on
@@ -1158,7 +1158,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
         if (bid % xs) return;
         bid /= xs;
     }
-    const bool start = (flags & TREE_START) != 0;
+    constexpr bool start = (flags & TREE_START) != 0;
     if (bid >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
         const uint32_t count = start ? (uint32_t)(h.n_rels - 1) : counters[k];
         if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
@@ -1175,12 +1175,14 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     const uint32_t s_pre = (!start && seg < spw && idx0 < cap) ? lvl_sets[idx0] : 0u;
     for (int i = threadIdx.x; i < h.words; i += BLOCK) smem[i] = blob[i];
     __syncthreads();
-    if (start && bid == 0) {                /* the base relations (read from the memo by later levels) */
+    if constexpr (start) {
+        if (bid == 0) {                     /* the base relations (read from the memo by later levels) */
         if ((int)threadIdx.x < h.n_rels) {
             store_base(mr.vals, 1u << threadIdx.x, h, smem, (int)threadIdx.x);
             mr.left[1u << threadIdx.x] = 0u;
         }
         if (threadIdx.x == 0) counters[2] = (uint32_t)(h.n_rels - 1);
+        }
     }
     if ((uint64_t)bid * (BLOCK / WAVE) * spw >= count) return;
     View<uint32_t> q = make_view(h, smem);
@@ -1189,7 +1191,7 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
     eval_level_tree<ECW, FAST, LPE>(h, smem, q, k, bicc != 0, lvl_sets, lvl_slot, count, mr, w0,
                                     (uint64_t)ev_blocks * (BLOCK / WAVE), idx0 < count ? s_pre : 0u, start,
                                     root_rec);
-    if (flags & TREE_WALK) {                /* one set, one workgroup: the plan walk */
+    if constexpr ((flags & TREE_WALK) != 0) {    /* one set, one workgroup: the plan walk */
         __syncthreads();
         if (threadIdx.x < WAVE) {
             const int n = h.n_rels, lane = (int)threadIdx.x;
@@ -2864,7 +2866,7 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     const uint64_t spw2 = tree_spw(k, 2);
     /* levels of at most g_tune.lpe2_waves waves at two lanes per edge leave
      * the GPU latency-bound (a few waves per SIMD): the two-lane form */
-    const int lpe = spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= g_tune.lpe2_waves ? 2 : 1;
+    const int lpe = (flags || (spw2 > 0 && (sets_k + spw2 - 1) / spw2 <= g_tune.lpe2_waves)) ? 2 : 1;
     const uint64_t spw = tree_spw(k, lpe);
     const uint64_t waves = (sets_k + spw - 1) / spw;
     /* two-lane form: one wave-iteration per wave (latency).  One-lane form:
@@ -2885,19 +2887,26 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
     lds = std::max(lds, tree_lds(h.words, lpe));
     if (flags & TREE_WALK) lds = std::max(lds, tree_lds(h.words, lpe) + (WALK_WS_WORDS + 8) * 4);
     const uint32_t cap = (uint32_t)std::min<uint64_t>(sets_k, 0xffffffffull);
-#define GQ_TREE_LAUNCH(E, F, P)                                                                                \
-    hipLaunchKernelGGL((k_eval_tree<E, F, P>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
-                       counters, ev, next, cap, next_slot, xs, flags, res_host)
+#define GQ_TREE_LAUNCH(E, F, P, M)                                                                            \
+    hipLaunchKernelGGL((k_eval_tree<E, F, P, M>), grid, dim3(BLOCK), lds, s, h, blob, k, bicc, sets, slots, mr, \
+                       counters, ev, next, cap, next_slot, xs, res_host)
+    /* the start and walk forms are kernels of their own (the common level
+     * kernel keeps its smaller code); both run one-set-per-few-lanes levels
+     * (level 2: n - 1 sets; the last level: one set), i.e. two lanes per edge */
+#define GQ_TREE_MODES(E, F)                                                                                   \
+    if (flags == (TREE_START | TREE_WALK)) GQ_TREE_LAUNCH(E, F, 2, TREE_START | TREE_WALK);                   \
+    else if (flags == TREE_START) GQ_TREE_LAUNCH(E, F, 2, TREE_START);                                        \
+    else if (flags == TREE_WALK) GQ_TREE_LAUNCH(E, F, 2, TREE_WALK);                                          \
+    else if (lpe == 2) GQ_TREE_LAUNCH(E, F, 2, 0u);                                                           \
+    else GQ_TREE_LAUNCH(E, F, 1, 0u)
     if (h.tree_fast) {
-        if (lpe == 2) GQ_TREE_LAUNCH(1, true, 2);
-        else GQ_TREE_LAUNCH(1, true, 1);
+        GQ_TREE_MODES(1, true);
     } else if (h.ecw == 1) {
-        if (lpe == 2) GQ_TREE_LAUNCH(1, false, 2);
-        else GQ_TREE_LAUNCH(1, false, 1);
+        GQ_TREE_MODES(1, false);
     } else {
-        if (lpe == 2) GQ_TREE_LAUNCH(0, false, 2);
-        else GQ_TREE_LAUNCH(0, false, 1);
+        GQ_TREE_MODES(0, false);
     }
+#undef GQ_TREE_MODES
 #undef GQ_TREE_LAUNCH
 }
 
