@@ -32,6 +32,7 @@ extern "C" {
 #define GQ_MAX_RELS 256          /* UnionDP / IDP drivers (host side)          */
 #define GQ_MAX_WORDS 4           /* GQ_MAX_RELS / 64                           */
 #define GQ_MAX_DP_RELS 32        /* largest single DP handed to the GPU kernels */
+#define GQ_NOALLOC_MAX_RELS 20   /* largest query gq_plan_noalloc plans        */
 
 typedef enum gq_status {
     GQ_OK = 0,
@@ -238,6 +239,24 @@ gq_status gq_init(int32_t device);
  * numbering.  GQ_ALGO_DPSUB and GQ_ALGO_DPSIZE run on the GPU.
  */
 gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out);
+
+/*
+ * gq_plan without any allocation.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * plans like gq_plan -- same algorithms, same plans bit for bit -- but the
+ * DP's device state lives in a static device array of about 40 MB that every
+ * GPU the library runs on holds until the process ends, and the call makes no
+ * hipMalloc, hipFree or pinned-memory allocation (the query's conversion on
+ * the host uses the host heap only).  Limits: GQ_ALGO_DPSUB (plain or
+ * gpuqo_dpsub_bicc) on at most GQ_NOALLOC_MAX_RELS relations, no IDP
+ * decomposition, the dense memo; anything else returns GQ_ERR_UNSUPPORTED
+ * before any device work.  gq_init(cfg->device) must have been called
+ * (GQ_ERR_INVALID otherwise).  Calls on one GPU share its array, which keeps
+ * the last call's contents; they are serialised by the library's device
+ * lock, so overlapping calls wait rather than race.
+ */
+gq_status gq_plan_noalloc(const gq_query *q, const gq_config *cfg, gq_plan_out *out);
 
 /*
  * Plan n_queries independent queries in one batched device pass: all DPs of

@@ -102,10 +102,13 @@ double host_now_ms();
 enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
 
 /* Run one DP on the current device.  `n_iters` < n_rels selects the IDP1
- * level cap (the tree returned is the cheapest set of that size).  Returns a
+ * level cap (the tree returned is the cheapest set of that size).  With
+ * `noalloc` the DP runs in the library's static scratch array and allocates
+ * nothing (dense-memo DPsub/MPDP of at most GQ_NOALLOC_MAX_RELS relations,
+ * no IDP1 cap; the device context must already be initialised).  Returns a
  * gq_status. */
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
-           DPStats &stats);
+           DPStats &stats, bool noalloc = false);
 
 /* Run several DPs in one fused launch (one workgroup per DP).  Every input
  * must satisfy fused_eligible(). */

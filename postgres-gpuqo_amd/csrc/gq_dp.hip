@@ -1725,6 +1725,41 @@ __host__ __device__ __forceinline__ uint32_t task_sets_of(bool is_tree, bool bic
 }
 #define RES_WORDS (64 + 64 * (sizeof(DPNode) / 4))   /* counters + up to 64 plan nodes per DP */
 
+/* ---------------------------------------------------------------- allocation-free DPs
+ * gq_plan_noalloc runs a dense-memo DPsub/MPDP DP of at most
+ * GQ_NOALLOC_MAX_RELS relations entirely in `scratch`, a static device
+ * array that every GPU the library runs on holds from the library's load
+ * to the end of the process (NA_BYTES, about 40 MB).  Layout, 256-byte
+ * aligned: vals 16 B x 2^20 | left 4 B x 2^20 | connectivity bitmap |
+ * level lists 4 B x 2^20 | lvl_best 8 B x 2^20 | lvl_first 4 B x 2^20 |
+ * counters + plan nodes (RES_WORDS) | staged blob and level offsets |
+ * the big-set list (C(20,10) entries) and its prefix.  Host code never
+ * uses the array's name as an address: ctx_init takes each device's
+ * address once with hipGetSymbolAddress (g_scratch). */
+namespace na {
+constexpr size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NS = 1ull << GQ_NOALLOC_MAX_RELS;
+constexpr size_t MAXC = 184756;                         /* C(20, 10): the largest level */
+constexpr size_t STAGE_WORDS = 16384 + 36 + 32 + 128;   /* blob (64 KB LDS bound), offsets, level 1, job */
+constexpr size_t O_VALS = 0;
+constexpr size_t O_LEFT = O_VALS + al(16 * NS);
+constexpr size_t O_CONN = O_LEFT + al(4 * NS);
+constexpr size_t O_SETS = O_CONN + al(NS / 8);
+constexpr size_t O_BEST = O_SETS + al(4 * NS);
+constexpr size_t O_FIRST = O_BEST + al(8 * NS);
+constexpr size_t O_RES = O_FIRST + al(4 * NS);
+constexpr size_t O_BLOB = O_RES + al(RES_WORDS * 4);
+constexpr size_t O_BIG = O_BLOB + al(STAGE_WORDS * 4);
+constexpr size_t O_BIGP = O_BIG + al(MAXC * sizeof(BigEnt));
+constexpr size_t BYTES = O_BIGP + al((MAXC + 1) * 4);
+}  // namespace na
+static_assert(na::BYTES < (64ull << 20), "the allocation-free scratch stays below 64 MB");
+static __device__ __align__(256) unsigned char scratch[na::BYTES];
+
+/* device address of `scratch` per device (0 until ctx_init runs there) */
+#define GQ_MAX_DEVICES 64
+static uintptr_t g_scratch[GQ_MAX_DEVICES];
+
 __device__ __forceinline__ uint32_t task_sets(const BatchJob &J, int k) {
     return task_sets_of(J.h.is_tree != 0, J.bicc != 0, k, J.tree_iters);
 }
@@ -2498,6 +2533,9 @@ static int ensure(T **p, size_t &cap, size_t need) {
     return GQ_OK;
 }
 
+template <class T>
+static int ensure_host(T **p, size_t &cap, size_t need);
+
 static int ctx_init(DeviceCtx &c, int device) {
     if (c.device == device && c.stream) return GQ_OK;
     HIPCHK(hipSetDevice(device));
@@ -2508,6 +2546,11 @@ static int ctx_init(DeviceCtx &c, int device) {
     for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.gate_ev[k], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c.ev0));
     HIPCHK(hipEventCreate(&c.ev1));
+    if (device >= 0 && device < GQ_MAX_DEVICES && !g_scratch[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));   /* this device's copy of the array */
+        g_scratch[device] = (uintptr_t)p;
+    }
     c.device = device;
     /* counters[0..63] and the plan nodes right after them: one read-back */
     int rc = ensure(&c.b.counters, c.cap_counters, RES_WORDS);
@@ -2515,7 +2558,9 @@ static int ctx_init(DeviceCtx &c, int device) {
     c.b.nodes = (DPNode *)(c.b.counters + 64);
     /* coherent: the plan walk writes the results straight into it */
     HIPCHK(hipHostMalloc((void **)&c.h_res, RES_WORDS * 4, hipHostMallocMapped | hipHostMallocCoherent));
-    return GQ_OK;
+    /* pinned staging at its working size, so the allocation-free path never
+     * grows it */
+    return ensure_host(&c.h_stage, c.cap_h_stage, na::STAGE_WORDS);
 }
 
 double host_now_ms() {
@@ -2617,8 +2662,8 @@ static int stage_up(DeviceCtx &c, hipStream_t s, uint32_t *dst, size_t words) {
 }
 
 /* The DP's result (64 counters, then `nodes` plan nodes) into c.h_res. */
-static int results_down(DeviceCtx &c, hipStream_t s, int nodes) {
-    HIPCHK(hipMemcpyAsync(c.h_res, c.b.counters, (64 + (size_t)nodes * (sizeof(DPNode) / 4)) * 4,
+static int results_down(DeviceCtx &c, hipStream_t s, int nodes, const uint32_t *src = nullptr) {
+    HIPCHK(hipMemcpyAsync(c.h_res, src ? src : c.b.counters, (64 + (size_t)nodes * (sizeof(DPNode) / 4)) * 4,
                           hipMemcpyDeviceToHost, s));
     return GQ_OK;
 }
@@ -2626,6 +2671,12 @@ static int results_down(DeviceCtx &c, hipStream_t s, int nodes) {
 int device_init(int device) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
     return ctx_init(g_ctx, device);
+}
+
+bool device_ready(int device) {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    return g_ctx.device == device && g_ctx.stream != nullptr && device >= 0 && device < GQ_MAX_DEVICES &&
+           g_scratch[device] != 0;
 }
 
 
@@ -2910,16 +2961,38 @@ static void launch_eval_tree(const BlobHdr &h, uint64_t sets_k, size_t lds, hipS
 #undef GQ_TREE_LAUNCH
 }
 
-static MemoRef dense_memo(DeviceCtx &c) {
+static MemoRef dense_memo_of(const DevBufs &b) {
     MemoRef m;
-    m.vals = c.b.vals;
-    m.left = c.b.left;
+    m.vals = b.vals;
+    m.left = b.left;
     m.keys = nullptr;
     m.mask = 0;
-    m.err = c.b.counters + 49;
+    m.err = b.counters + 49;
     m.rank = nullptr;
     m.rn = 0;
     return m;
+}
+static MemoRef dense_memo(DeviceCtx &c) { return dense_memo_of(c.b); }
+
+/* The allocation-free buffers of `device` (inside `scratch`). */
+static int scratch_bufs(int device, DevBufs &b, BigEnt **big, uint32_t **big_prefix) {
+    if (device < 0 || device >= GQ_MAX_DEVICES || !g_scratch[device]) {
+        set_error("no scratch address for device %d (gq_init must run first)", device);
+        return GQ_ERR_INVALID;
+    }
+    char *base = (char *)g_scratch[device];
+    b.vals = (float4 *)(base + na::O_VALS);
+    b.left = (uint32_t *)(base + na::O_LEFT);
+    b.conn = (uint32_t *)(base + na::O_CONN);
+    b.lvl_sets = (uint32_t *)(base + na::O_SETS);
+    b.lvl_best = (unsigned long long *)(base + na::O_BEST);
+    b.lvl_first = (uint32_t *)(base + na::O_FIRST);
+    b.counters = (uint32_t *)(base + na::O_RES);
+    b.nodes = (DPNode *)(b.counters + 64);
+    b.blob = (uint32_t *)(base + na::O_BLOB);
+    *big = (BigEnt *)(base + na::O_BIG);
+    *big_prefix = (uint32_t *)(base + na::O_BIGP);
+    return GQ_OK;
 }
 
 /* IDP1 ties: the first of `cands` in the iteration order of the CPU DP's
@@ -3374,13 +3447,13 @@ void prepare_memo(DPInput &in, int requested, int max_memo_mb) {
 }
 
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
-           DPStats &st) {
+           DPStats &st, bool noalloc) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
     DeviceCtx &c = g_ctx;
     int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
     if (rc) return rc;
-    const bool counters_were_clean = c.counters_clean;
-    c.counters_clean = false;
+    const bool counters_were_clean = c.counters_clean && !noalloc;
+    if (!noalloc) c.counters_clean = false;
     const BlobHdr &h = in.hdr;
     const int n = h.n_rels;
     if (n < 1 || n > 32) {
@@ -3390,6 +3463,15 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     /* n_iters < n: IDP1's level cap; the result is the cheapest set of
      * n_iters relations */
     const int K = (n_iters >= 2 && n_iters < n) ? n_iters : n;
+    if (noalloc) {
+        /* everything below runs in `scratch`: one dense DP, no IDP1 cap */
+        if (n > GQ_NOALLOC_MAX_RELS || algo == DP_DPSIZE || K != n || in.memo != GQ_MEMO_DENSE) {
+            set_error("the allocation-free path plans dense-memo DPsub/MPDP DPs of at most %d relations",
+                      GQ_NOALLOC_MAX_RELS);
+            return GQ_ERR_UNSUPPORTED;
+        }
+        profile = false;
+    }
     if (in.memo == GQ_MEMO_HASH && algo != DP_DPSIZE) return run_dp_hash(c, in, algo, K, profile, tree, st);
     if (in.memo == GQ_MEMO_RANK && algo != DP_DPSIZE) return run_dp_rank(c, in, algo, K, profile, tree, st);
     const size_t blob_bytes = (size_t)h.words * 4;
@@ -3406,12 +3488,19 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     const uint64_t nsets_all = 1ull << n;
     uint64_t maxC = 1;
     for (int k = 1; k <= n; k++) maxC = std::max(maxC, binom64(n, k));
-    if ((rc = ensure(&c.b.vals, c.cap_memo, nsets_all))) return rc;
-    if ((rc = ensure(&c.b.left, c.cap_left, nsets_all))) return rc;
-    if ((rc = ensure(&c.b.conn, c.cap_conn, std::max<uint64_t>(1, nsets_all / 32)))) return rc;
-    if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, nsets_all))) return rc;
-    if ((rc = ensure(&c.b.lvl_best, c.cap_best, nsets_all))) return rc;
-    if ((rc = ensure(&c.b.lvl_first, c.cap_first, nsets_all))) return rc;
+    DevBufs nb{};
+    BigEnt *big_ents = nullptr;
+    uint32_t *big_prefix = nullptr;
+    if (noalloc) {
+        if ((rc = scratch_bufs(c.device, nb, &big_ents, &big_prefix))) return rc;
+    } else {
+        if ((rc = ensure(&c.b.vals, c.cap_memo, nsets_all))) return rc;
+        if ((rc = ensure(&c.b.left, c.cap_left, nsets_all))) return rc;
+        if ((rc = ensure(&c.b.conn, c.cap_conn, std::max<uint64_t>(1, nsets_all / 32)))) return rc;
+        if ((rc = ensure(&c.b.lvl_sets, c.cap_lvl, nsets_all))) return rc;
+        if ((rc = ensure(&c.b.lvl_best, c.cap_best, nsets_all))) return rc;
+        if ((rc = ensure(&c.b.lvl_first, c.cap_first, nsets_all))) return rc;
+    }
     /* device blob area: the query blob, then (at a 4-word boundary) the 33
      * level-list offsets; both go up in one copy */
     const size_t W4h = ((size_t)h.words + 3) & ~(size_t)3;
@@ -3419,9 +3508,18 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * order (32) | the big-set job descriptor */
     const size_t o_l1 = W4h + 36, o_sj = o_l1 + 32;
     const size_t stage_words = o_sj + ((sizeof(BatchJob) + 15) & ~(size_t)15) / 4;
-    if ((rc = ensure(&c.b.blob, c.cap_blob, stage_words))) return rc;
-    if ((rc = ensure_stage(c, stage_words))) return rc;
-    c.lvl_off = c.b.blob + W4h;
+    if (noalloc) {
+        if (stage_words > na::STAGE_WORDS || c.cap_h_stage < stage_words || maxC > na::MAXC) {
+            set_error("query description too large for the allocation-free path");
+            return GQ_ERR_UNSUPPORTED;
+        }
+    } else {
+        if ((rc = ensure(&c.b.blob, c.cap_blob, stage_words))) return rc;
+        if ((rc = ensure_stage(c, stage_words))) return rc;
+    }
+    /* the DP's buffers from here on: the context's, or `scratch` */
+    DevBufs &b = noalloc ? nb : c.b;
+    uint32_t *const lvl_off = b.blob + W4h;
 
     /* optional per-phase events: [3k] before filter, [3k+1] before eval, [3k+2] after eval */
     if (profile && (int)c.prof_ev.size() < 3 * 33) {
@@ -3451,25 +3549,25 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * block are deferred to the chunk-parallel big-set kernels, which see
      * this query as a batch of one. */
     const bool use_big = algo == DP_BICC && !h.is_tree && n >= 15;
-    BatchJob *const d_sj = (BatchJob *)(c.b.blob + o_sj);
+    BatchJob *const d_sj = (BatchJob *)(b.blob + o_sj);
     if (use_big) {
         BatchJob &sj = *(BatchJob *)(c.h_stage + o_sj);
         sj.h = h;
-        sj.blob = c.b.blob;
-        sj.vals = c.b.vals;
-        sj.left = c.b.left;
-        sj.conn = c.b.conn;
-        sj.lvl_sets = c.b.lvl_sets;
-        sj.counters = c.b.counters;
-        sj.lvl_best = c.b.lvl_best;
-        sj.lvl_first = c.b.lvl_first;
-        sj.nodes = c.b.nodes;
+        sj.blob = b.blob;
+        sj.vals = b.vals;
+        sj.left = b.left;
+        sj.conn = b.conn;
+        sj.lvl_sets = b.lvl_sets;
+        sj.counters = b.counters;
+        sj.lvl_best = b.lvl_best;
+        sj.lvl_first = b.lvl_first;
+        sj.nodes = b.nodes;
         memcpy(sj.lvl_off, off_h, sizeof(off_h));
         sj.conn_words = conn_words;
         sj.bicc = 1;
     }
     st.t_first_launch = host_now_ms();
-    if ((rc = stage_up(c, s, c.b.blob, stage_words))) return rc;
+    if ((rc = stage_up(c, s, b.blob, stage_words))) return rc;
     HIPCHK(hipEventRecord(c.ev0, s));
     /* Trees with exact level sizes and at most expand_max_n relations skip
      * the 2^n filter: level 2 comes from k_tree_start and each level's
@@ -3485,14 +3583,14 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
      * kernel or read-back copy runs */
     const bool fused = tree_exp && K == n;
     if (fused) {
-        if (!counters_were_clean) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, nullptr, 0u, c.b.counters);
+        if (!counters_were_clean) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, nullptr, 0u, b.counters);
     } else if (tree_exp) {
-        hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, c.b.blob, c.b.vals, c.b.left,
-                           c.b.lvl_sets + off_h[2], c.b.counters, nullptr);
+        hipLaunchKernelGGL(k_tree_start, dim3(1), dim3(WAVE), 0, s, h, b.blob, b.vals, b.left,
+                           b.lvl_sets + off_h[2], b.counters, nullptr);
     } else {
         uint32_t blocks = (uint32_t)std::min<uint64_t>((conn_words + BLOCK - 1) / BLOCK, 4096);
-        hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, c.b.conn, conn_words, c.b.counters);
-        hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, c.b.blob, c.b.vals, c.b.left, c.b.conn);
+        hipLaunchKernelGGL(k_clear, dim3(blocks), dim3(BLOCK), 0, s, b.conn, conn_words, b.counters);
+        hipLaunchKernelGGL(k_base, dim3(1), dim3(64), 0, s, h, b.blob, b.vals, b.left, b.conn);
     }
     if (algo == DP_DPSIZE) {
         if (n > 28) {
@@ -3518,9 +3616,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         const size_t lds_pairs = lds_eval + (DS_TA * 2 + DS_TB + DS_TA * DS_TB + 4) * 4;
         {
             uint32_t blocks = (uint32_t)std::min<uint64_t>((nsets_all + BLOCK - 1) / BLOCK, 4096);
-            hipLaunchKernelGGL(k_ds_init, dim3(blocks), dim3(BLOCK), 0, s, h, c.b.blob, (const int *)(c.b.blob + o_l1),
-                               d, nsets_all, c.b.vals,
-                               c.b.left, c.b.counters);
+            hipLaunchKernelGGL(k_ds_init, dim3(blocks), dim3(BLOCK), 0, s, h, b.blob, (const int *)(b.blob + o_l1),
+                               d, nsets_all, b.vals,
+                               b.left, b.counters);
         }
         uint64_t cand_ub = 0;
         LevelGate gate{c, s, paced_dp(in, algo)};
@@ -3539,16 +3637,16 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             }
             uint32_t pblocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(tiles_ub, 1), 8192);
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 1], s));
-            hipLaunchKernelGGL(k_ds_pairs, dim3(pblocks), dim3(BLOCK), lds_pairs, s, h, c.b.blob, k, d, c.b.vals,
-                               c.b.counters);
+            hipLaunchKernelGGL(k_ds_pairs, dim3(pblocks), dim3(BLOCK), lds_pairs, s, h, b.blob, k, d, b.vals,
+                               b.counters);
             uint64_t Ck = binom64(n, k);
             uint32_t sblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 4096);
-            hipLaunchKernelGGL(k_ds_rowcount, dim3(sblocks), dim3(BLOCK), 0, s, k, d, c.b.counters);
+            hipLaunchKernelGGL(k_ds_rowcount, dim3(sblocks), dim3(BLOCK), 0, s, k, d, b.counters);
             hipLaunchKernelGGL(k_ds_scan, dim3(1), dim3(1024), 0, s, d);
-            hipLaunchKernelGGL(k_ds_scatter, dim3(sblocks), dim3(BLOCK), 0, s, k, d, c.b.counters);
+            hipLaunchKernelGGL(k_ds_scatter, dim3(sblocks), dim3(BLOCK), 0, s, k, d, b.counters);
             hipLaunchKernelGGL(k_ds_rowsort, dim3(rblocks), dim3(BLOCK), 0, s, d);
-            hipLaunchKernelGGL(k_ds_finalize, dim3(sblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, d, c.b.vals,
-                               c.b.left, c.b.counters);
+            hipLaunchKernelGGL(k_ds_finalize, dim3(sblocks), dim3(BLOCK), lds_eval, s, h, b.blob, k, d, b.vals,
+                               b.left, b.counters);
             if (profile) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
             if ((rc = gate.after(k))) return rc;
         }
@@ -3566,18 +3664,18 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                 for (int k = 2; k <= K; k++) o.insert(o.end(), all.begin() + sz[40 + k], all.begin() + sz[40 + k] + sz[k]);
                 return (int)GQ_OK;
             };
-            rc = idp1_root(c, s, dense_memo(c), d.lset, d.sz + 40 + K, d.sz + K, n, K, created, &root);
+            rc = idp1_root(c, s, dense_memo_of(b), d.lset, d.sz + 40 + K, d.sz + K, n, K, created, &root);
             if (rc) return rc;
         }
         const int nodes_n = 2 * K - 1;
-        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n,
-                           c.b.counters + 48);
+        hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo_of(b), root, b.nodes, nodes_n,
+                           b.counters + 48);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c.ev1, s));
         tree.resize(nodes_n);
         const uint32_t *cnt = c.h_res;
         st.t_last_launch = host_now_ms();
-        if ((rc = results_down(c, s, nodes_n))) return rc;
+        if ((rc = results_down(c, s, nodes_n, b.counters))) return rc;
         HIPCHK(hipStreamSynchronize(s));
         st.t_synced = host_now_ms();
         memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
@@ -3608,25 +3706,29 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         return GQ_OK;
     }
     if (use_big) {
-        if ((rc = ensure(&c.big_ents, c.cap_big, maxC))) return rc;
-        if ((rc = ensure(&c.big_prefix, c.cap_big_prefix, maxC + 1))) return rc;
+        if (!noalloc) {
+            if ((rc = ensure(&c.big_ents, c.cap_big, maxC))) return rc;
+            if ((rc = ensure(&c.big_prefix, c.cap_big_prefix, maxC + 1))) return rc;
+            big_ents = c.big_ents;
+            big_prefix = c.big_prefix;
+        }
     }
     const uint64_t cand_total = nsets_all - (uint64_t)n - 1ull;
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[0], s));
     if (!tree_exp) {
         uint64_t waves = (nsets_all + WAVE - 1) / WAVE;
         uint32_t fblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 8192);
-        hipLaunchKernelGGL(k_filter_all, dim3(fblocks), dim3(BLOCK), 0, s, h, c.b.blob, c.lvl_off, c.b.lvl_sets,
-                           c.b.lvl_best, c.b.lvl_first, c.b.conn, conn_words, c.b.counters);
+        hipLaunchKernelGGL(k_filter_all, dim3(fblocks), dim3(BLOCK), 0, s, h, b.blob, lvl_off, b.lvl_sets,
+                           b.lvl_best, b.lvl_first, b.conn, conn_words, b.counters);
     }
     if (profile) HIPCHK(hipEventRecord(c.prof_ev[1], s));
     LevelGate gate{c, s, paced_dp(in, algo)};
     for (int k = 2; k <= K; k++) {
         if ((rc = gate.before(k))) return rc;
         uint64_t Ck = binom64(n, k);
-        uint32_t *sets_k = c.b.lvl_sets + off_h[k];
-        unsigned long long *best_k = c.b.lvl_best + off_h[k];
-        uint32_t *first_k = c.b.lvl_first + off_h[k];
+        uint32_t *sets_k = b.lvl_sets + off_h[k];
+        unsigned long long *best_k = b.lvl_best + off_h[k];
+        uint32_t *first_k = b.lvl_first + off_h[k];
         uint32_t lblocks = (uint32_t)std::min<uint64_t>((Ck + BLOCK - 1) / BLOCK, 8192);
         /* trees: one launch per level, timed as one span from the first
          * level to the last (events between launches would add their own
@@ -3637,9 +3739,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             /* exact level size of a tree (subtree polynomial), else C(n,k) */
             const uint64_t lvl = (int)in.tree_counts.size() == n + 1 ? in.tree_counts[k] : Ck;
             const uint32_t tf = fused ? ((k == 2 ? TREE_START : 0u) | (k == K ? TREE_WALK : 0u)) : 0u;
-            launch_eval_tree(h, lvl, lds_eval, s, c.b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
-                             dense_memo(c), c.b.counters,
-                             tree_exp && k < K ? c.b.lvl_sets + off_h[k + 1] : nullptr, nullptr, tf, c.h_res);
+            launch_eval_tree(h, lvl, lds_eval, s, b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
+                             dense_memo_of(b), b.counters,
+                             tree_exp && k < K ? b.lvl_sets + off_h[k + 1] : nullptr, nullptr, tf, c.h_res);
         } else if (algo == DP_BICC || T <= 4096) {
             uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
             uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k, g_tune.tree_iters)
@@ -3654,22 +3756,22 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             if (algo == DP_BICC) {
                 const bool big = use_big && k >= 15;
                 if (big) {
-                    bl.ents = c.big_ents;
-                    bl.count = c.b.counters + 50;
+                    bl.ents = big_ents;
+                    bl.count = b.counters + 50;
                     bl.cap = (uint32_t)maxC;
                     bl.gidx0 = off_h[k];
                 }
-                hipLaunchKernelGGL(k_eval_wg<true>, dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, c.b.blob, k, B, 0,
-                                   sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters, bl);
+                hipLaunchKernelGGL(k_eval_wg<true>, dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, b.blob, k, B, 0,
+                                   sets_k, b.vals, b.left, b.conn, b.counters, bl);
                 if (big) {
-                    hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, c.big_ents, c.b.counters + 50,
-                                       (uint32_t)maxC, c.big_prefix);
-                    hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, d_sj, c.big_ents,
-                                       c.b.counters + 50, c.big_prefix);
+                    hipLaunchKernelGGL(k_big_scan, dim3(1), dim3(1024), 0, s, big_ents, b.counters + 50,
+                                       (uint32_t)maxC, big_prefix);
+                    hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, d_sj, big_ents,
+                                       b.counters + 50, big_prefix);
                 }
             } else {
-                hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, c.b.blob, k, B,
-                                   conn_lds, sets_k, c.b.vals, c.b.left, c.b.conn, c.b.counters, bl);
+                hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, b.blob, k, B,
+                                   conn_lds, sets_k, b.vals, b.left, b.conn, b.counters, bl);
             }
         } else {
             uint32_t ipl = 16;
@@ -3677,32 +3779,32 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             uint32_t cps = (uint32_t)((T + per_wave - 1) / per_wave);
             uint64_t waves = Ck * cps;
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((waves + 3) / 4, 16384);
-            hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, cps, ipl,
-                               sets_k, best_k, first_k, c.b.vals, c.b.conn, c.b.counters);
-            hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, c.b.blob, k, sets_k,
-                               best_k, first_k, c.b.vals, c.b.left, c.b.counters);
+            hipLaunchKernelGGL(k_eval_dpsub_chunk, dim3(wblocks), dim3(BLOCK), lds_eval, s, h, b.blob, k, cps, ipl,
+                               sets_k, best_k, first_k, b.vals, b.conn, b.counters);
+            hipLaunchKernelGGL(k_finalize_dpsub, dim3(lblocks), dim3(BLOCK), lds_eval, s, h, b.blob, k, sets_k,
+                               best_k, first_k, b.vals, b.left, b.counters);
         }
         if (profile && (!h.is_tree || k == K)) HIPCHK(hipEventRecord(c.prof_ev[3 * k + 2], s));
         if ((rc = gate.after(k))) return rc;
     }
     uint32_t root = (n == 32) ? 0xffffffffu : ((1u << n) - 1u);
     if (K < n) {                                   /* IDP1: the cheapest set of K relations */
-        rc = idp1_root(c, s, dense_memo(c), c.b.lvl_sets + off_h[K], nullptr, c.b.counters + K, n, K,
+        rc = idp1_root(c, s, dense_memo_of(b), b.lvl_sets + off_h[K], nullptr, b.counters + K, n, K,
                        [&](std::vector<uint32_t> &o) -> int { o = dpsub_creation_order(in, K); return GQ_OK; }, &root);
         if (rc) return rc;
     }
     const int nodes_n = 2 * K - 1;
-    if (!fused) hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo(c), root, c.b.nodes, nodes_n,
-                                   c.b.counters + 48);
+    if (!fused) hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo_of(b), root, b.nodes, nodes_n,
+                                   b.counters + 48);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
     st.t_last_launch = host_now_ms();
-    if (!fused && (rc = results_down(c, s, nodes_n))) return rc;
+    if (!fused && (rc = results_down(c, s, nodes_n, b.counters))) return rc;
     HIPCHK(hipStreamSynchronize(s));
     st.t_synced = host_now_ms();
-    if (fused) c.counters_clean = true;            /* the walk zeroed them */
+    if (fused && !noalloc) c.counters_clean = true;   /* the walk zeroed them */
     memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
     if (cnt[49] != 0) {
         set_error("a connected set has more than 2^31 splits (unsupported on the GPU path)");

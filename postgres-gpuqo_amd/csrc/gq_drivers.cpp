@@ -300,7 +300,7 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
         }
         std::vector<DPNode> nodes;
         DPStats st;
-        rc = run_dp(ins[i], algo, hs[i]->n_iters, ctx.memo_kind, ctx.profile != 0, nodes, st);
+        rc = run_dp(ins[i], algo, hs[i]->n_iters, ctx.memo_kind, ctx.profile != 0, nodes, st, ctx.noalloc);
         ctx.dp_calls++;
         if (rc) return rc;
         ctx.add(st);

@@ -17,6 +17,7 @@ namespace gq {
  * node built during the call (trees share composite subtrees). */
 struct RunCtx {
     int algorithm, bicc, idp_type, idp_n_iters, memo_kind, profile, max_memo_mb;
+    bool noalloc = false;      /* gq_plan_noalloc: DPs in the static scratch array */
     DPStats stats;
     int dp_calls = 0;
     std::deque<QT<1>> arena1;

@@ -576,12 +576,13 @@ static int emit(const QT<W> *t, gq_plan_out *out) {
 }
 
 template <int W>
-static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out, bool noalloc = false) {
     auto t0 = std::chrono::steady_clock::now();
     HQuery<W> h;
     convert(q, cfg, h);                /* host preparation first, then the device */
     if (int e = device_init(cfg->device)) return e;
     RunCtx ctx(cfg);
+    ctx.noalloc = noalloc;
     QT<W> *t = nullptr;
     int rc = gpuqo_run_t(ctx, h, &t);
     out->n_nodes = 0;
@@ -684,6 +685,44 @@ static gq_status plan_impl(const gq_query *q, const gq_config *cfg, gq_plan_out 
 
 gq_status gq_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     return gq::guarded([&] { return plan_impl(q, cfg, out); });
+}
+
+static gq_status plan_noalloc_impl(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    int rc = gq::validate(q);
+    if (rc) return (gq_status)rc;
+    if (!cfg || !out || !out->nodes) {
+        gq::set_error("null config or output");
+        return GQ_ERR_INVALID;
+    }
+    if (cfg->gpuqo_algorithm != GQ_ALGO_DPSUB) {
+        gq::set_error("the allocation-free path plans GQ_ALGO_DPSUB (plain or bicc) only");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (q->n_rels > GQ_NOALLOC_MAX_RELS) {
+        gq::set_error("the allocation-free path plans at most %d relations (got %d)", GQ_NOALLOC_MAX_RELS,
+                      q->n_rels);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (cfg->gpuqo_idp_n_iters > 1 && cfg->gpuqo_idp_n_iters < q->n_rels) {
+        gq::set_error("the allocation-free path runs no IDP decomposition");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (cfg->memo_kind != GQ_MEMO_AUTO && cfg->memo_kind != GQ_MEMO_DENSE) {
+        gq::set_error("the allocation-free path uses the dense memo only");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (!gq::device_ready(cfg->device)) {
+        gq::set_error("gq_init(%d) must be called before gq_plan_noalloc", cfg->device);
+        return GQ_ERR_INVALID;
+    }
+    gq_config c = *cfg;
+    c.profile_kernels = 0;
+    c.gpuqo_max_memo_size_mb = std::max(c.gpuqo_max_memo_size_mb, 64);   /* 2^20 dense records fit */
+    return (gq_status)gq::plan_t<1>(q, &c, out, true);
+}
+
+gq_status gq_plan_noalloc(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    return gq::guarded([&] { return plan_noalloc_impl(q, cfg, out); });
 }
 
 static gq_status plan_batch_impl(int32_t n_queries, const gq_query *const *qs, const gq_config *cfg,

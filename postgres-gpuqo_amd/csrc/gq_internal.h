@@ -19,6 +19,9 @@ int validate(const gq_query *q);
 /* Bind the device context (stream, constant tables) to `device`. */
 int device_init(int device);
 
+/* The device context is bound to `device` (gq_init has run for it). */
+bool device_ready(int device);
+
 /* Interrupt flag (CHECK_FOR_INTERRUPTS), polled between DP calls and
  * between the level launches of one DP. */
 bool interrupted();
