@@ -334,8 +334,10 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
                       int32_t knobs, gq_tuning *out);
 
 /* Request an interrupt of the running call (CHECK_FOR_INTERRUPTS,
- * gpuqo.cuh:21-28); the call returns GQ_ERR_INTERRUPTED and frees its
- * device memory. */
+ * gpuqo.cuh:21-28): the planner stops enqueuing DP levels (the levels
+ * already enqueued finish), and the call returns GQ_ERR_INTERRUPTED; the
+ * device buffers stay cached for the next call.  gq_interrupt(0) clears the
+ * flag. */
 void gq_interrupt(int32_t on);
 
 const char *gq_last_error(void);
