@@ -104,6 +104,13 @@ struct Params {             /* GpuqoPlannerInfoParams subset (float), planner_in
 #define GQ_MF_INDEX 2u            /* index available                               */
 #define GQ_MF_UNIQUE 4u           /* index unique                                  */
 
+/* Three records per two-member class (View::pair_rec): {sel_lo, sel_hi,
+ * lower member as a float, 0}, then per member (lower, higher) the bucket
+ * constants {ndistinct, 1/ndistinct, skew, mcvfreq}; a member whose bucket
+ * size is the 0.1 default has {-inf, 0.1f, 1.0f, mcvfreq}, which gives
+ * exactly 0.1f below. */
+struct alignas(16) PairRec { float x, y, z, w; };
+
 template <class S>
 struct View {
     int n_rels, n_ec, ecw;
@@ -118,12 +125,18 @@ struct View {
     const uint8_t *ec_hasfk;      /* [n_ec] some member holds an FK                */
     const S *mem_fks;             /* [n_members]                                   */
     const float *mem_nd;          /* ndistinct after the rows/tuples adjustment    */
-    const float *mem_avgfreq;
+    const float *mem_inv_nd;      /* 1.0f / mem_nd                                 */
+    const float *mem_skew;        /* mcvfreq / avgfreq where it applies, else 1.0f */
     const float *mem_mcv;
     const uint8_t *mem_flags;
     const float *mem_idx_rows, *mem_idx_startup, *mem_idx_total;
     const float *sels;
     const float *qualtab;         /* qualtab[k] = cpu_operator_cost added k times  */
+    /* pairs != 0: every class has exactly two members; class e's lower
+     * member is relation ec_lo[e], and its selectivity is sel_lo[e] when
+     * that member is on the outer side of a join, sel_hi[e] otherwise */
+    int pairs;
+    const PairRec *pair_rec;      /* [3 * n_ec] when pairs                         */
 };
 
 struct Rel {                /* value part of JoinRelation (planner_info.cuh:22-28) */
@@ -369,24 +382,37 @@ GQ_HD void nestloop(const View<S> &q, const Rel &outer, float in_rows, float in_
 }
 
 /* __estimate_hash_bucketsize with the nbuckets-independent part precomputed
- * per member (cost_postgres.cuh:701-775). */
+ * per member (cost_postgres.cuh:701-775): fb = (float)nbuckets and
+ * inv_fb = 1.0f / fb come from the join, 1.0f / ndistinct and the skew
+ * quotient mcvfreq / avgfreq from the member (the quotients the reference
+ * computes, rounded once; a member without skew multiplies by 1.0f, which
+ * changes no bit). */
 template <class S>
-GQ_HD float bucketsize(const View<S> &q, int mi, int nbuckets) {
+GQ_HD float bucketsize(const View<S> &q, int mi, float fb, float inv_fb) {
     if (q.mem_flags[mi] & GQ_MF_BUCKET_DEFAULT) return 0.1f;
-    float nd = q.mem_nd[mi], avgfreq = q.mem_avgfreq[mi], mcv = q.mem_mcv[mi];
-    float fb = (float)nbuckets;
-    float est = nd > fb ? 1.0f / fb : 1.0f / nd;
-    if (avgfreq > 0.0f && mcv > avgfreq) est *= mcv / avgfreq;
+    float est = q.mem_nd[mi] > fb ? inv_fb : q.mem_inv_nd[mi];
+    est *= q.mem_skew[mi];
     if (est < 1.0e-6f) est = 1.0e-6f;
     else if (est > 1.0f) est = 1.0f;
     return est;
 }
 
-/* cost_hashjoin (cost_postgres.cuh:287-470).  `pv` lists the member slots of
- * the inner side in the crossing classes (see join_prov). */
-template <class S, class Prov>
-GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const Rel &inner, const Extra &ex,
-                    int n_quals, float qual_per_tuple, const Prov &pv, float *o_startup, float *o_total) {
+/* estimate_hash_bucket_stats (cost_postgres.cuh:620-668) folds one inner
+ * member: the minima do not depend on the order the members come in. */
+template <class S>
+GQ_HD void bucket_fold(const View<S> &q, int mi, float fb, float inv_fb, float *bsize, float *mcvfreq) {
+    const float t = bucketsize(q, mi, fb, inv_fb);
+    if (*bsize > t) *bsize = t;
+    if (*mcvfreq > q.mem_mcv[mi]) *mcvfreq = q.mem_mcv[mi];
+}
+
+/* cost_hashjoin (cost_postgres.cuh:287-470) with the inner's hash table size
+ * `hs` (virtualbuckets = nbuckets * nbatches) and the bucket statistics
+ * (bsize, mcvfreq) of its crossing members already computed (join_prov). */
+template <class S>
+GQ_HD void hashjoin(const View<S> &q, const Rel &outer, const Rel &inner, const Extra &ex, int n_quals,
+                    float qual_per_tuple, HashSize hs, float virtualbuckets, float bsize, float mcvfreq,
+                    float *o_startup, float *o_total) {
     float startup_cost = 0.0f, run_cost = 0.0f;
     float outer_path_rows = outer.rows, inner_path_rows = inner.rows;
     if (!q.p.enable_hashjoin) startup_cost += q.p.disable_cost;
@@ -395,22 +421,12 @@ GQ_HD void hashjoin(const View<S> &q, const S &inner_id, const Rel &outer, const
     startup_cost += inner.total;
     startup_cost += (q.p.cpu_operator_cost * (float)n_quals + q.p.cpu_tuple_cost) * inner_path_rows;
     run_cost += q.p.cpu_operator_cost * (float)n_quals * outer_path_rows;
-    HashSize hs = inner.hs ? hash_unpack(inner.hs) : hash_table_size(inner_path_rows, inner.width, q.p.work_mem);
     if (hs.nbatches > 1) {
         float outerpages = page_size(outer_path_rows, outer.width);
         float innerpages = page_size(inner_path_rows, inner.width);
         startup_cost += q.p.seq_page_cost * innerpages;
         run_cost += q.p.seq_page_cost * (innerpages + 2.0f * outerpages);
     }
-    float virtualbuckets = (float)hs.nbuckets * (float)hs.nbatches;
-    int nbk = f2i_x86(virtualbuckets);
-    /* estimate_hash_bucket_stats (:620-668) */
-    float bsize = 1.0f, mcvfreq = 1.0f;
-    pv.for_each_inner(q, inner_id, [&](int mi) {
-        float t = bucketsize(q, mi, nbk);
-        if (bsize > t) bsize = t;
-        if (mcvfreq > q.mem_mcv[mi]) mcvfreq = q.mem_mcv[mi];
-    });
     if (relation_byte_size(clamp_row_est(inner_path_rows * mcvfreq), inner.width) >
         (float)((long)q.p.work_mem * 1024L))
         startup_cost += q.p.disable_cost;
@@ -453,6 +469,45 @@ struct MaskProv {
     template <class S>
     GQ_HD float selectivity(const View<S> &q, const S &l, const S &r) const { return join_selectivity<ECW>(q, l, r, m); }
     GQ_HD int n_quals() const { return nq; }
+    /* One pass over the crossing classes: the selectivity (product in
+     * ascending class order) and, with `buckets`, the bucket statistics of
+     * the inner members. */
+    template <class S>
+    GQ_HD float scan(const View<S> &q, const S &l, const S &r, bool buckets, float fb, float inv_fb, float *bsize,
+                     float *mcvfreq) const {
+        float sel = 1.0f;
+        if (q.pairs) {
+            for_each_ec<ECW>(q, m, [&](int e) {
+                const PairRec *pr = q.pair_rec + 3 * e;
+                const PairRec c = pr[0], m0 = pr[1], m1 = pr[2];   /* three independent reads */
+                const bool lo_outer = bit(l, (int)c.z);
+                sel *= lo_outer ? c.x : c.y;
+                if (buckets) {
+                    const PairRec mm = lo_outer ? m1 : m0;           /* the inner member */
+                    float est = mm.x > fb ? inv_fb : mm.y;
+                    est *= mm.z;
+                    if (est < 1.0e-6f) est = 1.0e-6f;
+                    else if (est > 1.0f) est = 1.0f;
+                    if (*bsize > est) *bsize = est;
+                    if (*mcvfreq > mm.w) *mcvfreq = mm.w;
+                }
+            });
+        } else {
+            for_each_ec<ECW>(q, m, [&](int e) {
+                sel *= ec_selectivity(q, e, l, r);
+                if (buckets) {
+                    S ec = q.ec_relids[e];
+                    S mr = ec & r;
+                    while (!empty(mr)) {
+                        S in_id = lowbit(mr);
+                        bucket_fold(q, q.ec_moff[e] + rank_below(ec, in_id), fb, inv_fb, bsize, mcvfreq);
+                        mr = mr - in_id;
+                    }
+                }
+            });
+        }
+        return sel;
+    }
     /* f(member slot) for each member of `inner` in each class, in class order */
     template <class S, class F>
     GQ_HD void for_each_inner(const View<S> &q, const S &inner, F f) const {
@@ -474,6 +529,13 @@ struct EdgeProv {
     const uint16_t *mi;
     template <class S>
     GQ_HD float selectivity(const View<S> &, const S &, const S &) const { return sel; }
+    template <class S>
+    GQ_HD float scan(const View<S> &q, const S &, const S &, bool buckets, float fb, float inv_fb, float *bsize,
+                     float *mcvfreq) const {
+        if (buckets)
+            for (int i = 0; i < n_mi; i++) bucket_fold(q, (int)mi[i], fb, inv_fb, bsize, mcvfreq);
+        return sel;
+    }
     GQ_HD int n_quals() const { return nq; }
     template <class S, class F>
     GQ_HD void for_each_inner(const View<S> &, const S &, F f) const {
@@ -486,7 +548,21 @@ struct EdgeProv {
 template <class S, class Prov>
 GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const Prov &pv) {
     Rel out;
-    out.rows = join_rows(pv.selectivity(q, l, r), L.rows, R.rows);
+    /* the inner's hash table size first (it depends on the inner alone), so
+     * one pass over the crossing classes gives the selectivity and the
+     * bucket statistics together */
+    const bool hj = q.p.enable_hashjoin != 0;
+    HashSize hsz;
+    hsz.nbuckets = 0;
+    hsz.nbatches = 0;
+    float vbuckets = 0.0f, fb = 0.0f, inv_fb = 0.0f, bsize = 1.0f, mcvfreq = 1.0f;
+    if (hj) {
+        hsz = R.hs ? hash_unpack(R.hs) : hash_table_size(R.rows, R.width, q.p.work_mem);
+        vbuckets = (float)hsz.nbuckets * (float)hsz.nbatches;
+        fb = (float)f2i_x86(vbuckets);         /* the int nbuckets of estimate_hash_bucket_stats */
+        inv_fb = 1.0f / fb;
+    }
+    out.rows = join_rows(pv.scan(q, l, r, hj, fb, inv_fb, &bsize, &mcvfreq), L.rows, R.rows);
     out.width = L.width + R.width;
     out.hs = 0u;
     const int n_quals = pv.n_quals();
@@ -506,9 +582,9 @@ GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     ex.match_count = R.rows;
     float bs, bt;
     nestloop(q, L, R.rows, R.startup, R.total, ex, qual_pt, &bs, &bt);
-    if (q.p.enable_hashjoin) {
+    if (hj) {
         float hs, ht;
-        hashjoin(q, r, L, R, ex, n_quals, qual_pt, pv, &hs, &ht);
+        hashjoin(q, L, R, ex, n_quals, qual_pt, hsz, vbuckets, bsize, mcvfreq, &hs, &ht);
         if (ht < bt) { bs = hs; bt = ht; }
     }
     if (single && !bit(q.composite, ctz(r))) {       /* index nested loop :116-147 */

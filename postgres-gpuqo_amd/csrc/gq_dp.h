@@ -22,8 +22,11 @@ struct BlobHdr {
     uint32_t composite;
     int words;
     int o_rel_rows, o_rel_fksel, o_adj, o_inc, o_ec_relids, o_ec_moff, o_ec_soff, o_ec_hasfk;
-    int o_mem_fks, o_mem_nd, o_mem_avgfreq, o_mem_mcv, o_mem_flags, o_mem_idx_rows, o_mem_idx_startup,
-        o_mem_idx_total, o_sels, o_qualtab, o_base;
+    int o_mem_fks, o_mem_nd, o_mem_inv_nd, o_mem_skew, o_mem_mcv, o_mem_flags, o_mem_idx_rows,
+        o_mem_idx_startup, o_mem_idx_total, o_sels, o_qualtab, o_base;
+    /* every class has two members (View::pairs): three 16-byte records per
+     * class (PairRec), 4-word aligned */
+    int pairs, o_pair_rec;
     /* Tree graphs whose every vertex v > 0 has exactly one lower-numbered
      * neighbour (its parent; true after BFS relabelling of a tree): o_desc
      * holds each vertex's descendants (itself included), o_lglob the vertices

@@ -105,7 +105,8 @@ __device__ __forceinline__ View<uint32_t> make_view(const BlobHdr &h, const uint
     v.ec_hasfk = (const uint8_t *)(s + h.o_ec_hasfk);
     v.mem_fks = s + h.o_mem_fks;
     v.mem_nd = (const float *)(s + h.o_mem_nd);
-    v.mem_avgfreq = (const float *)(s + h.o_mem_avgfreq);
+    v.mem_inv_nd = (const float *)(s + h.o_mem_inv_nd);
+    v.mem_skew = (const float *)(s + h.o_mem_skew);
     v.mem_mcv = (const float *)(s + h.o_mem_mcv);
     v.mem_flags = (const uint8_t *)(s + h.o_mem_flags);
     v.mem_idx_rows = (const float *)(s + h.o_mem_idx_rows);
@@ -113,6 +114,8 @@ __device__ __forceinline__ View<uint32_t> make_view(const BlobHdr &h, const uint
     v.mem_idx_total = (const float *)(s + h.o_mem_idx_total);
     v.sels = (const float *)(s + h.o_sels);
     v.qualtab = (const float *)(s + h.o_qualtab);
+    v.pairs = h.pairs;
+    v.pair_rec = (const PairRec *)(s + h.o_pair_rec);
     return v;
 }
 
@@ -281,6 +284,36 @@ __device__ __forceinline__ uint32_t ord_key(float x) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+/* C(n, k) at [n * 33 + k], n, k <= 32 (uploaded by ctx_init) */
+__constant__ uint32_t c_binom[33 * 33];
+
+/* The t-th proper non-empty subset of k positions (t < 2^k - 2) in order of
+ * size, then of numeric value within a size (combinadic unranking).  The
+ * DPsub kernels enumerate a k-set's splits in this order -- each split keeps
+ * its numeric rank, so the first-valid and first-minimum keys are those of
+ * the reference's increasing-rank order -- because neighbouring lanes then
+ * cost splits with the same left size, i.e. (on dense graphs) the same
+ * number of crossing classes, and do not wait on each other. */
+__device__ __forceinline__ uint32_t split_by_size(uint32_t t, int k) {
+    const uint32_t *row = c_binom + k * 33;
+    int a = 1;
+    uint32_t c = row[1];
+    while (t >= c && a < k - 1) {
+        t -= c;
+        c = row[++a];
+    }
+    uint32_t m = 0;
+    for (int pos = k - 1; pos >= 0 && a > 0; pos--) {
+        const uint32_t b = c_binom[pos * 33 + a];
+        if (t >= b) {
+            m |= 1u << pos;
+            t -= b;
+            a--;
+        }
+    }
+    return m;
+}
+
 /* Adds the wave's pair count to counters[40..41]: one atomic per wave.  All
  * lanes of the wave must call it. */
 __device__ __forceinline__ void add_pairs(uint32_t *counters, uint32_t n) {
@@ -432,27 +465,26 @@ __global__ void __launch_bounds__(BLOCK) k_eval_dpsub_chunk(BlobHdr h, const uin
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; w < nwaves; w += wstride) {
         uint32_t idx = (uint32_t)(w / cps), part = (uint32_t)(w % cps);
         uint32_t S = lvl_sets[idx];
-        uint64_t j0 = 1 + (uint64_t)part * ipl * WAVE + (uint64_t)lane * ipl;
-        uint64_t j1 = j0 + ipl;
-        if (j1 > (uint64_t)T + 1) j1 = (uint64_t)T + 1;
+        /* the wave's ipl * WAVE consecutive splits in size order, lane-strided */
+        const uint64_t t0 = (uint64_t)part * ipl * WAVE + (uint64_t)lane;
         uint32_t first = ~0u;
         unsigned long long best = ~0ull;
-        if (j0 <= T) {
-            uint32_t l = pdep32((uint32_t)j0, S);
-            for (uint64_t j = j0; j < j1; j++) {
-                uint32_t r = S ^ l;
-                if (is_conn(conn, l) && is_conn(conn, r)) {
-                    Rel L = load_rel(vals, l), R = load_rel(vals, r);
-                    Rel jr = join(q, l, L, r, R);
-                    npairs++;
-                    bool nan = jr.total != jr.total;
-                    if (first == ~0u) first = ((uint32_t)j << 1) | (nan ? 1u : 0u);
-                    if (!nan) {
-                        unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | (uint32_t)j;
-                        if (key < best) best = key;
-                    }
+        for (uint32_t s = 0; s < ipl; s++) {
+            const uint64_t t = t0 + (uint64_t)s * WAVE;
+            if (t >= T) break;
+            const uint32_t j = split_by_size((uint32_t)t, k);
+            const uint32_t l = pdep32(j, S), r = S ^ l;
+            if (is_conn(conn, l) && is_conn(conn, r)) {
+                Rel L = load_rel(vals, l), R = load_rel(vals, r);
+                Rel jr = join(q, l, L, r, R);
+                npairs++;
+                bool nan = jr.total != jr.total;
+                const uint32_t f = (j << 1) | (nan ? 1u : 0u);
+                if (f < first) first = f;
+                if (!nan) {
+                    unsigned long long key = ((unsigned long long)ord_key(jr.total) << 32) | j;
+                    if (key < best) best = key;
                 }
-                l = S & (l - S);
             }
         }
         for (int off = WAVE / 2; off > 0; off >>= 1) {
@@ -1444,7 +1476,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                     uint32_t l = 0, i = 0, j = 0;
                     if (item < rend) {
                         i = (uint32_t)(item / T);
-                        j = (uint32_t)(item % T) + 1u;
+                        j = split_by_size((uint32_t)(item % T), k);
                         uint32_t S = s_set[i];
                         l = pdep32(j, S);
                         ok = is_conn(cb, l) && is_conn(cb, S ^ l);
@@ -2546,6 +2578,12 @@ static int ctx_init(DeviceCtx &c, int device) {
     for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.gate_ev[k], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c.ev0));
     HIPCHK(hipEventCreate(&c.ev1));
+    {
+        uint32_t tab[33 * 33];
+        for (int n = 0; n <= 32; n++)
+            for (int k = 0; k <= 32; k++) tab[n * 33 + k] = (uint32_t)binom64(n, k);
+        HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), tab, sizeof(tab)));
+    }
     if (device >= 0 && device < GQ_MAX_DEVICES && !g_scratch[device]) {
         void *p = nullptr;
         HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));   /* this device's copy of the array */

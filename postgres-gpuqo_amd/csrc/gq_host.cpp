@@ -73,7 +73,8 @@ void HostView<W>::build(const HQuery<W> &q) {
     }
     const int nm = moff;
     mem_nd.assign(nm, 0.0f);
-    mem_avgfreq.assign(nm, 0.0f);
+    mem_inv_nd.assign(nm, 0.0f);
+    mem_skew.assign(nm, 1.0f);
     mem_mcv.assign(nm, 0.0f);
     mem_flags.assign(nm, 0);
     mem_idx_rows.assign(nm, 0.0f);
@@ -94,7 +95,9 @@ void HostView<W>::build(const HQuery<W> &q) {
                 nd = clamp_row_est(nd);
             }
             mem_nd[m] = nd;
-            mem_avgfreq[m] = avgfreq;
+            mem_inv_nd[m] = 1.0f / nd;                              /* :751-754 */
+            if (avgfreq > 0.0f && var.mcvfreq > avgfreq)            /* :766-767 */
+                mem_skew[m] = var.mcvfreq / avgfreq;
         }
         mem_mcv[m] = var.mcvfreq;
         if (var.index_available) mem_flags[m] |= GQ_MF_INDEX;
@@ -126,7 +129,8 @@ void HostView<W>::build(const HQuery<W> &q) {
     v.ec_hasfk = ec_hasfk.data();
     v.mem_fks = q.fks.data();
     v.mem_nd = mem_nd.data();
-    v.mem_avgfreq = mem_avgfreq.data();
+    v.mem_inv_nd = mem_inv_nd.data();
+    v.mem_skew = mem_skew.data();
     v.mem_mcv = mem_mcv.data();
     v.mem_flags = mem_flags.data();
     v.mem_idx_rows = mem_idx_rows.data();
@@ -134,6 +138,40 @@ void HostView<W>::build(const HQuery<W> &q) {
     v.mem_idx_total = mem_idx_total.data();
     v.sels = q.sels.data();
     v.qualtab = qualtab.data();
+    /* two-member classes: the selectivity of each orientation, from the
+     * same estimate_ec_selectivity code (FK test included) */
+    v.pairs = 0;
+    v.pair_rec = nullptr;
+    bool pairs = n_ec > 0;
+    for (int e = 0; e < n_ec; e++)
+        if (popc(q.ec_relids[e]) != 2) pairs = false;
+    if (pairs) {
+        pair_rec.resize((size_t)3 * n_ec);
+        for (int e = 0; e < n_ec; e++) {
+            Set<W> lo = lowbit(q.ec_relids[e]), hi = q.ec_relids[e] - lo;
+            PairRec &c = pair_rec[3 * (size_t)e];
+            c.x = ec_selectivity(v, e, lo, hi);
+            c.y = ec_selectivity(v, e, hi, lo);
+            c.z = (float)ctz(lo);
+            c.w = 0.0f;
+            for (int k = 0; k < 2; k++) {
+                const int m = ec_moff[e] + k;
+                PairRec &r = pair_rec[3 * (size_t)e + 1 + k];
+                if (mem_flags[m] & GQ_MF_BUCKET_DEFAULT) {
+                    r.x = -INFINITY;
+                    r.y = 0.1f;
+                    r.z = 1.0f;
+                } else {
+                    r.x = mem_nd[m];
+                    r.y = mem_inv_nd[m];
+                    r.z = mem_skew[m];
+                }
+                r.w = mem_mcv[m];
+            }
+        }
+        v.pairs = 1;
+        v.pair_rec = pair_rec.data();
+    }
 }
 
 template struct HostView<1>;
@@ -160,7 +198,8 @@ static View<uint32_t> host_view(const BlobHdr &h, const uint32_t *s) {
     v.ec_hasfk = (const uint8_t *)(s + h.o_ec_hasfk);
     v.mem_fks = s + h.o_mem_fks;
     v.mem_nd = (const float *)(s + h.o_mem_nd);
-    v.mem_avgfreq = (const float *)(s + h.o_mem_avgfreq);
+    v.mem_inv_nd = (const float *)(s + h.o_mem_inv_nd);
+    v.mem_skew = (const float *)(s + h.o_mem_skew);
     v.mem_mcv = (const float *)(s + h.o_mem_mcv);
     v.mem_flags = (const uint8_t *)(s + h.o_mem_flags);
     v.mem_idx_rows = (const float *)(s + h.o_mem_idx_rows);
@@ -168,6 +207,8 @@ static View<uint32_t> host_view(const BlobHdr &h, const uint32_t *s) {
     v.mem_idx_total = (const float *)(s + h.o_mem_idx_total);
     v.sels = (const float *)(s + h.o_sels);
     v.qualtab = (const float *)(s + h.o_qualtab);
+    v.pairs = h.pairs;
+    v.pair_rec = (const PairRec *)(s + h.o_pair_rec);
     return v;
 }
 
@@ -268,7 +309,8 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     }
     h.o_mem_fks = take(std::max(1, nm));
     h.o_mem_nd = take(std::max(1, nm));
-    h.o_mem_avgfreq = take(std::max(1, nm));
+    h.o_mem_inv_nd = take(std::max(1, nm));
+    h.o_mem_skew = take(std::max(1, nm));
     h.o_mem_mcv = take(std::max(1, nm));
     h.o_mem_flags = take(std::max(1, (nm + 3) / 4));
     h.o_mem_idx_rows = take(std::max(1, nm));
@@ -277,7 +319,8 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     for (int m = 0; m < nm; m++) {
         b[h.o_mem_fks + m] = (uint32_t)q.fks[m].w[0];
         put_f(h.o_mem_nd, m, hv.mem_nd[m]);
-        put_f(h.o_mem_avgfreq, m, hv.mem_avgfreq[m]);
+        put_f(h.o_mem_inv_nd, m, hv.mem_inv_nd[m]);
+        put_f(h.o_mem_skew, m, hv.mem_skew[m]);
         put_f(h.o_mem_mcv, m, hv.mem_mcv[m]);
         ((uint8_t *)&b[h.o_mem_flags])[m] = hv.mem_flags[m];
         put_f(h.o_mem_idx_rows, m, hv.mem_idx_rows[m]);
@@ -288,6 +331,10 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     for (size_t i = 0; i < q.sels.size(); i++) put_f(h.o_sels, (int)i, q.sels[i]);
     h.o_qualtab = take(n_ec + 1);
     for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
+    h.pairs = hv.v.pairs;
+    while (b.size() & 3) b.push_back(0u);       /* 16-byte records: 4-word aligned in the staged blob */
+    h.o_pair_rec = take(h.pairs ? 12 * n_ec : 4);
+    if (h.pairs) memcpy(&b[h.o_pair_rec], hv.pair_rec.data(), sizeof(PairRec) * 3 * (size_t)n_ec);
     in.tree_counts.clear();
     in.memo = GQ_MEMO_DENSE;
     h.o_desc = take(32);

@@ -91,8 +91,9 @@ struct HQuery {
 /* Arrays the cost model reads, derived from an HQuery. */
 template <int W>
 struct HostView {
-    std::vector<float> rel_rows, rel_fksel, mem_nd, mem_avgfreq, mem_mcv, mem_idx_rows, mem_idx_startup,
+    std::vector<float> rel_rows, rel_fksel, mem_nd, mem_inv_nd, mem_skew, mem_mcv, mem_idx_rows, mem_idx_startup,
         mem_idx_total, qualtab;
+    std::vector<PairRec> pair_rec;
     std::vector<uint64_t> inc;
     std::vector<int> ec_moff, ec_soff, mem_rel;
     std::vector<uint8_t> ec_hasfk, mem_flags;
