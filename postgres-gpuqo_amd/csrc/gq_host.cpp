@@ -704,8 +704,7 @@ gq_status gq_config_default(gq_config *c) {
     c->effective_cache_size = 524288;
     c->work_mem = 4096;
     c->enable_seqscan = c->enable_indexscan = c->enable_tidscan = c->enable_sort = 1;
-    c->enable_hashagg = c->enable_nestloop = c->enable_hashjoin = 1;
-    c->enable_mergejoin = 0;
+    c->enable_hashagg = c->enable_nestloop = c->enable_mergejoin = c->enable_hashjoin = 1;   /* guc.c:999-1027 */
     c->memo_kind = GQ_MEMO_AUTO;
     c->device = 0;
     return GQ_OK;

@@ -165,7 +165,7 @@ def default_config(**overrides):
     c.enable_sort = 1
     c.enable_hashagg = 1
     c.enable_nestloop = 1
-    c.enable_mergejoin = 0
+    c.enable_mergejoin = 1              # guc.c:1009-1017 (not read by the join cost model)
     c.enable_hashjoin = 1
     c.memo_kind = MEMO["auto"]
     c.device = 0
