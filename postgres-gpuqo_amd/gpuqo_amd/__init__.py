@@ -187,7 +187,10 @@ def tuning_set(**kw):
     """Set some launch-shape knobs (the others keep their values)."""
     t = abi.Tuning()
     _check(lib().gq_tuning_get(C.byref(t)))
+    names = {f for f, _ in abi.Tuning._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise KeyError(k)
         setattr(t, k, v)
     _check(lib().gq_tuning_set(C.byref(t)))
 
