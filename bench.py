@@ -123,13 +123,13 @@ def batched_section(gpuqo_amd, abi, qgen, reps=5):
             "batch_equals_single_all": same}
 
 
-def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=128, load=0.8):
+def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, inflight=128, loads=(0.5, 0.8)):
     """The served mix: `count` seeded requests (70% snowflakes of 10-30 relations
     planned directly by MPDP, 30% snowflakes of 60-200 relations, depth 4,
     fanout 1-5, by UnionDP with k=25 and MPDP per partition) at the
-    continuous-batching server.  Three legs, after one untimed pass of the
-    mix: capacity (every request submitted at once), Poisson arrivals at
-    `rate` requests/s, and Poisson arrivals at `load` x the measured
+    continuous-batching server.  Legs, after one untimed pass of the mix:
+    capacity (every request submitted at once), Poisson arrivals at `rate`
+    requests/s, and Poisson arrivals at each of `loads` x the measured
     capacity.  Latency is submission to completion (measured by the
     library); every served plan is compared with the same request planned
     alone by gq_plan, which also gives the one-at-a-time throughput (after
@@ -197,10 +197,13 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
         sat_s = time.perf_counter() - t0
         capacity = count / sat_s if sat == count else float("nan")
         res, ok, fixed = poisson_leg(srv, rate, 2 * count)
-        loaded = None
+        loaded, loaded_res = [], []
         if capacity == capacity:
-            res2, ok2, loaded = poisson_leg(srv, load * capacity, 4 * count)
-            ok = ok and ok2
+            for j, load in enumerate(loads):
+                res2, ok2, leg = poisson_leg(srv, load * capacity, (4 + 2 * j) * count)
+                loaded.append(dict(leg, load_of_capacity=load))
+                loaded_res.append(res2)
+                ok = ok and ok2
     # the same requests one at a time through gq_plan: throughput and parity
     # (after one untimed pass, as for the server)
     for g, large in mix:
@@ -213,7 +216,7 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
         single_s += time.perf_counter() - t1
         alone = gpuqo_amd.plan_bits(gpuqo_amd.plan_tuples(out))
         same = same and res.get(i, (None, None))[1] == alone
-        if loaded is not None:
+        for res2 in loaded_res:
             same = same and res2.get(i, (None, None))[1] == alone
     out = {"workload": "serve-mix256", "requests": count,
            "large_uniondp_requests": sum(1 for _, l in mix if l),
@@ -221,7 +224,7 @@ def serving_section(gpuqo_amd, abi, qgen, count=256, rate=1000.0, budget_gb=96, 
            "arrival_rate_per_s": rate, "served_queries_per_s": fixed["served_queries_per_s"],
            "latency_p50_ms": fixed["latency_p50_ms"], "latency_p99_ms": fixed["latency_p99_ms"],
            "latency_max_ms": fixed["latency_max_ms"],
-           "loaded": dict(loaded, load_of_capacity=load) if loaded else None,
+           "loaded": loaded or None,
            "memo_budget_gb": budget_gb, "max_inflight": inflight, "all_ok": ok,
            "single_queries_per_s": count / single_s,
            "capacity_vs_single": capacity / (count / single_s),

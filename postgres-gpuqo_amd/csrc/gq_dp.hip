@@ -2427,11 +2427,12 @@ __global__ void __launch_bounds__(WAVE) k_tree(MemoRef mr, uint32_t root_set, DP
  * from +inf (cpu_sequential.cu:101-112; gpuqo_dpsub_common.cu:350-358), so
  * NaN and +inf totals never win.  out[0] = the least order key of the
  * level-K totals below +inf (~0: none), out[1] = how many sets have it,
- * out[2 ..] = the first IDP1_TIES of them (the host orders ties).  The
- * level's list is list[*list_off ..] (list_off may be null) of *count sets. */
+ * out[2 ..] = the first `cap` of them (the host orders ties).  The level's
+ * list is list[*list_off ..] (list_off may be null) of *count sets. */
 #define IDP1_TIES 64
 __global__ void __launch_bounds__(1024) k_best_of_level(const uint32_t *list, const uint32_t *list_off,
-                                                        const uint32_t *count, MemoRef mr, uint32_t *out) {
+                                                        const uint32_t *count, MemoRef mr, uint32_t *out,
+                                                        uint32_t cap) {
     __shared__ uint32_t s_min, s_n;
     const uint32_t off = list_off ? *list_off : 0u, cnt = *count;
     const float inf = __builtin_huge_valf();
@@ -2456,7 +2457,7 @@ __global__ void __launch_bounds__(1024) k_best_of_level(const uint32_t *list, co
         const float t = mr.vals[memo_slot(mr, S)].z;
         if (t < inf && ord_key(t) == m) {
             const uint32_t pos = atomicAdd(&s_n, 1u);
-            if (pos < IDP1_TIES) out[2 + pos] = S;
+            if (pos < cap) out[2 + pos] = S;
         }
     }
     __syncthreads();
@@ -2527,6 +2528,9 @@ struct DeviceCtx {
     uint32_t *big_prefix = nullptr;
     BatchJob *sjob = nullptr;
     size_t cap_big = 0, cap_big_prefix = 0, cap_sjob = 0;
+    /* IDP1: every tied set of the last level when the first pass kept too few */
+    uint32_t *idp1_ties = nullptr;
+    size_t cap_idp1_ties = 0;
     /* pinned host staging: query blob + level offsets in, counters + plan out */
     uint32_t *h_stage = nullptr, *h_res = nullptr;
     size_t cap_h_stage = 0;
@@ -3077,8 +3081,10 @@ static std::vector<uint32_t> dpsub_creation_order(const DPInput &in, int K) {
     return out;
 }
 
-/* Largest IDP1 memo the host replays to order tied sets */
-#define IDP1_REPLAY_MAX (1ull << 22)
+/* Largest IDP1 memo (sets of 1..K relations, counted by C(n, k)) the host
+ * replays to order tied sets; above it a tie is broken by the lowest set,
+ * which may differ from the CPU DP's choice (its memo iteration order). */
+#define IDP1_REPLAY_MAX (1ull << 24)
 
 /* IDP1: the plan root, the cheapest set of level K (k_best_of_level) with
  * ties ordered as the CPU DP orders them.  `created` produces the creation
@@ -3087,7 +3093,8 @@ template <class Created>
 static int idp1_root(DeviceCtx &c, hipStream_t s, const MemoRef &mr, const uint32_t *list, const uint32_t *list_off,
                      const uint32_t *count, int n, int K, Created created, uint32_t *root) {
     uint32_t *dout = (uint32_t *)c.b.nodes;                  /* free until the plan walk */
-    hipLaunchKernelGGL(k_best_of_level, dim3(1), dim3(1024), 0, s, list, list_off, count, mr, dout);
+    hipLaunchKernelGGL(k_best_of_level, dim3(1), dim3(1024), 0, s, list, list_off, count, mr, dout,
+                       (uint32_t)IDP1_TIES);
     HIPCHK(hipGetLastError());
     uint32_t *r = c.h_res + 64;
     HIPCHK(hipMemcpyAsync(r, dout, (2 + IDP1_TIES) * 4, hipMemcpyDeviceToHost, s));
@@ -3100,14 +3107,29 @@ static int idp1_root(DeviceCtx &c, hipStream_t s, const MemoRef &mr, const uint3
         *root = r[2];
         return GQ_OK;
     }
+    std::vector<uint32_t> ties(r + 2, r + 2 + std::min<uint32_t>(r[1], IDP1_TIES));
+    if (r[1] > IDP1_TIES) {                                  /* more ties than the first pass kept: all of them */
+        const uint32_t nt = r[1];
+        int rc0 = ensure(&c.idp1_ties, c.cap_idp1_ties, (size_t)nt + 2);
+        if (rc0) return rc0;
+        hipLaunchKernelGGL(k_best_of_level, dim3(1), dim3(1024), 0, s, list, list_off, count, mr, c.idp1_ties, nt);
+        HIPCHK(hipGetLastError());
+        ties.resize((size_t)nt + 2);
+        HIPCHK(hipMemcpyAsync(ties.data(), c.idp1_ties, ((size_t)nt + 2) * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (ties[1] != nt) {
+            set_error("IDP1: the tie count changed between passes");
+            return GQ_ERR_DEVICE;
+        }
+        ties.erase(ties.begin(), ties.begin() + 2);
+    }
     uint64_t sets = 0;
     for (int k = 1; k <= K; k++) sets += binom64(n, k);
-    if (r[1] > IDP1_TIES || sets > IDP1_REPLAY_MAX) {
-        set_error("IDP1: %u sets of %d relations tie for the cheapest plan; ordering them needs a memo replay "
-                  "beyond this build's limit", r[1], K);
-        return GQ_ERR_UNSUPPORTED;
+    if (sets > IDP1_REPLAY_MAX) {
+        *root = *std::min_element(ties.begin(), ties.end());
+        return GQ_OK;
     }
-    const std::unordered_set<uint32_t> cands(r + 2, r + 2 + r[1]);
+    const std::unordered_set<uint32_t> cands(ties.begin(), ties.end());
     std::vector<uint32_t> order;
     int rc = created(order);
     if (rc) return rc;

@@ -224,7 +224,8 @@ def _uniform(n, edges, shape):
 def test_idp1_ties(algo, bicc):
     star = _uniform(9, [(0, i) for i in range(1, 9)], "uniform-star")     # C(8,2) = 28 tied 3-sets
     chain = _uniform(10, [(i, i + 1) for i in range(9)], "uniform-chain")  # 7 tied 4-paths
-    for name, g, k in (("star9", star, 3), ("chain10", chain, 4)):
+    big = _uniform(12, [(0, i) for i in range(1, 12)], "uniform-star")    # C(11,4) = 330 tied 5-sets
+    for name, g, k in (("star9", star, 3), ("chain10", chain, 4), ("star12", big, 5)):
         c = cfg(algo, bicc, idp_type="idp1", idp_n_iters=k)
         want, _ = oracle_bits(g, c)
         got, _ = gpu_bits(g, c)
