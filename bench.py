@@ -479,7 +479,9 @@ def main():
     else:
         # DPsub/MPDP (80 P + 40 S + 16 C): two memo probes of an 8-byte key
         # and a 32-byte JoinRelation per pair, one memo write per set, one
-        # 8-byte candidate write + read per unranked set.
+        # 8-byte candidate write + read per generated set (C: the 2^n - n - 1
+        # integers the filter tests, or the connected sets that expansion
+        # builds on tree graphs, which need no filter).
         eval_bytes = 80.0 * P + 40.0 * S
         filt_bytes = 16.0 * Cn
     eval_kernel = {"mpdp-snowflake20": "k_eval_tree", "star12-dpsub": "k_eval_tree",

@@ -3897,6 +3897,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         }
     }
     st.candidates = cand_total;
+    if (tree_exp) {                        /* no 2^n scan: the level lists were built by expansion */
+        st.candidates = 0;
+        for (int k = 2; k <= K; k++) st.candidates += in.tree_counts[k];
+    }
     st.sets = 0;
     for (int k = 2; k <= K; k++) st.sets += cnt[k];
     st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
