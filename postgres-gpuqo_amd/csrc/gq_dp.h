@@ -60,7 +60,12 @@ struct DPInput {
     /* memo layout chosen for this DP: GQ_MEMO_DENSE, or GQ_MEMO_HASH /
      * GQ_MEMO_RANK (trees) */
     int memo = 1;
+    /* plain DPsub on a sparse cyclic graph: the splits of each set are
+     * enumerated through its biconnected blocks, as MPDP does, and ranked by
+     * DPsub's j (dpsub_by_blocks) */
+    bool dpsub_blocks = false;
 };
+
 
 /* GQ_MEMO_AUTO resolves to the rank memo for tree graphs whose dense memo
  * (36 B x 2^n) would exceed max_memo_mb; GQ_MEMO_HASH and GQ_MEMO_RANK
@@ -103,6 +108,14 @@ struct DPStats {
 double host_now_ms();
 
 enum DPAlgo { DP_DPSUB = 0, DP_BICC = 1, DP_DPSIZE = 2 };
+
+/* Split enumeration of a DP: 0 every subset of a set (DPsub), 1 the blocks
+ * of a set with MPDP ranks, 2 the blocks with DPsub ranks (the same valid
+ * splits as 0, so the same plan). */
+inline int dp_mode(const DPInput &in, DPAlgo algo) {
+    if (algo == DP_BICC) return 1;
+    return (algo == DP_DPSUB && in.dpsub_blocks && !in.hdr.is_tree) ? 2 : 0;
+}
 
 /* Run one DP on the current device.  `n_iters` < n_rels selects the IDP1
  * level cap (the tree returned is the cheapest set of that size).  With

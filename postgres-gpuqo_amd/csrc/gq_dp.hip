@@ -1355,9 +1355,11 @@ __host__ __device__ constexpr size_t eval_lds_dpsub(int words, int conn_words_ld
 /* Evaluate level k: batches bt0, bt0 + bstride, ... of B sets each (at most
  * BMAX_B for MPDP on general graphs).  The blob is staged at smem[0..words)
  * (and, for DPsub, `cb` points to the connectivity bitmap, staged or
- * global).  Returns the pairs costed; raises *err when a set has more than
- * 2^31 splits. */
-template <bool BICC>
+ * global).  JRANK (with BICC, general graphs): the block enumeration gives
+ * the valid splits and each is ranked by DPsub's j = pext(L, S), so the
+ * result is DPsub's (dp_mode 2).  Returns the pairs costed; raises *err
+ * when a set has more than 2^31 splits. */
+template <bool BICC, bool JRANK = false>
 __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint32_t> &q, int k, uint32_t B,
                                const uint32_t *cb, const uint32_t *lvl_sets, uint32_t count, float4 *vals,
                                uint32_t *left, uint32_t bt0, uint32_t bstride, const BigList &bl) {
@@ -1398,7 +1400,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                 s_nblk[tid] = nblk;
                 unsigned long long full;
                 cnt = set_items(s_blocks + tid * 32, nblk, &full);
-                if (full > 0x7fffffffull) {         /* split rank would not fit 31 bits */
+                if (full > 0x7fffffffull || (JRANK && k > 31)) {   /* split rank would not fit 31 bits */
                     s_misc[1] = 1;
                     cnt = 0;
                 } else if (bl.ents && cnt > BIG_SPLITS) {
@@ -1438,7 +1440,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                 const uint32_t sub = S & desc[c];
                 const uint32_t o = local & 1u;
                 const uint32_t L = o ? sub : (S ^ sub), R = S ^ L;
-                const uint32_t rank = BICC ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
+                const uint32_t rank = (BICC && !JRANK) ? (2u * tree_post(S, c, desc, lglob) + o) : pext32(L, S);
                 Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
                 npairs++;
                 bool nan = jr.total != jr.total;
@@ -1455,9 +1457,10 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                 }
                 const uint32_t i = lo, S = s_set[i];
                 uint32_t L, R;
-                const uint32_t rank = bicc_item(S, s_blocks + i * 32, s_own + i * BSET_W, s_subb + i * BSET_W,
-                                                (uint32_t)(item - s_off[i]), adj, &L, &R);
+                const uint32_t brank = bicc_item(S, s_blocks + i * 32, s_own + i * BSET_W, s_subb + i * BSET_W,
+                                                 (uint32_t)(item - s_off[i]), adj, &L, &R);
                 if ((L | R) == S) {
+                    const uint32_t rank = JRANK ? pext32(L, S) : brank;
                     Rel jr = join(q, L, load_rel(vals, L), R, load_rel(vals, R));
                     npairs++;
                     bool nan = jr.total != jr.total;
@@ -1515,7 +1518,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
             if (f != ~0u) {
                 uint32_t rank = (f & 1u) ? (f >> 1) : (uint32_t)(s_best[tid] & 0xffffffffull);
                 uint32_t S = s_set[tid], L, R;
-                if (tree && BICC) {
+                if (tree && BICC && !JRANK) {
                     const uint32_t *desc = smem + h.o_desc, *lglob = smem + h.o_lglob;
                     uint32_t rest = S & (S - 1u), sub = 0;
                     while (rest) {
@@ -1528,7 +1531,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
                     }
                     L = (rank & 1u) ? sub : (S ^ sub);
                     R = S ^ L;
-                } else if (BICC) {
+                } else if (BICC && !JRANK) {
                     bicc_split(S, s_blocks + tid * 32, s_own + tid * BSET_W, s_subb + tid * BSET_W, rank, &L, &R);
                 } else {
                     L = pdep32(rank, S);
@@ -1544,7 +1547,7 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
     return npairs;
 }
 
-template <bool BICC>
+template <bool BICC, bool JRANK = false>
 __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
                                                     int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
                                                     uint32_t *left, const uint32_t *conn, uint32_t *counters,
@@ -1562,8 +1565,8 @@ __global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *bl
     __syncthreads();
     View<uint32_t> q = make_view(h, smem);
     const uint32_t *cb = conn_words_lds ? conn_l : conn;
-    uint32_t npairs = eval_level<BICC>(h, smem, q, k, B, cb, lvl_sets, counters[k], vals, left, blockIdx.x,
-                                       gridDim.x, bl);
+    uint32_t npairs = eval_level<BICC, JRANK>(h, smem, q, k, B, cb, lvl_sets, counters[k], vals, left, blockIdx.x,
+                                              gridDim.x, bl);
     if (tid == 0 && s_misc[1]) atomicOr(&counters[49], 1u);
     add_pairs(counters, npairs);
 }
@@ -1978,12 +1981,20 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
             const MemoRef mr = job_memo(J);
             const uint32_t *slots = J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr;
             if (h.ecw == 1)
-                eval_level_tree<1, false, 1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                eval_level_tree<1, false, 1>(h, smem, q, k, J.bicc == 1, sets, slots, m, mr, threadIdx.x / WAVE,
                                                 BLOCK / WAVE);
             else
-                eval_level_tree<0, false, 1>(h, smem, q, k, J.bicc != 0, sets, slots, m, mr, threadIdx.x / WAVE,
+                eval_level_tree<0, false, 1>(h, smem, q, k, J.bicc == 1, sets, slots, m, mr, threadIdx.x / WAVE,
                                                 BLOCK / WAVE);
             np = 0;                            /* counted on the host */
+        } else if (J.bicc == 2) {                /* DPsub splits through the blocks, evaluated in place */
+            BigList bl;
+            bl.ents = nullptr;
+            bl.count = nullptr;
+            bl.cap = 0;
+            bl.job = 0;
+            bl.gidx0 = 0;
+            np = eval_level<true, true>(h, smem, q, k, spt, J.conn, sets, m, J.vals, J.left, 0, 1, bl);
         } else if (J.bicc) {
             BigList bl;
             bl.ents = big;
@@ -2046,7 +2057,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs,
     if (cnt <= first) return;
     const uint32_t m = cnt - first < spt ? cnt - first : spt;
     const MemoRef mr = job_memo(J);
-    eval_level_tree<1, true, 1>(h, smem, q, k, J.bicc != 0, J.lvl_sets + J.lvl_off[k] + first,
+    eval_level_tree<1, true, 1>(h, smem, q, k, J.bicc == 1, J.lvl_sets + J.lvl_off[k] + first,
                                    J.lvl_slot ? J.lvl_slot + J.lvl_off[k] + first : nullptr, m, mr,
                                    threadIdx.x / WAVE, BLOCK / WAVE);
 }
@@ -2830,7 +2841,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             J.lvl_off[k] = (uint32_t)acc;
             if (k >= 2 && k <= h.n_rels) acc += binom64(h.n_rels, k);
         }
-        J.bicc = algos[j] == DP_BICC ? 1 : 0;
+        J.bicc = dp_mode(*ins[j], algos[j]);
         J.tree_iters = tn.tree_iters;
     }
     hipStream_t s = c.stream;
@@ -2873,7 +2884,7 @@ static int run_dp_batch_impl(const std::vector<const DPInput *> &ins, const std:
             else {                        /* tasks of a bound on the level size */
                 const uint64_t lvl = (h.is_tree && (int)ins[j]->tree_counts.size() == h.n_rels + 1)
                                          ? ins[j]->tree_counts[k] : binom64(h.n_rels, k);
-                const uint64_t spt = task_sets_of(h.is_tree != 0, algos[j] == DP_BICC, k, tn.tree_iters);
+                const uint64_t spt = task_sets_of(h.is_tree != 0, dp_mode(*ins[j], algos[j]) != 0, k, tn.tree_iters);
                 ub_gen += (lvl + spt - 1) / spt;
             }
         }
@@ -3523,6 +3534,7 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     /* n_iters < n: IDP1's level cap; the result is the cheapest set of
      * n_iters relations */
     const int K = (n_iters >= 2 && n_iters < n) ? n_iters : n;
+    const int mode = dp_mode(in, algo);
     if (noalloc) {
         /* everything below runs in `scratch`: one dense DP, no IDP1 cap */
         if (n > GQ_NOALLOC_MAX_RELS || algo == DP_DPSIZE || K != n || in.memo != GQ_MEMO_DENSE) {
@@ -3802,10 +3814,10 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
             launch_eval_tree(h, lvl, lds_eval, s, b.blob, k, algo == DP_BICC ? 1 : 0, sets_k, nullptr,
                              dense_memo_of(b), b.counters,
                              tree_exp && k < K ? b.lvl_sets + off_h[k + 1] : nullptr, nullptr, tf, c.h_res);
-        } else if (algo == DP_BICC || T <= 4096) {
-            uint64_t items_est = (algo == DP_BICC || h.is_tree) ? 2ull * (k - 1) : T;
-            uint32_t B = algo == DP_BICC ? task_sets_of(false, true, k, g_tune.tree_iters)
-                                         : (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
+        } else if (mode != 0 || T <= 4096) {
+            uint64_t items_est = (mode != 0 || h.is_tree) ? 2ull * (k - 1) : T;
+            uint32_t B = mode != 0 ? task_sets_of(false, true, k, g_tune.tree_iters)
+                                   : (uint32_t)std::min<uint64_t>(BMAX, std::max<uint64_t>(1, (768 + items_est - 1) / items_est));
             uint32_t wblocks = (uint32_t)std::min<uint64_t>((Ck + B - 1) / B, 4096);
             BigList bl;
             bl.ents = nullptr;
@@ -3829,6 +3841,9 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                     hipLaunchKernelGGL(k_big_eval, dim3(1024), dim3(BLOCK), big_lds(h.words), s, d_sj, big_ents,
                                        b.counters + 50, big_prefix);
                 }
+            } else if (mode == 2) {          /* DPsub splits through the blocks (dpsub_by_blocks) */
+                hipLaunchKernelGGL((k_eval_wg<true, true>), dim3(wblocks), dim3(BLOCK), lds_bicc, s, h, b.blob, k,
+                                   B, 0, sets_k, b.vals, b.left, b.conn, b.counters, bl);
             } else {
                 hipLaunchKernelGGL(k_eval_wg<false>, dim3(wblocks), dim3(BLOCK), lds_dpsub, s, h, b.blob, k, B,
                                    conn_lds, sets_k, b.vals, b.left, b.conn, b.counters, bl);
@@ -4223,7 +4238,7 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
         J.lvl_off[k] = (uint32_t)acc;
         if (k >= 2 && k <= n) acc += in.tree_counts[k];
     }
-    J.bicc = algo == DP_BICC ? 1 : 0;
+    J.bicc = dp_mode(in, algo);
     J.tree_iters = tuning_get().tree_iters;
     J.cur_k = 2;
     e->incoming.push_back(j);
@@ -4270,7 +4285,7 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
         J.lvl_off[k] = (uint32_t)acc;
         if (k >= 2 && k <= n) acc += binom64(n, k);
     }
-    J.bicc = algo == DP_BICC ? 1 : 0;
+    J.bicc = dp_mode(in, algo);
     J.tree_iters = tuning_get().tree_iters;
     J.cur_k = 2;
     j->hash = false;
@@ -4360,7 +4375,7 @@ int engine_enqueue(StepEngine *e, int nn) {
         pttp[i + 1] = (uint32_t)ub_tree;
         const uint64_t src = j->hash ? j->counts[j->cur_k - 1] : 0;
         pexp[i + 1] = pexp[i] + (uint32_t)((src + BLOCK - 1) / BLOCK);
-        if (j->J.bicc && !h.is_tree && j->cur_k >= 15) {
+        if (j->J.bicc == 1 && !h.is_tree && j->cur_k >= 15) {
             big_cap += binom64(j->n, j->cur_k);
             big_lds_max = std::max(big_lds_max, big_lds(h.words));
         }

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <atomic>
 #include <chrono>
 #include <cfloat>
@@ -251,6 +252,58 @@ static void fill_tree_edges(const BlobHdr &h, std::vector<uint32_t> &b) {
     }
 }
 
+/* Plain DPsub on a sparse cyclic graph: a split (L, S - L) with both sides
+ * connected cuts edges of one biconnected block of S only (two cut edges in
+ * different blocks would close a cycle through both), so the MPDP block
+ * enumeration visits exactly DPsub's valid splits; ranked by DPsub's
+ * j = pext(L, S) they give DPsub's plan while the 2^k - 2 subsets of a set
+ * are never tested.  Taken when the blocks of the whole graph (a set's
+ * blocks never exceed them) enumerate at most 2^n / 64 splits -- m(m - 1)
+ * arcs of a simple cycle of m relations, 2^m - 2 subsets of any other block
+ * -- and no block but a simple cycle has more than 16 relations. */
+static bool dpsub_by_blocks(int n, const std::vector<Set<1>> &adjs) {
+    if (n < 12 || n > 31) return false;
+    uint32_t adj[32];
+    for (int i = 0; i < n; i++) adj[i] = (uint32_t)adjs[i].w[0];
+    int disc[32], low[32], timer = 0;
+    for (int i = 0; i < n; i++) disc[i] = -1;
+    std::vector<std::pair<int, int>> st;
+    uint64_t items = 0;
+    bool ok = true;
+    std::function<void(int, int)> dfs = [&](int u, int p) {
+        disc[u] = low[u] = timer++;
+        for (uint32_t r = adj[u]; r; r &= r - 1u) {
+            const int v = __builtin_ctz(r);
+            if (disc[v] < 0) {
+                st.emplace_back(u, v);
+                dfs(v, u);
+                low[u] = std::min(low[u], low[v]);
+                if (low[v] >= disc[u]) {                /* u separates the block holding edge (u, v) */
+                    uint32_t vs = 0;
+                    int edges = 0;
+                    while (true) {
+                        const std::pair<int, int> e = st.back();
+                        st.pop_back();
+                        vs |= (1u << e.first) | (1u << e.second);
+                        edges++;
+                        if (e.first == u && e.second == v) break;
+                    }
+                    const int m = __builtin_popcount(vs);
+                    const bool cycle = m >= 3 && edges == m;
+                    if (!cycle && m > 16) ok = false;
+                    items += cycle ? (uint64_t)m * (m - 1) : (1ull << m) - 2ull;
+                }
+            } else if (v != p && disc[v] < disc[u]) {
+                st.emplace_back(u, v);
+                low[u] = std::min(low[u], disc[v]);
+            }
+        }
+    };
+    dfs(0, -1);
+    for (int i = 0; i < n; i++) ok = ok && disc[i] >= 0;     /* connected */
+    return ok && items * 64 <= (1ull << n);
+}
+
 int build_dp_input(const HQuery<1> &q, DPInput &in) {
     const int n = q.n;
     if (n > 32) {
@@ -439,6 +492,7 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     while (b.size() & 3) b.push_back(0u);
     h.words = (int)b.size();
     if (h.tree_fast) fill_tree_edges(h, b);
+    in.dpsub_blocks = !h.is_tree && dpsub_by_blocks(n, q.adj);
     /* Level-1 list of cpu_dpsize: iteration order of a libstdc++
      * unordered_map holding the base relations, inserted in index order, keyed
      * by the reference's relation ids (relation i -> bit i+1), whose hash is

@@ -35,3 +35,35 @@ def small_graphs():
         out.append((f"clique{n}", qgen.clique(n, 41 + n)))
         out.append((f"snow{n}", qgen.snowflake(n, 51 + n)))
     return out
+
+
+def _figure8(n, seed):
+    """Two cycles sharing relation 0."""
+    a = n // 2
+    edges = [(i, i + 1) for i in range(a - 1)] + [(a - 1, 0)]
+    prev = 0
+    for i in range(a, n):
+        edges.append((prev, i))
+        prev = i
+    edges.append((n - 1, 0))
+    return qgen.graph(n, edges, seed, "figure8")
+
+
+def _triangle_ring(t, seed):
+    """t triangles in a ring, consecutive ones sharing a relation: one
+    biconnected block of 2t relations."""
+    n = 2 * t
+    edges = []
+    for i in range(t):
+        a, b, c = 2 * i, 2 * i + 1, (2 * i + 2) % n
+        edges += [(a, b), (b, c), (a, c)]
+    return qgen.graph(n, edges, seed, "triangle-ring")
+
+
+def dpsub_sparse_graphs():
+    """Sparse cyclic graphs of 16-22 relations (name, graph) whose DPsub splits
+    are found through their biconnected blocks, and the triangle ring, one
+    16-relation block, which keeps the subset scan."""
+    return [("cycle18", qgen.cycle(18, 0xb18)), ("cycle22", qgen.cycle(22, 0xb22)),
+            ("cycle12+tail6", qgen.cycle_with_tail(12, 6, 0xb21)), ("figure8-17", _figure8(17, 0xb24)),
+            ("triangle-ring16", _triangle_ring(8, 0xb23))]

@@ -3,7 +3,7 @@ import pytest
 
 import gpuqo_amd
 
-from helpers import assert_same_plan, cfg, gpu_bits, oracle_bits, small_graphs
+from helpers import dpsub_sparse_graphs, assert_same_plan, cfg, gpu_bits, oracle_bits, small_graphs
 from gpuqo_amd import qgen
 
 pytestmark = pytest.mark.gpu
@@ -185,6 +185,32 @@ def test_batch_with_big_blocks():
     for g, (nodes, st) in zip(qs, res):
         want, ost = oracle_bits(g, c)
         assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"batch {g.shape}{g.n}")
+        assert st.pairs == ost.pairs, f"{g.shape}{g.n}"
+
+
+@pytest.mark.parametrize("name,g", dpsub_sparse_graphs(), ids=[c[0] for c in dpsub_sparse_graphs()])
+def test_dpsub_sparse_cyclic_through_blocks(name, g):
+    """Plain DPsub on sparse cyclic graphs of 12 or more relations finds the
+    valid splits of a set through its biconnected blocks and ranks them by
+    DPsub's j, instead of testing all 2^k - 2 subsets: plan, pairs and sets
+    equal cpu_dpsub's.  The triangle ring is one 16-relation block and keeps
+    the subset scan."""
+    c = cfg("dpsub", 0)
+    want, ost = oracle_bits(g, c)
+    got, gst = gpu_bits(g, c)
+    assert_same_plan(got, want, f"dpsub {name}")
+    assert (gst.pairs, gst.sets) == (ost.pairs, ost.sets)
+
+
+def test_dpsub_sparse_cyclic_batched():
+    """The same through gq_plan_batch, beside DPsub jobs that keep the subset
+    scan and tree jobs."""
+    import gpuqo_amd
+    qs = [g for _, g in dpsub_sparse_graphs()] + [qgen.clique(10, 0xb30), qgen.star(13, 0xb31), qgen.cycle(9, 0xb32)]
+    c = cfg("dpsub", 0)
+    for g, (nodes, st) in zip(qs, gpuqo_amd.plan_batch(qs, c)):
+        want, ost = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(nodes), want, f"batch dpsub {g.shape}{g.n}")
         assert st.pairs == ost.pairs, f"{g.shape}{g.n}"
 
 

@@ -5,7 +5,7 @@ import time
 
 import pytest
 
-from helpers import assert_same_plan, cfg, gpu_bits, oracle_bits
+from helpers import assert_same_plan, cfg, dpsub_sparse_graphs, gpu_bits, oracle_bits
 from gpuqo_amd import qgen
 
 pytestmark = pytest.mark.gpu
@@ -50,6 +50,19 @@ def test_server_mix_equals_alone_and_oracle():
         assert_same_plan(plan, alone, f"request {i} ({g.shape}{g.n}) vs alone")
         want, _ = oracle_bits(g, c)
         assert_same_plan(plan, want, f"request {i} ({g.shape}{g.n}) vs oracle")
+
+
+def test_server_dpsub_sparse_cyclic_requests():
+    """Plain DPsub requests on sparse cyclic graphs (their splits found
+    through the blocks) on the step engine equal the oracle's plans."""
+    import gpuqo_amd
+    c = cfg("dpsub", 0)
+    reqs = [(g, c) for _, g in dpsub_sparse_graphs()]
+    with gpuqo_amd.Server(c, 4 << 30, max_inflight=4) as srv:
+        got = _run(srv, reqs, 0.0)
+    for i, (g, _) in enumerate(reqs):
+        want, _ = oracle_bits(g, c)
+        assert_same_plan(gpuqo_amd.plan_bits(got[i]), want, f"served dpsub {g.shape}{g.n}")
 
 
 def test_server_tight_budget_admits_one_at_a_time():
