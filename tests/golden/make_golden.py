@@ -10,11 +10,15 @@ and child indices; plus the pair and set counters.
 
     python tests/golden/make_golden.py            # plans_v1.json
     python tests/golden/make_golden.py --large    # plans_large_v1.json
+    python tests/golden/make_golden.py --trees    # plans_trees_v1.json
 
 The large set holds the full-size workloads that take the oracle minutes:
 cpu_dpsub and cpu_dpsize on the 18-relation clique (about 3.9e8 csg-cmp
 pairs each, 153 classes) and cpu_dpsub_bicc on every query of the
 256-query mixed batch.  Its records are written without indentation.
+The tree set holds the snowflakes of 24-30 relations whose memos exceed the
+dense limit in the tests (rank and open-addressing memos), under MPDP:
+about 1.5 minutes of oracle time.
 """
 import hashlib
 import json
@@ -76,6 +80,14 @@ LARGE_CASES = [
 ]
 
 
+TREE_CASES = [
+    ("snowflake24-mpdp", "snowflake", [24, 0x24], {}, MPDP),
+    ("snowflake26-mpdp", "snowflake", [26, 0x26], {}, MPDP),
+    ("snowflake28d4f5-mpdp", "snowflake", [28, 0x28], {"depth": 4, "fanout": 5}, MPDP),
+    ("snowflake30-mpdp", "snowflake", [30, 0x30], {}, MPDP),
+]
+
+
 def make_batch_member(i):
     """Query i of the 256-query mixed batch under MPDP (the generator call is
     recorded as queries_batch256 plus the index)."""
@@ -108,9 +120,26 @@ def main_large(procs):
     print(f"wrote {len(recs)} records to {path}")
 
 
+def main_trees(procs):
+    from multiprocessing import Pool
+    with Pool(procs) as pool:
+        recs = pool.map(_make_large, TREE_CASES, chunksize=1)
+    out = {"format": "postgres-gpuqo_amd golden plans v1 (tree DPs above the dense memo limit)",
+           "float_encoding": "IEEE-754 binary32 bit patterns",
+           "records": recs}
+    path = os.path.join(HERE, "plans_trees_v1.json")
+    with open(path, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+        f.write("\n")
+    print(f"wrote {len(recs)} records to {path}")
+
+
 def main():
     if "--large" in sys.argv:
         main_large(int(os.environ.get("GOLDEN_PROCS", "6")))
+        return
+    if "--trees" in sys.argv:
+        main_trees(int(os.environ.get("GOLDEN_PROCS", "4")))
         return
     out = {"format": "postgres-gpuqo_amd golden plans v1",
            "float_encoding": "IEEE-754 binary32 bit patterns",

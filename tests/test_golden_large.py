@@ -26,6 +26,15 @@ with open(os.path.join(HERE, "golden", "plans_large_v1.json")) as f:
     LARGE = json.load(f)["records"]
 CLIQUE = [r for r in LARGE if r["generator"] == "query_clique18"]
 BATCH = sorted((r for r in LARGE if r["generator"] == "queries_batch256"), key=lambda r: r["index"])
+with open(os.path.join(HERE, "golden", "plans_trees_v1.json")) as f:
+    TREES = {r["label"]: r for r in json.load(f)["records"]}
+
+
+def tree_fixture(label):
+    """(graph, expected plan bits, record) of a tree fixture."""
+    r = TREES[label]
+    g = getattr(qgen, r["generator"])(*r["args"], **r["kwargs"])
+    return g, _want(r), r
 
 
 def _digest(g):
@@ -63,6 +72,45 @@ def test_large_fixture_inputs_unchanged():
     assert _digest(qgen.query_clique18()) == CLIQUE[0]["input_sha256"] == CLIQUE[1]["input_sha256"]
     for q, r in zip(batch_queries(), BATCH):
         assert q.n == r["n"] and _digest(q) == r["input_sha256"], r["label"]
+
+
+def test_tree_fixture_inventory_and_inputs():
+    assert sorted(TREES) == ["snowflake24-mpdp", "snowflake26-mpdp", "snowflake28d4f5-mpdp", "snowflake30-mpdp"]
+    for label, r in TREES.items():
+        g, want, _ = tree_fixture(label)
+        assert g.n == r["n"] and _digest(g) == r["input_sha256"], label
+        assert len(want) == 2 * g.n - 1
+        assert (r["pairs"], r["sets"]) == tree_pairs_and_sets(g), label
+
+
+def tree_pairs_and_sets(g):
+    """Ordered csg-cmp pairs and connected sets of two or more relations of
+    a tree query, from the subtree polynomials P_v(x) = x * prod over the
+    children c of (1 + P_c(x)): every connected set of k relations has
+    2(k - 1) ordered splits."""
+    adj = g.adjacency()
+    order, parent = [0], {0: -1}
+    for v in order:
+        for w in range(g.n):
+            if (adj[v] >> w) & 1 and w not in parent:
+                parent[w] = v
+                order.append(w)
+    poly, total = {}, {}
+    for v in reversed(order):
+        p = {1: 1}
+        for c in (w for w in order if parent[w] == v):
+            q = dict(poly[c])
+            q[0] = q.get(0, 0) + 1
+            r = {}
+            for a, x in p.items():
+                for b, y in q.items():
+                    r[a + b] = r.get(a + b, 0) + x * y
+            p = r
+        poly[v] = p
+        for k, m in p.items():
+            total[k] = total.get(k, 0) + m
+    return (sum(2 * (k - 1) * m for k, m in total.items() if k >= 2),
+            sum(m for k, m in total.items() if k >= 2))
 
 
 def test_oracle_reproduces_batch_sample():

@@ -71,12 +71,17 @@ def test_hash_request_on_a_cycle_keeps_the_dense_memo():
 
 
 def test_hash_and_rank_equal_dense_at_30_relations():
-    g = qgen.snowflake(30, 0x30)
+    """All three memos against the oracle's plan of the 30-relation
+    snowflake (tests/golden/plans_trees_v1.json)."""
+    from test_golden_large import tree_fixture
+    g, want, rec = tree_fixture("snowflake30-mpdp")
     dense, dst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_DENSE))
     hashed, hst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_HASH))
     ranked, rst = gpu_bits(g, cfg("dpsub", 1, memo_kind=GQ_MEMO_RANK))
-    assert_same_plan(hashed, dense, "snowflake-30 hash")
-    assert_same_plan(ranked, dense, "snowflake-30 rank")
+    assert_same_plan(dense, want, "snowflake-30 dense vs oracle")
+    assert_same_plan(hashed, want, "snowflake-30 hash vs oracle")
+    assert_same_plan(ranked, want, "snowflake-30 rank vs oracle")
+    assert (dst.pairs, dst.sets) == (rec["pairs"], rec["sets"])
     assert hst.pairs == dst.pairs and hst.sets == dst.sets
     assert rst.pairs == dst.pairs and rst.sets == dst.sets
     assert hst.memo_bytes * 50 < dst.memo_bytes          # 2^30 dense slots vs ~4M connected sets
@@ -101,10 +106,17 @@ def test_server_with_tree_memo_requests(kind):
             tag, st, nodes, stats = r
             assert st == 0, f"request {tag}: status {st}"
             got[tag] = (nodes, stats)
+    from test_golden_large import tree_fixture
+    fixtures = {0: "snowflake26-mpdp", 2: "snowflake24-mpdp", 4: "snowflake28d4f5-mpdp"}
     for i, g in enumerate(reqs):
         alone, ast = gpu_bits(g, direct)
         assert_same_plan(gpuqo_amd.plan_bits(got[i][0]), alone, f"request {i} ({g.shape}{g.n})")
         assert got[i][1].memo_kind == ast.memo_kind
+        if i in fixtures:                         # the oracle's plans of the large ones
+            fg, want, rec = tree_fixture(fixtures[i])
+            assert fg.n == g.n
+            assert_same_plan(alone, want, f"request {i} ({g.shape}{g.n}) vs oracle")
+            assert got[i][1].pairs == rec["pairs"]
 
 
 @pytest.mark.parametrize("idp,k,n", [("idp2", 8, 20), ("union", 25, 40), ("idp1", 6, 16)])
