@@ -201,7 +201,11 @@ typedef struct gq_plan_stats {
     uint64_t sets;            /* S: connected sets evaluated (size >= 2)   */
     uint64_t pairs;           /* P: ordered valid csg-cmp pairs costed     */
     uint64_t memo_bytes;      /* device bytes held by the memo             */
-    double gpu_ms;            /* device time of the DP (HIP events)        */
+    double gpu_ms;            /* device time of the DP: HIP events around
+                                 its launches, or, for a dense tree DP planned
+                                 whole without profile_kernels, the device
+                                 clock from its first launch's start to the
+                                 end of its plan walk                       */
     double wall_ms;           /* host wall time of the whole call          */
     int32_t n_dp_calls;       /* DPs run (1 for a plain query)             */
     int32_t memo_kind;        /* GQ_MEMO_DENSE, _HASH or _RANK used        */

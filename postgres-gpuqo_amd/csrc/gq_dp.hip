@@ -1163,8 +1163,10 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
  * base relations from the blob, and workgroup 0 stores the base relations
  * and counters[2] (the counters must be zero on entry).  TREE_WALK: the last
  * level (one set): workgroup 0 then walks the plan from it and writes the
- * 64 counters (word 48: the walk's status) and the plan nodes straight to
- * res_host (coherent pinned host memory), and zeroes the device counters. */
+ * 64 counters and the plan nodes straight to res_host (coherent pinned host
+ * memory) -- words 56..57 and 58..59 being the device clock at the DP's
+ * start and end -- then, after a system-scope release, word 48 (the walk's
+ * status), and zeroes the device counters. */
 #define TREE_START 1u
 #define TREE_WALK 2u
 
@@ -1191,6 +1193,11 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
         bid /= xs;
     }
     constexpr bool start = (flags & TREE_START) != 0;
+    if (start && bid == 0 && threadIdx.x == 0) {     /* the DP's start on the device clock */
+        const uint64_t t0 = wall_clock64();
+        counters[56] = (uint32_t)t0;
+        counters[57] = (uint32_t)(t0 >> 32);
+    }
     if (bid >= ev_blocks) {                 /* adjacency (n words, inside the blob's LDS room) */
         const uint32_t count = start ? (uint32_t)(h.n_rels - 1) : counters[k];
         if ((int)threadIdx.x < h.n_rels) smem[threadIdx.x] = blob[h.o_adj + threadIdx.x];
@@ -1230,8 +1237,15 @@ __global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *
             const uint32_t root = n == 32 ? 0xffffffffu : ((1u << n) - 1u);
             const uint32_t r = walk_tree_ws(mr, root, (DPNode *)(res_host + 64), 2 * n - 1, smem + o_ws, root_rec);
             const uint32_t cv = (start && lane == 2) ? (uint32_t)(n - 1) : counters[lane];
-            res_host[lane] = lane == 48 ? r : cv;
+            const uint64_t t1 = wall_clock64();       /* the DP's end; its start is counters[56..57] */
+            const uint32_t v = lane == 58 ? (uint32_t)t1 : lane == 59 ? (uint32_t)(t1 >> 32) : cv;
+            if (lane != 48) res_host[lane] = v;
             counters[lane] = 0u;
+            /* the status word last: a host polling it finds the nodes and
+             * counters already in place */
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 48) __hip_atomic_store(res_host + 48, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
     /* no pair counter: every split of a tree set is valid, so the host
@@ -2520,6 +2534,7 @@ struct DeviceCtx {
      * leaves them so): the next fused tree DP needs no clearing launch */
     bool counters_clean = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int wall_khz = 0;                   /* rate of the device clock wall_clock64 reads */
     DevBufs b{};
     size_t cap_memo = 0, cap_left = 0, cap_lvl = 0, cap_best = 0, cap_first = 0, cap_blob = 0, cap_conn = 0,
            cap_nodes = 0, cap_counters = 0;
@@ -2593,6 +2608,7 @@ static int ctx_init(DeviceCtx &c, int device) {
     for (int k = 0; k < 34; k++) HIPCHK(hipEventCreateWithFlags(&c.gate_ev[k], hipEventDisableTiming));
     HIPCHK(hipEventCreate(&c.ev0));
     HIPCHK(hipEventCreate(&c.ev1));
+    HIPCHK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, device));
     {
         uint32_t tab[33 * 33];
         for (int n = 0; n <= 32; n++)
@@ -2614,6 +2630,37 @@ static int ctx_init(DeviceCtx &c, int device) {
     /* pinned staging at its working size, so the allocation-free path never
      * grows it */
     return ensure_host(&c.h_stage, c.cap_h_stage, na::STAGE_WORDS);
+}
+
+/* The status word of a fused tree DP's walk (h_res[48]) before the walk has
+ * written it; the walk writes a node count (at most 63) or ~0u. */
+#define RES_PENDING 0xfffffffdu
+
+/* Wait for the walk's status word in pinned memory instead of for the
+ * stream: the last launch writes the plan, the counters and then the status
+ * word, a few microseconds before the stream reports the launch complete.
+ * Later work on the stream is ordered after that launch anyway.  Every 4096
+ * polls the stream is queried, so a failed launch (or one that ended with no
+ * status) ends the wait with an error. */
+static int wait_status(DeviceCtx &c, hipStream_t s) {
+    for (uint64_t i = 0;; i++) {
+        if (__atomic_load_n(&c.h_res[48], __ATOMIC_ACQUIRE) != RES_PENDING) break;
+        if ((i & 4095u) == 4095u) {
+            const hipError_t q = hipStreamQuery(s);
+            if (q == hipErrorNotReady) continue;
+            (void)hipGetLastError();
+            if (q != hipSuccess) {
+                set_error("HIP error %s while waiting for the plan", hipGetErrorString(q));
+                return GQ_ERR_DEVICE;
+            }
+            if (__atomic_load_n(&c.h_res[48], __ATOMIC_ACQUIRE) != RES_PENDING) break;
+            set_error("the plan walk left no status");
+            return GQ_ERR_DEVICE;
+        }
+        __builtin_ia32_pause();
+    }
+    (void)hipGetLastError();                  /* the stream's not-ready answers are no error */
+    return GQ_OK;
 }
 
 double host_now_ms() {
@@ -3638,9 +3685,6 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         sj.conn_words = conn_words;
         sj.bicc = 1;
     }
-    st.t_first_launch = host_now_ms();
-    if ((rc = stage_up(c, s, b.blob, stage_words))) return rc;
-    HIPCHK(hipEventRecord(c.ev0, s));
     /* Trees with exact level sizes and at most expand_max_n relations skip
      * the 2^n filter: level 2 comes from k_tree_start and each level's
      * launch builds the next level's list by expansion (no connectivity
@@ -3652,8 +3696,15 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
                           n <= (int)g_tune.expand_max_n;
     /* fused: level 2 starts the DP (TREE_START) and the last level walks the
      * plan into pinned host memory (TREE_WALK), so no start kernel, walk
-     * kernel or read-back copy runs */
+     * kernel or read-back copy runs.  Unprofiled, the host then waits for the
+     * walk's status word (wait_status) and takes the device time from the
+     * device clock the first and last launches read, not from events. */
     const bool fused = tree_exp && K == n;
+    const bool poll = fused && !profile;
+    if (poll) c.h_res[48] = RES_PENDING;
+    st.t_first_launch = host_now_ms();
+    if ((rc = stage_up(c, s, b.blob, stage_words))) return rc;
+    if (!poll) HIPCHK(hipEventRecord(c.ev0, s));
     if (fused) {
         if (!counters_were_clean) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, s, nullptr, 0u, b.counters);
     } else if (tree_exp) {
@@ -3872,12 +3923,16 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
     if (!fused) hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, dense_memo_of(b), root, b.nodes, nodes_n,
                                    b.counters + 48);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c.ev1, s));
+    if (!poll) HIPCHK(hipEventRecord(c.ev1, s));
     tree.resize(nodes_n);
     const uint32_t *cnt = c.h_res;
     st.t_last_launch = host_now_ms();
     if (!fused && (rc = results_down(c, s, nodes_n, b.counters))) return rc;
-    HIPCHK(hipStreamSynchronize(s));
+    if (poll) {
+        if ((rc = wait_status(c, s))) return rc;
+    } else {
+        HIPCHK(hipStreamSynchronize(s));
+    }
     st.t_synced = host_now_ms();
     if (fused && !noalloc) c.counters_clean = true;   /* the walk zeroed them */
     memcpy(tree.data(), c.h_res + 64, sizeof(DPNode) * nodes_n);
@@ -3890,7 +3945,13 @@ int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool prof
         return GQ_ERR_DEVICE;
     }
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    if (poll) {
+        const uint64_t t0 = (uint64_t)cnt[56] | ((uint64_t)cnt[57] << 32);
+        const uint64_t t1 = (uint64_t)cnt[58] | ((uint64_t)cnt[59] << 32);
+        ms = c.wall_khz > 0 && t1 >= t0 ? (float)((double)(t1 - t0) / (double)c.wall_khz) : 0.0f;
+    } else {
+        HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    }
     st.gpu_ms = ms;
     if (profile) {
         float a = 0.0f;
