@@ -11,6 +11,7 @@ and child indices; plus the pair and set counters.
     python tests/golden/make_golden.py            # plans_v1.json
     python tests/golden/make_golden.py --large    # plans_large_v1.json
     python tests/golden/make_golden.py --trees    # plans_trees_v1.json
+    python tests/golden/make_golden.py --served   # plans_served_v1.json
 
 The large set holds the full-size workloads that take the oracle minutes:
 cpu_dpsub and cpu_dpsize on the 18-relation clique (about 3.9e8 csg-cmp
@@ -18,7 +19,10 @@ pairs each, 153 classes) and cpu_dpsub_bicc on every query of the
 256-query mixed batch.  Its records are written without indentation.
 The tree set holds the snowflakes of 24-30 relations whose memos exceed the
 dense limit in the tests (rank and open-addressing memos), under MPDP:
-about 1.5 minutes of oracle time.
+about 1.5 minutes of oracle time.  The served set holds every request of
+the 256-request served mix (qgen.serving_mix with the serve256 seed): the
+direct MPDP snowflakes of 10-30 relations and the UnionDP (k = 25)
+snowflakes of 60-200 relations, about half an hour of oracle time.
 """
 import hashlib
 import json
@@ -88,6 +92,21 @@ TREE_CASES = [
 ]
 
 
+SERVED_DIRECT = MPDP
+SERVED_UNION = {"algorithm": "dpsub", "dpsub_bicc": 1, "idp_type": "union", "idp_n_iters": 25}
+
+
+def make_served_member(i):
+    """Request i of the served mix, with its own configuration."""
+    g, large = qgen.serving_mix(256, qgen.workload_seed("serve256"))[i]
+    over = SERVED_UNION if large else SERVED_DIRECT
+    st, nodes, stats = oracle.plan(g, abi.default_config(**over))
+    assert st == 0, (i, st)
+    plan = [[list(r), a, b, c, w, l, rr] for (r, a, b, c, w, l, rr) in gpuqo_amd.plan_bits(nodes)]
+    return {"label": f"serve256-{i:03d}-{g.shape}{g.n}", "generator": "serving_mix", "index": i, "n": g.n,
+            "input_sha256": digest(g), "config": over, "plan": plan, "pairs": stats.pairs, "sets": stats.sets}
+
+
 def make_batch_member(i):
     """Query i of the 256-query mixed batch under MPDP (the generator call is
     recorded as queries_batch256 plus the index)."""
@@ -134,7 +153,24 @@ def main_trees(procs):
     print(f"wrote {len(recs)} records to {path}")
 
 
+def main_served(procs):
+    from multiprocessing import Pool
+    with Pool(procs) as pool:
+        recs = pool.map(make_served_member, range(256), chunksize=1)
+    out = {"format": "postgres-gpuqo_amd golden plans v1 (the 256-request served mix)",
+           "float_encoding": "IEEE-754 binary32 bit patterns",
+           "records": recs}
+    path = os.path.join(HERE, "plans_served_v1.json")
+    with open(path, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+        f.write("\n")
+    print(f"wrote {len(recs)} records to {path}")
+
+
 def main():
+    if "--served" in sys.argv:
+        main_served(int(os.environ.get("GOLDEN_PROCS", "6")))
+        return
     if "--large" in sys.argv:
         main_large(int(os.environ.get("GOLDEN_PROCS", "6")))
         return

@@ -28,6 +28,17 @@ CLIQUE = [r for r in LARGE if r["generator"] == "query_clique18"]
 BATCH = sorted((r for r in LARGE if r["generator"] == "queries_batch256"), key=lambda r: r["index"])
 with open(os.path.join(HERE, "golden", "plans_trees_v1.json")) as f:
     TREES = {r["label"]: r for r in json.load(f)["records"]}
+with open(os.path.join(HERE, "golden", "plans_served_v1.json")) as f:
+    SERVED = sorted(json.load(f)["records"], key=lambda r: r["index"])
+
+_MIX = None
+
+
+def served_mix():
+    global _MIX
+    if _MIX is None:
+        _MIX = qgen.serving_mix(256, qgen.workload_seed("serve256"))
+    return _MIX
 
 
 def tree_fixture(label):
@@ -113,6 +124,18 @@ def tree_pairs_and_sets(g):
             sum(m for k, m in total.items() if k >= 2))
 
 
+def test_served_fixture_inventory_and_inputs():
+    """The served mix (config 5): 175 direct MPDP requests and 81 UnionDP
+    (k = 25) requests, every one recorded with the configuration it is
+    served with."""
+    assert [r["index"] for r in SERVED] == list(range(256))
+    for (g, large), r in zip(served_mix(), SERVED):
+        assert g.n == r["n"] and _digest(g) == r["input_sha256"], r["label"]
+        assert (r["config"].get("idp_type") == "union") == large, r["label"]
+        assert len(r["plan"]) == 2 * g.n - 1
+    assert sum(1 for r in SERVED if r["config"].get("idp_type") == "union") == 81
+
+
 def test_oracle_reproduces_batch_sample():
     """The 24 smallest batch queries (the whole batch takes the oracle
     minutes)."""
@@ -155,3 +178,34 @@ def test_gpu_batch256_one_at_a_time_matches_oracle_fixtures():
         nodes, stats = gpuqo_amd.plan(q, c)
         assert_same_plan(gpuqo_amd.plan_bits(nodes), _want(r), r["label"])
         assert stats.pairs == r["pairs"], r["label"]
+
+
+@pytest.mark.gpu
+def test_gpu_served_mix_alone_matches_oracle_fixtures():
+    """Every request of the served mix planned alone through gq_plan (the
+    UnionDP ones of 60-200 relations included) equals the oracle's plan."""
+    for (g, _), r in zip(served_mix(), SERVED):
+        nodes, stats = gpuqo_amd.plan(g, abi.default_config(**r["config"]))
+        assert_same_plan(gpuqo_amd.plan_bits(nodes), _want(r), r["label"])
+        assert stats.pairs == r["pairs"], r["label"]
+
+
+@pytest.mark.gpu
+def test_gpu_served_mix_through_the_server_matches_oracle_fixtures():
+    """The whole served mix submitted at once to the continuous-batching
+    server (96 GB arena, 128 request threads, as bench.py serves it): every
+    plan equals the oracle's."""
+    direct = abi.default_config(algorithm="dpsub", dpsub_bicc=1)
+    with gpuqo_amd.Server(direct, 96 << 30, max_inflight=128) as srv:
+        for i, ((g, _), r) in enumerate(zip(served_mix(), SERVED)):
+            srv.submit(g, i, abi.default_config(**r["config"]))
+        got = {}
+        while len(got) < len(SERVED):
+            res = srv.poll(wait_ms=120000)
+            assert res is not None, f"server returned nothing after {len(got)} requests"
+            tag, st, nodes, stats = res
+            assert st == 0, f"request {tag}: status {st}"
+            got[tag] = (nodes, stats)
+    for i, r in enumerate(SERVED):
+        assert_same_plan(gpuqo_amd.plan_bits(got[i][0]), _want(r), r["label"] + " served")
+        assert got[i][1].pairs == r["pairs"], r["label"]
