@@ -27,16 +27,49 @@ namespace gq {
 template <int WI, int WO>
 struct Remapper {
     std::vector<RemapEl<WI>> tr;
-    explicit Remapper(const std::vector<RemapEl<WI>> &t) : tr(t) {}
+    /* every entry maps one relation and none carries a subtree (a pure
+     * relabelling, such as the BFS one): the maps below then take a table
+     * lookup per member instead of a pass over the entries, with the same
+     * results */
+    bool perm = true;
+    int16_t to_of[64 * WI], from_of[64 * WO];
+    explicit Remapper(const std::vector<RemapEl<WI>> &t) : tr(t) {
+        for (auto &x : to_of) x = -1;
+        for (auto &x : from_of) x = -1;
+        for (const auto &e : tr) {
+            if (popc(e.from) != 1 || e.qt) {
+                perm = false;
+                break;
+            }
+            to_of[ctz(e.from)] = (int16_t)e.to;
+            from_of[e.to] = (int16_t)ctz(e.from);
+        }
+    }
 
     Set<WO> fwd(const Set<WI> &id) const {
         Set<WO> o;
+        if (perm) {
+            for (Set<WI> t = id; !empty(t);) {
+                const int f = ctz(t);
+                t.unset(f);
+                if (to_of[f] >= 0) o.set(to_of[f]);
+            }
+            return o;
+        }
         for (const auto &e : tr)
             if (intersects(e.from, id)) o.set(e.to);
         return o;
     }
     Set<WO> fwd_plain(const Set<WI> &id, const HQuery<WI> &old) const {   /* remapRelidNoComposite */
         Set<WO> o;
+        if (perm) {
+            for (Set<WI> t = id; !empty(t);) {
+                const int f = ctz(t);
+                t.unset(f);
+                if (to_of[f] >= 0 && !old.composite[f]) o.set(to_of[f]);
+            }
+            return o;
+        }
         for (const auto &e : tr) {
             if (popc(e.from) > 1 || old.composite[ctz(e.from)]) continue;
             if (intersects(e.from, id)) o.set(e.to);
@@ -45,11 +78,23 @@ struct Remapper {
     }
     Set<WI> inv(const Set<WO> &id) const {
         Set<WI> o;
+        if (perm) {
+            for (Set<WO> t = id; !empty(t);) {
+                const int v = ctz(t);
+                t.unset(v);
+                if (from_of[v] >= 0) o.set(from_of[v]);
+            }
+            return o;
+        }
         for (const auto &e : tr)
             if (bit(id, e.to)) o |= e.from;
         return o;
     }
     bool dropped(const Set<WI> &ec) const {
+        if (perm) {                                 /* ec within one mapped relation */
+            const int c = popc(ec);
+            return c == 0 ? !tr.empty() : (c == 1 && to_of[ctz(ec)] >= 0);
+        }
         for (const auto &e : tr)
             if (subset_of(ec, e.from)) return true;
         return false;
@@ -62,8 +107,8 @@ struct Remapper {
 
     /* remapPlannerInfo (gpuqo_remapper.cu:200-276) */
     void remap(const HQuery<WI> &old, HQuery<WO> &h) const {
-        HostView<WI> ov;
-        ov.build(old);
+        HostView<WI> ov;                            /* built when a merged class needs it */
+        bool ov_built = false;
         const int n = (int)tr.size();
         h = HQuery<WO>();
         h.n = n;
@@ -124,10 +169,15 @@ struct Remapper {
                         Set<WI> rfr = inv(rto);
                         int rf = rank_below(ecf, rfr);
                         float sel;
-                        if (popc(lfr) == popc(lto) && popc(rfr) == popc(rto))
+                        if (popc(lfr) == popc(lto) && popc(rfr) == popc(rto)) {
                             sel = old.sels[old_soff + eq_class_index(lf, rf, s_from)];
-                        else
+                        } else {
+                            if (!ov_built) {
+                                ov.build(old);
+                                ov_built = true;
+                            }
                             sel = ec_selectivity(ov.v, (int)i, lfr, rfr);
+                        }
                         h.sels[sb + eq_class_index(lt, rt, s_to)] = sel;
                     }
                     if (popc(lfr) == 1 && !old.composite[ctz(lfr)])
@@ -145,7 +195,7 @@ struct Remapper {
     /* remapQueryTree (:278-301) */
     QT<WI> *back(RunCtx &ctx, const QT<WO> *t) const {
         if (!t) return nullptr;
-        if (popc(t->id) == 1) {
+        if (popc(t->id) == 1 && !perm) {
             int idx = ctz(t->id);
             for (const auto &e : tr)
                 if (e.qt && e.to == idx) return e.qt;
