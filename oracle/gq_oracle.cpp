@@ -875,10 +875,83 @@ static void dpsub_enumerate_sets(const Info<W> &info, F visit) {
     }
 }
 
+/* The connected sets of sizes 2 .. n_iters, size by size, each size in
+ * increasing order: every neighbour v outside a set S of size k gives
+ * S + v of size k + 1 (duplicates dropped).  A checker-side stand-in for
+ * dpsub_enumerate_sets on DPs whose pass over all 2^n integers would not
+ * finish (the wide DPs above 32 relations).  It visits the same connected
+ * sets, each once, and every set after all of its proper subsets (sizes
+ * ascend).  A set's entry depends only on its splits, and their entries are
+ * final when it is visited, so the memo ends identical
+ * (oracle_plan_connected; pinned against the 2^n pass on small graphs). */
+template <int W, typename F>
+static void connected_enumerate_sets(const Info<W> &info, F visit) {
+    std::vector<Bits<W>> cur, nxt;
+    for (int i = 0; i < info.n_rels; i++) cur.push_back(Bits<W>::nth(i + 1));
+    for (int k = 1; k < info.n_iters; k++) {
+        nxt.clear();
+        for (const Bits<W> &S : cur) {
+            Bits<W> nb;
+            for (Bits<W> t = S; !t.empty();) {
+                const int u = t.lowestPos();
+                t.unset(u);
+                nb |= info.edge_table[u - 1];
+            }
+            nb -= S;
+            while (!nb.empty()) {
+                const int v = nb.lowestPos();
+                nb.unset(v);
+                Bits<W> T = S;
+                T.set(v);
+                nxt.push_back(T);
+            }
+        }
+        std::sort(nxt.begin(), nxt.end());
+        nxt.erase(std::unique(nxt.begin(), nxt.end()), nxt.end());
+        cur.swap(nxt);
+        for (const Bits<W> &S : cur) visit(S);
+    }
+}
+
+template <int W>
+static QueryTree<W> *cpu_dpsub_bicc(const Info<W> &info, Counters &c, bool connected_only = false);
+template <int W, typename F>
+static void bicc_pairs(const Bits<W> &S, const std::vector<Bits<W>> &et, Counters &c, F emit);
+
+/* cpu_dpsub over connected_enumerate_sets: the valid splits of a set are
+ * those with both sides connected, which are exactly the block splits
+ * (bicc_pairs: two cut edges in different blocks would close a cycle
+ * through both), taken in cpu_dpsub's order, left side ascending.  c.checks
+ * still counts the 2^|S| - 2 subsets cpu_dpsub tries. */
+template <int W>
+static QueryTree<W> *cpu_dpsub_connected(const Info<W> &info, Counters &c) {
+    Memo<W> memo;
+    init_memo(info, memo);
+    std::vector<std::pair<Bits<W>, Bits<W>>> ps;
+    connected_enumerate_sets(info, [&](const Bits<W> &S) {
+        c.sets++;
+        c.checks += (1ULL << S.size()) - 2ULL;
+        ps.clear();
+        Counters unused;
+        bicc_pairs(S, info.edge_table, unused, [&](const Bits<W> &l, const Bits<W> &r) { ps.emplace_back(l, r); });
+        std::sort(ps.begin(), ps.end(),
+                  [](const std::pair<Bits<W>, Bits<W>> &a, const std::pair<Bits<W>, Bits<W>> &b) {
+                      return a.first < b.first;
+                  });
+        for (const auto &p : ps) {
+            auto lf = memo.find(p.first), rf = memo.find(p.second);
+            JR<W> *o;
+            do_join(*lf->second, *rf->second, info, memo, &o, c);
+        }
+    });
+    return finish_memo(info, memo);
+}
+
 /* cpu_dpsub: DPsubCPUAlgorithm::enumerate_subsets, gpuqo_cpu_dpsub.cu:32-79 */
 template <int W>
-static QueryTree<W> *cpu_dpsub(const Info<W> &info, Counters &c) {
+static QueryTree<W> *cpu_dpsub(const Info<W> &info, Counters &c, bool connected_only = false) {
     static_assert(W == 1, "DPsub oracle works on <64 relations");
+    if (connected_only) return cpu_dpsub_connected(info, c);
     Memo<W> memo;
     init_memo(info, memo);
     dpsub_enumerate_sets(info, [&](const Bits<W> &S) {
@@ -972,18 +1045,20 @@ static void bicc_pairs(const Bits<W> &S, const std::vector<Bits<W>> &et, Counter
 }
 
 template <int W>
-static QueryTree<W> *cpu_dpsub_bicc(const Info<W> &info, Counters &c) {
+static QueryTree<W> *cpu_dpsub_bicc(const Info<W> &info, Counters &c, bool connected_only) {
     static_assert(W == 1, "DPsub oracle works on <64 relations");
     Memo<W> memo;
     init_memo(info, memo);
-    dpsub_enumerate_sets(info, [&](const Bits<W> &S) {
+    auto per_set = [&](const Bits<W> &S) {
         if (S.size() > 1) c.sets++;
         bicc_pairs(S, info.edge_table, c, [&](const Bits<W> &l, const Bits<W> &r) {
             auto lf = memo.find(l), rf = memo.find(r);
             JR<W> *o;
             do_join(*lf->second, *rf->second, info, memo, &o, c);
         });
-    });
+    };
+    if (connected_only) connected_enumerate_sets(info, per_set);
+    else dpsub_enumerate_sets(info, per_set);
     return finish_memo(info, memo);
 }
 
@@ -1514,6 +1589,7 @@ struct RunCtx {
     int n_threads, chunk, pairs_per_depbuf;
     Counters c;
     int dp_calls = 0;
+    bool connected_only = false;     /* DPsub / MPDP over connected_enumerate_sets */
 };
 
 template <int W>
@@ -1530,11 +1606,12 @@ QueryTree<1> *run_switch<1>(RunCtx &ctx, const Info<1> &info) {     /* gpuqo_mai
     ctx.dp_calls++;
     switch (ctx.algorithm) {
     case GQ_ALGO_DPSUB:
-        return ctx.bicc ? cpu_dpsub_bicc(info, ctx.c) : cpu_dpsub(info, ctx.c);
+        return ctx.bicc ? cpu_dpsub_bicc(info, ctx.c, ctx.connected_only)
+                        : cpu_dpsub(info, ctx.c, ctx.connected_only);
     case GQ_ALGO_CPU_DPSUB:
-        return cpu_dpsub(info, ctx.c);
+        return cpu_dpsub(info, ctx.c, ctx.connected_only);
     case GQ_ALGO_CPU_DPSUB_BICC:
-        return cpu_dpsub_bicc(info, ctx.c);
+        return cpu_dpsub_bicc(info, ctx.c, ctx.connected_only);
     case GQ_ALGO_DPSIZE:
     case GQ_ALGO_CPU_DPSIZE:
         return cpu_dpsize(info, ctx.c);
@@ -1949,8 +2026,9 @@ static int emit_tree(const QueryTree<W> *qt, gq_plan_out *out) {   /* pre-order 
 }
 
 template <int W>
-static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out, bool connected_only = false) {
     RunCtx ctx;
+    ctx.connected_only = connected_only;
     ctx.algorithm = cfg->gpuqo_algorithm;
     ctx.bicc = cfg->gpuqo_dpsub_bicc != 0;
     ctx.idp_type = cfg->gpuqo_idp_type;
@@ -1988,6 +2066,14 @@ int oracle_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     if (!q || !cfg || !out || q->n_rels < 1 || q->n_rels >= GQ_MAX_RELS) return GQ_ERR_INVALID;
     if (q->n_rels < 64) return oracle::plan_t<1>(q, cfg, out);
     return oracle::plan_t<4>(q, cfg, out);
+}
+
+/* oracle_plan with DPsub and MPDP visiting the connected sets size by
+ * size (connected_enumerate_sets) instead of all 2^n integers: the same
+ * plans, for DPs above 32 relations whose 2^n pass would not finish. */
+int oracle_plan_connected(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
+    if (!q || !cfg || !out || q->n_rels < 1 || q->n_rels >= 64) return GQ_ERR_INVALID;
+    return oracle::plan_t<1>(q, cfg, out, true);
 }
 
 /* Cost one join (outer = l, inner = r) of two already-costed relations with

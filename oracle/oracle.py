@@ -25,6 +25,8 @@ def lib():
         L = C.CDLL(_LIB)
         L.oracle_plan.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.POINTER(abi.PlanOut)]
         L.oracle_plan.restype = C.c_int
+        L.oracle_plan_connected.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.POINTER(abi.PlanOut)]
+        L.oracle_plan_connected.restype = C.c_int
         L.oracle_join_cost.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                        C.POINTER(C.c_float)]
@@ -60,6 +62,19 @@ def plan(query, cfg):
     nodes = (abi.PlanNode * cap)()
     out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
     st = lib().oracle_plan(C.byref(query.to_c()), C.byref(cfg), C.byref(out))
+    return st, plan_tuples(out), out.stats
+
+
+def plan_connected(query, cfg):
+    """plan() with DPsub / MPDP visiting only the connected sets, size by
+    size (oracle_plan_connected): the same plans, for DPs above 32
+    relations.  At most 63 relations."""
+    from gpuqo_amd import abi, plan_tuples
+    n = query.n
+    cap = 2 * n
+    nodes = (abi.PlanNode * cap)()
+    out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
+    st = lib().oracle_plan_connected(C.byref(query.to_c()), C.byref(cfg), C.byref(out))
     return st, plan_tuples(out), out.stats
 
 

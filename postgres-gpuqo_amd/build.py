@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libgpuqo_amd.so")
-SOURCES = ["gq_dp.hip", "gq_host.cpp", "gq_drivers.cpp", "gq_server.cpp"]
+SOURCES = ["gq_dp.hip", "gq_wide.hip", "gq_host.cpp", "gq_drivers.cpp", "gq_server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
          "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang",

@@ -59,8 +59,10 @@
 #include <unordered_set>
 #include <vector>
 
+#include "gq_device.h"
 #include "gq_dp.h"
 #include "gq_internal.h"
+#include "gq_wide.h"
 
 namespace gq {
 
@@ -128,30 +130,13 @@ __device__ __forceinline__ bool is_conn(const uint32_t *conn, uint32_t x) {
     return (conn[x >> 5] >> (x & 31)) & 1u;
 }
 
-/* Memo record: rows, startup, total and a width word, which is the width
- * itself or, bit 31 set, hash_pack() of the relation (the width in its low
- * 21 bits).  Widths are below 2^31 (base widths below 2^26, at most 32
- * relations), so bit 31 tells the two forms apart. */
-__device__ __forceinline__ int rec_width(uint32_t w) { return (w >> 31) ? (int)(w & 0x1fffffu) : (int)w; }
+/* Memo records (rel_of / record_of, gq_device.h) of slot x. */
+__device__ __forceinline__ Rel load_rel(const float4 *vals, uint32_t x) { return rel_of(vals[x]); }
 
-__device__ __forceinline__ Rel load_rel(const float4 *vals, uint32_t x) {
-    float4 v = vals[x];
-    Rel r;
-    r.rows = v.x;
-    r.startup = v.y;
-    r.total = v.z;
-    const uint32_t w = __float_as_uint(v.w);
-    r.width = rec_width(w);
-    r.hs = (w >> 31) ? w : 0u;
-    return r;
-}
-
-/* pack = false keeps the plain width (readers then size the hash table per
- * join): for latency-bound levels, where the winning lane alone would run
- * hash_table_size on the critical path. */
+/* pack = false: for latency-bound levels, where the winning lane alone would
+ * run hash_table_size on the critical path. */
 __device__ __forceinline__ void store_rel(float4 *vals, uint32_t x, const Rel &r, int work_mem, bool pack = true) {
-    const uint32_t hs = pack ? hash_pack(r.rows, r.width, work_mem) : 0u;
-    vals[x] = make_float4(r.rows, r.startup, r.total, __uint_as_float(hs ? hs : (uint32_t)r.width));
+    vals[x] = record_of(r, work_mem, pack);
 }
 
 /* base relation i of the blob into memo slot x */
@@ -275,13 +260,6 @@ __device__ __forceinline__ uint32_t pdep32(uint32_t j, uint32_t mask) {
         mask &= mask - 1;
     }
     return res;
-}
-
-/* unsigned order of non-NaN floats; -0 and +0 map to the same key */
-__device__ __forceinline__ uint32_t ord_key(float x) {
-    if (x == 0.0f) x = 0.0f;
-    uint32_t u = __float_as_uint(x);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
 /* C(n, k) at [n * 33 + k], n, k <= 32 (uploaded by ctx_init) */
@@ -2777,6 +2755,13 @@ bool device_ready(int device) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
     return g_ctx.device == device && g_ctx.stream != nullptr && device >= 0 && device < GQ_MAX_DEVICES &&
            g_scratch[device] != 0;
+}
+
+int with_device_stream(const std::function<int(int, void *)> &f) {
+    std::lock_guard<std::mutex> g(g_ctx.mu);
+    int rc = ctx_init(g_ctx, g_ctx.device < 0 ? 0 : g_ctx.device);
+    if (rc) return rc;
+    return f(g_ctx.device, (void *)g_ctx.stream);
 }
 
 

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "gq_drivers.h"
+#include "gq_wide.h"
 
 namespace gq {
 
@@ -305,6 +306,52 @@ static QT<1> *nodes_to_tree(RunCtx &ctx, const std::vector<DPNode> &nodes) {
     return made.empty() ? nullptr : made[0];
 }
 
+static QT<1> *wide_nodes_to_tree(RunCtx &ctx, const std::vector<WideNode> &nodes) {
+    std::vector<QT<1> *> made(nodes.size(), nullptr);
+    for (size_t i = 0; i < nodes.size(); i++) {
+        QT<1> *t = ctx.node<1>();
+        t->id.w[0] = nodes[i].set;
+        t->rows = nodes[i].rows;
+        t->startup = nodes[i].startup;
+        t->total = nodes[i].total;
+        t->width = nodes[i].width;
+        made[i] = t;
+    }
+    for (size_t i = 0; i < nodes.size(); i++) {
+        made[i]->left = nodes[i].left >= 0 ? made[nodes[i].left] : nullptr;
+        made[i]->right = nodes[i].right >= 0 ? made[nodes[i].right] : nullptr;
+    }
+    return made.empty() ? nullptr : made[0];
+}
+
+/* One DP over 33..64 relations (a tree join graph): the wide DP. */
+static int run_wide(RunCtx &ctx, const HQuery<1> &h, DPAlgo algo, QT<1> **out) {
+    *out = nullptr;
+    if (algo == DP_DPSIZE) {
+        set_error("DPsize runs on the GPU for at most %d relations (got %d)", GQ_MAX_DP_RELS, h.n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (h.n_iters != h.n) {
+        set_error("IDP1's level-capped DPs run on the GPU for at most %d relations (got %d)", GQ_MAX_DP_RELS, h.n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (ctx.noalloc) {
+        set_error("the allocation-free path plans at most %d relations", GQ_NOALLOC_MAX_RELS);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    WideInput wi;
+    int rc = build_wide_input(h, wi);
+    if (rc) return rc;
+    std::vector<WideNode> nodes;
+    DPStats st;
+    rc = run_wide_dp(wi, algo, ctx.max_memo_mb, nodes, st);
+    ctx.dp_calls++;
+    if (rc) return rc;
+    ctx.add(st);
+    *out = wide_nodes_to_tree(ctx, nodes);
+    return GQ_OK;
+}
+
 static QT<1> *single_leaf(RunCtx &ctx, const HQuery<1> &h) {
     QT<1> *t = ctx.node<1>();
     t->id = Set<1>::nth(0);
@@ -336,6 +383,11 @@ static int run_many(RunCtx &ctx, const std::vector<const HQuery<1> *> &hs, std::
     for (size_t i = 0; i < hs.size(); i++) {
         if (hs[i]->n == 1) {
             out[i] = single_leaf(ctx, *hs[i]);
+            continue;
+        }
+        if (hs[i]->n > GQ_MAX_DP_RELS) {
+            int rc = run_wide(ctx, *hs[i], algo, &out[i]);
+            if (rc) return rc;
             continue;
         }
         int rc = build_dp_input(*hs[i], ins[i]);

@@ -1,0 +1,792 @@
+/*
+ * gq_wide.hip -- the wide DP: DPsub / MPDP over one tree query of 33 to 64
+ * relations, relation i = bit i of a uint64_t set, on CDNA4 (gfx950, wave64).
+ *
+ * The reference's GPU DPsub takes such queries as Bitmapset64
+ * (gpuqo_main_internal.cu:178-184): per level it unranks every k-subset of
+ * the n relations (gpuqo_dpsub.cuh:95-126, C(n, k) of them), keeps the
+ * connected ones (gpuqo_dpsub_filtered_kernels.cuh:30-109) and evaluates
+ * them (:111-240) against an open-addressing memo
+ * (gpuqo_hashtable.cuh:119-166).  Above 32 relations that pass over 2^n
+ * candidates is what bounds it; here no candidate that is not a connected
+ * set is ever formed:
+ *
+ *   levels   in a tree numbered breadth-first, the highest member v of a
+ *            connected set T of two or more relations has its parent in T
+ *            and is a leaf of T, so T - v is connected: T is built exactly
+ *            once, from S = T - v, as S + v for a child v of a member of S
+ *            with v above every member of S.  The host knows each level's
+ *            size in advance (the subtree-count polynomial), so the lists
+ *            and the memo are sized exactly, with no host synchronisation.
+ *   memo     open addressing, 32-byte slots {key, left child, record},
+ *            load at most 1/2, Fibonacci hashing and linear probing.  A
+ *            set's slot is claimed (atomicCAS on the key) when the set is
+ *            generated, one level ahead; lookups only ever look for keys of
+ *            earlier launches, whose probe chains later inserts never
+ *            shorten.
+ *   kw_level one launch per level k: the connected sets of size k are
+ *            evaluated -- k - 1 lanes per set, lane j takes the edge above
+ *            the j-th non-top member c and costs both orientations of the
+ *            split (S - sub(c), sub(c)) -- and, in the same launch, their
+ *            extensions are inserted as level k + 1.  The lanes of a set
+ *            reduce (first valid split, first minimum of (cost, rank)) with
+ *            shuffles; ranks follow the reference's enumeration orders
+ *            (below).
+ *   kw_walk  one wave walks the memo breadth-first from the full set.
+ *
+ * Split orders (ties keep the first split found, gpuqo_cpu_common.cuh:
+ * 189-211):
+ *   MPDP   blocks in find_blocks emission order (gpuqo_cpu_dpsub_bicc.cu:
+ *          65-169): in a tree every edge is a block, emitted in the
+ *          post-order of its lower end c in the DFS from the lowest member
+ *          of S with ascending neighbours; within the block (parent(c), c)
+ *          lb = {parent(c)} first (:180-232).  Rank 2 post(c) + o, post(c) =
+ *          |S & desc(c)| - 1 + |S & lglob(c)|.
+ *   DPsub  left side ascending (gpuqo_cpu_dpsub.cu:32-79), the rank being
+ *          the left side itself; the valid splits of a tree set are exactly
+ *          its edge splits.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "gq_device.h"
+#include "gq_host.h"
+#include "gq_internal.h"
+#include "gq_wide.h"
+
+namespace gq {
+
+#define WW 64                 /* wave */
+#define WBLOCK 256            /* threads per workgroup of kw_level */
+#define WGRID_MAX 8192u       /* workgroups of a kw_level launch (they loop over the level) */
+
+/* ------------------------------------------------------------ host side */
+
+int build_wide_input(const HQuery<1> &q, WideInput &in) {
+    const int n = q.n;
+    if (n < 2 || n > GQ_WIDE_MAX_RELS) {
+        set_error("the wide DP takes 2..%d relations (got %d)", GQ_WIDE_MAX_RELS, n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    std::vector<int> parent(n, -1);
+    for (int v = 1; v < n; v++) {
+        const uint64_t lower = q.adj[v].w[0] & ((1ull << v) - 1ull);
+        if (__builtin_popcountll(lower) != 1) {
+            set_error("a single DP over %d relations needs a tree join graph (relation %d closes a cycle); "
+                      "plan larger cyclic queries with gpuqo_idp_type / gpuqo_idp_n_iters", n, v);
+            return GQ_ERR_UNSUPPORTED;
+        }
+        parent[v] = __builtin_ctzll(lower);
+    }
+    HostView<1> hv;
+    hv.build(q);
+    const View<Set<1>> &v = hv.v;
+    const int n_ec = v.n_ec, ecw = v.ecw, nm = (int)q.vars.size();
+    if (ecw > GQ_MAX_ECW) {
+        set_error("GPU DP supports at most %d equivalence classes (got %d)", 64 * GQ_MAX_ECW, n_ec);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    WideHdr &h = in.hdr;
+    memset(&h, 0, sizeof(h));
+    std::vector<uint32_t> &b = in.blob;
+    b.clear();
+    auto take = [&](int words) {
+        const int o = (int)b.size();
+        b.resize(b.size() + words, 0u);
+        return o;
+    };
+    auto align = [&](int k) { while (b.size() % k) b.push_back(0u); };
+    auto put_f = [&](int o, int i, float x) { memcpy(&b[o + i], &x, 4); };
+    auto put_u64 = [&](int o, int i, uint64_t x) {
+        b[o + 2 * i] = (uint32_t)x;
+        b[o + 2 * i + 1] = (uint32_t)(x >> 32);
+    };
+    h.n_rels = n;
+    h.n_ec = n_ec;
+    h.n_mem = nm;
+    h.ecw = ecw;
+    h.p = q.p;
+    h.composite = v.composite.w[0];
+    h.o_rel_rows = take(64);
+    h.o_rel_fksel = take(64);
+    h.o_base = take(4 * 64);
+    for (int i = 0; i < n; i++) {
+        put_f(h.o_rel_rows, i, hv.rel_rows[i]);
+        put_f(h.o_rel_fksel, i, hv.rel_fksel[i]);
+        put_f(h.o_base, 4 * i, q.rows[i]);
+        put_f(h.o_base, 4 * i + 1, q.startup[i]);
+        put_f(h.o_base, 4 * i + 2, q.total[i]);
+        b[h.o_base + 4 * i + 3] = (uint32_t)q.width[i];
+    }
+    align(2);
+    h.o_inc = take(2 * n * ecw);
+    for (int i = 0; i < n * ecw; i++) put_u64(h.o_inc, i, hv.inc[i]);
+    h.o_ec_relids = take(2 * std::max(1, n_ec));
+    h.o_ec_moff = take(std::max(1, n_ec));
+    h.o_ec_soff = take(std::max(1, n_ec));
+    h.o_ec_hasfk = take(std::max(1, (n_ec + 3) / 4));
+    for (int e = 0; e < n_ec; e++) {
+        put_u64(h.o_ec_relids, e, q.ec_relids[e].w[0]);
+        b[h.o_ec_moff + e] = (uint32_t)hv.ec_moff[e];
+        b[h.o_ec_soff + e] = (uint32_t)hv.ec_soff[e];
+        ((uint8_t *)&b[h.o_ec_hasfk])[e] = hv.ec_hasfk[e];
+    }
+    align(2);
+    h.o_mem_fks = take(2 * std::max(1, nm));
+    h.o_mem_nd = take(std::max(1, nm));
+    h.o_mem_inv_nd = take(std::max(1, nm));
+    h.o_mem_skew = take(std::max(1, nm));
+    h.o_mem_mcv = take(std::max(1, nm));
+    h.o_mem_flags = take(std::max(1, (nm + 3) / 4));
+    h.o_mem_idx_rows = take(std::max(1, nm));
+    h.o_mem_idx_startup = take(std::max(1, nm));
+    h.o_mem_idx_total = take(std::max(1, nm));
+    for (int m = 0; m < nm; m++) {
+        put_u64(h.o_mem_fks, m, q.fks[m].w[0]);
+        put_f(h.o_mem_nd, m, hv.mem_nd[m]);
+        put_f(h.o_mem_inv_nd, m, hv.mem_inv_nd[m]);
+        put_f(h.o_mem_skew, m, hv.mem_skew[m]);
+        put_f(h.o_mem_mcv, m, hv.mem_mcv[m]);
+        ((uint8_t *)&b[h.o_mem_flags])[m] = hv.mem_flags[m];
+        put_f(h.o_mem_idx_rows, m, hv.mem_idx_rows[m]);
+        put_f(h.o_mem_idx_startup, m, hv.mem_idx_startup[m]);
+        put_f(h.o_mem_idx_total, m, hv.mem_idx_total[m]);
+    }
+    h.o_sels = take(std::max<int>(1, (int)q.sels.size()));
+    for (size_t i = 0; i < q.sels.size(); i++) put_f(h.o_sels, (int)i, q.sels[i]);
+    h.o_qualtab = take(n_ec + 1);
+    for (int i = 0; i <= n_ec; i++) put_f(h.o_qualtab, i, hv.qualtab[i]);
+    h.pairs = v.pairs;
+    align(4);
+    h.o_pair_rec = take(h.pairs ? 12 * n_ec : 4);
+    if (h.pairs) memcpy(&b[h.o_pair_rec], hv.pair_rec.data(), sizeof(PairRec) * 3 * (size_t)n_ec);
+    /* tree tables: descendants, left-global sets, children */
+    std::vector<uint64_t> desc(n, 0), lg(n, 0), kids(n, 0);
+    for (int u = n - 1; u >= 0; u--) {
+        desc[u] |= 1ull << u;
+        if (u > 0) {
+            desc[parent[u]] |= desc[u];
+            kids[parent[u]] |= 1ull << u;
+        }
+    }
+    for (int u = 1; u < n; u++) {              /* parents precede children */
+        uint64_t l = lg[parent[u]];
+        for (int s = 1; s < u; s++)
+            if (parent[s] == parent[u]) l |= desc[s];
+        lg[u] = l;
+    }
+    h.o_desc = take(2 * 64);
+    h.o_lglob = take(2 * 64);
+    h.o_kids = take(2 * 64);
+    for (int u = 0; u < n; u++) {
+        put_u64(h.o_desc, u, desc[u]);
+        put_u64(h.o_lglob, u, lg[u]);
+        put_u64(h.o_kids, u, kids[u]);
+    }
+    /* per-edge split tables when every class of two or more members is one
+     * tree edge and no edge carries more than 4 classes: the crossing
+     * classes of the split at edge (parent(c), c) are then exactly those on
+     * that edge, so the selectivity (estimate_join_selectivity, the product
+     * in ascending class order of ec_selectivity, which reads the class's
+     * members on each side only) and the inner member slots depend on (c,
+     * orientation) alone */
+    std::vector<std::vector<int>> on_edge(n);
+    bool fast = true;
+    for (int e = 0; e < n_ec && fast; e++) {
+        const uint64_t rel = q.ec_relids[e].w[0];
+        const int pc = __builtin_popcountll(rel);
+        if (pc < 2) continue;
+        const int a = __builtin_ctzll(rel), c = 63 - __builtin_clzll(rel);
+        if (pc != 2 || parent[c] != a) fast = false;
+        else on_edge[c].push_back(e);
+    }
+    for (int c = 1; c < n && fast; c++)
+        if (on_edge[c].size() > 4) fast = false;
+    h.tree_fast = fast ? 1 : 0;
+    h.o_tedge = 0;
+    if (fast) {
+        h.o_tedge = take(8 * 64);
+        for (int c = 1; c < n; c++) {
+            const Set<1> sc = Set<1>::nth(c), sp = Set<1>::nth(parent[c]);
+            float sel0 = 1.0f, sel1 = 1.0f;
+            uint16_t mi0[4] = {0, 0, 0, 0}, mi1[4] = {0, 0, 0, 0};
+            int k = 0;
+            for (int e : on_edge[c]) {                   /* ascending class order */
+                sel0 *= ec_selectivity(v, e, sp, sc);
+                sel1 *= ec_selectivity(v, e, sc, sp);
+                const Set<1> ec = q.ec_relids[e];
+                mi0[k] = (uint16_t)(hv.ec_moff[e] + rank_below(ec, sc));
+                mi1[k] = (uint16_t)(hv.ec_moff[e] + rank_below(ec, sp));
+                k++;
+            }
+            uint32_t *t = b.data() + h.o_tedge + 8 * c;
+            memcpy(&t[0], &sel0, 4);
+            memcpy(&t[1], &sel1, 4);
+            t[2] = (uint32_t)k | ((uint32_t)k << 8);
+            t[3] = 0;
+            memcpy(&t[4], mi0, 8);
+            memcpy(&t[6], mi1, 8);
+        }
+    }
+    align(4);
+    h.words = (int)b.size();
+    /* connected sets per size: P_u(x) = x prod_{c child of u} (1 + P_c(x)),
+     * summed over u (each subtree counted at its top vertex); every
+     * coefficient counts sets of relations, so it fits 64 bits */
+    std::vector<std::vector<uint64_t>> P(n);
+    in.counts.assign(n + 1, 0ull);
+    for (int u = n - 1; u >= 0; u--) {
+        std::vector<uint64_t> poly(n + 1, 0ull);
+        poly[1] = 1;
+        for (uint64_t ch = kids[u]; ch; ch &= ch - 1) {
+            const int c = __builtin_ctzll(ch);
+            std::vector<uint64_t> r(n + 1, 0ull);
+            for (int a = 1; a <= n; a++) {
+                if (!poly[a]) continue;
+                r[a] += poly[a];                          /* c not taken */
+                for (int bb = 1; a + bb <= n; bb++) r[a + bb] += poly[a] * P[c][bb];
+            }
+            poly.swap(r);
+        }
+        P[u] = poly;
+        for (int k = 0; k <= n; k++) in.counts[k] += poly[k];
+    }
+    return GQ_OK;
+}
+
+/* memo slots: a power of two at least twice the connected sets (and 256) */
+static uint32_t wide_slots_log2(const WideInput &in) {
+    uint64_t total = 0;
+    for (uint64_t c : in.counts) total += c;
+    uint32_t lg = 8;
+    while (lg < 31 && (1ull << lg) < 2 * total) lg++;
+    return lg;
+}
+
+uint64_t wide_dp_bytes(const WideInput &in) {
+    uint64_t total = 0;
+    for (uint64_t c : in.counts) total += c;
+    const uint32_t lg = wide_slots_log2(in);
+    return (32ull << lg) + 4ull * total;
+}
+
+/* ------------------------------------------------------------ device side */
+
+struct alignas(16) WSlot {
+    unsigned long long key;    /* the set; 0 = empty */
+    unsigned long long left;   /* outer side of the chosen split (0 for base relations) */
+    float4 rec;                /* rows, startup, total, width word (rel_of / record_of) */
+};
+static_assert(sizeof(WSlot) == 32, "32-byte memo slots");
+
+struct WTab {
+    WSlot *s;
+    uint32_t mask, lg;
+};
+
+__device__ __forceinline__ uint32_t wslot0(uint64_t key, uint32_t lg) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64u - lg));
+}
+
+/* slot of a key inserted by an earlier launch; ~0u if it is missing */
+__device__ __forceinline__ uint32_t wfind(const WTab &t, uint64_t key) {
+    uint32_t h = wslot0(key, t.lg);
+    for (uint32_t i = 0; i <= t.mask; i++) {
+        const unsigned long long k = t.s[h].key;
+        if (k == key) return h;
+        if (k == 0ull) break;
+        h = (h + 1u) & t.mask;
+    }
+    return ~0u;
+}
+
+/* claims a slot for key (each set is generated once); ~0u if the table is full */
+__device__ __forceinline__ uint32_t winsert(const WTab &t, uint64_t key) {
+    uint32_t h = wslot0(key, t.lg);
+    for (uint32_t i = 0; i <= t.mask; i++) {
+        const unsigned long long old = atomicCAS(&t.s[h].key, 0ull, (unsigned long long)key);
+        if (old == 0ull || old == key) return h;
+        h = (h + 1u) & t.mask;
+    }
+    return ~0u;
+}
+
+/* position of the member of x that has j members below it (j < popcount(x)) */
+__device__ __forceinline__ int select64(uint64_t x, uint32_t j) {
+    int p = 0;
+    uint32_t c = __builtin_popcount((uint32_t)x);
+    if (j >= c) {
+        j -= c;
+        x >>= 32;
+        p = 32;
+    }
+    uint32_t y = (uint32_t)x;
+    c = __builtin_popcount(y & 0xffffu);
+    if (j >= c) { j -= c; y >>= 16; p += 16; }
+    c = __builtin_popcount(y & 0xffu);
+    if (j >= c) { j -= c; y >>= 8; p += 8; }
+    c = __builtin_popcount(y & 0xfu);
+    if (j >= c) { j -= c; y >>= 4; p += 4; }
+    c = __builtin_popcount(y & 0x3u);
+    if (j >= c) { j -= c; y >>= 2; p += 2; }
+    return (j >= (y & 1u)) ? p + 1 : p;
+}
+
+__device__ __forceinline__ unsigned long long shfl_down64(unsigned long long x, uint32_t o) {
+    const uint32_t lo = __shfl_down((uint32_t)x, o), hi = __shfl_down((uint32_t)(x >> 32), o);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long x, uint32_t src) {
+    const uint32_t lo = __shfl((uint32_t)x, (int)src), hi = __shfl((uint32_t)(x >> 32), (int)src);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ View<uint64_t> make_wview(const WideHdr &h, const uint32_t *s) {
+    View<uint64_t> v;
+    v.n_rels = h.n_rels;
+    v.n_ec = h.n_ec;
+    v.ecw = h.ecw;
+    v.p = h.p;
+    v.rel_rows = (const float *)(s + h.o_rel_rows);
+    v.rel_fksel = (const float *)(s + h.o_rel_fksel);
+    v.composite = h.composite;
+    v.inc = (const uint64_t *)(s + h.o_inc);
+    v.ec_relids = (const uint64_t *)(s + h.o_ec_relids);
+    v.ec_moff = (const int *)(s + h.o_ec_moff);
+    v.ec_soff = (const int *)(s + h.o_ec_soff);
+    v.ec_hasfk = (const uint8_t *)(s + h.o_ec_hasfk);
+    v.mem_fks = (const uint64_t *)(s + h.o_mem_fks);
+    v.mem_nd = (const float *)(s + h.o_mem_nd);
+    v.mem_inv_nd = (const float *)(s + h.o_mem_inv_nd);
+    v.mem_skew = (const float *)(s + h.o_mem_skew);
+    v.mem_mcv = (const float *)(s + h.o_mem_mcv);
+    v.mem_flags = (const uint8_t *)(s + h.o_mem_flags);
+    v.mem_idx_rows = (const float *)(s + h.o_mem_idx_rows);
+    v.mem_idx_startup = (const float *)(s + h.o_mem_idx_startup);
+    v.mem_idx_total = (const float *)(s + h.o_mem_idx_total);
+    v.sels = (const float *)(s + h.o_sels);
+    v.qualtab = (const float *)(s + h.o_qualtab);
+    v.pairs = h.pairs;
+    v.pair_rec = (const PairRec *)(s + h.o_pair_rec);
+    return v;
+}
+
+/* error bits of a wide DP (misc[WERR]) */
+#define WERR_PROBE 1u      /* a lookup missed a set of an earlier level */
+#define WERR_FULL 2u       /* an insert found no free slot, or a level list overflowed */
+#define WERR_WALK 4u       /* the plan walk met an inconsistent entry */
+#define WERR 66            /* misc[0..65]: sets generated per level; misc[66]: error bits */
+
+/* Base relations (level 1) and their extensions (level 2). */
+__global__ void __launch_bounds__(WW) kw_base(WideHdr h, const uint32_t *blob, WTab t, uint32_t *lst,
+                                              uint32_t off2, uint32_t cap2, uint32_t *misc) {
+    const int i = threadIdx.x;
+    if (i >= h.n_rels) return;
+    const uint64_t S = 1ull << i;
+    const uint32_t sl = winsert(t, S);
+    if (sl == ~0u) {
+        atomicOr(&misc[WERR], WERR_FULL);
+        return;
+    }
+    const float *bv = (const float *)(blob + h.o_base) + 4 * i;
+    Rel r;
+    r.rows = bv[0];
+    r.startup = bv[1];
+    r.total = bv[2];
+    r.width = __float_as_int(bv[3]);
+    r.hs = 0u;
+    t.s[sl].rec = record_of(r, h.p.work_mem);
+    t.s[sl].left = 0ull;
+    lst[i] = sl;
+    const uint64_t *kids = (const uint64_t *)(blob + h.o_kids);
+    for (uint64_t c = kids[i]; c; c &= c - 1ull) {
+        const uint32_t s2 = winsert(t, S | (c & (0ull - c)));
+        const uint32_t pos = atomicAdd(&misc[2], 1u);
+        if (s2 == ~0u || pos >= cap2) {
+            atomicOr(&misc[WERR], WERR_FULL);
+            continue;
+        }
+        lst[off2 + pos] = s2;
+    }
+}
+
+/* Level k: evaluate its n_k sets (list at off_k) and, below the top level,
+ * insert their extensions as level k + 1 (list at off_n, cap_n of them). */
+template <bool DPSUB, bool FAST>
+__global__ void __launch_bounds__(WBLOCK) kw_level(WideHdr h, const uint32_t *blob, WTab t, uint32_t *lst, int k,
+                                                   uint32_t off_k, uint32_t n_k, uint32_t off_n, uint32_t cap_n,
+                                                   uint32_t *misc) {
+    extern __shared__ __align__(16) uint32_t wsm[];
+    for (int i = threadIdx.x; i < h.words; i += blockDim.x) wsm[i] = blob[i];
+    __syncthreads();
+    const View<uint64_t> q = make_wview(h, wsm);
+    const uint64_t *desc = (const uint64_t *)(wsm + h.o_desc);
+    const uint64_t *lglob = (const uint64_t *)(wsm + h.o_lglob);
+    const uint64_t *kids = (const uint64_t *)(wsm + h.o_kids);
+    const uint32_t L = (uint32_t)(k - 1), spw = WW / L;
+    const uint32_t lane = threadIdx.x & (WW - 1), seg = lane / L, j = lane - seg * L, lead = seg * L;
+    const bool in_seg = seg < spw;
+    const uint32_t wave = blockIdx.x * (WBLOCK / WW) + threadIdx.x / WW, nwaves = gridDim.x * (WBLOCK / WW);
+    const bool expand = k < h.n_rels;
+    const uint32_t NONE = 0xffffffffu;
+    for (uint32_t base = wave * spw; base < n_k; base += nwaves * spw) {
+        const uint32_t idx = base + seg;
+        const bool live = in_seg && idx < n_k;
+        uint32_t slot = 0;
+        uint64_t S = 0, sub = 0, oth = 0;
+        int c = 0;
+        unsigned long long tie0 = ~0ull, tie1 = ~0ull;
+        bool nan0 = false, nan1 = false;
+        Rel r0, r1;
+        if (live) {
+            slot = lst[off_k + idx];
+            S = t.s[slot].key;
+            const uint64_t top = S & (0ull - S);
+            c = select64(S ^ top, j);
+            sub = S & desc[c];
+            oth = S ^ sub;
+            const uint32_t is = wfind(t, sub), io = wfind(t, oth);
+            if (is == ~0u || io == ~0u) {
+                atomicOr(&misc[WERR], WERR_PROBE);
+            } else {
+                const Rel Vs = rel_of(t.s[is].rec), Vo = rel_of(t.s[io].rec);
+                if (FAST) {
+                    const uint32_t *te = wsm + h.o_tedge + 8 * c;
+                    EdgeProv p0, p1;
+                    p0.sel = __uint_as_float(te[0]);
+                    p1.sel = __uint_as_float(te[1]);
+                    p0.nq = p1.nq = (int)(te[2] & 0xffu);
+                    p0.n_mi = p1.n_mi = (int)((te[2] >> 8) & 0xffu);
+                    p0.mi = (const uint16_t *)(te + 4);
+                    p1.mi = (const uint16_t *)(te + 6);
+                    r0 = join_prov(q, oth, Vo, sub, Vs, p0);      /* parent's side outer */
+                    r1 = join_prov(q, sub, Vs, oth, Vo, p1);
+                } else {
+                    r0 = join<0>(q, oth, Vo, sub, Vs);
+                    r1 = join<0>(q, sub, Vs, oth, Vo);
+                }
+                if (DPSUB) {
+                    tie0 = oth;
+                    tie1 = sub;
+                } else {
+                    const uint32_t post =
+                        (uint32_t)(__builtin_popcountll(S & desc[c]) - 1 + __builtin_popcountll(S & lglob[c]));
+                    tie0 = 2ull * post;
+                    tie1 = 2ull * post + 1ull;
+                }
+                nan0 = r0.total != r0.total;
+                nan1 = r1.total != r1.total;
+            }
+        }
+        /* (a) the first split (lowest rank) with its NaN flag; (b) the first
+         * minimum of (cost, rank) over non-NaN costs */
+        unsigned long long tA = tie0 < tie1 ? tie0 : tie1;
+        bool nA = tie0 < tie1 ? nan0 : nan1;
+        uint32_t oB = NONE;
+        unsigned long long tB = ~0ull;
+        if (tie0 != ~0ull && !nan0) { oB = ord_key(r0.total); tB = tie0; }
+        if (tie1 != ~0ull && !nan1) {
+            const uint32_t o1 = ord_key(r1.total);
+            if (o1 < oB || (o1 == oB && tie1 < tB)) { oB = o1; tB = tie1; }
+        }
+        for (uint32_t o = 1; o < L; o <<= 1) {
+            const unsigned long long xtA = shfl_down64(tA, o), xtB = shfl_down64(tB, o);
+            const bool xnA = __shfl_down((int)nA, o) != 0;
+            const uint32_t xoB = __shfl_down(oB, o);
+            if (j + o < L) {
+                if (xtA < tA) { tA = xtA; nA = xnA; }
+                if (xoB < oB || (xoB == oB && xtB < tB)) { oB = xoB; tB = xtB; }
+            }
+        }
+        const unsigned long long win0 = (nA || oB == NONE) ? tA : tB;
+        const unsigned long long win = shfl64(win0, lead);
+        if (live && win != ~0ull) {
+            if (tie0 == win) {
+                t.s[slot].rec = record_of(r0, h.p.work_mem);
+                t.s[slot].left = oth;
+            } else if (tie1 == win) {
+                t.s[slot].rec = record_of(r1, h.p.work_mem);
+                t.s[slot].left = sub;
+            }
+        }
+        if (!expand) continue;
+        /* extensions: children of members of S above its highest member */
+        unsigned long long acc = 0ull;
+        if (live) {
+            acc = kids[c];
+            if (j == 0) acc |= kids[__builtin_ctzll(S)];
+        }
+        for (uint32_t o = 1; o < L; o <<= 1) {
+            const unsigned long long x = shfl_down64(acc, o);
+            if (j + o < L) acc |= x;
+        }
+        acc = shfl64(acc, lead);
+        uint64_t C = 0ull;
+        if (live) {
+            const int mx = 63 - __builtin_clzll(S);
+            C = mx >= 63 ? 0ull : (acc & (~0ull << (mx + 1)));
+        }
+        const uint32_t nc = (uint32_t)__builtin_popcountll(C);
+        const uint32_t mine = (live && j < nc) ? (nc - j + L - 1u) / L : 0u;
+        uint32_t incl = mine;                                  /* wave-inclusive scan */
+        for (uint32_t o = 1; o < WW; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t total = __shfl(incl, WW - 1);
+        uint32_t first = 0;
+        if (lane == WW - 1 && total) first = atomicAdd(&misc[k + 1], total);
+        first = __shfl(first, WW - 1);
+        uint32_t pos = first + incl - mine;
+        for (uint32_t tt = j; tt < nc && live; tt += L, pos++) {
+            const uint64_t T = S | (1ull << select64(C, tt));
+            const uint32_t s2 = winsert(t, T);
+            if (s2 == ~0u || pos >= cap_n) {
+                atomicOr(&misc[WERR], WERR_FULL);
+                continue;
+            }
+            lst[off_n + pos] = s2;
+        }
+    }
+}
+
+/* The plan, breadth-first from the full set: node i's children are
+ * allocated after the nodes of its frontier. */
+__global__ void __launch_bounds__(WW) kw_walk(WTab t, int n, WideNode *out, uint32_t *misc) {
+    __shared__ unsigned long long ns[128];
+    const uint32_t lane = threadIdx.x;
+    if (lane == 0) ns[0] = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    __syncthreads();
+    uint32_t fb = 0, fe = 1;
+    while (fb < fe) {
+        if (fe - fb > WW) {
+            if (lane == 0) atomicOr(&misc[WERR], WERR_WALK);
+            break;
+        }
+        const uint32_t i = fb + lane;
+        const bool live = i < fe;
+        uint64_t S = 0, Lset = 0;
+        Rel r;
+        r.rows = r.startup = r.total = 0.0f;
+        r.width = 0;
+        bool internal = false;
+        if (live) {
+            S = ns[i];
+            const uint32_t sl = wfind(t, S);
+            if (sl == ~0u) {
+                atomicOr(&misc[WERR], WERR_WALK);
+            } else {
+                r = rel_of(t.s[sl].rec);
+                Lset = t.s[sl].left;
+                internal = __builtin_popcountll(S) > 1;
+                if (internal && (Lset == 0ull || (Lset & ~S) != 0ull || Lset == S)) {
+                    atomicOr(&misc[WERR], WERR_WALK);
+                    internal = false;
+                }
+            }
+        }
+        const unsigned long long bal = __ballot(internal);
+        const uint32_t ci = fe + 2u * (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        const uint32_t fe2 = fe + 2u * (uint32_t)__popcll(bal);
+        if (fe2 > 128u) {
+            if (lane == 0) atomicOr(&misc[WERR], WERR_WALK);
+            break;
+        }
+        if (live) {
+            WideNode nd;
+            nd.set = S;
+            nd.rows = r.rows;
+            nd.startup = r.startup;
+            nd.total = r.total;
+            nd.width = r.width;
+            nd.left = internal ? (int32_t)ci : -1;
+            nd.right = internal ? (int32_t)ci + 1 : -1;
+            out[i] = nd;
+            if (internal) {
+                ns[ci] = Lset;
+                ns[ci + 1] = S ^ Lset;
+            }
+        }
+        __syncthreads();
+        fb = fe;
+        fe = fe2;
+    }
+    if (lane == 0) misc[WERR + 1] = fe;
+}
+
+/* ------------------------------------------------------------ runner */
+
+struct WideBufs {
+    int device = -1;
+    WSlot *tab = nullptr;
+    uint32_t *lst = nullptr, *blob = nullptr, *misc = nullptr;
+    WideNode *nodes = nullptr;
+    size_t cap_tab = 0, cap_lst = 0, cap_blob = 0;
+    uint32_t *h_misc = nullptr;
+    WideNode *h_nodes = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+static WideBufs g_wide;      /* used under the device context's lock only */
+
+#define WCHK(x)                                                                            \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, #x); \
+            return GQ_ERR_DEVICE;                                                          \
+        }                                                                                  \
+    } while (0)
+
+template <class T>
+static int wensure(T **p, size_t &cap, size_t need) {
+    if (*p && cap >= need) return GQ_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    const size_t bytes = std::max<size_t>(need, 1) * sizeof(T);
+    if (hipMalloc((void **)p, bytes) != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        set_error("hipMalloc of %zu bytes failed", bytes);
+        return GQ_ERR_NOMEM;
+    }
+    cap = std::max<size_t>(need, 1);
+    return GQ_OK;
+}
+
+static int wide_init(WideBufs &w, int device) {
+    if (w.device == device) return GQ_OK;
+    if (w.device >= 0) {
+        set_error("the wide DP runs on one device per process (device %d, asked for %d)", w.device, device);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    WCHK(hipMalloc((void **)&w.misc, 72 * 4));
+    WCHK(hipMalloc((void **)&w.nodes, 128 * sizeof(WideNode)));
+    WCHK(hipHostMalloc((void **)&w.h_misc, 72 * 4, hipHostMallocDefault));
+    WCHK(hipHostMalloc((void **)&w.h_nodes, 128 * sizeof(WideNode), hipHostMallocDefault));
+    WCHK(hipEventCreate(&w.e0));
+    WCHK(hipEventCreate(&w.e1));
+    w.device = device;
+    return GQ_OK;
+}
+
+template <bool DPSUB, bool FAST>
+static void launch_level(const WideHdr &h, uint32_t grid, size_t lds, hipStream_t s, const uint32_t *blob, WTab t,
+                         uint32_t *lst, int k, uint32_t off_k, uint32_t n_k, uint32_t off_n, uint32_t cap_n,
+                         uint32_t *misc) {
+    hipLaunchKernelGGL((kw_level<DPSUB, FAST>), dim3(grid), dim3(WBLOCK), lds, s, h, blob, t, lst, k, off_k, n_k,
+                       off_n, cap_n, misc);
+}
+
+int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<WideNode> &nodes, DPStats &st) {
+    const WideHdr &h = in.hdr;
+    const int n = h.n_rels;
+    if (algo == DP_DPSIZE) {
+        set_error("DPsize runs on the GPU for at most %d relations (got %d)", GQ_MAX_DP_RELS, n);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    const size_t lds = ((size_t)h.words * 4 + 15) & ~(size_t)15;
+    if (lds > 64 * 1024) {
+        set_error("query description too large for LDS (%zu bytes)", lds);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    uint64_t total = 0, pairs = 0;
+    for (int k = 1; k <= n; k++) {
+        total += in.counts[k];
+        pairs += 2ull * (uint64_t)(k - 1) * in.counts[k];
+    }
+    const uint64_t bytes = wide_dp_bytes(in);
+    if (bytes > ((uint64_t)std::max(max_memo_mb, 1) << 20) || total >= (1ull << 30)) {
+        set_error("the DP's %llu connected sets need %llu MB of memo, above gpuqo_max_memo_size_mb (%d)",
+                  (unsigned long long)total, (unsigned long long)(bytes >> 20), max_memo_mb);
+        return GQ_ERR_NOMEM;
+    }
+    const uint32_t lg = wide_slots_log2(in);
+    std::vector<uint32_t> off(n + 2, 0);                  /* level k's list starts at off[k] */
+    for (int k = 2; k <= n + 1; k++) off[k] = off[k - 1] + (uint32_t)in.counts[k - 1];
+    return with_device_stream([&](int device, void *sv) -> int {
+        hipStream_t s = (hipStream_t)sv;
+        WideBufs &w = g_wide;
+        int rc = wide_init(w, device);
+        if (rc) return rc;
+        if ((rc = wensure(&w.tab, w.cap_tab, (size_t)1 << lg))) return rc;
+        if ((rc = wensure(&w.lst, w.cap_lst, (size_t)total))) return rc;
+        if ((rc = wensure(&w.blob, w.cap_blob, in.blob.size()))) return rc;
+        const double t_first = host_now_ms();
+        WCHK(hipMemcpyAsync(w.blob, in.blob.data(), in.blob.size() * 4, hipMemcpyHostToDevice, s));
+        WCHK(hipMemsetAsync(w.tab, 0, sizeof(WSlot) << lg, s));
+        WCHK(hipMemsetAsync(w.misc, 0, 72 * 4, s));
+        WCHK(hipEventRecord(w.e0, s));
+        WTab t;
+        t.s = w.tab;
+        t.lg = lg;
+        t.mask = (uint32_t)(((uint64_t)1 << lg) - 1u);
+        hipLaunchKernelGGL(kw_base, dim3(1), dim3(WW), 0, s, h, (const uint32_t *)w.blob, t, w.lst, off[2],
+                           (uint32_t)in.counts[2], w.misc);
+        int launches = 1;
+        for (int k = 2; k <= n; k++) {
+            const uint32_t n_k = (uint32_t)in.counts[k];
+            const uint32_t spw = WW / (uint32_t)(k - 1);
+            const uint64_t waves = (n_k + spw - 1) / spw;
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>((waves + WBLOCK / WW - 1) / (WBLOCK / WW), WGRID_MAX));
+            const uint32_t off_n = k < n ? off[k + 1] : 0u, cap_n = k < n ? (uint32_t)in.counts[k + 1] : 0u;
+            const bool dpsub = algo == DP_DPSUB, fast = h.tree_fast != 0;
+            if (dpsub && fast)
+                launch_level<true, true>(h, grid, lds, s, w.blob, t, w.lst, k, off[k], n_k, off_n, cap_n, w.misc);
+            else if (dpsub)
+                launch_level<true, false>(h, grid, lds, s, w.blob, t, w.lst, k, off[k], n_k, off_n, cap_n, w.misc);
+            else if (fast)
+                launch_level<false, true>(h, grid, lds, s, w.blob, t, w.lst, k, off[k], n_k, off_n, cap_n, w.misc);
+            else
+                launch_level<false, false>(h, grid, lds, s, w.blob, t, w.lst, k, off[k], n_k, off_n, cap_n, w.misc);
+            launches++;
+        }
+        hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, n, w.nodes, w.misc);
+        WCHK(hipGetLastError());
+        WCHK(hipEventRecord(w.e1, s));
+        const double t_last = host_now_ms();
+        WCHK(hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s));
+        WCHK(hipMemcpyAsync(w.h_nodes, w.nodes, (size_t)(2 * n - 1) * sizeof(WideNode), hipMemcpyDeviceToHost, s));
+        WCHK(hipStreamSynchronize(s));
+        const double t_synced = host_now_ms();
+        const uint32_t err = w.h_misc[WERR];
+        if (err) {
+            set_error("wide DP memo inconsistency (error bits %u)", err);
+            return GQ_ERR_DEVICE;
+        }
+        for (int k = 3; k <= n; k++)
+            if (w.h_misc[k] != (uint32_t)in.counts[k]) {
+                set_error("wide DP generated %u sets of size %d, expected %llu", w.h_misc[k], k,
+                          (unsigned long long)in.counts[k]);
+                return GQ_ERR_DEVICE;
+            }
+        if (w.h_misc[2] != (uint32_t)in.counts[2] || w.h_misc[WERR + 1] != (uint32_t)(2 * n - 1)) {
+            set_error("wide DP plan walk returned %u nodes", w.h_misc[WERR + 1]);
+            return GQ_ERR_DEVICE;
+        }
+        nodes.assign(w.h_nodes, w.h_nodes + (2 * n - 1));
+        float ms = 0.0f;
+        WCHK(hipEventElapsedTime(&ms, w.e0, w.e1));
+        st = DPStats();
+        st.candidates = total - in.counts[1];
+        st.sets = total - in.counts[1];
+        st.pairs = pairs;
+        st.memo_bytes = bytes;
+        st.memo_kind = GQ_MEMO_HASH;
+        st.gpu_ms = ms;
+        st.eval_ms = ms;
+        st.eval_launches = launches;
+        st.t_first_launch = t_first;
+        st.t_last_launch = t_last;
+        st.t_synced = t_synced;
+        return GQ_OK;
+    });
+}
+
+}  // namespace gq

@@ -267,6 +267,22 @@ def graph(n, edges, seed, shape="graph"):
     return g.build()
 
 
+def hub_chains(n, arms, seed):
+    """A hub (relation 0) with `arms` chains of relations hanging off it, the
+    n - 1 others dealt out evenly (the first arms one longer), built chain
+    by chain; rows logU[1e3,1e7], selectivity logU[1e-5,1e-1] per edge."""
+    edges = []
+    nxt = 1
+    per, extra = divmod(n - 1, arms)
+    for a in range(arms):
+        prev = 0
+        for _ in range(per + (1 if a < extra else 0)):
+            edges.append((prev, nxt))
+            prev = nxt
+            nxt += 1
+    return graph(n, edges, seed, "hub-chains")
+
+
 def cycle_with_tail(c, t, seed):
     """A cycle of c relations with a path of t more hanging off relation 0."""
     edges = [(i, i + 1) for i in range(c - 1)] + [(c - 1, 0)]
