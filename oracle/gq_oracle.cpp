@@ -2068,12 +2068,14 @@ int oracle_plan(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
     return oracle::plan_t<4>(q, cfg, out);
 }
 
-/* oracle_plan with DPsub and MPDP visiting the connected sets size by
- * size (connected_enumerate_sets) instead of all 2^n integers: the same
- * plans, for DPs above 32 relations whose 2^n pass would not finish. */
+/* oracle_plan with DPsub and MPDP (also inside IDP1 / IDP2 / UnionDP)
+ * visiting the connected sets size by size (connected_enumerate_sets)
+ * instead of all 2^n integers: the same plans, for DPs above 32 relations
+ * whose 2^n pass would not finish. */
 int oracle_plan_connected(const gq_query *q, const gq_config *cfg, gq_plan_out *out) {
-    if (!q || !cfg || !out || q->n_rels < 1 || q->n_rels >= 64) return GQ_ERR_INVALID;
-    return oracle::plan_t<1>(q, cfg, out, true);
+    if (!q || !cfg || !out || q->n_rels < 1 || q->n_rels >= GQ_MAX_RELS) return GQ_ERR_INVALID;
+    if (q->n_rels < 64) return oracle::plan_t<1>(q, cfg, out, true);
+    return oracle::plan_t<4>(q, cfg, out, true);
 }
 
 /* Cost one join (outer = l, inner = r) of two already-costed relations with

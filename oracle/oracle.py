@@ -68,7 +68,7 @@ def plan(query, cfg):
 def plan_connected(query, cfg):
     """plan() with DPsub / MPDP visiting only the connected sets, size by
     size (oracle_plan_connected): the same plans, for DPs above 32
-    relations.  At most 63 relations."""
+    relations."""
     from gpuqo_amd import abi, plan_tuples
     n = query.n
     cap = 2 * n

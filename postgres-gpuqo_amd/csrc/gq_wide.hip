@@ -72,12 +72,21 @@ int build_wide_input(const HQuery<1> &q, WideInput &in) {
         set_error("the wide DP takes 2..%d relations (got %d)", GQ_WIDE_MAX_RELS, n);
         return GQ_ERR_UNSUPPORTED;
     }
+    int edges = 0;
+    for (int v = 0; v < n; v++) edges += __builtin_popcountll(q.adj[v].w[0]);
+    if (edges != 2 * (n - 1)) {
+        set_error("a single DP over %d relations needs a tree join graph (this one has %d edges); plan larger "
+                  "cyclic queries with gpuqo_idp_type / gpuqo_idp_n_iters", n, edges / 2);
+        return GQ_ERR_UNSUPPORTED;
+    }
     std::vector<int> parent(n, -1);
     for (int v = 1; v < n; v++) {
         const uint64_t lower = q.adj[v].w[0] & ((1ull << v) - 1ull);
         if (__builtin_popcountll(lower) != 1) {
-            set_error("a single DP over %d relations needs a tree join graph (relation %d closes a cycle); "
-                      "plan larger cyclic queries with gpuqo_idp_type / gpuqo_idp_n_iters", n, v);
+            /* gq_plan's BFS relabelling numbers parents first, and so does
+             * the ascending renumbering of a subtree of such a tree (IDP2) */
+            set_error("the wide DP needs a tree numbered parents first (relation %d of %d has %d lower "
+                      "neighbours)", v, n, __builtin_popcountll(lower));
             return GQ_ERR_UNSUPPORTED;
         }
         parent[v] = __builtin_ctzll(lower);

@@ -12,6 +12,7 @@ and child indices; plus the pair and set counters.
     python tests/golden/make_golden.py --large    # plans_large_v1.json
     python tests/golden/make_golden.py --trees    # plans_trees_v1.json
     python tests/golden/make_golden.py --served   # plans_served_v1.json
+    python tests/golden/make_golden.py --wide     # plans_wide_v1.json
 
 The large set holds the full-size workloads that take the oracle minutes:
 cpu_dpsub and cpu_dpsize on the 18-relation clique (about 3.9e8 csg-cmp
@@ -22,7 +23,12 @@ dense limit in the tests (rank and open-addressing memos), under MPDP:
 about 1.5 minutes of oracle time.  The served set holds every request of
 the 256-request served mix (qgen.serving_mix with the serve256 seed): the
 direct MPDP snowflakes of 10-30 relations and the UnionDP (k = 25)
-snowflakes of 60-200 relations, about half an hour of oracle time.
+snowflakes of 60-200 relations, about half an hour of oracle time.  The
+wide set holds single DPs of 33-63 relations (tree join graphs: chains, a
+hub of chains, deep snowflakes; an IDP2 run whose sub-DP has 36
+relations) under DPsub and MPDP, planned by oracle.plan_connected (the
+oracle visiting only the connected sets, which gives the same plans as its
+2^n pass): about 3 minutes of oracle time.
 """
 import hashlib
 import json
@@ -90,6 +96,51 @@ TREE_CASES = [
     ("snowflake28d4f5-mpdp", "snowflake", [28, 0x28], {"depth": 4, "fanout": 5}, MPDP),
     ("snowflake30-mpdp", "snowflake", [30, 0x30], {}, MPDP),
 ]
+
+
+DPSUB = {"algorithm": "dpsub", "dpsub_bicc": 0}
+WIDE_CASES = [
+    ("chain63-mpdp", "chain", [63, 0x63], {}, MPDP),
+    ("chain63-dpsub", "chain", [63, 0x63], {}, DPSUB),
+    ("hubchains48x6-mpdp", "hub_chains", [48, 6, 0x48], {}, MPDP),
+    ("hubchains48x6-dpsub", "hub_chains", [48, 6, 0x48], {}, DPSUB),
+    ("snowflake34d6f2-mpdp", "snowflake", [34, 0x34], {"depth": 6, "fanout": 2}, MPDP),
+    ("snowflake34d6f2-dpsub", "snowflake", [34, 0x34], {"depth": 6, "fanout": 2}, DPSUB),
+    ("snowflake40d6f2-mpdp", "snowflake", [40, 0x40], {"depth": 6, "fanout": 2}, MPDP),
+    ("snowflake40d6f2-dpsub", "snowflake", [40, 0x40], {"depth": 6, "fanout": 2}, DPSUB),
+    ("snowflake44d8f2-mpdp", "snowflake", [44, 0x44], {"depth": 8, "fanout": 2}, MPDP),
+    ("snowflake44d8f2-dpsub", "snowflake", [44, 0x44], {"depth": 8, "fanout": 2}, DPSUB),
+    ("snowflake48d6f2-idp2-k36", "snowflake", [48, 0x4836], {"depth": 6, "fanout": 2},
+     {"algorithm": "dpsub", "dpsub_bicc": 1, "idp_type": "idp2", "idp_n_iters": 36}),
+]
+
+
+def make_wide(label, gen, args, kwargs, over):
+    g = getattr(qgen, gen)(*args, **kwargs)
+    st, nodes, stats = oracle.plan_connected(g, abi.default_config(**over))
+    assert st == 0, (label, st)
+    plan = [[list(r), a, b, c, w, l, rr] for (r, a, b, c, w, l, rr) in gpuqo_amd.plan_bits(nodes)]
+    return {"label": label, "generator": gen, "args": args, "kwargs": kwargs, "n": g.n,
+            "input_sha256": digest(g), "config": over, "plan": plan,
+            "pairs": stats.pairs, "sets": stats.sets}
+
+
+def _make_wide(c):
+    return make_wide(*c)
+
+
+def main_wide(procs):
+    from multiprocessing import Pool
+    with Pool(procs) as pool:
+        recs = pool.map(_make_wide, WIDE_CASES, chunksize=1)
+    out = {"format": "postgres-gpuqo_amd golden plans v1 (single DPs of 33-63 relations)",
+           "float_encoding": "IEEE-754 binary32 bit patterns",
+           "records": recs}
+    path = os.path.join(HERE, "plans_wide_v1.json")
+    with open(path, "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+        f.write("\n")
+    print(f"wrote {len(recs)} records to {path}")
 
 
 SERVED_DIRECT = MPDP
@@ -173,6 +224,9 @@ def main():
         return
     if "--large" in sys.argv:
         main_large(int(os.environ.get("GOLDEN_PROCS", "6")))
+        return
+    if "--wide" in sys.argv:
+        main_wide(int(os.environ.get("GOLDEN_PROCS", "6")))
         return
     if "--trees" in sys.argv:
         main_trees(int(os.environ.get("GOLDEN_PROCS", "4")))

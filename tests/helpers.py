@@ -67,3 +67,33 @@ def dpsub_sparse_graphs():
     return [("cycle18", qgen.cycle(18, 0xb18)), ("cycle22", qgen.cycle(22, 0xb22)),
             ("cycle12+tail6", qgen.cycle_with_tail(12, 6, 0xb21)), ("figure8-17", _figure8(17, 0xb24)),
             ("triangle-ring16", _triangle_ring(8, 0xb23))]
+
+
+def tree_pairs_and_sets(g):
+    """Ordered csg-cmp pairs and connected sets of two or more relations of
+    a tree query, from the subtree polynomials P_v(x) = x * prod over the
+    children c of (1 + P_c(x)): every connected set of k relations has
+    2(k - 1) ordered splits."""
+    adj = g.adjacency()
+    order, parent = [0], {0: -1}
+    for v in order:
+        for w in range(g.n):
+            if (adj[v] >> w) & 1 and w not in parent:
+                parent[w] = v
+                order.append(w)
+    poly, total = {}, {}
+    for v in reversed(order):
+        p = {1: 1}
+        for c in (w for w in order if parent[w] == v):
+            q = dict(poly[c])
+            q[0] = q.get(0, 0) + 1
+            r = {}
+            for a, x in p.items():
+                for b, y in q.items():
+                    r[a + b] = r.get(a + b, 0) + x * y
+            p = r
+        poly[v] = p
+        for k, m in p.items():
+            total[k] = total.get(k, 0) + m
+    return (sum(2 * (k - 1) * m for k, m in total.items() if k >= 2),
+            sum(m for k, m in total.items() if k >= 2))

@@ -19,7 +19,7 @@ import pytest
 import gpuqo_amd
 import oracle
 from gpuqo_amd import abi, qgen
-from helpers import assert_same_plan
+from helpers import assert_same_plan, tree_pairs_and_sets
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 with open(os.path.join(HERE, "golden", "plans_large_v1.json")) as f:
@@ -92,36 +92,6 @@ def test_tree_fixture_inventory_and_inputs():
         assert g.n == r["n"] and _digest(g) == r["input_sha256"], label
         assert len(want) == 2 * g.n - 1
         assert (r["pairs"], r["sets"]) == tree_pairs_and_sets(g), label
-
-
-def tree_pairs_and_sets(g):
-    """Ordered csg-cmp pairs and connected sets of two or more relations of
-    a tree query, from the subtree polynomials P_v(x) = x * prod over the
-    children c of (1 + P_c(x)): every connected set of k relations has
-    2(k - 1) ordered splits."""
-    adj = g.adjacency()
-    order, parent = [0], {0: -1}
-    for v in order:
-        for w in range(g.n):
-            if (adj[v] >> w) & 1 and w not in parent:
-                parent[w] = v
-                order.append(w)
-    poly, total = {}, {}
-    for v in reversed(order):
-        p = {1: 1}
-        for c in (w for w in order if parent[w] == v):
-            q = dict(poly[c])
-            q[0] = q.get(0, 0) + 1
-            r = {}
-            for a, x in p.items():
-                for b, y in q.items():
-                    r[a + b] = r.get(a + b, 0) + x * y
-            p = r
-        poly[v] = p
-        for k, m in p.items():
-            total[k] = total.get(k, 0) + m
-    return (sum(2 * (k - 1) * m for k, m in total.items() if k >= 2),
-            sum(m for k, m in total.items() if k >= 2))
 
 
 def test_served_fixture_inventory_and_inputs():
