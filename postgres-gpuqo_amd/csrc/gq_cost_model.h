@@ -19,6 +19,7 @@
 #define GQ_COST_MODEL_H
 
 #include <stdint.h>
+#include <string.h>
 #include <math.h>
 
 #if defined(__HIPCC__)
@@ -168,9 +169,29 @@ GQ_HD float clamp_row_est(float nrows) {               /* cost_postgres.cuh:850-
     return nrows <= 1.0f ? 1.0f : rintf(nrows);
 }
 GQ_HD unsigned long maxalign8(long len) { return ((unsigned long)len + 7ul) & ~7ul; }
+/* The C ABI admits widths of 0 .. 2^31 - 1 only (base widths are checked
+ * non-negative and every sum of them below 2^31), so maxalign8(width) + 24
+ * is below 2^32 and its u32 conversion rounds the same value as the
+ * reference's conversion of the 64-bit one (a 64-bit integer conversion is
+ * a long instruction sequence on the GPU). */
 GQ_HD float relation_byte_size(float tuples, int width) {   /* :822-826 */
-    return tuples * (float)(maxalign8(width) + 24ul);
+    return tuples * (float)(uint32_t)(maxalign8(width) + 24ul);
 }
+
+GQ_HD uint32_t f_bits(float x) { uint32_t b; memcpy(&b, &x, 4); return b; }
+GQ_HD float bits_f(uint32_t b) { float x; memcpy(&x, &b, 4); return x; }
+
+/* 1 / x for x = +-2^e, a normal float whose reciprocal is normal too, else
+ * 0.  Then the reciprocal is exact, so x * (1 / x) is never needed: a / x
+ * and a * recip_pow2(x) are the same correctly rounded value of the same
+ * real quotient, for every a (subnormal, infinite and NaN a included). */
+GQ_HD float recip_pow2(float x) {
+    const uint32_t b = f_bits(x), e = (b >> 23) & 255u;
+    if ((b & 0x7fffffu) || e == 0u || e > 253u) return 0.0f;
+    return bits_f((b & 0x80000000u) | ((254u - e) << 23));
+}
+/* a / x through the exact reciprocal when x is a power of two */
+GQ_HD float div_pow2(float a, float x, float rx) { return rx != 0.0f ? a * rx : a / x; }
 GQ_HD float page_size(float tuples, int width) {            /* :832-840 */
     return ceilf(relation_byte_size(tuples, width) / 8192.0f);
 }
@@ -313,7 +334,10 @@ GQ_HD float join_rows(float sel, float lrows, float rrows) {
     return rows > 1.0f ? roundf(rows) : 1.0f;
 }
 
-struct Extra { bool inner_unique, indexed; float joinrows, outer_match_frac, match_count; };
+/* outer_match_frac and scan_frac (the inner_scan_frac of cost_nestloop and
+ * cost_hashjoin, 2 / (match_count + 1)) are set when inner_unique only:
+ * nothing else reads them */
+struct Extra { bool inner_unique, indexed; float joinrows, outer_match_frac, match_count, scan_frac; };
 
 /* ExecChooseHashTableSize depends on the inner relation alone (rows,
  * width) and the query's work_mem, so the memo keeps it with the relation:
@@ -352,13 +376,14 @@ GQ_HD void nestloop(const View<S> &q, const Rel &outer, float in_rows, float in_
     if (ex.inner_unique) {
         float outer_matched_rows = rintf(outer_path_rows * ex.outer_match_frac);
         float outer_unmatched_rows = outer_path_rows - outer_matched_rows;
-        float inner_scan_frac = 2.0f / (ex.match_count + 1.0f);
+        const float inner_scan_frac = ex.scan_frac;
         ntuples = outer_matched_rows * inner_path_rows * inner_scan_frac;
         if (ex.indexed) {
             run_cost += inner_run_cost * inner_scan_frac;
             if (outer_matched_rows > 1.0f)
                 run_cost += (outer_matched_rows - 1.0f) * inner_rescan_run_cost * inner_scan_frac;
-            run_cost += outer_unmatched_rows * inner_rescan_run_cost / inner_path_rows;
+            run_cost += div_pow2(outer_unmatched_rows * inner_rescan_run_cost, inner_path_rows,
+                                 recip_pow2(inner_path_rows));
         } else {
             ntuples += outer_unmatched_rows * inner_path_rows;
             run_cost += inner_run_cost;
@@ -411,8 +436,8 @@ GQ_HD void bucket_fold(const View<S> &q, int mi, float fb, float inv_fb, float *
  * (bsize, mcvfreq) of its crossing members already computed (join_prov). */
 template <class S>
 GQ_HD void hashjoin(const View<S> &q, const Rel &outer, const Rel &inner, const Extra &ex, int n_quals,
-                    float qual_per_tuple, HashSize hs, float virtualbuckets, float bsize, float mcvfreq,
-                    float *o_startup, float *o_total) {
+                    float qual_per_tuple, HashSize hs, float virtualbuckets, float inv_vb, float bsize,
+                    float mcvfreq, float *o_startup, float *o_total) {
     float startup_cost = 0.0f, run_cost = 0.0f;
     float outer_path_rows = outer.rows, inner_path_rows = inner.rows;
     if (!q.p.enable_hashjoin) startup_cost += q.p.disable_cost;
@@ -434,11 +459,11 @@ GQ_HD void hashjoin(const View<S> &q, const Rel &outer, const Rel &inner, const 
     startup_cost += 0.0f;   /* hash_qual_cost.startup (always 0), both branches */
     if (ex.inner_unique) {
         float outer_matched_rows = rintf(outer_path_rows * ex.outer_match_frac);
-        float inner_scan_frac = 2.0f / (ex.match_count + 1.0f);
+        const float inner_scan_frac = ex.scan_frac;
         run_cost += qual_per_tuple * outer_matched_rows *
                     clamp_row_est(inner_path_rows * bsize * inner_scan_frac) * 0.5f;
         run_cost += qual_per_tuple * (outer_path_rows - outer_matched_rows) *
-                    clamp_row_est(inner_path_rows / virtualbuckets) * 0.05f;
+                    clamp_row_est(div_pow2(inner_path_rows, virtualbuckets, inv_vb)) * 0.05f;
         hashjointuples = outer_matched_rows;
     } else {
         run_cost += qual_per_tuple * outer_path_rows * clamp_row_est(inner_path_rows * bsize) * 0.5f;
@@ -532,8 +557,25 @@ struct EdgeProv {
     template <class S>
     GQ_HD float scan(const View<S> &q, const S &, const S &, bool buckets, float fb, float inv_fb, float *bsize,
                      float *mcvfreq) const {
-        if (buckets)
-            for (int i = 0; i < n_mi; i++) bucket_fold(q, (int)mi[i], fb, inv_fb, bsize, mcvfreq);
+        if (buckets) {
+            if (q.pairs) {
+                /* member m of a two-member class e is m = 2e + j, and its
+                 * bucket constants are pair_rec[3e + 1 + j] (one 16-byte
+                 * read; the 0.1 default members give exactly 0.1) */
+                for (int i = 0; i < n_mi; i++) {
+                    const int m = (int)mi[i];
+                    const PairRec mm = q.pair_rec[3 * (m >> 1) + 1 + (m & 1)];
+                    float est = mm.x > fb ? inv_fb : mm.y;
+                    est *= mm.z;
+                    if (est < 1.0e-6f) est = 1.0e-6f;
+                    else if (est > 1.0f) est = 1.0f;
+                    if (*bsize > est) *bsize = est;
+                    if (*mcvfreq > mm.w) *mcvfreq = mm.w;
+                }
+            } else {
+                for (int i = 0; i < n_mi; i++) bucket_fold(q, (int)mi[i], fb, inv_fb, bsize, mcvfreq);
+            }
+        }
         return sel;
     }
     GQ_HD int n_quals() const { return nq; }
@@ -555,12 +597,14 @@ GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     HashSize hsz;
     hsz.nbuckets = 0;
     hsz.nbatches = 0;
-    float vbuckets = 0.0f, fb = 0.0f, inv_fb = 0.0f, bsize = 1.0f, mcvfreq = 1.0f;
+    float vbuckets = 0.0f, inv_vb = 0.0f, fb = 0.0f, inv_fb = 0.0f, bsize = 1.0f, mcvfreq = 1.0f;
     if (hj) {
         hsz = R.hs ? hash_unpack(R.hs) : hash_table_size(R.rows, R.width, q.p.work_mem);
         vbuckets = (float)hsz.nbuckets * (float)hsz.nbatches;
+        inv_vb = recip_pow2(vbuckets);         /* sizes are powers of two: exact reciprocals */
         fb = (float)f2i_x86(vbuckets);         /* the int nbuckets of estimate_hash_bucket_stats */
-        inv_fb = 1.0f / fb;
+        inv_fb = recip_pow2(fb);
+        if (inv_fb == 0.0f) inv_fb = 1.0f / fb;
     }
     out.rows = join_rows(pv.scan(q, l, r, hj, fb, inv_fb, &bsize, &mcvfreq), L.rows, R.rows);
     out.width = L.width + R.width;
@@ -578,13 +622,18 @@ GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, cons
     }
     ex.indexed = false;
     ex.joinrows = out.rows;
-    ex.outer_match_frac = out.rows / (L.rows * R.rows);
     ex.match_count = R.rows;
+    ex.outer_match_frac = 0.0f;
+    ex.scan_frac = 0.0f;
+    if (ex.inner_unique) {
+        ex.outer_match_frac = out.rows / (L.rows * R.rows);
+        ex.scan_frac = 2.0f / (ex.match_count + 1.0f);
+    }
     float bs, bt;
     nestloop(q, L, R.rows, R.startup, R.total, ex, qual_pt, &bs, &bt);
     if (hj) {
         float hs, ht;
-        hashjoin(q, L, R, ex, n_quals, qual_pt, hsz, vbuckets, bsize, mcvfreq, &hs, &ht);
+        hashjoin(q, L, R, ex, n_quals, qual_pt, hsz, vbuckets, inv_vb, bsize, mcvfreq, &hs, &ht);
         if (ht < bt) { bs = hs; bt = ht; }
     }
     if (single && !bit(q.composite, ctz(r))) {       /* index nested loop :116-147 */

@@ -170,14 +170,28 @@ struct MemoRef {
 /* Rank memo of a tree graph whose every vertex has a lower-numbered parent
  * (BlobHdr::is_tree): connected set S with lowest member t has slot
  *     off[t] + sum over u in S - t of Q[t][u]
- * (rk = off[rn] then Q[rn][rn]; the tables are built by attach_rank_tables,
- * which shows that this numbers the connected sets 0, 1, ... without gaps). */
-__host__ __device__ __forceinline__ uint32_t rank_slot(const uint32_t *rk, int rn, uint32_t S) {
+ * (rk = off[rn], Q[rn][rn], inv[rn], tz[rn], then the byte tables of W; the
+ * tables are built by attach_rank_tables, which shows that this numbers the
+ * connected sets 0, 1, ... without gaps).  The sum over S - t is
+ * W(S) / w[t] - 1 with W(S) = sum over u in S of Q[0][u] (w[t] = Q[0][t],
+ * w[0] = 1), four table lookups instead of a loop over S. */
+__device__ __forceinline__ uint32_t rank_wsum(const uint32_t *rk, int rn, uint32_t x) {
+    const uint32_t *bt = rk + rn + rn * rn + 2 * rn;
+    uint32_t w = bt[x & 255u];
+    if (rn > 8) w += bt[256 + ((x >> 8) & 255u)];
+    if (rn > 16) w += bt[512 + ((x >> 16) & 255u)];
+    if (rn > 24) w += bt[768 + (x >> 24)];
+    return w;
+}
+/* 1 + the sum over u in x - t of Q[t][u], t the lowest member of the
+ * connected set x */
+__device__ __forceinline__ uint32_t rank_digits(const uint32_t *rk, int rn, uint32_t x, int t) {
+    const uint32_t *inv = rk + rn + rn * rn, *tz = inv + rn;
+    return (rank_wsum(rk, rn, x) >> tz[t]) * inv[t];
+}
+__device__ __forceinline__ uint32_t rank_slot(const uint32_t *rk, int rn, uint32_t S) {
     const int t = __builtin_ctz(S);
-    const uint32_t *q = rk + rn + t * rn;
-    uint32_t r = rk[t];
-    for (uint32_t x = S & (S - 1u); x; x &= x - 1u) r += q[__builtin_ctz(x)];
-    return r;
+    return rk[t] + rank_digits(rk, rn, S, t) - 1u;
 }
 
 __device__ __forceinline__ uint32_t memo_hash(uint32_t x) {
@@ -425,8 +439,10 @@ __global__ void __launch_bounds__(BLOCK) k_filter_all(BlobHdr h, const uint32_t 
 /* DPsub, split-parallel: a wave owns `ipl * 64` consecutive split ranks of
  * one set; lane t scans ranks [j0 + t*ipl, j0 + (t+1)*ipl) in order.  Each
  * lane keeps (first valid rank, NaN flag) and the first minimum among
- * non-NaN costs; both reduce associatively across lanes and waves. */
-__global__ void __launch_bounds__(BLOCK) k_eval_dpsub_chunk(BlobHdr h, const uint32_t *blob, int k, uint32_t cps,
+ * non-NaN costs; both reduce associatively across lanes and waves.  Five
+ * waves per SIMD (at most 96 VGPRs, no spills; left alone the compiler
+ * takes 103 and four waves: clique-18 DPsub 29.2 against 31.1 ms). */
+__global__ void __launch_bounds__(BLOCK, 5) k_eval_dpsub_chunk(BlobHdr h, const uint32_t *blob, int k, uint32_t cps,
                                                              uint32_t ipl, const uint32_t *lvl_sets,
                                                              unsigned long long *lvl_best, uint32_t *lvl_first,
                                                              const float4 *vals, const uint32_t *conn,
@@ -963,11 +979,9 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
             const uint32_t rest = S ^ sub;
             uint32_t ss, sr;
             if (rk) {                                   /* slots of sub (lowest member c) and of S - sub */
-                const uint32_t *qc = rk + rn + c * rn;
-                uint32_t sig = 0;
-                for (uint32_t y = sub & ~(1u << c); y; y &= y - 1u) sig += qc[__builtin_ctz(y)];
-                ss = rk[c] + sig;
-                sr = rS - rk[rn + __builtin_ctz(S) * rn + c] * (1u + sig);
+                const uint32_t m1 = rank_digits(rk, rn, sub, c);   /* 1 + sig */
+                ss = rk[c] + m1 - 1u;
+                sr = rS - rk[rn + __builtin_ctz(S) * rn + c] * m1;
             } else {
                 ss = memo_slot(mr, sub);
                 sr = memo_slot(mr, rest);
@@ -1539,8 +1553,10 @@ __device__ uint32_t eval_level(const BlobHdr &h, uint32_t *smem, const View<uint
     return npairs;
 }
 
+/* Five waves per SIMD: at most 96 VGPRs, without spills (98-100 left alone,
+ * four waves). */
 template <bool BICC, bool JRANK = false>
-__global__ void __launch_bounds__(BLOCK) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
+__global__ void __launch_bounds__(BLOCK, 5) k_eval_wg(BlobHdr h, const uint32_t *blob, int k, uint32_t B,
                                                     int conn_words_lds, const uint32_t *lvl_sets, float4 *vals,
                                                     uint32_t *left, const uint32_t *conn, uint32_t *counters,
                                                     BigList bl) {
@@ -2021,8 +2037,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
  * sizes of tree queries (a device total above the grid would leave tasks
  * undone and raises error bit 32 of job 0).  A kernel of its own with only
  * this evaluator and no task loop keeps few registers and many waves per
- * SIMD.  Pairs are not counted: every split of a tree set is valid. */
-__global__ void __launch_bounds__(BLOCK) k_batch_eval_tree(const BatchJob *jobs, int nj, int k_fixed,
+ * SIMD: at least four per SIMD (at most 128 VGPRs; left alone the compiler
+ * takes 135, three waves, and the served mix plans 11-13% slower).  Pairs
+ * are not counted: every split of a tree set is valid. */
+__global__ void __launch_bounds__(BLOCK, 4) k_batch_eval_tree(const BatchJob *jobs, int nj, int k_fixed,
                                                             const uint32_t *tpg) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const uint32_t t = blockIdx.x;
@@ -3504,7 +3522,17 @@ int resolve_memo(const DPInput &in, int requested, int max_memo_mb) {
  * A split at edge (p(c), c) of set S with top t gives the part below c,
  * sub, with slot off[c] + sig, sig = sum over u in sub - c of Q[c][u], and
  * since Q[t][u] = Q[t][c] Q[c][u] below c, the rest has slot
- * slot(S) - Q[t][c] (1 + sig). */
+ * slot(S) - Q[t][c] (1 + sig).
+ *
+ * sig without a loop over sub: with w[u] = Q[0][u] (w[0] = 1), every u
+ * below c has w[u] = w[c] Q[c][u], so W(sub) = sum over u in sub of w[u]
+ * = w[c] (1 + sig).  W is linear in the bits of sub: four byte tables
+ * B[j][v] = sum of w over the bits of v at byte j give it in four lookups.
+ * W(sub) is below 2^31 (it is at most the slot of the set made of sub and
+ * the path from 0 to c, and every slot is), so the u32 sums are exact, and
+ * the exact quotient 1 + sig = W / w[c] is (W >> tz[c]) * inv[c] mod 2^32,
+ * with w[c] = 2^tz[c] times an odd number whose inverse mod 2^32 is
+ * inv[c].  The blob gains inv[n], tz[n] and B[ceil(n / 8)][256] after Q. */
 static void attach_rank_tables(DPInput &in) {
     BlobHdr &h = in.hdr;
     const int n = h.n_rels;
@@ -3522,7 +3550,8 @@ static void attach_rank_tables(DPInput &in) {
         N[v] = prod;
     }
     const int o = (int)b.size();
-    b.resize((size_t)o + n + (size_t)n * n, 0u);
+    const int nbt = (n + 7) / 8;
+    b.resize((size_t)o + n + (size_t)n * n + 2 * (size_t)n + 256 * (size_t)nbt, 0u);
     uint64_t acc = 0;
     for (int t = 0; t < n; t++) {
         b[o + t] = (uint32_t)acc;
@@ -3532,6 +3561,17 @@ static void attach_rank_tables(DPInput &in) {
             if (parent[u] == t) q[u] = (uint32_t)M[u];
             else if (q[parent[u]]) q[u] = (uint32_t)((uint64_t)q[parent[u]] * M[u]);
         }
+    }
+    uint32_t *inv = b.data() + o + n + (size_t)n * n, *tz = inv + n, *bt = tz + n;
+    for (int c = 0; c < n; c++) {
+        const uint32_t w = c ? b[o + n + c] : 1u;     /* Q[0][c]: never 0, every vertex lies below 0 */
+        tz[c] = (uint32_t)__builtin_ctz(w);
+        const uint32_t odd = w >> tz[c];
+        uint32_t x = odd;                               /* Newton: each step doubles the correct low bits */
+        for (int i = 0; i < 5; i++) x *= 2u - odd * x;
+        inv[c] = x;
+        for (int v = 0; v < 256; v++)
+            if ((v >> (c & 7)) & 1) bt[(c >> 3) * 256 + v] += w;
     }
     while (b.size() & 3) b.push_back(0u);
     h.o_rank = o;
