@@ -1277,7 +1277,9 @@ __global__ void __launch_bounds__(WAVE) k_tree_start(BlobHdr h, const uint32_t *
 }
 
 /* One DP's device state, as the batch kernels and the big-set kernels see it
- * (a single-query run is a batch of one). */
+ * (a single-query run is a batch of one).  No kernel writes a descriptor (they
+ * write through its pointers only), so the evaluation kernels take the
+ * array as __restrict__ and read its fields with scalar loads. */
 struct BatchJob {
     BlobHdr h;
     const uint32_t *blob;
@@ -1639,7 +1641,7 @@ __global__ void __launch_bounds__(1024) k_big_scan(const BigEnt *ents, uint32_t 
  * per wave and merges them into the set's lvl_first / lvl_best with global
  * atomics.  The workgroup finishing a set's last chunk (counted in
  * BigEnt::done) rebuilds the winning split and writes the memo. */
-__global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *jobs, BigEnt *ents, const uint32_t *hdr,
+__global__ void __launch_bounds__(BLOCK) k_big_eval(const BatchJob *__restrict__ jobs, BigEnt *ents, const uint32_t *hdr,
                                                      const uint32_t *prefix) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const uint32_t nb = hdr[1];
@@ -1943,8 +1945,8 @@ __global__ void __launch_bounds__(1024) k_step_plan(const BatchJob *jobs, int nj
  * engine).  `tpg` is the task prefix of this launch; it is staged in LDS at
  * word `tp_lds` (0: read from global memory).  Non-tree MPDP sets with more
  * than BIG_SPLITS splits go to the big-set list (`big`, `big_hdr`). */
-__global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int nj, int k_fixed,
-                                                       const uint32_t *tpg, uint32_t tp_lds, BigEnt *big,
+__global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *__restrict__ jobs, int nj, int k_fixed,
+                                                       const uint32_t *__restrict__ tpg, uint32_t tp_lds, BigEnt *big,
                                                        uint32_t big_cap, uint32_t *big_hdr) {
     /* The dynamic LDS must be the kernel's only LDS: a static __shared__
      * variable would move its base off the 8-byte alignment that the 64-bit
@@ -2037,11 +2039,14 @@ __global__ void __launch_bounds__(BLOCK) k_batch_eval(const BatchJob *jobs, int 
  * sizes of tree queries (a device total above the grid would leave tasks
  * undone and raises error bit 32 of job 0).  A kernel of its own with only
  * this evaluator and no task loop keeps few registers and many waves per
- * SIMD: at least four per SIMD (at most 128 VGPRs; left alone the compiler
- * takes 135, three waves, and the served mix plans 11-13% slower).  Pairs
- * are not counted: every split of a tree set is valid. */
-__global__ void __launch_bounds__(BLOCK, 4) k_batch_eval_tree(const BatchJob *jobs, int nj, int k_fixed,
-                                                            const uint32_t *tpg) {
+ * SIMD: five (at most 96 VGPRs, no spills).  The job descriptors are
+ * read-only here (__restrict__), so the job's header, memo and list
+ * pointers are scalar loads kept in SGPRs; without that the compiler keeps
+ * them in VGPRs (135 of them, three waves per SIMD, and the served mix
+ * plans 20% slower).  Pairs are not counted: every split of a tree set is
+ * valid. */
+__global__ void __launch_bounds__(BLOCK, 5) k_batch_eval_tree(const BatchJob *__restrict__ jobs, int nj,
+                                                               int k_fixed, const uint32_t *__restrict__ tpg) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const uint32_t t = blockIdx.x;
     const uint32_t total = tpg[nj];
@@ -2120,8 +2125,8 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
  * sizes; jobs with neither keys nor rank tables have no tasks.  A task belongs to one job, so the whole workgroup works for it.
  * Order within a level list is immaterial: each set's best split is
  * computed independently. */
-__global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *jobs, int nj, int k_fixed,
-                                                   const uint32_t *prefix) {
+__global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *__restrict__ jobs, int nj, int k_fixed,
+                                                   const uint32_t *__restrict__ prefix) {
     __shared__ uint32_t s_adj[32];
     __shared__ uint32_t part[17];
     const uint32_t total = prefix[nj];
