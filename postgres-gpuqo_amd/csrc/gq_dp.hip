@@ -1298,6 +1298,7 @@ struct BatchJob {
     uint32_t hmask;
     uint32_t *lvl_slot;
     uint32_t tree_iters;  /* wave-iterations of sets per tree task (Tuning::tree_iters when the job was made) */
+    uint32_t expand;      /* dense-memo tree job whose level lists come from k_expand (no filter) */
 };
 
 __device__ __forceinline__ MemoRef job_memo(const BatchJob &J) {
@@ -2122,7 +2123,10 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
  * present would be a duplicate: error bit 16).  Level k is level k_fixed of
  * every job, or each job's J.cur_k when k_fixed = 0; `prefix[j]` = first
  * task (BLOCK source sets) of job j, host-computed from the exact level
- * sizes; jobs with neither keys nor rank tables have no tasks.  A task belongs to one job, so the whole workgroup works for it.
+ * sizes; jobs with neither keys nor rank tables have no tasks, except
+ * dense-memo tree jobs marked `expand` (their slot is the set itself, so
+ * only the list is written).  A task belongs to one job, so the whole
+ * workgroup works for it.
  * Order within a level list is immaterial: each set's best split is
  * computed independently. */
 __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *__restrict__ jobs, int nj, int k_fixed,
@@ -2142,7 +2146,7 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *__restrict__ j
         const int n = J.h.n_rels;
         const uint32_t src_count = (k - 1 == 1) ? (uint32_t)n : J.counters[k - 1];
         const uint32_t i = (t - prefix[lo]) * BLOCK + threadIdx.x;
-        const bool have = i < src_count && (J.keys != nullptr || J.h.o_rank != 0);
+        const bool have = i < src_count && (J.keys != nullptr || J.h.o_rank != 0 || J.expand != 0);
         const uint32_t S = have ? ((k - 1 == 1) ? (1u << i) : J.lvl_sets[J.lvl_off[k - 1] + i]) : 0u;
         /* rank memo: the source's slot (off[i] for a singleton); each child
          * S + v, v above every member, has slot slot(S) + Q[top(S)][v] */
@@ -2170,9 +2174,9 @@ __global__ void __launch_bounds__(BLOCK) k_expand(const BatchJob *__restrict__ j
                     continue;
                 }
                 J.lvl_slot[J.lvl_off[k] + pos] = slot;
-            } else {
+            } else if (rq) {
                 J.lvl_slot[J.lvl_off[k] + pos] = sl + rq[J.h.n_rels + __builtin_ctz(S) * J.h.n_rels + v];
-            }
+            }                                      /* dense memo: the slot is the set */
             J.lvl_sets[J.lvl_off[k] + pos] = T;
             pos++;
         }
@@ -4069,8 +4073,9 @@ struct EJob {
     BatchJob J;
     int words;
     bool hash;
+    bool expand;                         /* level lists by k_expand (hash, rank, dense trees): counts exact */
     uint64_t cap;                        /* open-addressing capacity */
-    std::vector<uint64_t> counts;        /* hash: connected sets per size */
+    std::vector<uint64_t> counts;        /* connected sets per size (tree graphs) */
     std::vector<uint32_t> blob;          /* query blob, copied to the device by the DP's first step */
     int memo_kind = GQ_MEMO_DENSE;
 };
@@ -4309,6 +4314,7 @@ static bool engine_admit_hash(StepEngine *e, const DPInput &in, DPAlgo algo, uin
     j->words = h.words;
     j->blob.assign(in.blob.begin(), in.blob.begin() + h.words);
     j->hash = true;
+    j->expand = true;
     j->memo_kind = rank ? GQ_MEMO_RANK : GQ_MEMO_HASH;
     j->cap = rank ? 0 : cap;
     j->counts = in.tree_counts;
@@ -4382,6 +4388,12 @@ bool engine_admit(StepEngine *e, const DPInput &in, DPAlgo algo, uint64_t key) {
     j->hash = false;
     j->cap = 0;
     j->counts = in.tree_counts;
+    /* a tree with the per-edge split table and exact level sizes builds each
+     * level by expansion as it reaches it, instead of filtering all 2^n
+     * integers when admitted (its evaluator never reads the connectivity
+     * bitmap) */
+    j->expand = fast_tree_job(h) && (int)in.tree_counts.size() == n + 1;
+    J.expand = j->expand ? 1u : 0u;
     e->incoming.push_back(j);
     return true;
 }
@@ -4415,11 +4427,11 @@ int engine_enqueue(StepEngine *e, int nn) {
     const int j_new = na - nn;
     std::vector<uint32_t> fin_idx;
     size_t blob_words = 0;
-    bool any_hash = false;
+    bool any_expand = false;
     for (int i = 0; i < na; i++) {
         const EJob *j = e->active[i];
         if (j->cur_k >= j->n) fin_idx.push_back((uint32_t)i);
-        any_hash = any_hash || j->hash;
+        any_expand = any_expand || j->expand;
         if (i >= j_new) blob_words += j->blob.size();
     }
     const int nf = (int)fin_idx.size();
@@ -4464,7 +4476,7 @@ int engine_enqueue(StepEngine *e, int nn) {
             return GQ_ERR_UNSUPPORTED;
         }
         pttp[i + 1] = (uint32_t)ub_tree;
-        const uint64_t src = j->hash ? j->counts[j->cur_k - 1] : 0;
+        const uint64_t src = j->expand ? j->counts[j->cur_k - 1] : 0;
         pexp[i + 1] = pexp[i] + (uint32_t)((src + BLOCK - 1) / BLOCK);
         if (j->J.bicc == 1 && !h.is_tree && j->cur_k >= 15) {
             big_cap += binom64(j->n, j->cur_k);
@@ -4480,8 +4492,8 @@ int engine_enqueue(StepEngine *e, int nn) {
     for (int i = 0; i < nn; i++) {
         const EJob *j = e->active[j_new + i];
         const uint64_t ns = 1ull << j->n;
-        /* open-addressing jobs build their level lists by expansion */
-        pchunk[i + 1] = pchunk[i] + (j->hash ? 0u : (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK));
+        /* expansion jobs build their level lists step by step */
+        pchunk[i + 1] = pchunk[i] + (j->expand ? 0u : (uint32_t)((ns + FILT_CHUNK - 1) / FILT_CHUNK));
         pbsrc[i] = (uint32_t)bw;
         memcpy(pblob + bw, j->blob.data(), 4 * j->blob.size());
         bw += j->blob.size();
@@ -4522,8 +4534,8 @@ int engine_enqueue(StepEngine *e, int nn) {
             hipLaunchKernelGGL(k_batch_filter, dim3(fgrid), dim3(BLOCK), 0, s, d_jobs + j_new, nn,
                                (const uint32_t *)(D + o_chunk));
     }
-    /* 2. open-addressing jobs generate the level they evaluate now */
-    if (any_hash)
+    /* 2. expansion jobs generate the level they evaluate now */
+    if (any_expand)
         hipLaunchKernelGGL(k_expand, dim3(std::max<uint32_t>(1, std::min<uint32_t>(pexp[na], 4096))), dim3(BLOCK), 0,
                            s, d_jobs, na, 0, (const uint32_t *)(D + o_exp));
     /* 3. one level of every job: tree jobs on s (k_batch_eval_tree, task
@@ -4592,7 +4604,7 @@ int engine_collect(StepEngine *e, int si, std::vector<EngineDone> &done) {
         d.rc = GQ_OK;
         const int n = j->n;
         bool counts_ok = true;
-        if (j->hash)
+        if (j->expand)
             for (int k = 2; k <= n; k++) counts_ok = counts_ok && cnt[k] == j->counts[k];
         if (cnt[49]) {
             d.rc = (cnt[49] & 1u) ? GQ_ERR_UNSUPPORTED : GQ_ERR_DEVICE;
@@ -4602,7 +4614,7 @@ int engine_collect(StepEngine *e, int si, std::vector<EngineDone> &done) {
             const DPNode *nodes = (const DPNode *)(cnt + 64);
             d.tree.assign(nodes, nodes + 2 * n - 1);
             for (int k = 2; k <= n; k++) d.st.sets += cnt[k];
-            d.st.candidates = j->hash ? d.st.sets : (1ull << n) - (uint64_t)n - 1ull;
+            d.st.candidates = j->expand ? d.st.sets : (1ull << n) - (uint64_t)n - 1ull;
             d.st.pairs = (uint64_t)cnt[40] | ((uint64_t)cnt[41] << 32);
             if (j->J.h.is_tree && (int)j->counts.size() == n + 1) d.st.pairs = tree_pairs(j->counts);
             d.st.memo_bytes = j->bytes;
