@@ -1,15 +1,18 @@
 /*
  * gq_wide.h -- host-side interface of the wide DP: DPsub / MPDP over one
- * tree query of 33 to 64 relations on 64-bit relation sets (gq_wide.hip).
+ * query of 33 to 64 relations on 64-bit relation sets (gq_wide.hip).
  * Internal to the library; the public C ABI is include/gpuqo_amd.h.
  *
  * The reference runs its GPU DPsub over Bitmapset64 for queries of up to 63
  * relations (gpuqo_main_internal.cu:178-184), unranking every k-subset of
  * every level (gpuqo_dpsub.cuh:95-126) and filtering the connected ones.
- * Here the level lists hold only the connected sets, each built once from a
- * set of the level below; which queries that is practical for is set by
- * their number of connected sets, so the wide DP takes tree join graphs,
- * whose connected sets per size the host counts exactly.
+ * Here the level lists hold only the connected sets; which queries that is
+ * practical for is set by their number of connected sets.  Tree join
+ * graphs build each set once from a set of the level below, with level
+ * sizes the host counts exactly; other graphs (cycles, chords) build each
+ * set from every set of the level below that it extends, keep the copy
+ * whose memo insert comes first, and learn each level's size after it is
+ * built.
  */
 #ifndef GQ_WIDE_H
 #define GQ_WIDE_H
@@ -37,6 +40,7 @@ struct HQuery;
  *   o_lglob  per vertex v, the vertices of the subtrees of the smaller
  *            siblings of v and of each of its ancestors (u64);
  *   o_kids   per vertex v, its children (u64);
+ *   o_adj    per vertex v, its neighbours (u64; every graph);
  *   o_tedge  (tree_fast) per vertex c, the split at edge (parent(c), c):
  *            [0] selectivity with c's side inner, [1] with it outer,
  *            [2] n_quals | n_members << 8, [4..5] / [6..7] the inner member
@@ -51,12 +55,15 @@ struct WideHdr {
     int pairs, o_pair_rec;
     int o_desc, o_lglob, o_kids;
     int tree_fast, o_tedge;
+    int is_tree;    /* a tree numbered parents first: the tables above are set */
+    int o_adj;
 };
 
 struct WideInput {
     WideHdr hdr;
     std::vector<uint32_t> blob;
-    /* connected sets of each size k = 0..n (subtree-count polynomial) */
+    /* trees: connected sets of each size k = 0..n (subtree-count
+     * polynomial); empty for other graphs */
     std::vector<uint64_t> counts;
 };
 

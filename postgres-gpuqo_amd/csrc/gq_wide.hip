@@ -63,6 +63,8 @@ namespace gq {
 #define WW 64                 /* wave */
 #define WBLOCK 256            /* threads per workgroup of kw_level */
 #define WGRID_MAX 8192u       /* workgroups of a kw_level launch (they loop over the level) */
+#define WG_ECW 4              /* class-mask words of the general path (256 classes) */
+#define WG_BLOCK_MAX 24       /* largest biconnected block that is not a simple cycle (2^m - 2 subsets) */
 
 /* ------------------------------------------------------------ host side */
 
@@ -74,29 +76,24 @@ int build_wide_input(const HQuery<1> &q, WideInput &in) {
     }
     int edges = 0;
     for (int v = 0; v < n; v++) edges += __builtin_popcountll(q.adj[v].w[0]);
-    if (edges != 2 * (n - 1)) {
-        set_error("a single DP over %d relations needs a tree join graph (this one has %d edges); plan larger "
-                  "cyclic queries with gpuqo_idp_type / gpuqo_idp_n_iters", n, edges / 2);
-        return GQ_ERR_UNSUPPORTED;
-    }
+    /* a tree numbered parents first (gq_plan's BFS relabelling does that,
+     * and so does the ascending renumbering of a subtree of such a tree in
+     * IDP2) takes the tree tables; any other connected graph the general
+     * path */
+    bool tree = edges == 2 * (n - 1);
     std::vector<int> parent(n, -1);
-    for (int v = 1; v < n; v++) {
+    for (int v = 1; v < n && tree; v++) {
         const uint64_t lower = q.adj[v].w[0] & ((1ull << v) - 1ull);
-        if (__builtin_popcountll(lower) != 1) {
-            /* gq_plan's BFS relabelling numbers parents first, and so does
-             * the ascending renumbering of a subtree of such a tree (IDP2) */
-            set_error("the wide DP needs a tree numbered parents first (relation %d of %d has %d lower "
-                      "neighbours)", v, n, __builtin_popcountll(lower));
-            return GQ_ERR_UNSUPPORTED;
-        }
-        parent[v] = __builtin_ctzll(lower);
+        if (__builtin_popcountll(lower) != 1) tree = false;
+        else parent[v] = __builtin_ctzll(lower);
     }
     HostView<1> hv;
     hv.build(q);
     const View<Set<1>> &v = hv.v;
     const int n_ec = v.n_ec, ecw = v.ecw, nm = (int)q.vars.size();
-    if (ecw > GQ_MAX_ECW) {
-        set_error("GPU DP supports at most %d equivalence classes (got %d)", 64 * GQ_MAX_ECW, n_ec);
+    if (ecw > (tree ? GQ_MAX_ECW : WG_ECW)) {
+        set_error("the wide DP supports at most %d equivalence classes on %s (got %d)",
+                  64 * (tree ? GQ_MAX_ECW : WG_ECW), tree ? "trees" : "graphs with cycles", n_ec);
         return GQ_ERR_UNSUPPORTED;
     }
     WideHdr &h = in.hdr;
@@ -173,6 +170,16 @@ int build_wide_input(const HQuery<1> &q, WideInput &in) {
     align(4);
     h.o_pair_rec = take(h.pairs ? 12 * n_ec : 4);
     if (h.pairs) memcpy(&b[h.o_pair_rec], hv.pair_rec.data(), sizeof(PairRec) * 3 * (size_t)n_ec);
+    h.o_adj = take(2 * 64);
+    for (int u = 0; u < n; u++) put_u64(h.o_adj, u, q.adj[u].w[0]);
+    h.is_tree = tree ? 1 : 0;
+    if (!tree) {
+        h.tree_fast = 0;
+        align(4);
+        h.words = (int)b.size();
+        in.counts.clear();
+        return GQ_OK;
+    }
     /* tree tables: descendants, left-global sets, children */
     std::vector<uint64_t> desc(n, 0), lg(n, 0), kids(n, 0);
     for (int u = n - 1; u >= 0; u--) {
@@ -628,6 +635,430 @@ __global__ void __launch_bounds__(WW) kw_walk(WTab t, int n, WideNode *out, uint
     if (lane == 0) misc[WERR + 1] = fe;
 }
 
+/* ------------------------------------------------- general join graphs
+ *
+ * Sets of a graph with cycles: a set T of k + 1 relations is an extension
+ * S + v of each of its connected k-subsets S, so it is generated once per
+ * such S; the first to claim its memo key appends it to level k + 1
+ * (winsert_fresh).  Each set is evaluated by one wave:
+ *   1. lane 0 splits it into biconnected blocks in find_blocks emission
+ *      order (gpuqo_cpu_dpsub_bicc.cu:65-169; dfs_blocks64);
+ *   2. the lanes take the blocks' splits strided: a simple cycle's arcs, or
+ *      every non-empty proper subset lb of another block whose two parts
+ *      are connected inside it (valid exactly then: a path between two
+ *      members of a block never leaves it), with both sides grown by what
+ *      hangs off their members (block_side64) -- the reference's
+ *      grow(lowbit(lb), S - rb) / grow(lowbit(rb), S - lb) (:180-232);
+ *   3. the crossing classes of a split are the classes on both sides, from
+ *      per-member and per-block incidence masks the lanes build first;
+ *   4. a butterfly reduction keeps (first valid split, with its NaN flag)
+ *      and (first minimum of cost, rank) -- rank the split's place in the
+ *      block order (MPDP) or its left side (DPsub, whose splits come in
+ *      ascending left side, gpuqo_cpu_dpsub.cu:32-79; a valid split of
+ *      DPsub has both sides connected, so it is one of these block
+ *      splits) -- and the lane holding the winner writes the memo.
+ * Blocks that are not simple cycles take at most WG_BLOCK_MAX relations
+ * (WERR_BIG otherwise). */
+#define WERR_BIG 8u        /* a block above WG_BLOCK_MAX that is not a cycle, or MPDP ranks past 64 bits */
+#define WPAIRS 68          /* misc[68..69]: valid splits costed (u64) */
+
+struct alignas(16) WgScratch {           /* one wave's set, in LDS */
+    unsigned long long blocks[64], own[64], subb[65];
+    unsigned long long incown[64 * WG_ECW], incout[64 * WG_ECW];
+    unsigned long long cyc;
+    uint32_t dfs[64];
+    int nblk;
+};
+
+/* claims the slot of key; *fresh when this call inserted it */
+__device__ __forceinline__ uint32_t winsert_fresh(const WTab &t, uint64_t key, bool *fresh) {
+    uint32_t h = wslot0(key, t.lg);
+    for (uint32_t i = 0; i <= t.mask; i++) {
+        const unsigned long long old = atomicCAS(&t.s[h].key, 0ull, (unsigned long long)key);
+        if (old == 0ull || old == key) {
+            *fresh = old == 0ull;
+            return h;
+        }
+        h = (h + 1u) & t.mask;
+    }
+    *fresh = false;
+    return ~0u;
+}
+
+__device__ __forceinline__ uint64_t grow64(uint64_t from, uint64_t subset, const uint64_t *adj) {
+    uint64_t V = 0, N = from;
+    while (N) {
+        const int i = __builtin_ctzll(N);
+        V |= 1ull << i;
+        N = (N | adj[i]) & subset & ~V;
+    }
+    return V;
+}
+
+__device__ __forceinline__ uint64_t pdep64(uint64_t j, uint64_t mask) {
+    uint64_t r = 0;
+    for (uint64_t bit = 1; mask; mask &= mask - 1ull, bit <<= 1)
+        if (j & bit) r |= mask & (0ull - mask);
+    return r;
+}
+
+__device__ __forceinline__ uint64_t pext64(uint64_t x, uint64_t mask) {
+    uint64_t r = 0, bit = 1;
+    for (; mask; mask &= mask - 1ull, bit <<= 1)
+        if (x & mask & (0ull - mask)) r |= bit;
+    return r;
+}
+
+/* Biconnected blocks of connected S: the DFS from S's lowest member with
+ * ascending neighbours; a block is emitted when the tree edge (p, u)
+ * finishes with low[u] >= disc[p] -- find_blocks' order.  blocks[b] = the
+ * block, subb[b] = the vertices below its root p (its members but p and
+ * what hangs off them), own[v] = v and what hangs below it outside v's own
+ * block towards the root (own[root] = S); bit b of cyc: block b is a
+ * simple cycle.  dfs[v] packs discovery time, low and DFS parent (8 bits
+ * each); the vertex stack runs down from subb[64] (stack depth plus blocks
+ * emitted stays within |S| <= 64). */
+__device__ int dfs_blocks64(uint64_t S, const uint64_t *adj, WgScratch &W) {
+#define STK(i) W.subb[64 - (i)]
+    int nblk = 0;
+    const int root = __builtin_ctzll(S);
+    for (uint64_t x = S; x; x &= x - 1ull) W.own[__builtin_ctzll(x)] = 0ull;
+    uint64_t visited = 1ull << root;
+    uint32_t t = 1;
+    int sp = 0;
+    STK(sp++) = (unsigned long long)root;
+    uint32_t du = 0u | (255u << 8) | (255u << 16);
+    W.dfs[root] = du;
+    uint64_t au = adj[root] & S;
+    int u = root, cursor = -1;
+    while (true) {
+        const uint64_t gt = cursor < 0 ? ~0ull : (cursor >= 63 ? 0ull : (~0ull << (cursor + 1)));
+        const uint64_t rem = au & gt;
+        if (rem) {
+            const int v = __builtin_ctzll(rem);
+            if (!((visited >> v) & 1ull)) {
+                du = t | (255u << 8) | ((uint32_t)u << 16);
+                W.dfs[v] = du;
+                t++;
+                visited |= 1ull << v;
+                STK(sp++) = (unsigned long long)v;
+                au = adj[v] & S;
+                u = v;
+                cursor = -1;
+            } else {
+                if ((int)(du >> 16) != v) {       /* not the tree edge to u's parent */
+                    const uint32_t dv = W.dfs[v];
+                    if ((dv & 0xffu) < (du & 0xffu)) {
+                        const uint32_t low = (du >> 8) & 0xffu;
+                        const uint32_t nl = (dv & 0xffu) < low ? (dv & 0xffu) : low;
+                        du = (du & 0xffff00ffu) | (nl << 8);
+                        W.dfs[u] = du;
+                    }
+                }
+                cursor = v;
+            }
+            continue;
+        }
+        if (u == root) break;
+        const int p = (int)(du >> 16);
+        uint32_t dp = W.dfs[p];
+        const uint64_t ap = adj[p] & S;
+        const uint32_t lowu = (du >> 8) & 0xffu;
+        if (lowu >= (dp & 0xffu)) {
+            uint64_t rest = 0, sub = 0;
+            int v;
+            do {                                   /* pop down to u */
+                v = (int)STK(--sp);
+                const uint64_t o = W.own[v] | (1ull << v);
+                W.own[v] = o;
+                rest |= 1ull << v;
+                sub |= o;
+            } while (v != u);
+            W.blocks[nblk] = rest | (1ull << p);
+            W.subb[nblk] = sub;
+            nblk++;
+            W.own[p] |= sub;
+        }
+        const uint32_t lowp = (dp >> 8) & 0xffu;
+        if (lowu < lowp) {
+            dp = (dp & 0xffff00ffu) | (lowu << 8);
+            W.dfs[p] = dp;
+        }
+        cursor = u;
+        u = p;
+        du = dp;
+        au = ap;
+    }
+#undef STK
+    W.own[root] = S;
+    uint64_t cyc = 0;
+    for (int b = 0; b < nblk; b++) {
+        const uint64_t blk = W.blocks[b];
+        bool c = __builtin_popcountll(blk) >= 3;
+        for (uint64_t x = blk; c && x; x &= x - 1ull) c = __builtin_popcountll(adj[__builtin_ctzll(x)] & blk) == 2;
+        if (c) cyc |= 1ull << b;
+    }
+    W.cyc = cyc;
+    return nblk;
+}
+
+/* grow(lowbit(x), S - (blk - x)) for a part x of block blk (below-root set
+ * sb) that is connected inside the block */
+__device__ __forceinline__ uint64_t block_side64(uint64_t S, uint64_t x, uint64_t sb, const unsigned long long *own) {
+    uint64_t r = (x & ~sb) ? (S & ~sb) : 0ull;
+    for (uint64_t y = x & sb; y; y &= y - 1ull) r |= own[__builtin_ctzll(y)];
+    return r;
+}
+
+/* classes touching side x of block b (mask words m[0..ecw)) */
+__device__ __forceinline__ void side_classes(const WgScratch &W, int b, uint64_t x, uint64_t sb, int ecw,
+                                             uint64_t *m) {
+    for (int w = 0; w < ecw; w++) m[w] = (x & ~sb) ? W.incout[b * WG_ECW + w] : 0ull;
+    for (uint64_t y = x & sb; y; y &= y - 1ull) {
+        const int v = __builtin_ctzll(y);
+        for (int w = 0; w < ecw; w++) m[w] |= W.incown[v * WG_ECW + w];
+    }
+}
+
+__device__ __forceinline__ unsigned long long xor64(unsigned long long x, int o) {
+    const uint32_t lo = __shfl_xor((uint32_t)x, o), hi = __shfl_xor((uint32_t)(x >> 32), o);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+/* Base relations (level 1, list at 0) and the edges (level 2, list at n). */
+__global__ void __launch_bounds__(WW) kw_gbase(WideHdr h, const uint32_t *blob, WTab t, uint32_t *lst,
+                                               uint32_t cap2, uint32_t *misc) {
+    const int i = threadIdx.x;
+    if (i >= h.n_rels) return;
+    const uint64_t S = 1ull << i;
+    bool fresh;
+    const uint32_t sl = winsert_fresh(t, S, &fresh);
+    if (sl == ~0u) {
+        atomicOr(&misc[WERR], WERR_FULL);
+        return;
+    }
+    const float *bv = (const float *)(blob + h.o_base) + 4 * i;
+    Rel r;
+    r.rows = bv[0];
+    r.startup = bv[1];
+    r.total = bv[2];
+    r.width = __float_as_int(bv[3]);
+    r.hs = 0u;
+    t.s[sl].rec = record_of(r, h.p.work_mem);
+    t.s[sl].left = 0ull;
+    lst[i] = sl;
+    const uint64_t *adj = (const uint64_t *)(blob + h.o_adj);
+    const uint64_t up = i >= 63 ? 0ull : (adj[i] & (~0ull << (i + 1)));
+    for (uint64_t c = up; c; c &= c - 1ull) {
+        const uint32_t s2 = winsert_fresh(t, S | (c & (0ull - c)), &fresh);
+        const uint32_t pos = atomicAdd(&misc[2], 1u);
+        if (s2 == ~0u || pos >= cap2) {
+            atomicOr(&misc[WERR], WERR_FULL);
+            continue;
+        }
+        lst[h.n_rels + pos] = s2;
+    }
+}
+
+/* Level k of a general graph: one wave per set (see above); below the top
+ * level the set's extensions not yet in the memo become level k + 1 (list
+ * at off_n, room for cap_n). */
+template <bool DPSUB>
+__global__ void __launch_bounds__(WBLOCK) kw_glevel(WideHdr h, const uint32_t *blob, WTab t, uint32_t *lst, int k,
+                                                    uint32_t off_k, uint32_t n_k, uint32_t off_n, uint32_t cap_n,
+                                                    uint32_t *misc) {
+    extern __shared__ __align__(16) uint32_t wsm[];
+    for (int i = threadIdx.x; i < h.words; i += blockDim.x) wsm[i] = blob[i];
+    __syncthreads();
+    const View<uint64_t> q = make_wview(h, wsm);
+    const uint64_t *adj = (const uint64_t *)(wsm + h.o_adj);
+    const uint64_t *inc = q.inc;
+    const int ecw = h.ecw;
+    WgScratch &W = ((WgScratch *)(wsm + ((h.words + 3) & ~3)))[threadIdx.x / WW];
+    const uint32_t lane = threadIdx.x & (WW - 1);
+    const uint32_t wave = blockIdx.x * (WBLOCK / WW) + threadIdx.x / WW, nwaves = gridDim.x * (WBLOCK / WW);
+    const bool expand = k < h.n_rels;
+    const uint32_t NONE = 0xffffffffu;
+    for (uint32_t idx = wave; idx < n_k; idx += nwaves) {
+        const uint32_t slot = lst[off_k + idx];
+        const uint64_t S = t.s[slot].key;
+        if (lane == 0) W.nblk = dfs_blocks64(S, adj, W);
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const int nblk = W.nblk;
+        /* incidence of what each member owns and of each block's outside */
+        if ((S >> lane) & 1ull) {
+            for (int w = 0; w < ecw; w++) {
+                uint64_t x = 0;
+                for (uint64_t y = W.own[lane]; y; y &= y - 1ull) x |= inc[__builtin_ctzll(y) * ecw + w];
+                W.incown[lane * WG_ECW + w] = x;
+            }
+        }
+        if ((int)lane < nblk) {
+            const uint64_t out = S & ~W.subb[lane];
+            for (int w = 0; w < ecw; w++) {
+                uint64_t x = 0;
+                for (uint64_t y = out; y; y &= y - 1ull) x |= inc[__builtin_ctzll(y) * ecw + w];
+                W.incout[lane * WG_ECW + w] = x;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        /* the set's enumerated splits and the rank room of the blocks */
+        unsigned long long total = 0, rsum = 0;
+        bool big = false;
+        for (int b = 0; b < nblk; b++) {
+            const int m = __builtin_popcountll(W.blocks[b]);
+            const bool cy = (W.cyc >> b) & 1ull;
+            if (!cy && m > WG_BLOCK_MAX) big = true;
+            else {
+                total += cy ? (unsigned long long)m * (m - 1) : (1ull << m) - 2ull;
+                /* MPDP ranks count 2^m - 2 per block: a block of 64 (only a
+                 * cycle through every relation) or a sum past 64 bits has
+                 * no rank room; DPsub ranks by the left side instead */
+                if (!DPSUB) {
+                    const unsigned long long f = m >= 64 ? ~0ull : (1ull << m) - 2ull;
+                    if (m >= 64 || rsum + f < rsum) big = true;
+                    rsum += f;
+                }
+            }
+        }
+        if (big) {
+            if (lane == 0) atomicOr(&misc[WERR], WERR_BIG);
+            continue;
+        }
+        /* per lane: first valid split (rank, NaN flag, record, left) and the
+         * first minimum of (cost, rank) over non-NaN costs */
+        unsigned long long fr = ~0ull, mr = ~0ull, fl = 0, ml = 0;
+        bool fnan = false;
+        uint32_t mo = NONE, npairs = 0;
+        Rel frec, mrec;
+        int cb = 0;
+        unsigned long long acc_i = 0, acc_r = 0;
+        for (unsigned long long it = lane; it < total; it += WW) {
+            uint64_t blk = W.blocks[cb];
+            int m = __builtin_popcountll(blk);
+            bool cy = (W.cyc >> cb) & 1ull;
+            unsigned long long items = cy ? (unsigned long long)m * (m - 1) : (1ull << m) - 2ull;
+            while (it >= acc_i + items) {               /* lanes move through the blocks in order */
+                acc_i += items;
+                acc_r += m >= 64 ? 0ull : (1ull << m) - 2ull;
+                cb++;
+                blk = W.blocks[cb];
+                m = __builtin_popcountll(blk);
+                cy = (W.cyc >> cb) & 1ull;
+                items = cy ? (unsigned long long)m * (m - 1) : (1ull << m) - 2ull;
+            }
+            const unsigned long long local = it - acc_i;
+            uint64_t lb;
+            unsigned long long rank;
+            if (cy) {        /* arc (start, len) from the lowest member towards its lower neighbour */
+                const uint32_t m1 = (uint32_t)m - 1u;
+                const uint32_t start = (uint32_t)(local / m1), len = (uint32_t)(local % m1) + 1u;
+                uint64_t cur = blk & (0ull - blk), prev = 0;
+                lb = 0;
+                for (uint32_t st = 0; st < start + len; st++) {
+                    if (st >= start) lb |= cur;
+                    const uint64_t nb = adj[__builtin_ctzll(cur)] & blk & ~prev;
+                    prev = cur;
+                    cur = nb & (0ull - nb);
+                }
+                rank = acc_r + pext64(lb, blk) - 1ull;
+            } else {
+                lb = pdep64(local + 1ull, blk);
+                rank = acc_r + local;
+            }
+            if (m >= 64) rank = 0;                      /* DPsub only: ranked by L below */
+            const uint64_t rb = blk ^ lb;
+            if (!cy && m > 2 &&
+                (grow64(lb & (0ull - lb), lb, adj) != lb || grow64(rb & (0ull - rb), rb, adj) != rb))
+                continue;
+            const uint64_t sb = W.subb[cb];
+            const uint64_t L = block_side64(S, lb, sb, W.own), R = block_side64(S, rb, sb, W.own);
+            if (DPSUB) rank = L;
+            const uint32_t il = wfind(t, L), ir = wfind(t, R);
+            if (il == ~0u || ir == ~0u) {
+                atomicOr(&misc[WERR], WERR_PROBE);
+                continue;
+            }
+            uint64_t mL[WG_ECW], mR[WG_ECW], mm[GQ_MAX_ECW];
+            side_classes(W, cb, lb, sb, ecw, mL);
+            side_classes(W, cb, rb, sb, ecw, mR);
+            int nq = 0;
+            for (int w = 0; w < ecw; w++) {
+                mm[w] = mL[w] & mR[w];
+                nq += __builtin_popcountll(mm[w]);
+            }
+            const Rel r = join_mask<0>(q, L, rel_of(t.s[il].rec), R, rel_of(t.s[ir].rec), mm, nq);
+            npairs++;
+            const bool nan = r.total != r.total;
+            if (rank < fr) {
+                fr = rank;
+                fnan = nan;
+                frec = r;
+                fl = L;
+            }
+            if (!nan) {
+                const uint32_t o = ord_key(r.total);
+                if (o < mo || (o == mo && rank < mr)) {
+                    mo = o;
+                    mr = rank;
+                    mrec = r;
+                    ml = L;
+                }
+            }
+        }
+        /* butterfly: every lane ends with the set's two winners */
+        unsigned long long rfr = fr, rmr = mr;
+        bool rfnan = fnan;
+        uint32_t rmo = mo, rp = npairs;
+        for (int o = 1; o < WW; o <<= 1) {
+            const unsigned long long xfr = xor64(rfr, o), xmr = xor64(rmr, o);
+            const bool xfn = __shfl_xor((int)rfnan, o) != 0;
+            const uint32_t xmo = __shfl_xor(rmo, o);
+            rp += __shfl_xor(rp, o);
+            if (xfr < rfr) {
+                rfr = xfr;
+                rfnan = xfn;
+            }
+            if (xmo < rmo || (xmo == rmo && xmr < rmr)) {
+                rmo = xmo;
+                rmr = xmr;
+            }
+        }
+        const bool nanfirst = rfnan || rmo == NONE;
+        if (nanfirst ? (rfr != ~0ull && fr == rfr) : (mr == rmr)) {
+            t.s[slot].rec = record_of(nanfirst ? frec : mrec, h.p.work_mem);
+            t.s[slot].left = nanfirst ? fl : ml;
+        }
+        if (lane == 0 && rp) atomicAdd((unsigned long long *)(misc + WPAIRS), (unsigned long long)rp);
+        if (!expand) continue;
+        /* extensions S + v, v a neighbour of S: the first claim of a key
+         * appends the set to level k + 1 */
+        uint64_t nb = ((S >> lane) & 1ull) ? adj[lane] : 0ull;
+        for (int o = 1; o < WW; o <<= 1) nb |= xor64(nb, o);
+        const uint64_t C = nb & ~S;
+        const uint32_t nc = (uint32_t)__builtin_popcountll(C);
+        for (uint32_t base = 0; base < nc; base += WW) {
+            bool fresh = false;
+            uint32_t s2 = ~0u;
+            if (base + lane < nc) {
+                s2 = winsert_fresh(t, S | (1ull << select64(C, base + lane)), &fresh);
+                if (s2 == ~0u) atomicOr(&misc[WERR], WERR_FULL);
+            }
+            const unsigned long long bal = __ballot(fresh);
+            uint32_t first = 0;
+            if (lane == 0 && bal) first = atomicAdd(&misc[k + 1], (uint32_t)__popcll(bal));
+            first = __shfl(first, 0);
+            if (fresh) {
+                const uint32_t pos = first + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                if (pos < cap_n) lst[off_n + pos] = s2;
+                else atomicOr(&misc[WERR], WERR_FULL);
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------ runner */
 
 struct WideBufs {
@@ -692,6 +1123,123 @@ static void launch_level(const WideHdr &h, uint32_t grid, size_t lds, hipStream_
                        off_n, cap_n, misc);
 }
 
+/* A wide DP over a graph with cycles: the level sizes are learnt one level
+ * at a time (the host reads each level's count before launching it), in a
+ * memo of 2^lg 32-byte slots and a list of 2^(lg-1) entries sized from
+ * gpuqo_max_memo_size_mb; a DP that outgrows them fails with GQ_ERR_NOMEM.
+ * The interrupt flag is tested before every level. */
+static int run_wide_general(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<WideNode> &nodes,
+                            DPStats &st) {
+    const WideHdr &h = in.hdr;
+    const int n = h.n_rels;
+    const size_t lds = (((size_t)h.words * 4 + 15) & ~(size_t)15) + (WBLOCK / WW) * sizeof(WgScratch);
+    if (lds > 64 * 1024) {
+        set_error("query description too large for LDS (%zu bytes)", lds);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    const uint64_t budget = (uint64_t)std::max(max_memo_mb, 1) << 20;
+    uint32_t lg = 8;
+    while (lg < 31 && (32ull << (lg + 1)) + (4ull << lg) <= budget) lg++;
+    const uint64_t slots = 1ull << lg, list_cap = slots / 2;
+    return with_device_stream([&](int device, void *sv) -> int {
+        hipStream_t s = (hipStream_t)sv;
+        WideBufs &w = g_wide;
+        int rc = wide_init(w, device);
+        if (rc) return rc;
+        if ((rc = wensure(&w.tab, w.cap_tab, (size_t)slots))) return rc;
+        if ((rc = wensure(&w.lst, w.cap_lst, (size_t)list_cap))) return rc;
+        if ((rc = wensure(&w.blob, w.cap_blob, in.blob.size()))) return rc;
+        const double t_first = host_now_ms();
+        WCHK(hipMemcpyAsync(w.blob, in.blob.data(), in.blob.size() * 4, hipMemcpyHostToDevice, s));
+        WCHK(hipMemsetAsync(w.tab, 0, sizeof(WSlot) * slots, s));
+        WCHK(hipMemsetAsync(w.misc, 0, 72 * 4, s));
+        WCHK(hipEventRecord(w.e0, s));
+        WTab t;
+        t.s = w.tab;
+        t.lg = lg;
+        t.mask = (uint32_t)(slots - 1u);
+        hipLaunchKernelGGL(kw_gbase, dim3(1), dim3(WW), 0, s, h, (const uint32_t *)w.blob, t, w.lst,
+                           (uint32_t)(list_cap - (uint64_t)n), w.misc);
+        int launches = 1;
+        uint64_t off_k = (uint64_t)n, sets = 0;
+        bool stopped = false;
+        for (int k = 2; k <= n; k++) {
+            WCHK(hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s));
+            WCHK(hipStreamSynchronize(s));
+            if (w.h_misc[WERR]) {          /* reported below; no further level or walk */
+                stopped = true;
+                break;
+            }
+            if (interrupted()) {
+                set_error("interrupted before level %d", k);
+                return GQ_ERR_INTERRUPTED;
+            }
+            const uint32_t n_k = w.h_misc[k];
+            if (n_k == 0 || (k == n && n_k != 1)) {
+                set_error("wide DP: %u connected sets of size %d", n_k, k);
+                return GQ_ERR_DEVICE;
+            }
+            sets += n_k;
+            const uint64_t off_n = off_k + n_k;
+            const uint32_t cap_n = k < n ? (uint32_t)(list_cap - std::min<uint64_t>(off_n, list_cap)) : 0u;
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(((uint64_t)n_k + WBLOCK / WW - 1) / (WBLOCK / WW), WGRID_MAX));
+            if (algo == DP_DPSUB)
+                hipLaunchKernelGGL(kw_glevel<true>, dim3(grid), dim3(WBLOCK), lds, s, h, (const uint32_t *)w.blob, t,
+                                   w.lst, k, (uint32_t)off_k, n_k, (uint32_t)off_n, cap_n, w.misc);
+            else
+                hipLaunchKernelGGL(kw_glevel<false>, dim3(grid), dim3(WBLOCK), lds, s, h, (const uint32_t *)w.blob,
+                                   t, w.lst, k, (uint32_t)off_k, n_k, (uint32_t)off_n, cap_n, w.misc);
+            launches++;
+            off_k = off_n;
+        }
+        if (!stopped) hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, n, w.nodes, w.misc);
+        WCHK(hipGetLastError());
+        WCHK(hipEventRecord(w.e1, s));
+        const double t_last = host_now_ms();
+        WCHK(hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s));
+        WCHK(hipMemcpyAsync(w.h_nodes, w.nodes, (size_t)(2 * n - 1) * sizeof(WideNode), hipMemcpyDeviceToHost, s));
+        WCHK(hipStreamSynchronize(s));
+        const double t_synced = host_now_ms();
+        const uint32_t err = w.h_misc[WERR];
+        if (err & WERR_FULL) {
+            set_error("the DP's connected sets outgrow the memo of gpuqo_max_memo_size_mb (%d)", max_memo_mb);
+            return GQ_ERR_NOMEM;
+        }
+        if (err & WERR_BIG) {
+            set_error("a set of the DP has a biconnected block of more than %d relations that is not a simple "
+                      "cycle (or, under MPDP, a cycle of all 64); plan it with gpuqo_idp_type / "
+                      "gpuqo_idp_n_iters", WG_BLOCK_MAX);
+            return GQ_ERR_UNSUPPORTED;
+        }
+        if (err) {
+            set_error("wide DP memo inconsistency (error bits %u)", err);
+            return GQ_ERR_DEVICE;
+        }
+        if (w.h_misc[WERR + 1] != (uint32_t)(2 * n - 1)) {
+            set_error("wide DP plan walk returned %u nodes", w.h_misc[WERR + 1]);
+            return GQ_ERR_DEVICE;
+        }
+        nodes.assign(w.h_nodes, w.h_nodes + (2 * n - 1));
+        float ms = 0.0f;
+        WCHK(hipEventElapsedTime(&ms, w.e0, w.e1));
+        st = DPStats();
+        st.candidates = sets;
+        st.sets = sets;
+        st.pairs = (uint64_t)w.h_misc[WPAIRS] | ((uint64_t)w.h_misc[WPAIRS + 1] << 32);
+        st.memo_bytes = (32ull << lg) + 4ull * list_cap;
+        st.memo_kind = GQ_MEMO_HASH;
+        st.gpu_ms = ms;
+        st.eval_ms = ms;
+        st.eval_launches = launches;
+        st.t_first_launch = t_first;
+        st.t_last_launch = t_last;
+        st.t_synced = t_synced;
+        return GQ_OK;
+    });
+}
+
+
 int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<WideNode> &nodes, DPStats &st) {
     const WideHdr &h = in.hdr;
     const int n = h.n_rels;
@@ -699,6 +1247,7 @@ int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<W
         set_error("DPsize runs on the GPU for at most %d relations (got %d)", GQ_MAX_DP_RELS, n);
         return GQ_ERR_UNSUPPORTED;
     }
+    if (!h.is_tree) return run_wide_general(in, algo, max_memo_mb, nodes, st);
     const size_t lds = ((size_t)h.words * 4 + 15) & ~(size_t)15;
     if (lds > 64 * 1024) {
         set_error("query description too large for LDS (%zu bytes)", lds);

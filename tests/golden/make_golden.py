@@ -13,6 +13,7 @@ and child indices; plus the pair and set counters.
     python tests/golden/make_golden.py --trees    # plans_trees_v1.json
     python tests/golden/make_golden.py --served   # plans_served_v1.json
     python tests/golden/make_golden.py --wide     # plans_wide_v1.json
+    python tests/golden/make_golden.py --wide-cyclic   # plans_wide_cyclic_v1.json
 
 The large set holds the full-size workloads that take the oracle minutes:
 cpu_dpsub and cpu_dpsize on the 18-relation clique (about 3.9e8 csg-cmp
@@ -28,7 +29,10 @@ wide set holds single DPs of 33-63 relations (tree join graphs: chains, a
 hub of chains, deep snowflakes; an IDP2 run whose sub-DP has 36
 relations) under DPsub and MPDP, planned by oracle.plan_connected (the
 oracle visiting only the connected sets, which gives the same plans as its
-2^n pass): about 3 minutes of oracle time.
+2^n pass): about 3 minutes of oracle time.  The wide cyclic set holds
+single DPs of 34-40 relations whose join graphs have cycles: a 20-cycle
+with a 14-relation tail, 4-cycles joined by paths, a hub of three chains
+with triangles; under DPsub and MPDP, by oracle.plan_connected.
 """
 import hashlib
 import json
@@ -115,6 +119,38 @@ WIDE_CASES = [
 ]
 
 
+def _squares_edges(n):
+    """A path 0..n-1 with a chord (i, i + 3) at every i divisible by 6: 4-cycles
+    joined by paths of three edges."""
+    return [[i, i + 1] for i in range(n - 1)] + [[i, i + 3] for i in range(0, n - 3, 6)]
+
+
+def _hub_triangle_edges(arm):
+    """Relation 0 with three chains of `arm` relations; in each chain a
+    chord between its 2nd and 4th relations and one between its 7th and
+    9th (triangles), and a chord between the first relations of chains one
+    and two (a triangle with the hub)."""
+    edges, firsts = [], []
+    for a in range(3):
+        base = 1 + a * arm
+        firsts.append(base)
+        edges.append([0, base])
+        edges += [[base + i, base + i + 1] for i in range(arm - 1)]
+        edges += [[base + 1, base + 3], [base + 6, base + 8]]
+    edges.append([firsts[0], firsts[1]])
+    return edges
+
+
+WIDE_CYCLIC_CASES = [
+    ("cycle20tail14-mpdp", "cycle_with_tail", [20, 14, 0x2014], {}, MPDP),
+    ("cycle20tail14-dpsub", "cycle_with_tail", [20, 14, 0x2014], {}, DPSUB),
+    ("squares36-mpdp", "graph", [36, _squares_edges(36), 0x36, "squares"], {}, MPDP),
+    ("squares36-dpsub", "graph", [36, _squares_edges(36), 0x36, "squares"], {}, DPSUB),
+    ("hubtri40-mpdp", "graph", [40, _hub_triangle_edges(13), 0x40, "hub-triangles"], {}, MPDP),
+    ("hubtri40-dpsub", "graph", [40, _hub_triangle_edges(13), 0x40, "hub-triangles"], {}, DPSUB),
+]
+
+
 def make_wide(label, gen, args, kwargs, over):
     g = getattr(qgen, gen)(*args, **kwargs)
     st, nodes, stats = oracle.plan_connected(g, abi.default_config(**over))
@@ -129,14 +165,15 @@ def _make_wide(c):
     return make_wide(*c)
 
 
-def main_wide(procs):
+def main_wide(procs, cyclic=False):
     from multiprocessing import Pool
     with Pool(procs) as pool:
-        recs = pool.map(_make_wide, WIDE_CASES, chunksize=1)
-    out = {"format": "postgres-gpuqo_amd golden plans v1 (single DPs of 33-63 relations)",
+        recs = pool.map(_make_wide, WIDE_CYCLIC_CASES if cyclic else WIDE_CASES, chunksize=1)
+    what = "with cycles, 34-40" if cyclic else "33-63"
+    out = {"format": f"postgres-gpuqo_amd golden plans v1 (single DPs of {what} relations)",
            "float_encoding": "IEEE-754 binary32 bit patterns",
            "records": recs}
-    path = os.path.join(HERE, "plans_wide_v1.json")
+    path = os.path.join(HERE, "plans_wide_cyclic_v1.json" if cyclic else "plans_wide_v1.json")
     with open(path, "w") as f:
         json.dump(out, f, separators=(",", ":"))
         f.write("\n")
@@ -224,6 +261,9 @@ def main():
         return
     if "--large" in sys.argv:
         main_large(int(os.environ.get("GOLDEN_PROCS", "6")))
+        return
+    if "--wide-cyclic" in sys.argv:
+        main_wide(int(os.environ.get("GOLDEN_PROCS", "6")), cyclic=True)
         return
     if "--wide" in sys.argv:
         main_wide(int(os.environ.get("GOLDEN_PROCS", "6")))

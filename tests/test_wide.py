@@ -187,12 +187,23 @@ def test_gpu_wide_in_a_batch_matches_alone():
         assert_same_plan(gpuqo_amd.plan_bits(nodes), gpuqo_amd.plan_bits(alone), q.shape)
 
 
+def chord_block_graph():
+    """34 relations: a 25-cycle with the chord (0, 12) -- one biconnected
+    block of 25 that is not a simple cycle -- and a 9-relation tail."""
+    edges = [(i, (i + 1) % 25) for i in range(25)] + [(0, 12)]
+    prev = 0
+    for i in range(25, 34):
+        edges.append((prev, i))
+        prev = i
+    return qgen.graph(34, edges, 0x3425)
+
+
 @pytest.mark.gpu
 def test_gpu_wide_unsupported_cases():
     tree = qgen.snowflake(40, 0x40, depth=6, fanout=2)
-    with pytest.raises(gpuqo_amd.GpuqoError) as e:          # a cyclic join graph above 32 relations
-        gpuqo_amd.plan(qgen.cycle(40, 1), abi.default_config(algorithm="dpsub", dpsub_bicc=1))
-    assert e.value.status == abi.GQ_ERR_UNSUPPORTED and "tree join graph" in str(e.value)
+    with pytest.raises(gpuqo_amd.GpuqoError) as e:          # a block of 25 that is not a cycle
+        gpuqo_amd.plan(chord_block_graph(), abi.default_config(algorithm="dpsub", dpsub_bicc=1))
+    assert e.value.status == abi.GQ_ERR_UNSUPPORTED and "biconnected block" in str(e.value)
     with pytest.raises(gpuqo_amd.GpuqoError) as e:          # DPsize above 32 relations
         gpuqo_amd.plan(tree, abi.default_config(algorithm="dpsize"))
     assert e.value.status == abi.GQ_ERR_UNSUPPORTED
