@@ -296,6 +296,53 @@ gq_status gq_server_poll(gq_server *s, int32_t wait, int32_t *done,
 gq_status gq_server_destroy(gq_server *s);
 
 /*
+ * Memo snapshots.  They replace the reference's per-node HashTable::get
+ * walk of the memo (gpuqo_hashtable.cu:148-178, dpsub_buildQueryTree in
+ * gpuqo_query_tree.cuh:46-72) with a device-resident copy of the whole
+ * memo: every connected set of the query (the base relations included),
+ * with its best join -- outer side, rows, startup and total cost, width --
+ * in the caller's relation numbering, kept in device memory until
+ * gq_snapshot_free.  The plan of any connected sub-query (the join of a
+ * connected subset of the relations) is then walked from it on the device
+ * without planning again: that entry is the DP's optimum for those
+ * relations alone.  A snapshot can be written out in the format below and
+ * read back, on this process or another.
+ *
+ * gq_plan_snapshot: gq_plan (the same plan, bit for bit, in *out) plus the
+ * snapshot of its DP.  One DPsub or MPDP DP of 2..64 relations: no IDP
+ * (gpuqo_idp_n_iters), no DPsize (GQ_ERR_UNSUPPORTED).
+ * gq_snapshot_subplan: the plan of the connected sub-query `relids` (one
+ * word, GQ_MAX_WORDS read as gq_query's; bits outside the query's
+ * relations or a set that is not connected give GQ_ERR_INVALID); stats
+ * are zero.
+ * gq_snapshot_export: *bytes = the export's size; with buf and cap >=
+ * *bytes, writes it (GQ_ERR_CAPACITY when cap is smaller).
+ * gq_snapshot_import: a snapshot from an export (GQ_ERR_INVALID when the
+ * buffer is not one).
+ *
+ * Export format, little-endian:
+ *   bytes 0..7   magic "GQMEMO1" and a zero byte;
+ *   8..11        version (1);       12..15  relations n (1..64);
+ *   16..23       records m;         24..27  bytes per record (32);
+ *   28..63       zero;
+ *   then m records of 32 bytes, ordered by relation count, then by set:
+ *     u64 set (bit i = the caller's relation i), u64 outer side of the
+ *     set's best join (0 for a base relation), f32 rows, f32 startup cost,
+ *     f32 total cost, i32 width.
+ */
+typedef struct gq_snapshot gq_snapshot;
+#define GQ_SNAPSHOT_MAGIC "GQMEMO1"
+#define GQ_SNAPSHOT_VERSION 1u
+#define GQ_SNAPSHOT_HEADER_BYTES 64u
+#define GQ_SNAPSHOT_RECORD_BYTES 32u
+gq_status gq_plan_snapshot(const gq_query *q, const gq_config *cfg, gq_plan_out *out, gq_snapshot **snap);
+gq_status gq_snapshot_subplan(const gq_snapshot *snap, const uint64_t *relids, gq_plan_out *out);
+gq_status gq_snapshot_info(const gq_snapshot *snap, int32_t *n_rels, uint64_t *n_sets, uint64_t *device_bytes);
+gq_status gq_snapshot_export(const gq_snapshot *snap, void *buf, uint64_t cap, uint64_t *bytes);
+gq_status gq_snapshot_import(const void *buf, uint64_t bytes, gq_snapshot **snap);
+gq_status gq_snapshot_free(gq_snapshot *snap);
+
+/*
  * Launch-shape knobs of the device DP, per process.  None of them changes a
  * plan, a cost or a count.  They take the place of the reference's
  * hand-set gpuqo_n_parallel / gpuqo_scratchpad_size_mb /

@@ -3598,9 +3598,182 @@ void prepare_memo(DPInput &in, int requested, int max_memo_mb) {
     }
 }
 
+/* ------------------------------------------------ memo snapshot (32-bit DPs)
+ *
+ * gq_plan_snapshot's copy of a finished DP's memo (dense, rank or open
+ * addressing) into a table of 64-bit sets in the caller's numbering
+ * (SnapTable, gq_wide.h): the connected sets are enumerated again level by
+ * level, each from every set it extends (the first claim of its key keeps
+ * it), and each takes its record and left side from the memo. */
+struct Perm32 {
+    int8_t p[32];
+};
+
+__device__ __forceinline__ uint64_t perm32(uint32_t x, const Perm32 &pm) {
+    uint64_t r = 0;
+    for (; x; x &= x - 1u) r |= 1ull << pm.p[__builtin_ctz(x)];
+    return r;
+}
+
+/* cnt[k]: sets of size k copied; cnt[63]: error bits (1 table full or a
+ * set twice, 2 list overflow, 4 a memo probe missed) */
+__global__ void __launch_bounds__(WAVE) ks32_base(BlobHdr h, MemoRef mr, WTab to, Perm32 pm, uint32_t *lst,
+                                                  uint32_t *cnt) {
+    const int i = threadIdx.x;
+    if (i >= h.n_rels) return;
+    const uint32_t S = 1u << i;
+    bool fresh;
+    const uint32_t sl = winsert_fresh(to, perm32(S, pm), &fresh);
+    if (sl == ~0u || !fresh) {
+        atomicOr(&cnt[63], 1u);
+        return;
+    }
+    const uint32_t ms = memo_slot(mr, S);
+    to.s[sl].rec = mr.vals[ms];
+    to.s[sl].left = 0ull;
+    lst[i] = S;
+    atomicAdd(&cnt[1], 1u);
+}
+
+__global__ void __launch_bounds__(BLOCK) ks32_level(BlobHdr h, const uint32_t *blob, MemoRef mr, WTab to,
+                                                    Perm32 pm, const uint32_t *cur, uint32_t n_cur, uint32_t *nxt,
+                                                    uint32_t cap, uint32_t *cnt, int k) {
+    const uint32_t *adj = blob + h.o_adj;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n_cur; i += gridDim.x * BLOCK) {
+        const uint32_t S = cur[i];
+        uint32_t nb = 0;
+        for (uint32_t x = S; x; x &= x - 1u) nb |= adj[__builtin_ctz(x)];
+        for (uint32_t C = nb & ~S; C; C &= C - 1u) {
+            const uint32_t T = S | (C & (0u - C));
+            bool fresh;
+            const uint32_t sl = winsert_fresh(to, perm32(T, pm), &fresh);
+            if (sl == ~0u) {
+                atomicOr(&cnt[63], 1u);
+                continue;
+            }
+            if (!fresh) continue;
+            const uint32_t pos = atomicAdd(&cnt[k + 1], 1u);
+            if (pos >= cap) {
+                atomicOr(&cnt[63], 2u);
+                continue;
+            }
+            nxt[pos] = T;
+            const uint32_t ms = memo_slot(mr, T);
+            to.s[sl].rec = mr.vals[ms];
+            to.s[sl].left = perm32(mr.left[ms], pm);
+        }
+    }
+}
+
+/* With the device context held, right after run_dp_locked planned `in`:
+ * g_snap_req->out gets the memo's `total` connected sets. */
+static int dp_snapshot(DeviceCtx &c, const DPInput &in, uint64_t total) {
+    SnapReq &r = *g_snap_req;
+    const BlobHdr &h = in.hdr;
+    const int n = h.n_rels;
+    hipStream_t s = c.stream;
+    Perm32 pm;
+    memcpy(pm.p, r.perm, 32);
+    MemoRef mr = dense_memo(c);
+    if (in.memo == GQ_MEMO_HASH) {
+        uint64_t cap = 64;                         /* as run_dp_hash sized it */
+        while (cap < 2 * total) cap <<= 1;
+        mr.keys = c.hkeys;
+        mr.mask = (uint32_t)(cap - 1);
+    } else if (in.memo == GQ_MEMO_RANK) {
+        mr.rank = c.b.blob + h.o_rank;
+        mr.rn = n;
+    }
+    SnapTable &t = r.out;
+    t.lg = snap_lg(total);
+    t.n_rels = n;
+    t.device = c.device;
+    const size_t tbytes = sizeof(WSlot) << t.lg;
+    uint32_t *lst = nullptr, *cnt = nullptr;
+    if (hipMalloc(&t.tab, tbytes) != hipSuccess || hipMalloc((void **)&lst, 4 * total) != hipSuccess ||
+        hipMalloc((void **)&cnt, 64 * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        if (lst) (void)hipFree(lst);
+        snapshot_free(t);
+        set_error("hipMalloc for the memo snapshot failed");
+        return GQ_ERR_NOMEM;
+    }
+    mr.err = cnt + 63;
+    WTab to;
+    to.s = (WSlot *)t.tab;
+    to.lg = t.lg;
+    to.mask = (uint32_t)((1ull << t.lg) - 1ull);
+    uint32_t hc[64];
+    int rc = GQ_OK;
+    if (hipMemsetAsync(t.tab, 0, tbytes, s) != hipSuccess || hipMemsetAsync(cnt, 0, 64 * 4, s) != hipSuccess) {
+        rc = GQ_ERR_DEVICE;
+    } else {
+        hipLaunchKernelGGL(ks32_base, dim3(1), dim3(WAVE), 0, s, h, mr, to, pm, lst, cnt);
+        uint64_t off = 0, done = (uint64_t)n;
+        for (int k = 1; k < n && rc == GQ_OK; k++) {
+            if (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = GQ_ERR_DEVICE;
+                break;
+            }
+            if (hc[63]) break;
+            const uint32_t nk = hc[k];
+            const uint64_t nxt = off + nk;
+            if (nk == 0 || nxt > total) {
+                hc[63] |= 2u;
+                break;
+            }
+            hipLaunchKernelGGL(ks32_level, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(
+                                             ((uint64_t)nk + BLOCK - 1) / BLOCK, 4096))),
+                               dim3(BLOCK), 0, s, h, (const uint32_t *)c.b.blob, mr, to, pm, lst + off, nk,
+                               lst + nxt, (uint32_t)(total - nxt), cnt, k);
+            off = nxt;
+        }
+        if (rc == GQ_OK && (hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                            hipStreamSynchronize(s) != hipSuccess))
+            rc = GQ_ERR_DEVICE;
+        if (rc == GQ_OK) {
+            done = 0;
+            for (int k = 1; k <= n; k++) done += hc[k];
+            if (hc[63] || done != total) {
+                set_error("memo snapshot: %llu of %llu connected sets copied (error bits %u)",
+                          (unsigned long long)done, (unsigned long long)total, hc[63]);
+                rc = GQ_ERR_DEVICE;
+            }
+        } else {
+            set_error("memo snapshot: a device call failed");
+        }
+    }
+    (void)hipFree(lst);
+    (void)hipFree(cnt);
+    if (rc) {
+        snapshot_free(t);
+        return rc;
+    }
+    t.n_sets = total;
+    r.done = true;
+    return GQ_OK;
+}
+
+static int run_dp_locked(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile,
+                         std::vector<DPNode> &tree, DPStats &st, bool noalloc);
+
 int run_dp(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile, std::vector<DPNode> &tree,
            DPStats &st, bool noalloc) {
     std::lock_guard<std::mutex> g(g_ctx.mu);
+    int rc = run_dp_locked(in, algo, n_iters, memo_kind, profile, tree, st, noalloc);
+    if (rc == GQ_OK && g_snap_req && g_snap_req->n == in.hdr.n_rels) {
+        if (noalloc || algo == DP_DPSIZE || (n_iters >= 2 && n_iters < in.hdr.n_rels)) {
+            set_error("a memo snapshot takes one DPsub/MPDP DP without a level cap");
+            return GQ_ERR_UNSUPPORTED;
+        }
+        rc = dp_snapshot(g_ctx, in, st.sets + (uint64_t)in.hdr.n_rels);
+    }
+    return rc;
+}
+
+static int run_dp_locked(const DPInput &in, DPAlgo algo, int n_iters, int memo_kind, bool profile,
+                         std::vector<DPNode> &tree, DPStats &st, bool noalloc) {
     DeviceCtx &c = g_ctx;
     int rc = ctx_init(c, c.device < 0 ? 0 : c.device);
     if (rc) return rc;

@@ -24,6 +24,7 @@
 
 #include "gq_drivers.h"
 #include "gq_host.h"
+#include "gq_wide.h"
 
 namespace gq {
 
@@ -720,7 +721,72 @@ static int plan_t(const gq_query *q, const gq_config *cfg, gq_plan_out *out, boo
     return r;
 }
 
+/* ------------------------------------------------------------ snapshots */
+
+/* A snapshot's breadth-first plan nodes as a pre-order gq_plan_out. */
+static int emit_wide_nodes(const std::vector<WideNode> &nodes, gq_plan_out *out) {
+    std::vector<QT<1>> made(nodes.size());
+    for (size_t i = 0; i < nodes.size(); i++) {
+        made[i].id = Set<1>();
+        made[i].id.w[0] = nodes[i].set;
+        made[i].rows = nodes[i].rows;
+        made[i].startup = nodes[i].startup;
+        made[i].total = nodes[i].total;
+        made[i].width = nodes[i].width;
+        made[i].left = nodes[i].left >= 0 ? &made[nodes[i].left] : nullptr;
+        made[i].right = nodes[i].right >= 0 ? &made[nodes[i].right] : nullptr;
+    }
+    out->n_nodes = 0;
+    const int r = made.empty() ? -1 : emit(&made[0], out);
+    if (r == -2) {
+        set_error("output capacity %d below the plan's %zu nodes", out->capacity, nodes.size());
+        return GQ_ERR_CAPACITY;
+    }
+    return r < 0 ? GQ_ERR_INTERNAL : GQ_OK;
+}
+
+static int plan_snapshot_impl(const gq_query *q, const gq_config *cfg, gq_plan_out *out, SnapTable *snap) {
+    if (q->n_rels < 2 || q->n_rels > 64) {
+        set_error("a memo snapshot takes one DP of 2..64 relations (got %d)", q->n_rels);
+        return GQ_ERR_UNSUPPORTED;
+    }
+    if (cfg->gpuqo_algorithm != GQ_ALGO_DPSUB ||
+        (cfg->gpuqo_idp_n_iters > 1 && cfg->gpuqo_idp_n_iters < q->n_rels)) {
+        set_error("a memo snapshot takes one DPsub/MPDP DP (gpuqo_algorithm dpsub, no IDP)");
+        return GQ_ERR_UNSUPPORTED;
+    }
+    /* the DP numbering: the BFS relabelling plan_t will apply */
+    HQuery<1> h;
+    convert(q, cfg, h);
+    PlainPrepared pp;
+    int rc = prepare_plain(h, pp);
+    if (rc) return rc;
+    SnapReq req;
+    req.n = q->n_rels;
+    memset(req.perm, -1, sizeof(req.perm));
+    for (const auto &e : pp.bfs) req.perm[e.to] = (int8_t)ctz(e.from);
+    g_snap_req = &req;
+    rc = plan_t<1>(q, cfg, out);
+    g_snap_req = nullptr;
+    if (rc) {
+        snapshot_free(req.out);
+        return rc;
+    }
+    if (!req.done) {
+        snapshot_free(req.out);
+        set_error("no DP ran for the snapshot");
+        return GQ_ERR_INTERNAL;
+    }
+    *snap = req.out;
+    return GQ_OK;
+}
+
 }  // namespace gq
+
+/* The opaque snapshot handle of the C ABI. */
+struct gq_snapshot {
+    gq::SnapTable t;
+};
 
 /* ================================================================ C ABI */
 
@@ -1048,5 +1114,172 @@ gq_status gq_autotune(int32_t n_queries, const gq_query *const *qs, const gq_con
 const char *gq_last_error(void) { return gq::g_err; }
 
 const char *gq_version(void) { return "gpuqo-mi355x 0.1 (gfx950)"; }
+
+/* ------------------------------------------------------------ snapshots */
+
+gq_status gq_plan_snapshot(const gq_query *q, const gq_config *cfg, gq_plan_out *out, gq_snapshot **snap) {
+    return gq::guarded([&]() -> int {
+        if (!snap) {
+            gq::set_error("null snapshot pointer");
+            return GQ_ERR_INVALID;
+        }
+        *snap = nullptr;
+        int rc = gq::validate(q);
+        if (rc) return rc;
+        if (!cfg || !out || !out->nodes) {
+            gq::set_error("null config or output");
+            return GQ_ERR_INVALID;
+        }
+        gq::SnapTable t;
+        rc = gq::plan_snapshot_impl(q, cfg, out, &t);
+        if (rc) return rc;
+        gq_snapshot *h = new (std::nothrow) gq_snapshot;
+        if (!h) {
+            gq::snapshot_free(t);
+            gq::set_error("host allocation failed");
+            return GQ_ERR_NOMEM;
+        }
+        h->t = t;
+        *snap = h;
+        return GQ_OK;
+    });
+}
+
+gq_status gq_snapshot_subplan(const gq_snapshot *snap, const uint64_t *relids, gq_plan_out *out) {
+    return gq::guarded([&]() -> int {
+        if (!snap || !relids || !out || !out->nodes) {
+            gq::set_error("null snapshot, relation set or output");
+            return GQ_ERR_INVALID;
+        }
+        std::vector<gq::WideNode> nodes;
+        int rc = gq::snapshot_walk(snap->t, relids[0], nodes);
+        if (rc) return rc;
+        memset(&out->stats, 0, sizeof(out->stats));
+        return gq::emit_wide_nodes(nodes, out);
+    });
+}
+
+gq_status gq_snapshot_info(const gq_snapshot *snap, int32_t *n_rels, uint64_t *n_sets, uint64_t *device_bytes) {
+    if (!snap) {
+        gq::set_error("null snapshot");
+        return GQ_ERR_INVALID;
+    }
+    if (n_rels) *n_rels = snap->t.n_rels;
+    if (n_sets) *n_sets = snap->t.n_sets;
+    if (device_bytes) *device_bytes = snap->t.tab ? (32ull << snap->t.lg) : 0ull;
+    return GQ_OK;
+}
+
+gq_status gq_snapshot_export(const gq_snapshot *snap, void *buf, uint64_t cap, uint64_t *bytes) {
+    return gq::guarded([&]() -> int {
+        if (!snap || !bytes) {
+            gq::set_error("null snapshot or size pointer");
+            return GQ_ERR_INVALID;
+        }
+        const uint64_t need = GQ_SNAPSHOT_HEADER_BYTES + GQ_SNAPSHOT_RECORD_BYTES * snap->t.n_sets;
+        *bytes = need;
+        if (!buf) return GQ_OK;                 /* size query */
+        if (cap < need) {
+            gq::set_error("export buffer of %llu bytes, %llu needed", (unsigned long long)cap,
+                          (unsigned long long)need);
+            return GQ_ERR_CAPACITY;
+        }
+        std::vector<gq::SnapRecord> recs;
+        int rc = gq::snapshot_read(snap->t, recs);
+        if (rc) return rc;
+        if (recs.size() != snap->t.n_sets) {
+            gq::set_error("snapshot holds %zu records, %llu expected", recs.size(),
+                          (unsigned long long)snap->t.n_sets);
+            return GQ_ERR_INTERNAL;
+        }
+        uint8_t *p = (uint8_t *)buf;
+        memset(p, 0, GQ_SNAPSHOT_HEADER_BYTES);
+        memcpy(p, GQ_SNAPSHOT_MAGIC, 8);
+        const uint32_t version = GQ_SNAPSHOT_VERSION, nr = (uint32_t)snap->t.n_rels,
+                       rb = GQ_SNAPSHOT_RECORD_BYTES;
+        const uint64_t ns = recs.size();
+        memcpy(p + 8, &version, 4);
+        memcpy(p + 12, &nr, 4);
+        memcpy(p + 16, &ns, 8);
+        memcpy(p + 24, &rb, 4);
+        uint8_t *r = p + GQ_SNAPSHOT_HEADER_BYTES;
+        for (const gq::SnapRecord &x : recs) {
+            memcpy(r, &x.set, 8);
+            memcpy(r + 8, &x.left, 8);
+            memcpy(r + 16, &x.rows, 4);
+            memcpy(r + 20, &x.startup, 4);
+            memcpy(r + 24, &x.total, 4);
+            memcpy(r + 28, &x.width, 4);
+            r += GQ_SNAPSHOT_RECORD_BYTES;
+        }
+        return GQ_OK;
+    });
+}
+
+gq_status gq_snapshot_import(const void *buf, uint64_t bytes, gq_snapshot **snap) {
+    return gq::guarded([&]() -> int {
+        if (!buf || !snap) {
+            gq::set_error("null buffer or snapshot pointer");
+            return GQ_ERR_INVALID;
+        }
+        *snap = nullptr;
+        const uint8_t *p = (const uint8_t *)buf;
+        uint32_t version = 0, nr = 0, rb = 0;
+        uint64_t ns = 0;
+        if (bytes < GQ_SNAPSHOT_HEADER_BYTES || memcmp(p, GQ_SNAPSHOT_MAGIC, 8) != 0) {
+            gq::set_error("not a memo snapshot (magic)");
+            return GQ_ERR_INVALID;
+        }
+        memcpy(&version, p + 8, 4);
+        memcpy(&nr, p + 12, 4);
+        memcpy(&ns, p + 16, 8);
+        memcpy(&rb, p + 24, 4);
+        if (version != GQ_SNAPSHOT_VERSION || rb != GQ_SNAPSHOT_RECORD_BYTES || nr < 1 || nr > 64 || ns < nr ||
+            ns > (1ull << 30) || bytes != GQ_SNAPSHOT_HEADER_BYTES + GQ_SNAPSHOT_RECORD_BYTES * ns) {
+            gq::set_error("memo snapshot header: version %u, %u relations, %llu records of %u bytes in %llu bytes",
+                          version, nr, (unsigned long long)ns, rb, (unsigned long long)bytes);
+            return GQ_ERR_INVALID;
+        }
+        const uint64_t all = nr >= 64 ? ~0ull : ((1ull << nr) - 1ull);
+        std::vector<gq::SnapRecord> recs((size_t)ns);
+        const uint8_t *r = p + GQ_SNAPSHOT_HEADER_BYTES;
+        for (uint64_t i = 0; i < ns; i++, r += GQ_SNAPSHOT_RECORD_BYTES) {
+            gq::SnapRecord &x = recs[i];
+            memcpy(&x.set, r, 8);
+            memcpy(&x.left, r + 8, 8);
+            memcpy(&x.rows, r + 16, 4);
+            memcpy(&x.startup, r + 20, 4);
+            memcpy(&x.total, r + 24, 4);
+            memcpy(&x.width, r + 28, 4);
+            const bool base = __builtin_popcountll(x.set) == 1;
+            const bool left_ok = base ? x.left == 0ull : (x.left && !(x.left & ~x.set) && x.left != x.set);
+            if (!x.set || (x.set & ~all) || x.width < 0 || !left_ok) {
+                gq::set_error("memo snapshot record %llu is malformed", (unsigned long long)i);
+                return GQ_ERR_INVALID;
+            }
+        }
+        gq::SnapTable t;
+        int rc = gq::snapshot_build(recs, (int)nr, t);
+        if (rc) return rc;
+        gq_snapshot *h = new (std::nothrow) gq_snapshot;
+        if (!h) {
+            gq::snapshot_free(t);
+            gq::set_error("host allocation failed");
+            return GQ_ERR_NOMEM;
+        }
+        h->t = t;
+        *snap = h;
+        return GQ_OK;
+    });
+}
+
+gq_status gq_snapshot_free(gq_snapshot *snap) {
+    return gq::guarded([&]() -> int {
+        if (!snap) return GQ_OK;
+        gq::snapshot_free(snap->t);
+        delete snap;
+        return GQ_OK;
+    });
+}
 
 }  // extern "C"

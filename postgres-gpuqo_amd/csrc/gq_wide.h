@@ -75,8 +75,9 @@ struct WideNode {
     int32_t width, left, right;
 };
 
-/* A relabelled query (BFS order from relation 0) of 2..64 relations whose
- * join graph is a tree; GQ_ERR_UNSUPPORTED otherwise. */
+/* A relabelled query (BFS order from relation 0) of 2..64 relations: a
+ * tree numbered parents first, or any connected graph (the general path);
+ * GQ_ERR_UNSUPPORTED above the class limits. */
 int build_wide_input(const HQuery<1> &q, WideInput &in);
 
 /* Device bytes the DP takes: the open-addressing memo (32 B slots, load at
@@ -86,6 +87,54 @@ uint64_t wide_dp_bytes(const WideInput &in);
 /* Runs the DP (DPsub or MPDP order) on the library's device stream;
  * GQ_ERR_NOMEM when wide_dp_bytes exceeds max_memo_mb. */
 int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<WideNode> &nodes, DPStats &st);
+
+/* ---------------------------------------------------------- memo snapshots
+ *
+ * A finished DP's memo copied into an open-addressing table of WSlot
+ * records (gq_device.h) keyed by every connected set of the query in the
+ * caller's relation numbering, with the outer side of the set's chosen
+ * split in the same numbering (0 for base relations): the plan of every
+ * connected sub-query can be walked from it without planning again. */
+struct SnapTable {
+    void *tab = nullptr;      /* WSlot[2^lg] on `device` */
+    uint32_t lg = 0;
+    uint64_t n_sets = 0;      /* connected sets, the base relations included */
+    int n_rels = 0, device = -1;
+};
+
+/* Set by gq_plan_snapshot around the one DP of its call: DP relation i is
+ * the caller's relation perm[i].  run_dp and run_wide_dp, when the DP they
+ * finish has n relations, build `out` on the device before they return
+ * (and set done). */
+struct SnapReq {
+    int n = 0;
+    int8_t perm[64];
+    SnapTable out;
+    bool done = false;
+};
+extern thread_local SnapReq *g_snap_req;
+
+/* Slots for n_sets sets at a load of at most 1/2. */
+uint32_t snap_lg(uint64_t n_sets);
+
+/* Plan of the connected sub-query `root` (caller's numbering) walked from
+ * the snapshot, breadth-first (WideNode); GQ_ERR_INVALID if root is not one
+ * of its sets. */
+int snapshot_walk(const SnapTable &t, uint64_t root, std::vector<WideNode> &nodes);
+
+/* Every record of the snapshot, sorted by (size, set): set, left, rows,
+ * startup, total, width. */
+struct SnapRecord {
+    uint64_t set, left;
+    float rows, startup, total;
+    int32_t width;
+};
+int snapshot_read(const SnapTable &t, std::vector<SnapRecord> &recs);
+
+/* A snapshot on the device from records (an imported one). */
+int snapshot_build(const std::vector<SnapRecord> &recs, int n_rels, SnapTable &t);
+
+void snapshot_free(SnapTable &t);
 
 /* f(device, stream) under the device context's lock, on its stream
  * (gq_dp.hip) */

@@ -292,45 +292,6 @@ uint64_t wide_dp_bytes(const WideInput &in) {
 
 /* ------------------------------------------------------------ device side */
 
-struct alignas(16) WSlot {
-    unsigned long long key;    /* the set; 0 = empty */
-    unsigned long long left;   /* outer side of the chosen split (0 for base relations) */
-    float4 rec;                /* rows, startup, total, width word (rel_of / record_of) */
-};
-static_assert(sizeof(WSlot) == 32, "32-byte memo slots");
-
-struct WTab {
-    WSlot *s;
-    uint32_t mask, lg;
-};
-
-__device__ __forceinline__ uint32_t wslot0(uint64_t key, uint32_t lg) {
-    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64u - lg));
-}
-
-/* slot of a key inserted by an earlier launch; ~0u if it is missing */
-__device__ __forceinline__ uint32_t wfind(const WTab &t, uint64_t key) {
-    uint32_t h = wslot0(key, t.lg);
-    for (uint32_t i = 0; i <= t.mask; i++) {
-        const unsigned long long k = t.s[h].key;
-        if (k == key) return h;
-        if (k == 0ull) break;
-        h = (h + 1u) & t.mask;
-    }
-    return ~0u;
-}
-
-/* claims a slot for key (each set is generated once); ~0u if the table is full */
-__device__ __forceinline__ uint32_t winsert(const WTab &t, uint64_t key) {
-    uint32_t h = wslot0(key, t.lg);
-    for (uint32_t i = 0; i <= t.mask; i++) {
-        const unsigned long long old = atomicCAS(&t.s[h].key, 0ull, (unsigned long long)key);
-        if (old == 0ull || old == key) return h;
-        h = (h + 1u) & t.mask;
-    }
-    return ~0u;
-}
-
 /* position of the member of x that has j members below it (j < popcount(x)) */
 __device__ __forceinline__ int select64(uint64_t x, uint32_t j) {
     int p = 0;
@@ -571,12 +532,13 @@ __global__ void __launch_bounds__(WBLOCK) kw_level(WideHdr h, const uint32_t *bl
     }
 }
 
-/* The plan, breadth-first from the full set: node i's children are
- * allocated after the nodes of its frontier. */
-__global__ void __launch_bounds__(WW) kw_walk(WTab t, int n, WideNode *out, uint32_t *misc) {
+/* The plan, breadth-first from the set `root` (the full set, or a
+ * connected sub-query of a memo snapshot): node i's children are allocated
+ * after the nodes of its frontier. */
+__global__ void __launch_bounds__(WW) kw_walk(WTab t, uint64_t root, WideNode *out, uint32_t *misc) {
     __shared__ unsigned long long ns[128];
     const uint32_t lane = threadIdx.x;
-    if (lane == 0) ns[0] = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+    if (lane == 0) ns[0] = root;
     __syncthreads();
     uint32_t fb = 0, fe = 1;
     while (fb < fe) {
@@ -669,21 +631,6 @@ struct alignas(16) WgScratch {           /* one wave's set, in LDS */
     uint32_t dfs[64];
     int nblk;
 };
-
-/* claims the slot of key; *fresh when this call inserted it */
-__device__ __forceinline__ uint32_t winsert_fresh(const WTab &t, uint64_t key, bool *fresh) {
-    uint32_t h = wslot0(key, t.lg);
-    for (uint32_t i = 0; i <= t.mask; i++) {
-        const unsigned long long old = atomicCAS(&t.s[h].key, 0ull, (unsigned long long)key);
-        if (old == 0ull || old == key) {
-            *fresh = old == 0ull;
-            return h;
-        }
-        h = (h + 1u) & t.mask;
-    }
-    *fresh = false;
-    return ~0u;
-}
 
 __device__ __forceinline__ uint64_t grow64(uint64_t from, uint64_t subset, const uint64_t *adj) {
     uint64_t V = 0, N = from;
@@ -1115,6 +1062,239 @@ static int wide_init(WideBufs &w, int device) {
     return GQ_OK;
 }
 
+/* ------------------------------------------------------------ snapshots */
+
+thread_local SnapReq *g_snap_req = nullptr;
+
+static uint64_t full_set(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+
+uint32_t snap_lg(uint64_t n_sets) {
+    uint32_t lg = 6;
+    while (lg < 31 && (1ull << lg) < 2 * n_sets) lg++;
+    return lg;
+}
+
+struct Perm64 {
+    int8_t p[64];
+};
+
+__device__ __forceinline__ uint64_t perm64(uint64_t x, const Perm64 &pm) {
+    uint64_t r = 0;
+    for (; x; x &= x - 1ull) r |= 1ull << pm.p[__builtin_ctzll(x)];
+    return r;
+}
+
+/* A wide DP's memo into a snapshot table, keys and left sides renumbered. */
+__global__ void __launch_bounds__(WBLOCK) ks_remap(WTab from, WTab to, Perm64 pm, uint32_t *misc) {
+    const uint64_t n = (uint64_t)from.mask + 1ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * WBLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WBLOCK) {
+        const unsigned long long key = from.s[i].key;
+        if (!key) continue;
+        const uint32_t sl = winsert(to, perm64(key, pm));
+        if (sl == ~0u) {
+            atomicOr(&misc[WERR], WERR_FULL);
+            continue;
+        }
+        to.s[sl].rec = from.s[i].rec;
+        to.s[sl].left = perm64(from.s[i].left, pm);
+        atomicAdd(&misc[0], 1u);
+    }
+}
+
+/* Records into an empty table (an imported snapshot). */
+__global__ void __launch_bounds__(WBLOCK) ks_fill(WTab to, const WSlot *recs, uint64_t n, uint32_t *misc) {
+    for (uint64_t i = (uint64_t)blockIdx.x * WBLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WBLOCK) {
+        bool fresh;
+        const uint32_t sl = winsert_fresh(to, recs[i].key, &fresh);
+        if (sl == ~0u || !fresh) {
+            atomicOr(&misc[WERR], sl == ~0u ? WERR_FULL : WERR_PROBE);
+            continue;
+        }
+        to.s[sl].rec = recs[i].rec;
+        to.s[sl].left = recs[i].left;
+    }
+}
+
+static int snap_alloc(SnapTable &t, uint64_t n_sets, int n_rels, int device, hipStream_t s) {
+    t.lg = snap_lg(n_sets);
+    t.n_rels = n_rels;
+    t.device = device;
+    t.n_sets = 0;
+    const size_t bytes = sizeof(WSlot) << t.lg;
+    if (hipMalloc(&t.tab, bytes) != hipSuccess) {
+        t.tab = nullptr;
+        (void)hipGetLastError();
+        set_error("hipMalloc of %zu bytes for the memo snapshot failed", bytes);
+        return GQ_ERR_NOMEM;
+    }
+    if (hipMemsetAsync(t.tab, 0, bytes, s) != hipSuccess) {
+        set_error("memo snapshot: clearing its table failed");
+        return GQ_ERR_DEVICE;
+    }
+    return GQ_OK;
+}
+
+void snapshot_free(SnapTable &t) {
+    if (t.tab) (void)hipFree(t.tab);
+    t.tab = nullptr;
+    t.n_sets = 0;
+}
+
+static WTab snap_wtab(const SnapTable &t) {
+    WTab w;
+    w.s = (WSlot *)t.tab;
+    w.lg = t.lg;
+    w.mask = (uint32_t)((1ull << t.lg) - 1ull);
+    return w;
+}
+
+/* The wide DP's memo (total sets) into g_snap_req's table; called on the
+ * DP's stream with the device context held. */
+static int wide_snapshot(const WTab &memo, uint64_t total, int n, int device, hipStream_t s) {
+    SnapReq &r = *g_snap_req;
+    Perm64 pm;
+    memcpy(pm.p, r.perm, 64);
+    int rc = snap_alloc(r.out, total, n, device, s);
+    if (rc) return rc;
+    WideBufs &w = g_wide;
+    if (hipMemsetAsync(w.misc, 0, 72 * 4, s) != hipSuccess) return GQ_ERR_DEVICE;
+    const uint64_t slots = (uint64_t)memo.mask + 1ull;
+    hipLaunchKernelGGL(ks_remap, dim3((uint32_t)std::min<uint64_t>((slots + WBLOCK - 1) / WBLOCK, 4096)),
+                       dim3(WBLOCK), 0, s, memo, snap_wtab(r.out), pm, w.misc);
+    if (hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        set_error("memo snapshot: copying the memo failed");
+        return GQ_ERR_DEVICE;
+    }
+    if (w.h_misc[WERR] || w.h_misc[0] != total) {
+        set_error("memo snapshot: %u of %llu sets copied (error bits %u)", w.h_misc[0],
+                  (unsigned long long)total, w.h_misc[WERR]);
+        return GQ_ERR_DEVICE;
+    }
+    r.out.n_sets = total;
+    r.done = true;
+    return GQ_OK;
+}
+
+int snapshot_walk(const SnapTable &t, uint64_t root, std::vector<WideNode> &nodes) {
+    const int k = __builtin_popcountll(root);
+    if (!t.tab || !root || (t.n_rels < 64 && (root >> t.n_rels))) {
+        set_error("not a set of the snapshot's %d relations", t.n_rels);
+        return GQ_ERR_INVALID;
+    }
+    return with_device_stream([&](int device, void *sv) -> int {
+        if (device != t.device) {
+            set_error("the snapshot lives on device %d, the library on %d", t.device, device);
+            return GQ_ERR_UNSUPPORTED;
+        }
+        hipStream_t s = (hipStream_t)sv;
+        WideBufs &w = g_wide;
+        int rc = wide_init(w, device);
+        if (rc) return rc;
+        WCHK(hipMemsetAsync(w.misc, 0, 72 * 4, s));
+        hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, snap_wtab(t), (uint64_t)root, w.nodes, w.misc);
+        WCHK(hipGetLastError());
+        WCHK(hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s));
+        WCHK(hipMemcpyAsync(w.h_nodes, w.nodes, (size_t)(2 * k - 1) * sizeof(WideNode), hipMemcpyDeviceToHost, s));
+        WCHK(hipStreamSynchronize(s));
+        if (w.h_misc[WERR] || w.h_misc[WERR + 1] != (uint32_t)(2 * k - 1)) {
+            set_error("relation set %#llx is not a connected set of the snapshot",
+                      (unsigned long long)root);
+            return GQ_ERR_INVALID;
+        }
+        nodes.assign(w.h_nodes, w.h_nodes + (2 * k - 1));
+        return GQ_OK;
+    });
+}
+
+int snapshot_read(const SnapTable &t, std::vector<SnapRecord> &recs) {
+    recs.clear();
+    if (!t.tab) {
+        set_error("empty snapshot");
+        return GQ_ERR_INVALID;
+    }
+    std::vector<WSlot> h((size_t)1 << t.lg);
+    int rc = with_device_stream([&](int device, void *sv) -> int {
+        if (device != t.device) {
+            set_error("the snapshot lives on device %d, the library on %d", t.device, device);
+            return GQ_ERR_UNSUPPORTED;
+        }
+        hipStream_t s = (hipStream_t)sv;
+        WCHK(hipMemcpyAsync(h.data(), t.tab, h.size() * sizeof(WSlot), hipMemcpyDeviceToHost, s));
+        WCHK(hipStreamSynchronize(s));
+        return GQ_OK;
+    });
+    if (rc) return rc;
+    recs.reserve(t.n_sets);
+    for (const WSlot &x : h) {
+        if (!x.key) continue;
+        SnapRecord r;
+        r.set = x.key;
+        r.left = x.left;
+        r.rows = x.rec.x;
+        r.startup = x.rec.y;
+        r.total = x.rec.z;
+        uint32_t wb;
+        memcpy(&wb, &x.rec.w, 4);
+        r.width = (wb >> 31) ? (int32_t)(wb & 0x1fffffu) : (int32_t)wb;   /* rec_width */
+        recs.push_back(r);
+    }
+    std::sort(recs.begin(), recs.end(), [](const SnapRecord &a, const SnapRecord &b) {
+        const int pa = __builtin_popcountll(a.set), pb = __builtin_popcountll(b.set);
+        return pa != pb ? pa < pb : a.set < b.set;
+    });
+    return GQ_OK;
+}
+
+int snapshot_build(const std::vector<SnapRecord> &recs, int n_rels, SnapTable &t) {
+    std::vector<WSlot> h(recs.size());
+    for (size_t i = 0; i < recs.size(); i++) {
+        h[i].key = recs[i].set;
+        h[i].left = recs[i].left;
+        h[i].rec = make_float4(recs[i].rows, recs[i].startup, recs[i].total, 0.0f);
+        const uint32_t wb = (uint32_t)recs[i].width;
+        memcpy(&h[i].rec.w, &wb, 4);
+    }
+    return with_device_stream([&](int device, void *sv) -> int {
+        hipStream_t s = (hipStream_t)sv;
+        WideBufs &w = g_wide;
+        int rc = wide_init(w, device);
+        if (rc) return rc;
+        if ((rc = snap_alloc(t, recs.size(), n_rels, device, s))) return rc;
+        WSlot *d = nullptr;
+        if (hipMalloc((void **)&d, std::max<size_t>(1, h.size()) * sizeof(WSlot)) != hipSuccess) {
+            (void)hipGetLastError();
+            snapshot_free(t);
+            set_error("hipMalloc for the imported records failed");
+            return GQ_ERR_NOMEM;
+        }
+        bool ok = hipMemcpyAsync(d, h.data(), h.size() * sizeof(WSlot), hipMemcpyHostToDevice, s) == hipSuccess &&
+                  hipMemsetAsync(w.misc, 0, 72 * 4, s) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(ks_fill, dim3((uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(
+                                         (h.size() + WBLOCK - 1) / WBLOCK, 4096))),
+                               dim3(WBLOCK), 0, s, snap_wtab(t), (const WSlot *)d, (uint64_t)h.size(), w.misc);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipMemcpyAsync(w.h_misc, w.misc, 72 * 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+                 hipStreamSynchronize(s) == hipSuccess;
+        }
+        (void)hipFree(d);
+        if (!ok) {
+            snapshot_free(t);
+            set_error("memo snapshot: building the table failed");
+            return GQ_ERR_DEVICE;
+        }
+        if (w.h_misc[WERR]) {
+            snapshot_free(t);
+            set_error("memo snapshot: %s", (w.h_misc[WERR] & WERR_PROBE) ? "a set appears twice"
+                                                                          : "the table is full");
+            return GQ_ERR_INVALID;
+        }
+        t.n_sets = h.size();
+        return GQ_OK;
+    });
+}
+
 template <bool DPSUB, bool FAST>
 static void launch_level(const WideHdr &h, uint32_t grid, size_t lds, hipStream_t s, const uint32_t *blob, WTab t,
                          uint32_t *lst, int k, uint32_t off_k, uint32_t n_k, uint32_t off_n, uint32_t cap_n,
@@ -1193,7 +1373,7 @@ static int run_wide_general(const WideInput &in, DPAlgo algo, int max_memo_mb, s
             launches++;
             off_k = off_n;
         }
-        if (!stopped) hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, n, w.nodes, w.misc);
+        if (!stopped) hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, full_set(n), w.nodes, w.misc);
         WCHK(hipGetLastError());
         WCHK(hipEventRecord(w.e1, s));
         const double t_last = host_now_ms();
@@ -1235,6 +1415,7 @@ static int run_wide_general(const WideInput &in, DPAlgo algo, int max_memo_mb, s
         st.t_first_launch = t_first;
         st.t_last_launch = t_last;
         st.t_synced = t_synced;
+        if (g_snap_req && g_snap_req->n == n) return wide_snapshot(t, sets + (uint64_t)n, n, device, s);
         return GQ_OK;
     });
 }
@@ -1305,7 +1486,7 @@ int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<W
                 launch_level<false, false>(h, grid, lds, s, w.blob, t, w.lst, k, off[k], n_k, off_n, cap_n, w.misc);
             launches++;
         }
-        hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, n, w.nodes, w.misc);
+        hipLaunchKernelGGL(kw_walk, dim3(1), dim3(WW), 0, s, t, full_set(n), w.nodes, w.misc);
         WCHK(hipGetLastError());
         WCHK(hipEventRecord(w.e1, s));
         const double t_last = host_now_ms();
@@ -1343,6 +1524,7 @@ int run_wide_dp(const WideInput &in, DPAlgo algo, int max_memo_mb, std::vector<W
         st.t_first_launch = t_first;
         st.t_last_launch = t_last;
         st.t_synced = t_synced;
+        if (g_snap_req && g_snap_req->n == n) return wide_snapshot(t, total, n, device, s);
         return GQ_OK;
     });
 }

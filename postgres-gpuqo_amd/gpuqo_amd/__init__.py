@@ -47,6 +47,14 @@ def lib():
         L.gq_tuning_set.argtypes = [C.POINTER(abi.Tuning)]
         L.gq_autotune.argtypes = [C.c_int32, C.POINTER(C.POINTER(abi.Query)), C.POINTER(abi.Config), C.c_int32,
                                   C.c_int32, C.POINTER(abi.Tuning)]
+        L.gq_plan_snapshot.argtypes = [C.POINTER(abi.Query), C.POINTER(abi.Config), C.POINTER(abi.PlanOut),
+                                       C.POINTER(C.c_void_p)]
+        L.gq_snapshot_subplan.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(abi.PlanOut)]
+        L.gq_snapshot_info.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]
+        L.gq_snapshot_export.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.gq_snapshot_import.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.gq_snapshot_free.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -128,6 +136,64 @@ def plan_batch(queries, cfg):
         outs[i].capacity = 2 * q.n
     _check(lib().gq_plan_batch(k, qptrs, C.byref(cfg), outs))
     return [(plan_tuples(outs[i]), outs[i].stats) for i in range(k)]
+
+
+class Snapshot:
+    """A device-resident memo snapshot (gq_plan_snapshot / gq_snapshot_*):
+    the plan of the query, then the plans of its connected sub-queries."""
+
+    def __init__(self, query=None, cfg=None, handle=None):
+        self.h = None
+        self.plan, self.stats = None, None
+        if handle is not None:
+            self.h = handle
+            return
+        cap = 2 * query.n
+        nodes = (abi.PlanNode * cap)()
+        out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
+        h = C.c_void_p()
+        _check(lib().gq_plan_snapshot(C.byref(query.to_c()), C.byref(cfg), C.byref(out), C.byref(h)))
+        self.h = h
+        self.plan, self.stats = plan_tuples(out), out.stats
+
+    def subplan(self, relids):
+        """Plan nodes of the connected sub-query `relids` (a relation bit mask)."""
+        n_rels, _sets, _bytes = self.info()
+        cap = 2 * n_rels
+        nodes = (abi.PlanNode * cap)()
+        out = abi.PlanOut(C.cast(nodes, C.POINTER(abi.PlanNode)), cap, 0)
+        words = (C.c_uint64 * abi.GQ_MAX_WORDS)(relids)
+        _check(lib().gq_snapshot_subplan(self.h, words, C.byref(out)))
+        return plan_tuples(out)
+
+    def info(self):
+        n, s, b = C.c_int32(), C.c_uint64(), C.c_uint64()
+        _check(lib().gq_snapshot_info(self.h, C.byref(n), C.byref(s), C.byref(b)))
+        return n.value, s.value, b.value
+
+    def export(self):
+        size = C.c_uint64()
+        _check(lib().gq_snapshot_export(self.h, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        _check(lib().gq_snapshot_export(self.h, buf, size.value, C.byref(size)))
+        return buf.raw[:size.value]
+
+    @staticmethod
+    def load(data):
+        h = C.c_void_p()
+        _check(lib().gq_snapshot_import(data, len(data), C.byref(h)))
+        return Snapshot(handle=h)
+
+    def close(self):
+        if self.h:
+            _check(lib().gq_snapshot_free(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Server:
