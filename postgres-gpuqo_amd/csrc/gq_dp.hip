@@ -2256,7 +2256,11 @@ __global__ void k_ds_zero_rows(DsBufs d) {
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_ds_pairs(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
+/* The pair scan of one DPsize level over (left list x right list) tiles.
+ * Four waves per SIMD (at most 128 VGPRs, 8 bytes spilled; left alone the
+ * compiler takes 135 and three waves: clique-18 DPsize 71.3 against
+ * 88.7 ms; five waves spill 136 bytes). */
+__global__ void __launch_bounds__(BLOCK, 4) k_ds_pairs(BlobHdr h, const uint32_t *blob, int k, DsBufs d,
                                                      const float4 *vals, uint32_t *counters) {
     extern __shared__ GQ_LDS_ALIGN uint32_t smem[];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1);
