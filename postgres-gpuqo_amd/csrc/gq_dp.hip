@@ -1070,7 +1070,14 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
                 lft = w1n ? sub : (S ^ sub);
                 if (w1n) j0 = j1;
                 if (LPE == 2) {
-                    store_rel(mr.vals, slot, j0, q.p.work_mem, false);
+                    /* the plain width keeps hash_table_size off the
+                     * latency-bound level's critical path; the large DPs
+                     * of the rank and open-addressing memos pack it, since
+                     * their big levels read these records (the served
+                     * mix's 40 snowflakes of 25-30 relations: 39.3 against
+                     * 34.7 ms packed; snowflake-20 dense: 0.141 against
+                     * 0.146 ms unpacked) */
+                    store_rel(mr.vals, slot, j0, q.p.work_mem, rk != nullptr || mr.keys != nullptr);
                     mr.left[slot] = lft;
                     if (root_rec) {                     /* one winning lane: the caller walks from this set */
                         root_rec[0] = lft;
