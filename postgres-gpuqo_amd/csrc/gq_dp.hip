@@ -787,8 +787,10 @@ __host__ __device__ constexpr size_t tree_lds(int words, int lpe) {
  * 2e+1 cost one split each (half the latency: the form for levels too
  * small to fill the GPU).  The segment reduces the first valid split (rank
  * and NaN flag) and the first minimum of the packed (cost, rank) key with
- * shuffles.  LPE 2: the winning lane writes its record with the plain
- * width.  LPE 1: the winners' records wait in the wave's LDS queue
+ * shuffles.  LPE 2: the winning lane writes its record itself, with the
+ * plain width for a dense memo and with the hash table size packed for the
+ * rank and open-addressing memos (whose big levels read these records).
+ * LPE 1: the winners' records wait in the wave's LDS queue
  * (tree_lds) and are written a wave at a time, one record per lane, with
  * the hash table size packed (store_rel) -- one hash_pack pass per WAVE
  * sets instead of one per iteration.  Waves w0, w0 + wstride, ... cover the
