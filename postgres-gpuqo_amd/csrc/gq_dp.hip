@@ -1840,15 +1840,14 @@ __global__ void k_batch_init(const BatchJob *jobs, int nj, const uint32_t *blob_
             if (J.keys) {                      /* keys cleared beforehand */
                 bool fresh;
                 slot = memo_insert(J.keys, J.hmask, s1, &fresh);
-                if (slot == ~0u) {
-                    atomicOr(&J.counters[49], 8u);
-                    continue;
-                }
+                if (slot == ~0u) atomicOr(&J.counters[49], 8u);
             } else if (J.h.o_rank) {           /* rank memo: off[tid] */
                 slot = blob[J.h.o_rank + tid];
             }
-            store_base(J.vals, slot, J.h, blob, tid);
-            J.left[slot] = 0;
+            if (slot != ~0u) {
+                store_base(J.vals, slot, J.h, blob, tid);
+                J.left[slot] = 0;
+            }
         }
     }
 }
