@@ -80,6 +80,19 @@ typedef enum gq_memo_kind {
                            slot computed from the set (no keys, no probes) */
 } gq_memo_kind;
 
+/* The join cost function: the reference's make variable cost_function
+ * (src/backend/optimizer/gpuqo/Makefile:67-72, gpuqo_cost.cuh:25-38), which
+ * selects GPUQO_COST_FUNCTION at build time; here a config field.  SIMPLE
+ * is gpuqo_cost_simple.cuh (base relation 0.2 x tuples, nested loop
+ * outer.total + outer.rows x inner.total, hash join joinrows + inner.total
+ * + outer.total), COUT is gpuqo_cost_cout.cuh (base relation 0, nested loop
+ * +inf, hash join as SIMPLE); both have zero startup costs. */
+typedef enum gq_cost_function {
+    GQ_COST_POSTGRES = 0,
+    GQ_COST_SIMPLE = 1,
+    GQ_COST_COUT = 2
+} gq_cost_function;
+
 /* PathCost, src/include/optimizer/gpuqo_common.h:81-84. */
 typedef struct gq_path_cost {
     float startup;
@@ -180,6 +193,8 @@ typedef struct gq_config {
     int32_t device;                               /* HIP device ordinal, 0     */
     int32_t profile_kernels;                      /* time each phase with HIP
                                                      events (stats.*_ms)       */
+    /* The reference's build-time cost function (Makefile cost_function). */
+    int32_t cost_function;                        /* GQ_COST_POSTGRES          */
 } gq_config;
 
 /* One node of the returned join tree (QueryTreeC,

@@ -153,6 +153,7 @@ struct Params {           /* GpuqoPlannerInfoParams, planner_info.cuh:79-96 */
     bool enable_seqscan, enable_indexscan, enable_tidscan, enable_sort,
          enable_hashagg, enable_nestloop, enable_mergejoin, enable_hashjoin;
     int work_mem;
+    int cost_function;    /* GPUQO_COST_FUNCTION (gpuqo_cost.cuh:25-38): 0 postgres, 1 simple, 2 cout */
 };
 
 template <int W>
@@ -457,6 +458,38 @@ static bool is_inner_unique(const Bits<W> &outer, const Bits<W> &inner, const In
     return false;
 }
 
+/* The simple cost function (gpuqo_cost_simple.cuh:26-74): BASEREL_COEFF 0.2,
+ * HASHJOIN_COEFF 1.0; every startup cost is 0. */
+static const float BASEREL_COEFF_ = 0.2f, HASHJOIN_COEFF_ = 1.0f;
+
+/* cost_baserel: cost_postgres.cuh:843-847 (the relation's own cost),
+ * cost_simple.cuh:30-37, cost_cout.cuh:26-33 */
+template <int W>
+static PathCost cost_baserel(const BaseRel<W> &b, const Params &p) {
+    if (p.cost_function == 1) return PathCost{0.0f, b.composite ? b.cost.total : BASEREL_COEFF_ * b.tuples};
+    if (p.cost_function == 2) return PathCost{0.0f, b.composite ? b.cost.total : 0.0f};
+    return b.cost;
+}
+
+/* cost_nestloop and cost_hashjoin of the build's cost function:
+ * simple (cost_simple.cuh:44-56, :58-70), C_out (cost_cout.cuh:40-52,
+ * :54-66), postgres (above) */
+template <int W>
+static PathCost nestloop_of(const Bits<W> &outer_id, const JoinRel &outer, const Bits<W> &inner_id,
+                            const JoinRel &inner, const CostExtra &extra, const Info<W> &info) {
+    if (info.params.cost_function == 1) return PathCost{0.0f, outer.cost.total + outer.rows * inner.cost.total};
+    if (info.params.cost_function == 2) return PathCost{0.0f, INFF_};
+    return cost_nestloop(outer_id, outer, inner_id, inner, extra, info);
+}
+template <int W>
+static PathCost hashjoin_of(const Bits<W> &outer_id, const JoinRel &outer, const Bits<W> &inner_id,
+                            const JoinRel &inner, const CostExtra &extra, const Info<W> &info) {
+    if (info.params.cost_function == 1)
+        return PathCost{0.0f, HASHJOIN_COEFF_ * extra.joinrows + inner.cost.total + outer.cost.total};
+    if (info.params.cost_function == 2) return PathCost{0.0f, extra.joinrows + inner.cost.total + outer.cost.total};
+    return cost_hashjoin(outer_id, outer, inner_id, inner, extra, info);
+}
+
 /* calc_join_cost: gpuqo_cost.cuh:76-158 */
 template <int W>
 static PathCost calc_join_cost(const Bits<W> &outer_id, const JoinRel &outer, const Bits<W> &inner_id,
@@ -470,10 +503,10 @@ static PathCost calc_join_cost(const Bits<W> &outer_id, const JoinRel &outer, co
     extra.match_count = inner.rows;
     min_cost.startup = 0.0f;
     min_cost.total = NANF_;
-    tmp = cost_nestloop(outer_id, outer, inner_id, inner, extra, info);
+    tmp = nestloop_of(outer_id, outer, inner_id, inner, extra, info);
     if (std::isnan(min_cost.total) || tmp.total < min_cost.total) min_cost = tmp;
     if (info.params.enable_hashjoin) {
-        tmp = cost_hashjoin(outer_id, outer, inner_id, inner, extra, info);
+        tmp = hashjoin_of(outer_id, outer, inner_id, inner, extra, info);
         if (tmp.total < min_cost.total) min_cost = tmp;
     }
     extra.indexed_join_quals = true;
@@ -492,7 +525,7 @@ static PathCost calc_join_cost(const Bits<W> &outer_id, const JoinRel &outer, co
                 ir.cost.startup = v.index_cost.startup;
                 ir.cost.total = v.index_cost.total;
                 ir.width = inner.width;   /* not read by cost_nestloop */
-                tmp = cost_nestloop(outer_id, outer, inner_id, ir, extra, info);
+                tmp = nestloop_of(outer_id, outer, inner_id, ir, extra, info);
                 if (tmp.total < min_cost.total) min_cost = tmp;
             }
         }
@@ -821,7 +854,7 @@ static void init_memo(const Info<W> &info, Memo<W> &memo) {     /* cpu_sequentia
         JR<W> *jr = new JR<W>();
         jr->id = info.base_rels[i].id;
         jr->lp = jr->rp = nullptr;
-        jr->v.cost = info.base_rels[i].cost;
+        jr->v.cost = cost_baserel(info.base_rels[i], info.params);
         jr->v.width = info.base_rels[i].width;
         jr->v.rows = info.base_rels[i].rows;
         jr->edges = info.edge_table[i];
@@ -1131,7 +1164,7 @@ static QueryTree<W> *parallel_cpu_dpsub(const Info<W> &info, bool bicc, int n_th
     for (int i = 0; i < info.n_rels; i++) {
         JR<W> *jr = new JR<W>();
         jr->id = info.base_rels[i].id; jr->lp = jr->rp = nullptr;
-        jr->v.cost = info.base_rels[i].cost; jr->v.width = info.base_rels[i].width;
+        jr->v.cost = cost_baserel(info.base_rels[i], info.params); jr->v.width = info.base_rels[i].width;
         jr->v.rows = info.base_rels[i].rows; jr->edges = info.edge_table[i];
         memo.buckets[1].insert(std::make_pair(jr->id, jr));
     }
@@ -1490,7 +1523,7 @@ static QueryTree<1> *dpe_dpccp(const Info<1> &info, int n_threads, int pairs_per
         DpeRel *d = new DpeRel();
         d->r.id = info.base_rels[i].id;
         d->r.lp = d->r.rp = nullptr;
-        d->r.v.cost = info.base_rels[i].cost;
+        d->r.v.cost = cost_baserel(info.base_rels[i], info.params);
         d->r.v.width = info.base_rels[i].width;
         d->r.v.rows = info.base_rels[i].rows;
         d->r.edges = info.edge_table[i];
@@ -1785,8 +1818,8 @@ static GraphEdge<W> *createGraphEdge(int li, int ri, const Info<W> &info) {   /*
     e->rows = e->selectivity * lr * rr;
     e->left_size = 1; e->right_size = 1; e->total_size = 2;
     JoinRel L, R;
-    L.cost = info.base_rels[li].cost; L.width = info.base_rels[li].width; L.rows = info.base_rels[li].rows;
-    R.cost = info.base_rels[ri].cost; R.width = info.base_rels[ri].width; R.rows = info.base_rels[ri].rows;
+    L.cost = cost_baserel(info.base_rels[li], info.params); L.width = info.base_rels[li].width; L.rows = info.base_rels[li].rows;
+    R.cost = cost_baserel(info.base_rels[ri], info.params); R.width = info.base_rels[ri].width; R.rows = info.base_rels[ri].rows;
     e->cost = calc_join_cost(e->left, L, e->right, R, e->rows, info);
     e->width = L.width + R.width;
     e->weight = e->rows;
@@ -1923,6 +1956,7 @@ static Params make_params(const gq_config *c) {
     p.enable_hashagg = c->enable_hashagg; p.enable_nestloop = c->enable_nestloop;
     p.enable_mergejoin = c->enable_mergejoin; p.enable_hashjoin = c->enable_hashjoin;
     p.work_mem = c->work_mem;
+    p.cost_function = c->cost_function;
     return p;
 }
 
@@ -2137,8 +2171,11 @@ int oracle_dp_first_levels(const gq_query *q, const gq_config *cfg, int max_leve
  *            {inner_unique, indexed_join_quals, joinrows, outer_match_frac,
  *            match_count};
  *   which 3: estimate_hash_bucket_stats (:620-668), nbuckets = extra_in[0];
- *   which 4: is_inner_unique (gpuqo_cost.cuh:43-74).
- * out = {startup, total} (0-2), {mcvfreq, bucketsize} (3), {flag} (4). */
+ *   which 4: is_inner_unique (gpuqo_cost.cuh:43-74);
+ *   which 5: cost_baserel of l's lowest relation (cost_postgres.cuh:843-847,
+ *            cost_simple.cuh:30-37, cost_cout.cuh:26-33).
+ * Which 0-2 and 5 follow cfg->cost_function.
+ * out = {startup, total} (0-2, 5), {mcvfreq, bucketsize} (3), {flag} (4). */
 int oracle_cost_probe(const gq_query *q, const gq_config *cfg, int which, const uint64_t *l_words,
                       const uint64_t *r_words, const float *lvals, const float *rvals, const float *extra_in,
                       uint64_t composite_mask, float *out) {
@@ -2159,14 +2196,19 @@ int oracle_cost_probe(const gq_query *q, const gq_config *cfg, int which, const 
         out[0] = c.startup; out[1] = c.total;
         break;
     }
-    case 1: { oracle::PathCost c = oracle::cost_nestloop(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
-    case 2: { oracle::PathCost c = oracle::cost_hashjoin(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
+    case 1: { oracle::PathCost c = oracle::nestloop_of(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
+    case 2: { oracle::PathCost c = oracle::hashjoin_of(l, L, r, R, ex, *info); out[0] = c.startup; out[1] = c.total; break; }
     case 3: {
         oracle::BucketStats b = oracle::estimate_hash_bucket_stats(l, r, (int)extra_in[0], *info);
         out[0] = b.mcvfreq; out[1] = b.bucketsize;
         break;
     }
     case 4: out[0] = oracle::is_inner_unique(l, r, *info) ? 1.0f : 0.0f; break;
+    case 5: {   /* cost_baserel of the lowest relation of l */
+        oracle::PathCost c = oracle::cost_baserel(info->base_rels[l.lowestPos() - 1], info->params);
+        out[0] = c.startup; out[1] = c.total;
+        break;
+    }
     default: st = GQ_ERR_INVALID;
     }
     delete info;

@@ -100,7 +100,7 @@ def dp_first_levels(query, cfg, max_levels):
     return st
 
 
-PROBE_JOIN, PROBE_NESTLOOP, PROBE_HASHJOIN, PROBE_BUCKET_STATS, PROBE_INNER_UNIQUE = range(5)
+PROBE_JOIN, PROBE_NESTLOOP, PROBE_HASHJOIN, PROBE_BUCKET_STATS, PROBE_INNER_UNIQUE, PROBE_BASEREL = range(6)
 
 
 def cost_probe(query, cfg, which, l_words, r_words, lvals=(0, 0, 0, 0), rvals=(0, 0, 0, 0), extra=(0, 0, 0, 0, 0),
@@ -108,8 +108,9 @@ def cost_probe(query, cfg, which, l_words, r_words, lvals=(0, 0, 0, 0), rvals=(0
     """One branch of the oracle's cost model (see oracle_cost_probe):
     which = PROBE_JOIN (extra[2] = join rows), PROBE_NESTLOOP / PROBE_HASHJOIN
     (extra = inner_unique, indexed_join_quals, joinrows, outer_match_frac,
-    match_count), PROBE_BUCKET_STATS (extra[0] = nbuckets) or
-    PROBE_INNER_UNIQUE.  Returns two floats (one for PROBE_INNER_UNIQUE)."""
+    match_count), PROBE_BUCKET_STATS (extra[0] = nbuckets),
+    PROBE_INNER_UNIQUE or PROBE_BASEREL (cost_baserel of l's lowest
+    relation).  Returns two floats (one for PROBE_INNER_UNIQUE)."""
     nw = query.n_words
     lw = (C.c_uint64 * nw)(*l_words)
     rw = (C.c_uint64 * nw)(*r_words)

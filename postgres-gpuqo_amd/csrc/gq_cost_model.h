@@ -98,7 +98,29 @@ struct Params {             /* GpuqoPlannerInfoParams subset (float), planner_in
     int work_mem;
     int enable_nestloop;
     int enable_hashjoin;
+    int cost_fn;            /* COST_FN_*: the reference's build-time GPUQO_COST_FUNCTION */
 };
+
+/* Cost functions (gpuqo_cost.cuh:25-38): postgres (gpuqo_cost_postgres.cuh),
+ * simple (gpuqo_cost_simple.cuh) and C_out (gpuqo_cost_cout.cuh). */
+#define COST_FN_POSTGRES 0
+#define COST_FN_SIMPLE 1
+#define COST_FN_COUT 2
+
+/* cost_baserel: the path cost of a base relation as a DP's memo starts
+ * with it.  postgres (cost_postgres.cuh:843-847): the relation's own cost;
+ * simple (cost_simple.cuh:30-37): startup 0, total 0.2f * tuples unless the
+ * relation is composite (an IDP or UnionDP sub-plan: its total); C_out
+ * (cost_cout.cuh:26-33): startup 0, total 0 unless composite. */
+GQ_HD void base_cost(const Params &p, float tuples, bool composite, float *startup, float *total) {
+    if (p.cost_fn == COST_FN_SIMPLE) {
+        *startup = 0.0f;
+        if (!composite) *total = 0.2f * tuples;
+    } else if (p.cost_fn == COST_FN_COUT) {
+        *startup = 0.0f;
+        if (!composite) *total = 0.0f;
+    }
+}
 
 /* Member flags */
 #define GQ_MF_BUCKET_DEFAULT 1u   /* stadistinct resolves to <= 0: bucket size 0.1 */
@@ -585,10 +607,46 @@ struct EdgeProv {
     }
 };
 
+/* calc_join_cost (gpuqo_cost.cuh:76-158) under the simple and C_out cost
+ * functions, whose startup costs are 0: the nested loop costs
+ * outer.total + outer.rows * inner.total (cost_simple.cuh:44-56) or +inf
+ * (cost_cout.cuh:40-52); the hash join, when enabled, joinrows +
+ * inner.total + outer.total (cost_simple.cuh:58-70 with HASHJOIN_COEFF
+ * 1.0f, an exact product; cost_cout.cuh:54-66); the index nested loop, for
+ * a non-composite single inner relation, the nested loop over each crossing
+ * member's index (its cost's total), in class order.  Each later candidate
+ * replaces the current one when strictly cheaper. */
+template <class S, class Prov>
+GQ_HD Rel join_plain(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const Prov &pv) {
+    Rel out;
+    float bsize = 1.0f, mcvfreq = 1.0f;
+    out.rows = join_rows(pv.scan(q, l, r, false, 0.0f, 0.0f, &bsize, &mcvfreq), L.rows, R.rows);
+    out.width = L.width + R.width;
+    out.hs = 0u;
+    const bool simple = q.p.cost_fn == COST_FN_SIMPLE;
+    float bt = simple ? L.total + L.rows * R.total : __builtin_inff();
+    if (q.p.enable_hashjoin) {
+        const float ht = out.rows + R.total + L.total;
+        if (ht < bt) bt = ht;
+    }
+    if (simple && popc(r) == 1 && !bit(q.composite, ctz(r))) {
+        pv.for_each_inner(q, r, [&](int mi) {
+            if (q.mem_flags[mi] & GQ_MF_INDEX) {
+                const float it = L.total + L.rows * q.mem_idx_total[mi];
+                if (it < bt) bt = it;
+            }
+        });
+    }
+    out.startup = 0.0f;
+    out.total = bt;
+    return out;
+}
+
 /* One join: rows (estimate_join_rows) and cost (calc_join_cost,
  * gpuqo_cost.cuh:76-158) of outer `l` joined with inner `r`. */
 template <class S, class Prov>
 GQ_HD Rel join_prov(const View<S> &q, const S &l, const Rel &L, const S &r, const Rel &R, const Prov &pv) {
+    if (__builtin_expect(q.p.cost_fn != COST_FN_POSTGRES, 0)) return join_plain(q, l, L, r, R, pv);
     Rel out;
     /* the inner's hash table size first (it depends on the inner alone), so
      * one pass over the crossing classes gives the selectivity and the

@@ -1178,8 +1178,11 @@ __device__ __forceinline__ void expand_level(const uint32_t *adj, const uint32_t
  * slots go to next_slot), BLOCK level-k sets each.  The level's list has room for `cap` entries: each lane loads its
  * first set before the level size is known, so that this load, the level
  * size and the blob staging are in flight together. */
+/* The one-lane per-edge-table form (the large levels) is bounded to five
+ * waves per SIMD (at most 96 VGPRs, no spills): the compiler's own choice
+ * can land just above 96 and drop it to four. */
 template <int ECW, bool FAST, int LPE, uint32_t flags>
-__global__ void __launch_bounds__(BLOCK) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
+__global__ void __launch_bounds__(BLOCK, (FAST && LPE == 1) ? 5 : 1) k_eval_tree(BlobHdr h, const uint32_t *blob, int k, int bicc,
                                                       const uint32_t *lvl_sets, const uint32_t *lvl_slot,
                                                       MemoRef mr, uint32_t *counters, uint32_t ev_blocks,
                                                       uint32_t *next, uint32_t cap, uint32_t *next_slot,

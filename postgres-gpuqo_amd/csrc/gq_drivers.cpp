@@ -358,6 +358,7 @@ static QT<1> *single_leaf(RunCtx &ctx, const HQuery<1> &h) {
     t->rows = h.rows[0];
     t->startup = h.startup[0];
     t->total = h.total[0];
+    base_cost(h.p, h.tuples[0], h.composite[0] != 0, &t->startup, &t->total);   /* cost_baserel */
     t->width = h.width[0];
     return t;
 }

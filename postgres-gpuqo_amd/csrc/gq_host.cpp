@@ -484,11 +484,13 @@ int build_dp_input(const HQuery<1> &q, DPInput &in) {
     h.o_base = take(4 * n);
     in.base.resize(n);
     for (int i = 0; i < n; i++) {
+        float st = q.startup[i], tot = q.total[i];
+        base_cost(q.p, q.tuples[i], q.composite[i] != 0, &st, &tot);   /* cost_baserel */
         put_f(h.o_base, 4 * i, q.rows[i]);
-        put_f(h.o_base, 4 * i + 1, q.startup[i]);
-        put_f(h.o_base, 4 * i + 2, q.total[i]);
+        put_f(h.o_base, 4 * i + 1, st);
+        put_f(h.o_base, 4 * i + 2, tot);
         b[h.o_base + 4 * i + 3] = (uint32_t)q.width[i];
-        in.base[i] = Rel{q.rows[i], q.startup[i], q.total[i], q.width[i]};
+        in.base[i] = Rel{q.rows[i], st, tot, q.width[i]};
     }
     while (b.size() & 3) b.push_back(0u);
     h.words = (int)b.size();
@@ -516,6 +518,7 @@ static Params make_params(const gq_config *c) {     /* setParams, gpuqo_planner_
     p.work_mem = c->work_mem;
     p.enable_nestloop = c->enable_nestloop;
     p.enable_hashjoin = c->enable_hashjoin;
+    p.cost_fn = c->cost_function;
     return p;
 }
 
@@ -524,6 +527,15 @@ static Set<W> set_from_words(const uint64_t *w, int nw) {
     Set<W> s;
     for (int i = 0; i < nw && i < W; i++) s.w[i] = w[i];
     return s;
+}
+
+int validate_config(const gq_config *c) {
+    if (c->cost_function != GQ_COST_POSTGRES && c->cost_function != GQ_COST_SIMPLE &&
+        c->cost_function != GQ_COST_COUT) {
+        set_error("cost_function %d is not GQ_COST_POSTGRES, GQ_COST_SIMPLE or GQ_COST_COUT", c->cost_function);
+        return GQ_ERR_INVALID;
+    }
+    return GQ_OK;
 }
 
 int validate(const gq_query *q) {
@@ -841,6 +853,7 @@ static gq_status plan_impl(const gq_query *q, const gq_config *cfg, gq_plan_out 
         gq::set_error("null config or output");
         return GQ_ERR_INVALID;
     }
+    if ((rc = gq::validate_config(cfg))) return (gq_status)rc;
     if (cfg->gpuqo_algorithm != GQ_ALGO_DPSUB && cfg->gpuqo_algorithm != GQ_ALGO_DPSIZE) {
         gq::set_error("algorithm %d is not a GPU algorithm (use dpsub or dpsize)", cfg->gpuqo_algorithm);
         return GQ_ERR_UNSUPPORTED;
@@ -860,6 +873,7 @@ static gq_status plan_noalloc_impl(const gq_query *q, const gq_config *cfg, gq_p
         gq::set_error("null config or output");
         return GQ_ERR_INVALID;
     }
+    if ((rc = gq::validate_config(cfg))) return (gq_status)rc;
     if (cfg->gpuqo_algorithm != GQ_ALGO_DPSUB) {
         gq::set_error("the allocation-free path plans GQ_ALGO_DPSUB (plain or bicc) only");
         return GQ_ERR_UNSUPPORTED;
@@ -897,7 +911,9 @@ static gq_status plan_batch_impl(int32_t n_queries, const gq_query *const *qs, c
         gq::set_error("bad batch arguments");
         return GQ_ERR_INVALID;
     }
-    int rc = gq::device_init(cfg->device);
+    int rc = gq::validate_config(cfg);
+    if (rc) return (gq_status)rc;
+    rc = gq::device_init(cfg->device);
     if (rc) return (gq_status)rc;
     /* Plain DPs (no IDP decomposition) that the fused kernel takes run in one
      * launch; every other query goes through gq_plan. */
@@ -1027,7 +1043,9 @@ static gq_status autotune_impl(int32_t n_queries, const gq_query *const *qs, con
         gq::set_error("bad autotune arguments");
         return GQ_ERR_INVALID;
     }
-    int rc = gq::device_init(cfg->device);
+    int rc = gq::validate_config(cfg);
+    if (rc) return (gq_status)rc;
+    rc = gq::device_init(cfg->device);
     if (rc) return (gq_status)rc;
     std::vector<std::vector<gq_plan_node>> bufs(n_queries);
     std::vector<gq_plan_out> outs(n_queries);
@@ -1130,6 +1148,7 @@ gq_status gq_plan_snapshot(const gq_query *q, const gq_config *cfg, gq_plan_out 
             gq::set_error("null config or output");
             return GQ_ERR_INVALID;
         }
+        if ((rc = gq::validate_config(cfg))) return rc;
         gq::SnapTable t;
         rc = gq::plan_snapshot_impl(q, cfg, out, &t);
         if (rc) return rc;

@@ -16,6 +16,10 @@ void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
  * GQ_ERR_INVALID with the message set). */
 int validate(const gq_query *q);
 
+/* Check the config fields with a closed set of values the DP reads
+ * (cost_function), GQ_OK or GQ_ERR_INVALID with the message set. */
+int validate_config(const gq_config *c);
+
 /* Bind the device context (stream, constant tables) to `device`. */
 int device_init(int device);
 

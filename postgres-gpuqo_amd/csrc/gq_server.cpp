@@ -441,6 +441,7 @@ static gq_status server_create_impl(const gq_config *cfg, uint64_t memo_budget_b
         set_error("bad server arguments");
         return GQ_ERR_INVALID;
     }
+    if (int rc = validate_config(cfg)) return (gq_status)rc;
     *out = nullptr;
     std::unique_ptr<gq_server> owner(new gq_server());
     Server &S = owner->s;
@@ -477,6 +478,7 @@ static gq_status server_submit_impl(gq_server *srv, const gq_query *q, const gq_
         return GQ_ERR_INVALID;
     }
     if (int rc = validate(q)) return (gq_status)rc;
+    if (int rc = validate_config(c)) return (gq_status)rc;
     std::unique_ptr<Request> r(new Request());
     r->tag = tag;
     r->cfg = *c;

@@ -123,9 +123,11 @@ int build_wide_input(const HQuery<1> &q, WideInput &in) {
     for (int i = 0; i < n; i++) {
         put_f(h.o_rel_rows, i, hv.rel_rows[i]);
         put_f(h.o_rel_fksel, i, hv.rel_fksel[i]);
+        float st = q.startup[i], tot = q.total[i];
+        base_cost(q.p, q.tuples[i], q.composite[i] != 0, &st, &tot);   /* cost_baserel */
         put_f(h.o_base, 4 * i, q.rows[i]);
-        put_f(h.o_base, 4 * i + 1, q.startup[i]);
-        put_f(h.o_base, 4 * i + 2, q.total[i]);
+        put_f(h.o_base, 4 * i + 1, st);
+        put_f(h.o_base, 4 * i + 2, tot);
         b[h.o_base + 4 * i + 3] = (uint32_t)q.width[i];
     }
     align(2);

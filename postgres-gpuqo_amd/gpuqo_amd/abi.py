@@ -29,6 +29,7 @@ ALGO = {
 }
 IDP = {"idp1": 0, "idp2": 1, "union": 2}
 MEMO = {"auto": 0, "dense": 1, "hash": 2, "rank": 3}
+COST_FUNCTION = {"postgres": 0, "simple": 1, "cout": 2}
 
 
 class PathCost(C.Structure):
@@ -94,7 +95,8 @@ class Config(C.Structure):
                 ("enable_mergejoin", C.c_uint8), ("enable_hashjoin", C.c_uint8),
                 ("memo_kind", C.c_int32),
                 ("device", C.c_int32),
-                ("profile_kernels", C.c_int32)]
+                ("profile_kernels", C.c_int32),
+                ("cost_function", C.c_int32)]
 
 
 class PlanNode(C.Structure):
@@ -170,6 +172,7 @@ def default_config(**overrides):
     c.memo_kind = MEMO["auto"]
     c.device = 0
     c.profile_kernels = 0
+    c.cost_function = COST_FUNCTION["postgres"]
     for k, v in overrides.items():
         if k == "algorithm":
             v = ALGO[v] if isinstance(v, str) else v
@@ -179,6 +182,8 @@ def default_config(**overrides):
             k = "gpuqo_idp_type"
         elif k == "memo_kind" and isinstance(v, str):
             v = MEMO[v]
+        elif k == "cost_function" and isinstance(v, str):
+            v = COST_FUNCTION[v]
         elif not hasattr(c, k) and hasattr(c, "gpuqo_" + k):
             k = "gpuqo_" + k
         if not hasattr(c, k):

@@ -179,6 +179,12 @@ gpuqo_amd_config_from_gucs(int algorithm, gq_config *cfg)
 	cfg->enable_nestloop = enable_nestloop;
 	cfg->enable_mergejoin = enable_mergejoin;
 	cfg->enable_hashjoin = enable_hashjoin;
+	/* the reference picks its cost function at build time (Makefile
+	 * cost_function=postgres|simple|cout -> -DGPUQO_COST_FUNCTION=0|1|2);
+	 * the same define selects it here */
+#ifdef GPUQO_COST_FUNCTION
+	cfg->cost_function = GPUQO_COST_FUNCTION;
+#endif
 }
 
 static QueryTreeC *

@@ -418,7 +418,29 @@ def test_each_guc_sets_its_own_field(guc, ctype, field, value):
 def test_every_config_field_has_a_guc():
     mapped = {g[2] for g in GUCS} | {"gpuqo_algorithm"}
     own = {"memo_kind", "device", "profile_kernels"}           # no GUC counterpart
-    assert mapped | own == {k for k, _ in abi.Config._fields_}
+    build_time = {"cost_function"}                             # a make variable in the reference
+    assert mapped | own | build_time == {k for k, _ in abi.Config._fields_}
+
+
+@pytest.mark.parametrize("define,want", [(None, 0), ("1", 1), ("2", 2)])
+def test_cost_function_define(tmp_path, define, want):
+    """The reference picks its cost function at build time (Makefile
+    cost_function -> -DGPUQO_COST_FUNCTION=0|1|2); the shim built with the
+    same define passes it as gq_config.cost_function."""
+    out = str(tmp_path / "libshim_cf.so")
+    lib_dir = os.path.join(HERE, "..", "postgres-gpuqo_amd", "lib")
+    cmd = ["gcc", "-std=gnu99", "-O1", "-fPIC", "-shared", "-I" + os.path.join(SHIM_DIR, "include"),
+           "-I" + os.path.join(HERE, "..", "include"), "-o", out,
+           os.path.join(HERE, "..", "postgres-gpuqo_amd", "shim", "gpuqo_run_amd.c"),
+           os.path.join(SHIM_DIR, "pg_standin.c"), "-L" + lib_dir, "-lgpuqo_amd", "-Wl,-rpath," + lib_dir]
+    if define is not None:
+        cmd.insert(1, "-DGPUQO_COST_FUNCTION=" + define)
+    subprocess.run(cmd, check=True)
+    gpuqo_amd.lib()
+    L = C.CDLL(out)
+    c = abi.Config()
+    L.gpuqo_amd_config_from_gucs(abi.ALGO["dpsub"], C.byref(c))
+    assert c.cost_function == want
 
 
 def members_of(b):
