@@ -1046,19 +1046,25 @@ __device__ void eval_level_tree(const BlobHdr &h, uint32_t *smem, const View<uin
             key = k0 < k1 ? k0 : k1;
         }
         /* segmented min over the set's lanes: after the step with offset
-         * `off`, lane t holds the minimum over [t, min(t + 2 off, per)) */
-        uint32_t sf = first;
+         * `off`, lane t holds the minimum over [t, min(t + 2 off, per)).
+         * The first valid split's NaN flag decides only when some lane's
+         * lower-ranked orientation costs NaN: otherwise no set's first split
+         * has the flag, and that reduction is skipped (wave-uniform). */
+        const int head = (int)(seg * per);               /* the set's first lane holds its minimum */
+        uint32_t sf = 0u;
+        if (__ballot(active && (first & 1u)) != 0ull) {
+            sf = first;
+            for (uint32_t off = 1; off < per; off <<= 1) {
+                const uint32_t f2 = __shfl_down(sf, off);
+                if (t + off < per) sf = f2 < sf ? f2 : sf;
+            }
+            sf = __shfl(sf, head < WAVE ? head : 0);
+        }
         unsigned long long sk = key;
         for (uint32_t off = 1; off < per; off <<= 1) {
-            const uint32_t f2 = __shfl_down(sf, off);
             const unsigned long long k2 = __shfl_down(sk, off);
-            if (t + off < per) {
-                sf = f2 < sf ? f2 : sf;
-                sk = k2 < sk ? k2 : sk;
-            }
+            if (t + off < per) sk = k2 < sk ? k2 : sk;
         }
-        const int head = (int)(seg * per);               /* the set's first lane holds its minimum */
-        sf = __shfl(sf, head < WAVE ? head : 0);
         sk = __shfl(sk, head < WAVE ? head : 0);
         bool win = false;
         uint32_t slot = 0, lft = 0;

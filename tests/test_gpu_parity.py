@@ -347,3 +347,28 @@ def test_spanning_tree_mode_past_64_relations(idp, k):
     want, _ = oracle_bits(g, c)
     got, _ = gpu_bits(g, c)
     assert_same_plan(got, want, f"chorded80 {idp} k={k} spanning tree")
+
+
+def _nan_base(g, rel):
+    g.build()
+    g.base[rel]["total"] = float("nan")
+    return g
+
+
+@pytest.mark.parametrize("name,g", [
+    ("snowflake12", _nan_base(qgen.snowflake(12, 61), 5)),
+    ("star10", _nan_base(qgen.star(10, 62), 0)),
+    ("snowflake22", _nan_base(qgen.snowflake(22, 63), 9)),
+    ("chain16", _nan_base(qgen.chain(16, 64), 7)),
+])
+def test_tree_nan_costs(name, g):
+    """A base relation with a NaN total makes every plan containing it cost
+    NaN, so the tree evaluators' first-valid rule (the lowest-ranked split
+    wins when its cost is NaN) decides those sets, in the two-lane levels,
+    the one-lane levels and the batch kernel."""
+    c = cfg("dpsub", 1)
+    want, _ = oracle_bits(g, c)
+    got, _ = gpu_bits(g, c)
+    assert_same_plan(got, want, f"{name} NaN base cost")
+    batch = gpuqo_amd.plan_batch([g, qgen.snowflake(14, 65)], c)
+    assert_same_plan(gpuqo_amd.plan_bits(batch[0][0]), want, f"{name} NaN base cost, batched")
