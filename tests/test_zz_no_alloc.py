@@ -84,3 +84,16 @@ def test_noalloc_rejects_what_it_cannot_plan():
     idp = abi.default_config(algorithm="dpsub", dpsub_bicc=1, idp_type="idp2", idp_n_iters=5)
     assert _noalloc(g, idp)[0] == abi.GQ_ERR_UNSUPPORTED
     assert _noalloc(g, abi.default_config(algorithm="dpsub", memo_kind=3))[0] == abi.GQ_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fn", ["simple", "cout"])
+def test_noalloc_cost_functions(fn):
+    """The allocation-free path plans under the simple and C_out cost
+    functions as gq_plan does."""
+    for g, kw in ((qgen.query_snowflake20(), MPDP), (qgen.clique(12, 0x7705), DPSUB)):
+        c = abi.default_config(cost_function=fn, **kw)
+        st, got, _ = _noalloc(g, c)
+        assert st == 0
+        want, _ = gpuqo_amd.plan(g, c)
+        assert_same_plan(got, gpuqo_amd.plan_bits(want), f"noalloc {fn}")
